@@ -1,0 +1,220 @@
+/*
+ * mf3d.h -- C ABI of the MI355X-native Darcy head solver (libmf3d_hip.so).
+ *
+ * Plain C: POD descriptors, caller-owned device pointers with explicit element
+ * counts, an explicit HIP stream (hipStream_t passed as void*, NULL = null
+ * stream), and an int status (0 = MF3D_OK). No exception crosses this ABI;
+ * mf3d_last_error() gives the detail text of the calling thread's last
+ * failure. Every entry point enqueues its work on the given stream and
+ * returns without synchronising unless its comment says otherwise.
+ *
+ * Each entry point names the MacroFlow3D function it replaces (paths relative
+ * to the MacroFlow3D source root, file:line of the declaration/definition).
+ */
+#ifndef MF3D_H
+#define MF3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF3D_ABI_VERSION 1
+
+typedef void* mf3d_stream; /* hipStream_t */
+
+/* Status codes. */
+enum {
+    MF3D_OK = 0,
+    MF3D_ERR_INVALID_ARGUMENT = 1, /* bad pointer, BC pair, non-positive size ... */
+    MF3D_ERR_SIZE_MISMATCH = 2,    /* buffer shorter than the grid, wrong 2:1 sizes */
+    MF3D_ERR_BREAKDOWN = 3,        /* (p, Ap) zero or not finite */
+    MF3D_ERR_HIP = 4,              /* a HIP runtime call failed */
+    MF3D_ERR_ALLOC = 5,            /* device allocation failed */
+    MF3D_ERR_SINGULAR = 6,         /* coarsest operator not definite (direct solve) */
+    MF3D_ERR_UNSUPPORTED = 7       /* combination not implemented */
+};
+
+/* Grid3D (src/core/Grid3D.hpp:32-55). Cell (i,j,k) is at i + nx*(j + ny*k). */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz;
+} mf3d_grid;
+
+/* BCType (src/core/BCSpec.hpp:13) */
+enum { MF3D_BC_DIRICHLET = 0, MF3D_BC_NEUMANN = 1, MF3D_BC_PERIODIC = 2 };
+
+/* BCSpecDevice (src/core/BCSpecDevice.cuh:12-20): faces xmin, xmax, ymin, ymax, zmin, zmax. */
+typedef struct {
+    uint8_t type[6];
+    double value[6];
+} mf3d_bc;
+
+/* HeadSolverType (src/physics/flow/solve_head.cuh:52-56) */
+enum { MF3D_SOLVER_CG = 0, MF3D_SOLVER_MG = 1, MF3D_SOLVER_PCG_MG = 2 };
+/* PinMode (src/numerics/pin_spec.hpp:56-60) */
+enum { MF3D_PIN_AUTO = 0, MF3D_PIN_ON = 1, MF3D_PIN_OFF = 2 };
+
+/* Extension enums (value 0 = reference behaviour). */
+enum { MF3D_GUESS_REFERENCE = 0, MF3D_GUESS_ZERO = 1, MF3D_GUESS_PROVIDED = 2 };
+enum { MF3D_COARSE_GSRB = 0, MF3D_COARSE_DIRECT = 1 };
+enum { MF3D_STOP_INITIAL_RESIDUAL = 0, MF3D_STOP_RHS_NORM = 1 };
+enum { MF3D_PRECOND_FP64 = 0, MF3D_PRECOND_FP32 = 1 };
+enum { MF3D_COEF_FP32 = 0, MF3D_COEF_BF16 = 1, MF3D_COEF_FP16 = 2 };
+
+/*
+ * HeadSolveConfig (src/physics/flow/solve_head.cuh:64-131), field for field,
+ * followed by extensions whose zero value keeps the reference behaviour.
+ */
+typedef struct {
+    int solver_type;     /* MF3D_SOLVER_* (default PCG_MG) */
+    int mg_levels;       /* 4 */
+    int mg_pre_smooth;   /* 2 */
+    int mg_post_smooth;  /* 2 */
+    int mg_max_cycles;   /* 20 */
+    int mg_coarse_iters; /* 50 */
+    int cg_max_iter;     /* 1000 */
+    int cg_check_every;  /* 10 */
+    double cg_rtol;      /* 1e-8 */
+    double rtol;         /* 1e-6 */
+    int pin_mode;        /* MF3D_PIN_* */
+    /* extensions */
+    int init_guess;        /* MF3D_GUESS_* */
+    int coarse_solver;     /* MF3D_COARSE_* */
+    int stop_norm;         /* MF3D_STOP_* */
+    int precond_precision; /* MF3D_PRECOND_* */
+    int coef_format;       /* MF3D_COEF_* (FP32 preconditioner only) */
+    int true_residual;     /* 1: recompute ||b - A h|| in FP64 after the solve */
+    int reserved[5];
+} mf3d_head_solve_config;
+
+/* HeadSolveResult (src/physics/flow/solve_head.cuh:136-141) + extensions. */
+typedef struct {
+    int num_iterations;
+    int converged;
+    double initial_residual;
+    double final_residual;
+    double bnorm;         /* ||b|| of the head system */
+    double true_residual; /* ||b - A h|| when requested, else -1 */
+    double setup_ms;      /* device time of hierarchy/coefficient setup */
+    double solve_ms;      /* device time of the iteration */
+    int reserved[4];
+} mf3d_head_solve_result;
+
+/* Opaque FlowWorkspace (src/physics/common/workspaces.cuh:111-167): MG
+ * hierarchy, PCG vectors, reduction scratch. Device memory is allocated
+ * lazily on first use for a grid and reused while the grid is unchanged
+ * (solve_head.cu:89-94). Not re-entrant: one stream at a time per workspace. */
+typedef struct mf3d_workspace mf3d_workspace;
+
+/* ---- library --------------------------------------------------------- */
+int mf3d_abi_version(void);
+const char* mf3d_status_string(int status);
+const char* mf3d_last_error(void);
+/* Fills cfg with the reference defaults of HeadSolveConfig (solve_head.cuh:64-96). */
+void mf3d_default_config(mf3d_head_solve_config* cfg);
+
+int mf3d_workspace_create(mf3d_workspace** ws);
+int mf3d_workspace_destroy(mf3d_workspace* ws);
+/* Device bytes currently held by the workspace. */
+size_t mf3d_workspace_bytes(const mf3d_workspace* ws);
+
+/* ---- physics --------------------------------------------------------- */
+/* physics::solve_head (src/physics/flow/solve_head.cuh:158-160,
+ * solve_head.cu:75-243). h: in/out (n >= grid cells); K: cell conductivity.
+ * Synchronises the stream before returning (the result is host data). */
+int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
+                    const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
+                    double* h, size_t n, mf3d_head_solve_result* result);
+
+/* physics::init_head_guess (solve_head.cuh:173-174, solve_head.cu:56-73). */
+int mf3d_init_head_guess(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, double* h,
+                         size_t n);
+
+/* physics::build_rhs_head (src/physics/flow/rhs_head.cuh:50-51, rhs_head.cu:121-194). */
+int mf3d_build_rhs_head(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
+                        const double* K, double* rhs, size_t n);
+
+/* physics::coarsen_K (src/physics/flow/coarsen_K.cuh:46-47, coarsen_K.cu:55-78). */
+int mf3d_coarsen_K(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_grid* fine,
+                   const double* K_fine, size_t n_fine, double* K_coarse, size_t n_coarse);
+
+/* physics::compute_velocity_from_head, compact MAC layout
+ * (src/physics/flow/velocity_from_head.cuh:54-55, velocity_from_head.cu:601-691).
+ * U: (nx+1)*ny*nz, V: nx*(ny+1)*nz, W: nx*ny*(nz+1). */
+int mf3d_velocity_compact(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
+                          const double* h, const double* K, size_t n, double* U, size_t nU,
+                          double* V, size_t nV, double* W, size_t nW);
+
+/* physics::compute_velocity_from_head, padded layout
+ * (velocity_from_head.cuh:72-73, velocity_from_head.cu:697-777). Each of U, V,
+ * W holds (nx+1)*(ny+1)*(nz+1) values indexed k*(ny+1)*(nx+1) + j*(nx+1) + i
+ * (padded_idx, src/physics/flow/padded_layout.cuh:30-33). Only physical faces
+ * are written; the caller zero-fills the padding once (fields.cuh:268-279). */
+int mf3d_velocity_padded(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
+                         const double* h, const double* K, size_t n, double* U, double* V,
+                         double* W, size_t n_padded);
+
+/* ---- multigrid ------------------------------------------------------- */
+/* multigrid::gsrb_smooth_3d (src/multigrid/smoothers/gsrb_3d.cuh:29-31, gsrb_3d.cu:368-576). */
+int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, const double* b,
+                        const double* K, size_t n, int num_iters, const mf3d_bc* bc, int pin);
+
+/* multigrid::compute_residual_3d (residual_3d.cuh:30-32, residual_3d.cu:378-503). */
+int mf3d_compute_residual_3d(mf3d_stream stream, const mf3d_grid* grid, const double* x,
+                             const double* b, const double* K, double* r, size_t n,
+                             const mf3d_bc* bc, int pin);
+
+/* multigrid::restrict_3d (src/multigrid/transfer/restrict_3d.cuh:14-15, restrict_3d.cu:55-81). */
+int mf3d_restrict_3d(mf3d_stream stream, const mf3d_grid* fine, const mf3d_grid* coarse,
+                     const double* r_fine, size_t n_fine, double* b_coarse, size_t n_coarse);
+
+/* multigrid::prolong_3d_add (src/multigrid/transfer/prolong_3d.cuh:14-15, prolong_3d.cu:520-621). */
+int mf3d_prolong_3d_add(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_grid* fine,
+                        const double* x_coarse, size_t n_coarse, double* x_fine, size_t n_fine);
+
+/* MGHierarchy set-up (src/multigrid/mg_types.hpp:75-98) inside the workspace,
+ * with K copied to level 0 and coarsened (solve_head.cu:89-114); builds the
+ * dense coarsest inverse when cfg->coarse_solver is DIRECT. */
+int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
+                  const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
+                  size_t n);
+/* Number of levels of the current hierarchy (0 before set-up). */
+int mf3d_mg_num_levels(const mf3d_workspace* ws);
+
+/* solvers::MultigridPreconditioner::apply (src/numerics/solvers/mg_preconditioner.cuh:44-71,
+ * mg_preconditioner.cu:16-45): z = one V-cycle (v_cycle.cu:36-74) from x = 0 with b = r. */
+int mf3d_mg_precondition(mf3d_workspace* ws, mf3d_stream stream, const double* r, double* z,
+                         size_t n);
+
+/* ---- numerics -------------------------------------------------------- */
+/* operators::VarCoeffLaplacian::apply (src/numerics/operators/varcoeff_laplacian.cuh:47-55,
+ * varcoeff_laplacian.cu:241-277). */
+int mf3d_varcoeff_apply(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, int pin,
+                        const double* K, const double* x, double* y, size_t n);
+
+/* blas::dot_device (src/numerics/blas/dot.cuh, dot.cu:9-36): *d_result = x.y
+ * (device pointer). Deterministic two-stage reduction; scratch from ws. */
+int mf3d_dot(mf3d_workspace* ws, mf3d_stream stream, const double* x, const double* y, size_t n,
+             double* d_result);
+/* blas::nrm2_device (nrm2.cu:9-36) */
+int mf3d_nrm2(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
+              double* d_result);
+/* blas::axpy (axpy.cu:7-29): y = a x + y */
+int mf3d_axpy(mf3d_stream stream, double a, const double* x, double* y, size_t n);
+/* blas::axpby (axpby.cu:7-29): y = a x + b y */
+int mf3d_axpby(mf3d_stream stream, double a, const double* x, double b, double* y, size_t n);
+/* blas::copy (copy.cu:7-29) */
+int mf3d_copy(mf3d_stream stream, const double* x, double* y, size_t n);
+/* blas::fill (fill.cu:7-27) */
+int mf3d_fill(mf3d_stream stream, double* x, size_t n, double value);
+/* blas::scal (scal.cu:7-27): x = a x */
+int mf3d_scal(mf3d_stream stream, double* x, size_t n, double a);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MF3D_H */

@@ -1,0 +1,292 @@
+// BLAS-1 kernels, deterministic two-stage reductions and the fused Krylov
+// vector updates (FP64), gfx950.
+//
+// Reductions: every first stage writes one partial per 256-thread block (wave64
+// shuffle tree, then LDS across the 4 waves); the second stage is a single
+// 1024-thread block summing the partials in a fixed order, so a result depends
+// only on the data and n. The second stage also finalises the Krylov scalars
+// (alpha, beta, convergence flag) on the device, so an iteration needs no host
+// round trip (the reference syncs the stream 2-3 times per PCG iteration,
+// src/numerics/blas/dot.cu:38-50, nrm2.cu:38-49).
+
+#include "kernels.hpp"
+
+namespace mf3d {
+namespace {
+
+constexpr int kBlock = kReduceBlock;
+
+inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <int BS>
+__device__ inline double block_sum(double v) {
+    __shared__ double part[BS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = lane < BS / 64 ? part[lane] : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    }
+    return v;
+}
+
+// blas::fill (src/numerics/blas/fill.cu:7-14)
+__global__ void __launch_bounds__(kBlock) fill_kernel(double* __restrict__ x, size_t n, double v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+// blas::copy (copy.cu:7-14)
+__global__ void __launch_bounds__(kBlock)
+    copy_kernel(const double* __restrict__ x, double* __restrict__ y, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+// blas::axpy (axpy.cu:7-14): y = a x + y
+__global__ void __launch_bounds__(kBlock)
+    axpy_kernel(double a, const double* __restrict__ x, double* __restrict__ y, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + y[i];
+}
+// blas::axpby (axpby.cu:7-14): y = a x + b y
+__global__ void __launch_bounds__(kBlock)
+    axpby_kernel(double a, const double* __restrict__ x, double b, double* __restrict__ y,
+                 size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+// blas::scal (scal.cu:7-14): x = a x
+__global__ void __launch_bounds__(kBlock) scal_kernel(double* __restrict__ x, size_t n, double a) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = a * x[i];
+}
+__global__ void __launch_bounds__(kBlock)
+    scale_copy_kernel(double a, const double* __restrict__ x, double* __restrict__ y, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    dot_partials_kernel(const double* __restrict__ x, const double* __restrict__ y, size_t n,
+                        double* __restrict__ partials) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = i < n ? x[i] * y[i] : 0.0;
+    const double s = block_sum<kBlock>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kFinalBlock)
+    finalize_kernel(const double* __restrict__ partials, unsigned np, int op,
+                    IterState* __restrict__ st, double* __restrict__ out) {
+    double v = 0.0;
+    for (unsigned i = threadIdx.x; i < np; i += kFinalBlock) v += partials[i];
+    const double S = block_sum<kFinalBlock>(v);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+    case FIN_STORE: *out = S; break;
+    case FIN_SQRT: *out = sqrt(S); break;
+    case FIN_BNORM: st->bnorm = sqrt(S); break;
+    case FIN_R0: // pcg.cuh:113-127
+        st->r0 = sqrt(S);
+        st->rr = S;
+        st->den = st->stop_norm == MF3D_STOP_RHS_NORM ? st->bnorm : st->r0;
+        if (st->r0 < 1e-15) st->stop = 1;
+        break;
+    case FIN_RZ0: st->rz = S; break;
+    case FIN_PAP: // pcg.cuh:146-147
+        if (st->stop) break;
+        st->pAp = S;
+        st->alpha = st->rz / S;
+        if (!isfinite(st->alpha)) st->stop = 2;
+        break;
+    case FIN_RR: // pcg.cuh:156-164
+        if (st->stop) break;
+        st->rr = S;
+        st->iter += 1;
+        if (st->iter % st->check_every == 0 || st->iter >= st->max_iter) {
+            if (sqrt(S) / st->den < st->rtol)
+                st->stop = 1;
+            else if (st->iter >= st->max_iter)
+                st->stop = 3;
+        }
+        break;
+    case FIN_RZ: // pcg.cuh:170-177
+        if (st->stop) break;
+        st->beta = S / st->rz;
+        st->rz = S;
+        break;
+    case FIN_CG_RR0: // cg.cuh:47-77: rr = (r,r), tol = r0 * rtol
+        st->rz = S;
+        st->rr = S;
+        st->r0 = sqrt(S);
+        st->den = st->stop_norm == MF3D_STOP_RHS_NORM ? st->bnorm : st->r0;
+        if (st->r0 <= st->den * st->rtol) st->stop = 1;
+        break;
+    case FIN_CG_PAP: { // cg.cuh:84-124, operator negated: (p, -Ap) = -S exactly
+        if (st->stop) break;
+        const double pAp = -S;
+        st->pAp = pAp;
+        if ((st->iter + 1) % st->check_every == 0 && (!isfinite(pAp) || fabs(pAp) < 1e-30)) {
+            st->iter += 1;
+            st->stop = 2;
+            break;
+        }
+        st->alpha = st->rz / pAp;
+        break;
+    }
+    case FIN_CG_RR: // cg.cuh:126-156
+        if (st->stop) break;
+        st->iter += 1;
+        st->rr = S;
+        if (st->iter % st->check_every == 0 && sqrt(S) <= st->den * st->rtol) {
+            st->stop = 1;
+            break;
+        }
+        if (st->iter >= st->max_iter) st->stop = 3;
+        st->beta = S / st->rz;
+        st->rz = S;
+        break;
+    default: break;
+    }
+}
+
+// PCG x/r update (pcg.cuh:149-153 as two axpy) fused with (r, r).
+__global__ void __launch_bounds__(kBlock)
+    pcg_update_xr_kernel(const IterState* __restrict__ st, const double* __restrict__ p,
+                         const double* __restrict__ Ap, double* __restrict__ x,
+                         double* __restrict__ r, size_t n, double* __restrict__ partials) {
+    if (st->stop) return;  // uniform across the grid
+    const double alpha = st->alpha;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        x[i] = alpha * p[i] + x[i];
+        const double ri = -alpha * Ap[i] + r[i];
+        r[i] = ri;
+        v = ri * ri;
+    }
+    const double s = block_sum<kBlock>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// PCG p = z + beta p (pcg.cuh:179, axpby(1, z, beta, p)).
+__global__ void __launch_bounds__(kBlock)
+    pcg_update_p_kernel(const IterState* __restrict__ st, const double* __restrict__ z,
+                        double* __restrict__ p, size_t n) {
+    if (st->stop) return;
+    const double beta = st->beta;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 1.0 * z[i] + beta * p[i];
+}
+
+// CG on the negated operator (update_x_and_r_kernel, src/numerics/blas/cg_kernels.cu:19-31):
+// Ap holds A p, the CG operator is -A, so r - alpha (-Ap) = r + alpha Ap.
+__global__ void __launch_bounds__(kBlock)
+    cg_update_xr_kernel(const IterState* __restrict__ st, const double* __restrict__ p,
+                        const double* __restrict__ Ap, double* __restrict__ x,
+                        double* __restrict__ r, size_t n, double* __restrict__ partials) {
+    if (st->stop) return;
+    const double alpha = st->alpha;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        x[i] = x[i] + alpha * p[i];
+        const double ri = r[i] - alpha * (-Ap[i]);
+        r[i] = ri;
+        v = ri * ri;
+    }
+    const double s = block_sum<kBlock>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// update_p_kernel (cg_kernels.cu:33-44): p = r + beta p
+__global__ void __launch_bounds__(kBlock)
+    cg_update_p_kernel(const IterState* __restrict__ st, const double* __restrict__ r,
+                       double* __restrict__ p, size_t n) {
+    if (st->stop) return;
+    const double beta = st->beta;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = r[i] + beta * p[i];
+}
+
+}  // namespace
+
+namespace kern {
+
+#define MF3D_LAUNCH_CHECK() MF3D_HIP_TRY(hipGetLastError())
+
+int fill(hipStream_t s, double* x, size_t n, double v) {
+    if (n == 0) return MF3D_OK;
+    fill_kernel<<<grid_for(n), kBlock, 0, s>>>(x, n, v);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int copy(hipStream_t s, const double* x, double* y, size_t n) {
+    if (n == 0) return MF3D_OK;
+    copy_kernel<<<grid_for(n), kBlock, 0, s>>>(x, y, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int axpy(hipStream_t s, double a, const double* x, double* y, size_t n) {
+    if (n == 0) return MF3D_OK;
+    axpy_kernel<<<grid_for(n), kBlock, 0, s>>>(a, x, y, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int axpby(hipStream_t s, double a, const double* x, double b, double* y, size_t n) {
+    if (n == 0) return MF3D_OK;
+    axpby_kernel<<<grid_for(n), kBlock, 0, s>>>(a, x, b, y, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int scal(hipStream_t s, double* x, size_t n, double a) {
+    if (n == 0) return MF3D_OK;
+    scal_kernel<<<grid_for(n), kBlock, 0, s>>>(x, n, a);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int scale_copy(hipStream_t s, double a, const double* x, double* y, size_t n) {
+    if (n == 0) return MF3D_OK;
+    scale_copy_kernel<<<grid_for(n), kBlock, 0, s>>>(a, x, y, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int dot_partials(hipStream_t s, const double* x, const double* y, size_t n, double* partials) {
+    dot_partials_kernel<<<grid_for(n), kBlock, 0, s>>>(x, y, n, partials);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int finalize(hipStream_t s, const double* partials, unsigned np, Finalize op, IterState* st,
+             double* out) {
+    finalize_kernel<<<1, kFinalBlock, 0, s>>>(partials, np, (int)op, st, out);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int pcg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
+                  double* x, double* r, size_t n, double* partials) {
+    pcg_update_xr_kernel<<<grid_for(n), kBlock, 0, s>>>(st, p, Ap, x, r, n, partials);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int pcg_update_p(hipStream_t s, const IterState* st, const double* z, double* p, size_t n) {
+    pcg_update_p_kernel<<<grid_for(n), kBlock, 0, s>>>(st, z, p, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
+                 double* x, double* r, size_t n, double* partials) {
+    cg_update_xr_kernel<<<grid_for(n), kBlock, 0, s>>>(st, p, Ap, x, r, n, partials);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int cg_update_p(hipStream_t s, const IterState* st, const double* r, double* p, size_t n) {
+    cg_update_p_kernel<<<grid_for(n), kBlock, 0, s>>>(st, r, p, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+
+}  // namespace kern
+}  // namespace mf3d

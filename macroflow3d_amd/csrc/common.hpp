@@ -1,0 +1,134 @@
+// Shared host/device definitions of libmf3d_hip (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/mf3d.h"
+
+namespace mf3d {
+
+// ---------------------------------------------------------------------------
+// Error plumbing: internal functions return an mf3d status; the detail text of
+// the last failure is kept per thread (mf3d_last_error).
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+
+#define MF3D_HIP_TRY(expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return ::mf3d::fail(MF3D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define MF3D_TRY(expr)                                                                       \
+    do {                                                                                     \
+        int s_ = (expr);                                                                     \
+        if (s_ != MF3D_OK) return s_;                                                        \
+    } while (0)
+
+inline size_t cells(const mf3d_grid& g) { return (size_t)g.nx * (size_t)g.ny * (size_t)g.nz; }
+
+// Boundary types only (values enter through the RHS and velocity kernels).
+struct BcTypes {
+    int t[6];
+};
+inline BcTypes bc_types(const mf3d_bc& bc) {
+    BcTypes o;
+    for (int i = 0; i < 6; ++i) o.t[i] = bc.type[i];
+    return o;
+}
+
+// Validation used by every entry point (BCSpec::validate, src/core/BCSpec.hpp:63-87).
+int validate_grid(const mf3d_grid* g);
+int validate_bc(const mf3d_bc* bc);
+
+// pin_enabled (src/numerics/pin_spec.hpp:110-120)
+inline bool pin_enabled(int mode, const mf3d_bc& bc) {
+    if (mode == MF3D_PIN_ON) return true;
+    if (mode == MF3D_PIN_OFF) return false;
+    for (int f = 0; f < 6; ++f)
+        if (bc.type[f] == MF3D_BC_DIRICHLET) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell stencil coefficients with boundary conditions, shared by host and
+// device. Semantics of bc_helpers::neighbor_* (src/multigrid/smoothers/
+// bc_stencil_helpers.cuh:33-267): periodic/interior -> harmonic mean of K,
+// Neumann -> 0, Dirichlet -> 2*K_C with ghost value 0.
+// dir: 0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+. Returns the neighbour's linear
+// index (or -1 for a Neumann/Dirichlet face) and its coefficient.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline long neighbour_coef(int nx, int ny, int nz, const BcTypes& bc,
+                                               const double* K, int i, int j, int k, int dir,
+                                               double Kc, double& coef) {
+    int ni = i, nj = j, nk = k;
+    bool at_bnd;
+    switch (dir) {
+    case 0: at_bnd = (i == 0); ni = at_bnd ? nx - 1 : i - 1; break;
+    case 1: at_bnd = (i == nx - 1); ni = at_bnd ? 0 : i + 1; break;
+    case 2: at_bnd = (j == 0); nj = at_bnd ? ny - 1 : j - 1; break;
+    case 3: at_bnd = (j == ny - 1); nj = at_bnd ? 0 : j + 1; break;
+    case 4: at_bnd = (k == 0); nk = at_bnd ? nz - 1 : k - 1; break;
+    default: at_bnd = (k == nz - 1); nk = at_bnd ? 0 : k + 1; break;
+    }
+    if (at_bnd && bc.t[dir] != MF3D_BC_PERIODIC) {
+        coef = (bc.t[dir] == MF3D_BC_DIRICHLET) ? 2.0 * Kc : 0.0;
+        return -1;
+    }
+    const long n = (long)ni + (long)nx * ((long)nj + (long)ny * (long)nk);
+    coef = 2.0 / (1.0 / Kc + 1.0 / K[n]);
+    return n;
+}
+
+__host__ __device__ inline bool interior_cell(int nx, int ny, int nz, int i, int j, int k) {
+    return i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident iteration state of the Krylov solvers. Scalars never leave
+// the device inside the iteration; the host reads this struct only at its
+// convergence checks.
+// ---------------------------------------------------------------------------
+struct IterState {
+    double rz;     // (r, z) of the current iterate
+    double pAp;    // (p, A p)
+    double alpha;  // rz / pAp
+    double beta;   // rz_new / rz
+    double rr;     // ||r||^2 of the recursive residual
+    double den;    // stopping denominator: ||r0|| or ||b||
+    double r0;     // ||r0||
+    double bnorm;  // ||b||
+    double rtol;   // relative tolerance
+    int iter;      // completed iterations
+    int stop;      // 0 running, 1 converged, 2 breakdown, 3 iteration cap
+    int check_every;
+    int max_iter;
+    int stop_norm; // MF3D_STOP_*
+    int pad;
+};
+
+// Reduction finalisers (what thread 0 does with the reduced sum S).
+enum Finalize : int {
+    FIN_STORE = 0,   // *out = S
+    FIN_SQRT = 1,    // *out = sqrt(S)
+    FIN_R0 = 2,      // r0 = sqrt(S); den from stop_norm; early exit if r0 < 1e-15
+    FIN_RZ0 = 3,     // rz = S
+    FIN_PAP = 4,     // pAp = S, alpha = rz / pAp
+    FIN_RR = 5,      // rr = S; ++iter; convergence / cap test
+    FIN_RZ = 6,      // beta = S / rz; rz = S
+    FIN_BNORM = 7,   // bnorm = sqrt(S)
+    FIN_CG_RR0 = 8,  // plain CG: rz = S (= r.r), r0 = sqrt(S), den
+    FIN_CG_PAP = 9,  // plain CG: pAp = S, breakdown test every check_every, alpha
+    FIN_CG_RR = 10   // plain CG: rr_new = S, ++iter, test, beta = S / rz, rz = S
+};
+
+constexpr int kReduceBlock = 256;   // threads per first-stage block
+constexpr int kFinalBlock = 1024;   // threads of the second stage
+
+}  // namespace mf3d

@@ -1,0 +1,64 @@
+// Host-side launchers of the gfx950 kernels (all enqueue on the given stream).
+#pragma once
+
+#include "common.hpp"
+
+namespace mf3d {
+namespace kern {
+
+// ---- stencil (stencil.hip) -------------------------------------------------
+int gsrb(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double* x,
+         const double* b, const double* K, int iters, bool zero_init);
+int residual(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* x,
+             const double* b, const double* K, double* r);
+int residual_restrict(hipStream_t s, const mf3d_grid& fine, const BcTypes& bc, bool pin,
+                      const double* x, const double* b, const double* K, double* b_coarse);
+unsigned reduce_blocks(size_t n);  // first-stage block count of the fused reductions
+int apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
+          const double* x, double* y, double* partials /* nullable */);
+int op_residual(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
+                const double* x, const double* b, double* r /* nullable */, double* partials);
+int restrict_avg(hipStream_t s, const mf3d_grid& coarse, const double* f, double* c);
+int prolong_add(hipStream_t s, const mf3d_grid& fine, const double* c, double* f);
+int coarsen(hipStream_t s, const mf3d_grid& coarse, const double* Kf, double* Kc);
+int rhs(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* K, double* out);
+int guess(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, int mode, double* h);
+int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
+int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv);
+
+// ---- fast fine-level kernels (stencil_fast.hip) ----------------------------
+// True when the z-marching kernels cover this level (size and BC limits).
+bool fast_ok(const mf3d_grid& g, const BcTypes& bc);
+int gsrb_fast(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double* x,
+              const double* b, const double* K, int iters, bool zero_init);
+
+// ---- BLAS-1 and reductions (blas.hip) ---------------------------------------
+int fill(hipStream_t s, double* x, size_t n, double v);
+int copy(hipStream_t s, const double* x, double* y, size_t n);
+int axpy(hipStream_t s, double a, const double* x, double* y, size_t n);
+int axpby(hipStream_t s, double a, const double* x, double b, double* y, size_t n);
+int scal(hipStream_t s, double* x, size_t n, double a);
+// first stage of (x, y) into partials[reduce_blocks(n)]
+int dot_partials(hipStream_t s, const double* x, const double* y, size_t n, double* partials);
+// second stage: sum partials[0..np) in a fixed order and finalise into st / out
+int finalize(hipStream_t s, const double* partials, unsigned np, Finalize op, IterState* st,
+             double* out);
+// PCG: if !st->stop { x += alpha p; r -= alpha Ap } with partials of (r, r)
+int pcg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
+                  double* x, double* r, size_t n, double* partials);
+// PCG: p = z + beta p
+int pcg_update_p(hipStream_t s, const IterState* st, const double* z, double* p, size_t n);
+// CG on -A: if !st->stop { x += alpha p; r -= alpha (-Ap) } with partials of (r, r)
+int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
+                 double* x, double* r, size_t n, double* partials);
+// CG: p = r + beta p (beta from st)
+int cg_update_p(hipStream_t s, const IterState* st, const double* r, double* p, size_t n);
+// y = a * x (used to negate vectors for the CG path)
+int scale_copy(hipStream_t s, double a, const double* x, double* y, size_t n);
+
+// ---- velocity (velocity.hip) -----------------------------------------------
+int velocity(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* h,
+             const double* K, double* U, double* V, double* W, bool padded);
+
+}  // namespace kern
+}  // namespace mf3d

@@ -1,0 +1,896 @@
+// Host orchestration of the head solve and the C ABI (include/mf3d.h).
+//
+// The iteration keeps every Krylov scalar on the device (IterState): the host
+// enqueues whole chunks of iterations and reads the state only at its
+// convergence checks. Device kernels stop updating x and r as soon as the
+// device-side test fires, so the reported iteration count is exact whatever
+// the chunk length.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+namespace mf3d {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error_cstr() { return g_last_error.c_str(); }
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int validate_grid(const mf3d_grid* g) {
+    if (!g) return fail(MF3D_ERR_INVALID_ARGUMENT, "grid is null");
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "grid dimensions must be positive");
+    if (!(g->dx > 0.0) || !std::isfinite(g->dx))
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "grid spacing must be positive and finite");
+    if ((double)g->nx * g->ny * g->nz > 4.0e9)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "grid larger than 2^32 cells");
+    return MF3D_OK;
+}
+
+// BCSpec::validate (src/core/BCSpec.hpp:63-87)
+int validate_bc(const mf3d_bc* bc) {
+    if (!bc) return fail(MF3D_ERR_INVALID_ARGUMENT, "bc is null");
+    for (int f = 0; f < 6; ++f) {
+        if (bc->type[f] > MF3D_BC_PERIODIC)
+            return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown boundary type");
+        if (!std::isfinite(bc->value[f]))
+            return fail(MF3D_ERR_INVALID_ARGUMENT, "BC values must be finite");
+    }
+    static const char* axis[3] = {"x", "y", "z"};
+    for (int a = 0; a < 3; ++a) {
+        const bool p0 = bc->type[2 * a] == MF3D_BC_PERIODIC;
+        const bool p1 = bc->type[2 * a + 1] == MF3D_BC_PERIODIC;
+        if (p0 != p1)
+            return fail(MF3D_ERR_INVALID_ARGUMENT,
+                        std::string("Periodic BC must be specified on both ") + axis[a] +
+                            "min and " + axis[a] + "max");
+    }
+    return MF3D_OK;
+}
+}  // namespace mf3d
+
+using namespace mf3d;
+
+// ---------------------------------------------------------------------------
+// workspace
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Level {
+    mf3d_grid g{};
+    size_t n = 0;
+    double* x = nullptr;
+    double* b = nullptr;
+    double* K = nullptr;
+};
+
+inline hipStream_t as_stream(mf3d_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+struct mf3d_workspace {
+    // hierarchy (MGHierarchy, src/multigrid/mg_types.hpp:75-98)
+    mf3d_grid fine{};
+    int req_levels = 0;
+    std::vector<Level> lv;
+    // dense coarsest inverse
+    double* coarse_inv = nullptr;  // n*n inverse followed by the n*(w+1) band factor
+    int coarse_n = 0;
+    size_t coarse_cap = 0;         // elements allocated at coarse_inv
+    // Krylov vectors (PCGWorkspace, src/numerics/solvers/pcg.cuh:45-62) + rhs
+    size_t nvec = 0;
+    double *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
+    // reductions
+    double* partials = nullptr;
+    size_t partials_cap = 0;
+    IterState* st = nullptr;
+    double* dscalar = nullptr;
+    IterState* host_st = nullptr;  // pinned
+    // set-up snapshot
+    mf3d_bc bc{};
+    BcTypes bct{};
+    bool pin = false;
+    int pre = 2, post = 2, coarse_iters = 50, coarse_solver = 0;
+    const double* K0 = nullptr;
+    bool ready = false;
+    size_t bytes = 0;
+    std::vector<void*> owned;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    int alloc(double** p, size_t n) {
+        void* q = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&q, n * sizeof(double));
+        if (e != hipSuccess)
+            return fail(MF3D_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+        owned.push_back(q);
+        bytes += n * sizeof(double);
+        *p = static_cast<double*>(q);
+        return MF3D_OK;
+    }
+    void release() {
+        for (void* q : owned) (void)hipFree(q);
+        owned.clear();
+        bytes = 0;
+        lv.clear();
+        coarse_inv = nullptr;
+        coarse_n = 0;
+        coarse_cap = 0;
+        nvec = 0;
+        rhs = r = z = p = Ap = nullptr;
+        partials = nullptr;
+        partials_cap = 0;
+        st = nullptr;
+        dscalar = nullptr;
+        ready = false;
+        req_levels = 0;
+        fine = mf3d_grid{};
+    }
+    ~mf3d_workspace() {
+        release();
+        if (host_st) (void)hipHostFree(host_st);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+bool same_grid(const mf3d_grid& a, const mf3d_grid& b) {
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dx == b.dx && a.dy == b.dy &&
+           a.dz == b.dz;
+}
+
+// Builds (or reuses) the hierarchy and vectors for this grid and level count.
+int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
+    if (levels < 1) return fail(MF3D_ERR_INVALID_ARGUMENT, "mg_levels must be >= 1");
+    if (ws->ready && same_grid(ws->fine, g) && ws->req_levels == levels) return MF3D_OK;
+    ws->release();
+    // halving rule of MGHierarchy (mg_types.hpp:75-98): stop before a dim < 2
+    std::vector<mf3d_grid> grids;
+    mf3d_grid cur = g;
+    for (int l = 0; l < levels; ++l) {
+        grids.push_back(cur);
+        if (l < levels - 1) {
+            cur.nx /= 2;
+            cur.ny /= 2;
+            cur.nz /= 2;
+            cur.dx *= 2.0;
+            cur.dy *= 2.0;
+            cur.dz *= 2.0;
+            if (cur.nx < 2 || cur.ny < 2 || cur.nz < 2) break;
+        }
+    }
+    for (size_t l = 0; l + 1 < grids.size(); ++l) {
+        const mf3d_grid& f = grids[l];
+        if ((f.nx & 1) || (f.ny & 1) || (f.nz & 1))
+            return fail(MF3D_ERR_SIZE_MISMATCH,
+                        "grid level " + std::to_string(l) + " (" + std::to_string(f.nx) + "x" +
+                            std::to_string(f.ny) + "x" + std::to_string(f.nz) +
+                            ") is odd but not the coarsest: 2:1 restriction needs even sizes");
+    }
+    ws->lv.resize(grids.size());
+    for (size_t l = 0; l < grids.size(); ++l) {
+        Level& L = ws->lv[l];
+        L.g = grids[l];
+        L.n = cells(grids[l]);
+        if (l > 0) {
+            MF3D_TRY(ws->alloc(&L.x, L.n));
+            MF3D_TRY(ws->alloc(&L.b, L.n));
+            MF3D_TRY(ws->alloc(&L.K, L.n));
+        }
+    }
+    const size_t n = cells(g);
+    ws->nvec = n;
+    MF3D_TRY(ws->alloc(&ws->rhs, n));
+    MF3D_TRY(ws->alloc(&ws->r, n));
+    MF3D_TRY(ws->alloc(&ws->z, n));
+    MF3D_TRY(ws->alloc(&ws->p, n));
+    MF3D_TRY(ws->alloc(&ws->Ap, n));
+    ws->partials_cap = kern::reduce_blocks(n) + 64;
+    MF3D_TRY(ws->alloc(&ws->partials, ws->partials_cap));
+    double* stp = nullptr;
+    MF3D_TRY(ws->alloc(&stp, (sizeof(IterState) + 7) / 8));
+    ws->st = reinterpret_cast<IterState*>(stp);
+    MF3D_TRY(ws->alloc(&ws->dscalar, 8));
+    if (!ws->host_st) MF3D_HIP_TRY(hipHostMalloc((void**)&ws->host_st, sizeof(IterState)));
+    for (auto& e : ws->ev)
+        if (!e) MF3D_HIP_TRY(hipEventCreate(&e));
+    ws->fine = g;
+    ws->req_levels = levels;
+    ws->ready = true;
+    return MF3D_OK;
+}
+
+int scratch_partials(mf3d_workspace* ws, size_t n) {
+    const size_t need = kern::reduce_blocks(n) + 64;
+    if (ws->partials && ws->partials_cap >= need) return MF3D_OK;
+    double* p = nullptr;
+    MF3D_TRY(ws->alloc(&p, need));
+    ws->partials = p;
+    ws->partials_cap = need;
+    if (!ws->dscalar) MF3D_TRY(ws->alloc(&ws->dscalar, 8));
+    return MF3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Dense coarsest solve: assemble M = -A of the coarsest level (smoother /
+// residual operator, pin included), banded Cholesky on the host, explicit
+// inverse on the device (band_inverse_kernel).
+// ---------------------------------------------------------------------------
+int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
+    Level& L = ws->lv.back();
+    const int n = (int)L.n;
+    if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
+    std::vector<double> Kc(n);
+    const double* Kdev = ws->lv.size() == 1 ? ws->K0 : L.K;
+    MF3D_HIP_TRY(hipMemcpyAsync(Kc.data(), Kdev, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    const mf3d_grid& g = L.g;
+    const double dx2 = g.dx * g.dx;
+    // band width from the neighbour offsets actually present
+    int w = 0;
+    std::vector<std::vector<std::pair<int, double>>> rows(n);
+    std::vector<double> diag(n, 0.0);
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+                const int c = i + g.nx * (j + g.ny * k);
+                double asum = 0.0;
+                for (int d = 0; d < 6; ++d) {
+                    double coef;
+                    const long nb =
+                        neighbour_coef(g.nx, g.ny, g.nz, ws->bct, Kc.data(), i, j, k, d, Kc[c], coef);
+                    asum += coef;
+                    if (nb >= 0) {
+                        rows[c].push_back({(int)nb, -coef / dx2});
+                        w = std::max(w, std::abs((int)nb - c));
+                    }
+                }
+                if (ws->pin && c == 0) asum *= 2.0;
+                diag[c] = asum / dx2;
+            }
+    const int W1 = w + 1;
+    std::vector<double> band((size_t)n * W1, 0.0);  // band[i*W1 + (i-j)] = M[i][j], j <= i
+    for (int c = 0; c < n; ++c) {
+        band[(size_t)c * W1] += diag[c];
+        for (auto& e : rows[c]) {
+            const int j = e.first;
+            if (j < c) band[(size_t)c * W1 + (c - j)] += 0.5 * e.second;
+            if (j > c) band[(size_t)j * W1 + (j - c)] += 0.5 * e.second;
+            if (j == c) band[(size_t)c * W1] += e.second;  // self-wrap (dimension 1)
+        }
+    }
+    // banded Cholesky in place
+    for (int j = 0; j < n; ++j) {
+        double d = band[(size_t)j * W1];
+        for (int p = std::max(0, j - w); p < j; ++p) {
+            const double l = band[(size_t)j * W1 + (j - p)];
+            d -= l * l;
+        }
+        if (!(d > 0.0))
+            return fail(MF3D_ERR_SINGULAR, "coarsest operator is not definite (pure Neumann/"
+                                           "periodic problem without pin?)");
+        const double Ljj = std::sqrt(d);
+        band[(size_t)j * W1] = Ljj;
+        for (int i = j + 1; i <= std::min(n - 1, j + w); ++i) {
+            double sacc = band[(size_t)i * W1 + (i - j)];
+            for (int p = std::max(0, i - w); p < j; ++p)
+                sacc -= band[(size_t)i * W1 + (i - p)] * band[(size_t)j * W1 + (j - p)];
+            band[(size_t)i * W1 + (i - j)] = sacc / Ljj;
+        }
+    }
+    const size_t need = (size_t)n * n + (size_t)n * W1;
+    if (!ws->coarse_inv || ws->coarse_cap < need) {
+        MF3D_TRY(ws->alloc(&ws->coarse_inv, need));
+        ws->coarse_cap = need;
+    }
+    ws->coarse_n = n;
+    double* dband = ws->coarse_inv + (size_t)n * n;
+    MF3D_HIP_TRY(hipMemcpyAsync(dband, band.data(), band.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+    MF3D_TRY(kern::band_inverse(s, n, w, dband, ws->coarse_inv));
+    // the host band must outlive the copy
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    return MF3D_OK;
+}
+
+// Hierarchy coefficients for a K field (solve_head.cu:106-114).
+int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc,
+             const mf3d_head_solve_config& cfg, const double* K) {
+    MF3D_TRY(ensure_workspace(ws, g, cfg.mg_levels));
+    ws->bc = bc;
+    ws->bct = bc_types(bc);
+    ws->pin = pin_enabled(cfg.pin_mode, bc);
+    ws->pre = cfg.mg_pre_smooth;
+    ws->post = cfg.mg_post_smooth;
+    ws->coarse_iters = cfg.mg_coarse_iters;
+    ws->coarse_solver = cfg.coarse_solver;
+    ws->K0 = K;
+    for (size_t l = 1; l < ws->lv.size(); ++l)
+        MF3D_TRY(kern::coarsen(s, ws->lv[l].g, l == 1 ? K : ws->lv[l - 1].K, ws->lv[l].K));
+    if (cfg.coarse_solver == MF3D_COARSE_DIRECT) MF3D_TRY(build_coarse_inverse(ws, s));
+    return MF3D_OK;
+}
+
+inline const double* level_K(const mf3d_workspace* ws, size_t l) {
+    return l == 0 ? ws->K0 : ws->lv[l].K;
+}
+
+int smooth(mf3d_workspace* ws, hipStream_t s, size_t l, double* x, const double* b, int iters,
+           bool zero_init) {
+    const Level& L = ws->lv[l];
+    if (kern::fast_ok(L.g, ws->bct))
+        return kern::gsrb_fast(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), iters, zero_init);
+    return kern::gsrb(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), iters, zero_init);
+}
+
+// v_cycle_recursive (src/multigrid/cycle/v_cycle.cu:36-74). zero_init: x is
+// zero on entry (the fill of v_cycle.cu:62 / mg_preconditioner.cu:37 is folded
+// into the first smoothing sweep).
+int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, double* x, const double* b,
+           bool zero_init) {
+    const size_t nl = ws->lv.size();
+    const Level& L = ws->lv[l];
+    if (l == nl - 1) {
+        if (ws->coarse_solver == MF3D_COARSE_DIRECT)
+            return kern::dense_gemv(s, (int)L.n, ws->coarse_inv, b, x);
+        return smooth(ws, s, l, x, b, ws->coarse_iters, zero_init);
+    }
+    Level& C = ws->lv[l + 1];
+    MF3D_TRY(smooth(ws, s, l, x, b, ws->pre, zero_init));
+    MF3D_TRY(kern::residual_restrict(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), C.b));
+    MF3D_TRY(vcycle(ws, s, l + 1, C.x, C.b, true));
+    MF3D_TRY(kern::prolong_add(s, L.g, C.x, x));
+    return smooth(ws, s, l, x, b, ws->post, false);
+}
+
+IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
+    IterState st;
+    std::memset(&st, 0, sizeof(st));
+    st.rtol = rtol;
+    st.check_every = std::max(1, cfg.cg_check_every);
+    st.max_iter = std::max(0, cfg.cg_max_iter);
+    st.stop_norm = cfg.stop_norm;
+    st.den = 1.0;
+    if (st.max_iter == 0) st.stop = 3;
+    return st;
+}
+
+int read_state(mf3d_workspace* ws, hipStream_t s) {
+    MF3D_HIP_TRY(hipMemcpyAsync(ws->host_st, ws->st, sizeof(IterState), hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    return MF3D_OK;
+}
+
+// Chunk length before the next host check: with the reference stopping rule
+// the device tests only every check_every iterations, so the host follows that
+// cadence; otherwise the length follows the observed contraction rate.
+struct Chunker {
+    int fixed = 0;
+    double prev_log = 0.0;
+    int prev_iter = 0;
+    bool have_prev = false;
+    int next(const IterState& st, double target) {
+        if (fixed > 0) return fixed - (st.iter % fixed);
+        int c = 4;
+        const double cur = 0.5 * std::log(std::max(st.rr, 1e-300));
+        if (have_prev && st.iter > prev_iter) {
+            const double rate = (cur - prev_log) / (st.iter - prev_iter);
+            if (rate < -1e-3) {
+                const double left = (std::log(target) - cur) / rate;
+                c = (int)std::ceil(left);
+            }
+        }
+        prev_log = cur;
+        prev_iter = st.iter;
+        have_prev = true;
+        return std::max(1, std::min(c, 32));
+    }
+};
+
+int state_upload(mf3d_workspace* ws, hipStream_t s, const IterState& st) {
+    *ws->host_st = st;
+    MF3D_HIP_TRY(hipMemcpyAsync(ws->st, ws->host_st, sizeof(IterState), hipMemcpyHostToDevice, s));
+    // host_st is reused as the download buffer: finish the upload first
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    return MF3D_OK;
+}
+
+// pcg_solve (src/numerics/solvers/pcg.cuh:95-185) with the MG preconditioner.
+int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
+            const mf3d_head_solve_config& cfg, const double* K, double* x,
+            mf3d_head_solve_result* res) {
+    const size_t n = cells(g);
+    const unsigned np = kern::reduce_blocks(n);
+    IterState st0 = make_state(cfg, cfg.cg_rtol);
+    MF3D_TRY(state_upload(ws, s, st0));
+    MF3D_TRY(kern::dot_partials(s, ws->rhs, ws->rhs, n, ws->partials));
+    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_BNORM, ws->st, nullptr));
+    MF3D_TRY(kern::op_residual(s, g, ws->bct, ws->pin, K, x, ws->rhs, ws->r, ws->partials));
+    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_R0, ws->st, nullptr));
+    MF3D_TRY(vcycle(ws, s, 0, ws->z, ws->r, true));
+    MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
+    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ0, ws->st, nullptr));
+    MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
+    MF3D_TRY(read_state(ws, s));
+    Chunker ch;
+    if (cfg.stop_norm == MF3D_STOP_INITIAL_RESIDUAL && cfg.cg_check_every > 1)
+        ch.fixed = cfg.cg_check_every;
+    const double target_rel = cfg.cg_rtol;
+    while (!ws->host_st->stop) {
+        const double target = target_rel * ws->host_st->den;
+        const int chunk = ch.next(*ws->host_st, target);
+        for (int c = 0; c < chunk; ++c) {
+            MF3D_TRY(kern::apply(s, g, ws->bct, ws->pin, K, ws->p, ws->Ap, ws->partials));
+            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_PAP, ws->st, nullptr));
+            MF3D_TRY(kern::pcg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
+            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RR, ws->st, nullptr));
+            MF3D_TRY(vcycle(ws, s, 0, ws->z, ws->r, true));
+            MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
+            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
+            MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
+        }
+        MF3D_TRY(read_state(ws, s));
+    }
+    const IterState& st = *ws->host_st;
+    res->num_iterations = st.iter;
+    res->converged = st.stop == 1;
+    res->initial_residual = st.r0;
+    res->final_residual = std::sqrt(st.rr);
+    res->bnorm = st.bnorm;
+    return st.stop == 2 ? fail(MF3D_ERR_BREAKDOWN, "PCG breakdown: (p, Ap) not finite") : MF3D_OK;
+}
+
+// cg_solve on NegatedOperator (src/numerics/solvers/cg.cuh:15-172,
+// solve_head.cu:137-180).
+int run_cg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
+           const mf3d_head_solve_config& cfg, const double* K, double* x,
+           mf3d_head_solve_result* res) {
+    const size_t n = cells(g);
+    const unsigned np = kern::reduce_blocks(n);
+    IterState st0 = make_state(cfg, cfg.cg_rtol);
+    MF3D_TRY(state_upload(ws, s, st0));
+    MF3D_TRY(kern::dot_partials(s, ws->rhs, ws->rhs, n, ws->partials));
+    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_BNORM, ws->st, nullptr));
+    double* bneg = ws->z;
+    MF3D_TRY(kern::scale_copy(s, -1.0, ws->rhs, bneg, n));
+    MF3D_TRY(kern::apply(s, g, ws->bct, ws->pin, K, x, ws->r, nullptr));
+    MF3D_TRY(kern::scal(s, ws->r, n, -1.0));
+    MF3D_TRY(kern::axpby(s, 1.0, bneg, -1.0, ws->r, n));
+    MF3D_TRY(kern::copy(s, ws->r, ws->p, n));
+    MF3D_TRY(kern::dot_partials(s, ws->r, ws->r, n, ws->partials));
+    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_RR0, ws->st, nullptr));
+    MF3D_TRY(read_state(ws, s));
+    Chunker ch;
+    if (cfg.stop_norm == MF3D_STOP_INITIAL_RESIDUAL && cfg.cg_check_every > 1)
+        ch.fixed = cfg.cg_check_every;
+    while (!ws->host_st->stop) {
+        const int chunk = ch.next(*ws->host_st, cfg.cg_rtol * ws->host_st->den);
+        for (int c = 0; c < chunk; ++c) {
+            MF3D_TRY(kern::apply(s, g, ws->bct, ws->pin, K, ws->p, ws->Ap, ws->partials));
+            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_PAP, ws->st, nullptr));
+            MF3D_TRY(kern::cg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
+            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_RR, ws->st, nullptr));
+            MF3D_TRY(kern::cg_update_p(s, ws->st, ws->r, ws->p, n));
+        }
+        MF3D_TRY(read_state(ws, s));
+    }
+    const IterState& st = *ws->host_st;
+    res->num_iterations = st.iter;
+    res->converged = st.stop == 1;
+    res->initial_residual = st.r0;
+    res->final_residual = st.stop == 2 ? -1.0 : std::sqrt(st.rr);
+    res->bnorm = st.bnorm;
+    return MF3D_OK;
+}
+
+// host-synchronous ||v|| with the workspace reduction scratch
+int host_nrm2(mf3d_workspace* ws, hipStream_t s, const double* v, size_t n, double* out) {
+    MF3D_TRY(kern::dot_partials(s, v, v, n, ws->partials));
+    MF3D_TRY(kern::finalize(s, ws->partials, kern::reduce_blocks(n), FIN_SQRT, nullptr,
+                            ws->dscalar));
+    MF3D_HIP_TRY(hipMemcpyAsync(out, ws->dscalar, sizeof(double), hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    return MF3D_OK;
+}
+
+// mg_solve (src/multigrid/cycle/v_cycle.cu:76-116)
+int run_mg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
+           const mf3d_head_solve_config& cfg, const double* K, double* x,
+           mf3d_head_solve_result* res) {
+    const size_t n = cells(g);
+    double bnorm = 0.0, r0 = 0.0, rn = 0.0;
+    MF3D_TRY(host_nrm2(ws, s, ws->rhs, n, &bnorm));
+    MF3D_TRY(kern::residual(s, g, ws->bct, ws->pin, x, ws->rhs, K, ws->r));
+    MF3D_TRY(host_nrm2(ws, s, ws->r, n, &r0));
+    const double den = cfg.stop_norm == MF3D_STOP_RHS_NORM ? bnorm : r0;
+    res->initial_residual = r0;
+    res->bnorm = bnorm;
+    for (int c = 0; c < cfg.mg_max_cycles; ++c) {
+        res->num_iterations = c + 1;
+        MF3D_TRY(vcycle(ws, s, 0, x, ws->rhs, false));
+        MF3D_TRY(kern::residual(s, g, ws->bct, ws->pin, x, ws->rhs, K, ws->r));
+        MF3D_TRY(host_nrm2(ws, s, ws->r, n, &rn));
+        res->final_residual = rn;
+        if (rn / den < cfg.rtol) {
+            res->converged = 1;
+            break;
+        }
+    }
+    return MF3D_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.f;
+    return ms;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int mf3d_abi_version(void) { return MF3D_ABI_VERSION; }
+
+const char* mf3d_status_string(int status) {
+    switch (status) {
+    case MF3D_OK: return "ok";
+    case MF3D_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case MF3D_ERR_SIZE_MISMATCH: return "size mismatch";
+    case MF3D_ERR_BREAKDOWN: return "breakdown";
+    case MF3D_ERR_HIP: return "HIP runtime error";
+    case MF3D_ERR_ALLOC: return "device allocation failed";
+    case MF3D_ERR_SINGULAR: return "singular coarsest operator";
+    case MF3D_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+    }
+}
+
+const char* mf3d_last_error(void) { return mf3d::last_error_cstr(); }
+
+void mf3d_default_config(mf3d_head_solve_config* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->solver_type = MF3D_SOLVER_PCG_MG;
+    cfg->mg_levels = 4;
+    cfg->mg_pre_smooth = 2;
+    cfg->mg_post_smooth = 2;
+    cfg->mg_max_cycles = 20;
+    cfg->mg_coarse_iters = 50;
+    cfg->cg_max_iter = 1000;
+    cfg->cg_check_every = 10;
+    cfg->cg_rtol = 1e-8;
+    cfg->rtol = 1e-6;
+    cfg->pin_mode = MF3D_PIN_AUTO;
+}
+
+int mf3d_workspace_create(mf3d_workspace** ws) {
+    if (!ws) return fail(MF3D_ERR_INVALID_ARGUMENT, "ws is null");
+    *ws = new (std::nothrow) mf3d_workspace();
+    return *ws ? MF3D_OK : fail(MF3D_ERR_ALLOC, "host allocation failed");
+}
+
+int mf3d_workspace_destroy(mf3d_workspace* ws) {
+    delete ws;
+    return MF3D_OK;
+}
+
+size_t mf3d_workspace_bytes(const mf3d_workspace* ws) { return ws ? ws->bytes : 0; }
+
+// An error left by an unrelated earlier HIP call (this thread's last-error
+// slot) must not be reported as a failure of the launches below.
+static void clear_stale_error() { (void)hipGetLastError(); }
+
+#define MF3D_CHECK_PTR(p)                                                       \
+    do {                                                                        \
+        if (!(p)) return fail(MF3D_ERR_INVALID_ARGUMENT, #p " is null");        \
+    } while (0)
+#define MF3D_CHECK_LEN(len, need, what)                                         \
+    do {                                                                        \
+        if ((len) < (need))                                                     \
+            return fail(MF3D_ERR_SIZE_MISMATCH, std::string(what) + " buffer too small"); \
+    } while (0)
+
+static int check_pair(const mf3d_grid* coarse, const mf3d_grid* fine) {
+    MF3D_TRY(validate_grid(coarse));
+    MF3D_TRY(validate_grid(fine));
+    if (fine->nx != 2 * coarse->nx || fine->ny != 2 * coarse->ny || fine->nz != 2 * coarse->nz)
+        return fail(MF3D_ERR_SIZE_MISMATCH, "fine grid must be exactly 2x the coarse grid");
+    return MF3D_OK;
+}
+
+static int check_config(const mf3d_head_solve_config* cfg) {
+    MF3D_CHECK_PTR(cfg);
+    if (cfg->solver_type < 0 || cfg->solver_type > 2)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown solver_type");
+    if (cfg->mg_levels < 1 || cfg->mg_pre_smooth < 0 || cfg->mg_post_smooth < 0 ||
+        cfg->mg_coarse_iters < 0 || cfg->cg_max_iter < 0 || cfg->mg_max_cycles < 0)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "negative iteration count or mg_levels < 1");
+    if (cfg->precond_precision != MF3D_PRECOND_FP64 && cfg->precond_precision != MF3D_PRECOND_FP32)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown precond_precision");
+    if (cfg->precond_precision == MF3D_PRECOND_FP32)
+        return fail(MF3D_ERR_UNSUPPORTED, "FP32 preconditioner not built in this library");
+    return MF3D_OK;
+}
+
+int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
+                    const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
+                    double* h, size_t n, mf3d_head_solve_result* result) {
+    clear_stale_error();
+    MF3D_CHECK_PTR(ws);
+    MF3D_CHECK_PTR(result);
+    std::memset(result, 0, sizeof(*result));
+    result->true_residual = -1.0;
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_TRY(check_config(cfg));
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(h);
+    const size_t nc = cells(*grid);
+    // solve_head.cu:85-87
+    if (n < nc) return fail(MF3D_ERR_SIZE_MISMATCH, "Head or K buffer too small");
+    hipStream_t s = as_stream(stream);
+    MF3D_TRY(ensure_workspace(ws, *grid, cfg->mg_levels));
+    MF3D_HIP_TRY(hipEventRecord(ws->ev[0], s));
+    MF3D_TRY(mg_setup(ws, s, *grid, *bc, *cfg, K));
+    MF3D_TRY(kern::rhs(s, *grid, *bc, K, ws->rhs));
+    // solve_head.cu:123 overwrites h with the reference guess; the extension
+    // keeps a caller-provided guess.
+    if (cfg->init_guess != MF3D_GUESS_PROVIDED)
+        MF3D_TRY(kern::guess(s, *grid, *bc, cfg->init_guess, h));
+    MF3D_HIP_TRY(hipEventRecord(ws->ev[1], s));
+    int st = MF3D_OK;
+    if (cfg->solver_type == MF3D_SOLVER_PCG_MG)
+        st = run_pcg(ws, s, *grid, *cfg, K, h, result);
+    else if (cfg->solver_type == MF3D_SOLVER_CG)
+        st = run_cg(ws, s, *grid, *cfg, K, h, result);
+    else
+        st = run_mg(ws, s, *grid, *cfg, K, h, result);
+    if (st != MF3D_OK) return st;
+    MF3D_HIP_TRY(hipEventRecord(ws->ev[2], s));
+    if (cfg->true_residual) {
+        MF3D_TRY(kern::op_residual(s, *grid, ws->bct, ws->pin, K, h, ws->rhs, nullptr,
+                                   ws->partials));
+        MF3D_TRY(kern::finalize(s, ws->partials, kern::reduce_blocks(nc), FIN_SQRT, nullptr,
+                                ws->dscalar));
+        MF3D_HIP_TRY(hipMemcpyAsync(&result->true_residual, ws->dscalar, sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+    }
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    result->setup_ms = elapsed(ws->ev[0], ws->ev[1]);
+    result->solve_ms = elapsed(ws->ev[1], ws->ev[2]);
+    return MF3D_OK;
+}
+
+int mf3d_init_head_guess(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, double* h,
+                         size_t n) {
+    clear_stale_error();
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(h);
+    MF3D_CHECK_LEN(n, cells(*grid), "head");
+    return kern::guess(as_stream(stream), *grid, *bc, MF3D_GUESS_REFERENCE, h);
+}
+
+int mf3d_build_rhs_head(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
+                        const double* K, double* rhs, size_t n) {
+    clear_stale_error();
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(rhs);
+    // rhs_head.cu:124-126
+    if (n < cells(*grid)) return fail(MF3D_ERR_SIZE_MISMATCH, "RHS or K buffer too small");
+    return kern::rhs(as_stream(stream), *grid, *bc, K, rhs);
+}
+
+int mf3d_coarsen_K(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_grid* fine,
+                   const double* K_fine, size_t n_fine, double* K_coarse, size_t n_coarse) {
+    clear_stale_error();
+    MF3D_TRY(check_pair(coarse, fine));
+    MF3D_CHECK_PTR(K_fine);
+    MF3D_CHECK_PTR(K_coarse);
+    MF3D_CHECK_LEN(n_fine, cells(*fine), "fine K");
+    MF3D_CHECK_LEN(n_coarse, cells(*coarse), "coarse K");
+    return kern::coarsen(as_stream(stream), *coarse, K_fine, K_coarse);
+}
+
+int mf3d_velocity_compact(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
+                          const double* h, const double* K, size_t n, double* U, size_t nU,
+                          double* V, size_t nV, double* W, size_t nW) {
+    clear_stale_error();
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(h);
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(U);
+    MF3D_CHECK_PTR(V);
+    MF3D_CHECK_PTR(W);
+    const mf3d_grid& g = *grid;
+    MF3D_CHECK_LEN(n, cells(g), "head/K");
+    MF3D_CHECK_LEN(nU, (size_t)(g.nx + 1) * g.ny * g.nz, "U");
+    MF3D_CHECK_LEN(nV, (size_t)g.nx * (g.ny + 1) * g.nz, "V");
+    MF3D_CHECK_LEN(nW, (size_t)g.nx * g.ny * (g.nz + 1), "W");
+    return kern::velocity(as_stream(stream), g, *bc, h, K, U, V, W, false);
+}
+
+int mf3d_velocity_padded(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
+                         const double* h, const double* K, size_t n, double* U, double* V,
+                         double* W, size_t n_padded) {
+    clear_stale_error();
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(h);
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(U);
+    MF3D_CHECK_PTR(V);
+    MF3D_CHECK_PTR(W);
+    const mf3d_grid& g = *grid;
+    MF3D_CHECK_LEN(n, cells(g), "head/K");
+    MF3D_CHECK_LEN(n_padded, (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1), "padded velocity");
+    return kern::velocity(as_stream(stream), g, *bc, h, K, U, V, W, true);
+}
+
+int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, const double* b,
+                        const double* K, size_t n, int num_iters, const mf3d_bc* bc, int pin) {
+    clear_stale_error();
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(x);
+    MF3D_CHECK_PTR(b);
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_LEN(n, cells(*grid), "x/b/K");
+    if (num_iters < 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "num_iters < 0");
+    const BcTypes bt = bc_types(*bc);
+    if (kern::fast_ok(*grid, bt))
+        return kern::gsrb_fast(as_stream(stream), *grid, bt, pin != 0, x, b, K, num_iters, false);
+    return kern::gsrb(as_stream(stream), *grid, bt, pin != 0, x, b, K, num_iters, false);
+}
+
+int mf3d_compute_residual_3d(mf3d_stream stream, const mf3d_grid* grid, const double* x,
+                             const double* b, const double* K, double* r, size_t n,
+                             const mf3d_bc* bc, int pin) {
+    clear_stale_error();
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(x);
+    MF3D_CHECK_PTR(b);
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(r);
+    MF3D_CHECK_LEN(n, cells(*grid), "x/b/K/r");
+    return kern::residual(as_stream(stream), *grid, bc_types(*bc), pin != 0, x, b, K, r);
+}
+
+int mf3d_restrict_3d(mf3d_stream stream, const mf3d_grid* fine, const mf3d_grid* coarse,
+                     const double* r_fine, size_t n_fine, double* b_coarse, size_t n_coarse) {
+    clear_stale_error();
+    MF3D_TRY(check_pair(coarse, fine));
+    MF3D_CHECK_PTR(r_fine);
+    MF3D_CHECK_PTR(b_coarse);
+    MF3D_CHECK_LEN(n_fine, cells(*fine), "fine");
+    MF3D_CHECK_LEN(n_coarse, cells(*coarse), "coarse");
+    return kern::restrict_avg(as_stream(stream), *coarse, r_fine, b_coarse);
+}
+
+int mf3d_prolong_3d_add(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_grid* fine,
+                        const double* x_coarse, size_t n_coarse, double* x_fine, size_t n_fine) {
+    clear_stale_error();
+    MF3D_TRY(check_pair(coarse, fine));
+    MF3D_CHECK_PTR(x_coarse);
+    MF3D_CHECK_PTR(x_fine);
+    MF3D_CHECK_LEN(n_fine, cells(*fine), "fine");
+    MF3D_CHECK_LEN(n_coarse, cells(*coarse), "coarse");
+    return kern::prolong_add(as_stream(stream), *fine, x_coarse, x_fine);
+}
+
+int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
+                  const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
+                  size_t n) {
+    clear_stale_error();
+    MF3D_CHECK_PTR(ws);
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_TRY(check_config(cfg));
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_LEN(n, cells(*grid), "K");
+    return mg_setup(ws, as_stream(stream), *grid, *bc, *cfg, K);
+}
+
+int mf3d_mg_num_levels(const mf3d_workspace* ws) {
+    return ws && ws->ready ? (int)ws->lv.size() : 0;
+}
+
+int mf3d_mg_precondition(mf3d_workspace* ws, mf3d_stream stream, const double* r, double* z,
+                         size_t n) {
+    clear_stale_error();
+    MF3D_CHECK_PTR(ws);
+    MF3D_CHECK_PTR(r);
+    MF3D_CHECK_PTR(z);
+    if (!ws->ready || !ws->K0)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "mf3d_mg_setup has not been called");
+    MF3D_CHECK_LEN(n, cells(ws->fine), "r/z");
+    return vcycle(ws, as_stream(stream), 0, z, r, true);
+}
+
+int mf3d_varcoeff_apply(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, int pin,
+                        const double* K, const double* x, double* y, size_t n) {
+    clear_stale_error();
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(x);
+    MF3D_CHECK_PTR(y);
+    MF3D_CHECK_LEN(n, cells(*grid), "x/y/K");
+    return kern::apply(as_stream(stream), *grid, bc_types(*bc), pin != 0, K, x, y, nullptr);
+}
+
+int mf3d_dot(mf3d_workspace* ws, mf3d_stream stream, const double* x, const double* y, size_t n,
+             double* d_result) {
+    clear_stale_error();
+    MF3D_CHECK_PTR(ws);
+    MF3D_CHECK_PTR(d_result);
+    hipStream_t s = as_stream(stream);
+    if (n == 0) return kern::fill(s, d_result, 1, 0.0);  // dot.cu:13-18
+    MF3D_CHECK_PTR(x);
+    MF3D_CHECK_PTR(y);
+    MF3D_TRY(scratch_partials(ws, n));
+    MF3D_TRY(kern::dot_partials(s, x, y, n, ws->partials));
+    return kern::finalize(s, ws->partials, kern::reduce_blocks(n), FIN_STORE, nullptr, d_result);
+}
+
+int mf3d_nrm2(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
+              double* d_result) {
+    clear_stale_error();
+    MF3D_CHECK_PTR(ws);
+    MF3D_CHECK_PTR(d_result);
+    hipStream_t s = as_stream(stream);
+    if (n == 0) return kern::fill(s, d_result, 1, 0.0);
+    MF3D_CHECK_PTR(x);
+    MF3D_TRY(scratch_partials(ws, n));
+    MF3D_TRY(kern::dot_partials(s, x, x, n, ws->partials));
+    return kern::finalize(s, ws->partials, kern::reduce_blocks(n), FIN_SQRT, nullptr, d_result);
+}
+
+int mf3d_axpy(mf3d_stream stream, double a, const double* x, double* y, size_t n) {
+    clear_stale_error();
+    if (n && (!x || !y)) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    return kern::axpy(as_stream(stream), a, x, y, n);
+}
+int mf3d_axpby(mf3d_stream stream, double a, const double* x, double b, double* y, size_t n) {
+    clear_stale_error();
+    if (n && (!x || !y)) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    return kern::axpby(as_stream(stream), a, x, b, y, n);
+}
+int mf3d_copy(mf3d_stream stream, const double* x, double* y, size_t n) {
+    clear_stale_error();
+    if (n && (!x || !y)) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    return kern::copy(as_stream(stream), x, y, n);
+}
+int mf3d_fill(mf3d_stream stream, double* x, size_t n, double value) {
+    clear_stale_error();
+    if (n && !x) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    return kern::fill(as_stream(stream), x, n, value);
+}
+int mf3d_scal(mf3d_stream stream, double* x, size_t n, double a) {
+    clear_stale_error();
+    if (n && !x) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    return kern::scal(as_stream(stream), x, n, a);
+}
+
+}  // extern "C"

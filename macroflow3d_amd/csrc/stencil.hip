@@ -1,0 +1,612 @@
+// Cell-centred 7-point variable-K stencil kernels (FP64), gfx950.
+//
+// One thread per cell (or per own-colour cell for the red-black smoother),
+// all boundary types handled in the same launch: the reference spends 27
+// launches per colour on interior/face/edge/vertex kernels
+// (src/multigrid/smoothers/gsrb_3d.cu:392-575); here a colour is one launch.
+// These kernels serve every level and every boundary mix; the z-marching
+// fine-level kernels in stencil_fast.hip take over on large levels.
+
+#include "kernels.hpp"
+
+namespace mf3d {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t n, int block = kBlock) {
+    return (unsigned)((n + block - 1) / block);
+}
+
+// GSRB value of one cell: (sum K_f x_nb - b dx^2) / sum K_f.
+// Interior: gsrb_interior_kernel (src/multigrid/smoothers/gsrb_3d.cu:29-73).
+// Boundary: gsrb_{face,edge,vertex}_kernel (gsrb_3d.cu:76-366), pin doubles
+// the diagonal of cell 0 (gsrb_3d.cu:360-362).
+template <bool ZERO>
+__device__ inline double gsrb_value(int nx, int ny, int nz, const BcTypes& bc, bool pin,
+                                    const double* __restrict__ x, const double* __restrict__ b,
+                                    const double* __restrict__ K, int i, int j, int k, long idx,
+                                    double dx2) {
+    const double KC = K[idx];
+    if (interior_cell(nx, ny, nz, i, j, k)) {
+        const long Nx = nx, st = (long)nx * ny;
+        const double iKC = 1.0 / KC;
+        const double K_xp = 2.0 / (iKC + 1.0 / K[idx + 1]);
+        const double K_xm = 2.0 / (iKC + 1.0 / K[idx - 1]);
+        const double K_yp = 2.0 / (iKC + 1.0 / K[idx + Nx]);
+        const double K_ym = 2.0 / (iKC + 1.0 / K[idx - Nx]);
+        const double K_zp = 2.0 / (iKC + 1.0 / K[idx + st]);
+        const double K_zm = 2.0 / (iKC + 1.0 / K[idx - st]);
+        double result = 0.0;
+        if (!ZERO) {
+            result = x[idx + 1] * K_xp;
+            result += x[idx - 1] * K_xm;
+            result += x[idx + Nx] * K_yp;
+            result += x[idx - Nx] * K_ym;
+            result += x[idx + st] * K_zp;
+            result += x[idx - st] * K_zm;
+        }
+        const double aC = K_xp + K_xm + K_yp + K_ym + K_zp + K_zm;
+        return (result - b[idx] * dx2) / aC;
+    }
+    double c[6], v[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const long n = neighbour_coef(nx, ny, nz, bc, K, i, j, k, d, KC, c[d]);
+        v[d] = (ZERO || n < 0) ? 0.0 : x[n];
+    }
+    const double result =
+        c[0] * v[0] + c[1] * v[1] + c[2] * v[2] + c[3] * v[3] + c[4] * v[4] + c[5] * v[5];
+    double aC = c[0] + c[1] + c[2] + c[3] + c[4] + c[5];
+    if (pin && idx == 0) aC *= 2.0;
+    return (result - b[idx] * dx2) / aC;
+}
+
+// One colour of one GSRB iteration. Thread t covers the pair of cells
+// (2*ii, 2*ii+1) of row (j,k) and updates the one of colour `colour`
+// ((i+j+k) even = red = 0). ZERO: x is taken as zero on entry, the partner
+// (other colour) cell is written 0 -- a fill(0) fused into the first sweep.
+template <bool ZERO>
+__global__ void __launch_bounds__(kBlock)
+    gsrb_colour_kernel(int nx, int ny, int nz, double dx2, BcTypes bc, int pin,
+                       double* __restrict__ x, const double* __restrict__ b,
+                       const double* __restrict__ K, int colour) {
+    const int nxh = (nx + 1) >> 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)nxh * ny * nz;
+    if (t >= total) return;
+    const int ii = (int)(t % nxh);
+    const long row = t / nxh;
+    const int j = (int)(row % ny);
+    const int k = (int)(row / ny);
+    const int par = (j + k + colour) & 1;
+    const int i = 2 * ii + par;
+    const long rowbase = (long)nx * row;
+    if (ZERO) {
+        const int ip = 2 * ii + (par ^ 1);
+        if (ip < nx) x[rowbase + ip] = 0.0;
+    }
+    if (i >= nx) return;
+    const long idx = rowbase + i;
+    x[idx] = gsrb_value<ZERO>(nx, ny, nz, bc, pin != 0, x, b, K, i, j, k, idx, dx2);
+}
+
+// Residual value r = b - A x of one cell.
+// Interior: residual_interior_kernel (src/multigrid/smoothers/residual_3d.cu:30-80).
+// Boundary: residual_{face,edge,vertex}_kernel (residual_3d.cu:83-376), pin at
+// cell 0 adds aC*x_C (residual_3d.cu:365-368).
+__device__ inline double residual_value(int nx, int ny, int nz, const BcTypes& bc, bool pin,
+                                        const double* __restrict__ x,
+                                        const double* __restrict__ b,
+                                        const double* __restrict__ K, int i, int j, int k,
+                                        long o, double dx2, double inv_dx2) {
+    const double xc = x[o], Kc = K[o];
+    if (interior_cell(nx, ny, nz, i, j, k)) {
+        const long Nx = nx, st = (long)nx * ny;
+        const double iKc = 1.0 / Kc;
+        double Ax = 0.0;
+        Ax -= 2.0 * (xc - x[o + 1]) / (iKc + 1.0 / K[o + 1]);
+        Ax -= 2.0 * (xc - x[o + Nx]) / (iKc + 1.0 / K[o + Nx]);
+        Ax -= 2.0 * (xc - x[o - 1]) / (iKc + 1.0 / K[o - 1]);
+        Ax -= 2.0 * (xc - x[o - Nx]) / (iKc + 1.0 / K[o - Nx]);
+        Ax -= 2.0 * (xc - x[o + st]) / (iKc + 1.0 / K[o + st]);
+        Ax -= 2.0 * (xc - x[o - st]) / (iKc + 1.0 / K[o - st]);
+        return b[o] - Ax * inv_dx2;
+    }
+    double c[6], v[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const long n = neighbour_coef(nx, ny, nz, bc, K, i, j, k, d, Kc, c[d]);
+        v[d] = n < 0 ? (bc.t[d] == MF3D_BC_NEUMANN ? xc : 0.0) : x[n];
+    }
+    double result = c[0] * (xc - v[0]) + c[1] * (xc - v[1]) + c[2] * (xc - v[2]) +
+                    c[3] * (xc - v[3]) + c[4] * (xc - v[4]) + c[5] * (xc - v[5]);
+    if (pin && o == 0) result += (c[0] + c[1] + c[2] + c[3] + c[4] + c[5]) * xc;
+    return b[o] - (-result / dx2);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    residual_kernel(int nx, int ny, int nz, double dx2, double inv_dx2, BcTypes bc, int pin,
+                    const double* __restrict__ x, const double* __restrict__ b,
+                    const double* __restrict__ K, double* __restrict__ r) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)nx * ny * nz;
+    if (o >= n) return;
+    const int i = (int)(o % nx);
+    const long q = o / nx;
+    const int j = (int)(q % ny);
+    const int k = (int)(q / ny);
+    r[o] = residual_value(nx, ny, nz, bc, pin != 0, x, b, K, i, j, k, o, dx2, inv_dx2);
+}
+
+// Residual fused with restriction: one thread per coarse cell evaluates the
+// residual of its 8 children and averages them in the order of
+// restrict_kernel (src/multigrid/transfer/restrict_3d.cu:36-49); the fine
+// residual is never stored.
+__global__ void __launch_bounds__(kBlock)
+    residual_restrict_kernel(int nx, int ny, int nz, double dx2, double inv_dx2, BcTypes bc,
+                             int pin, const double* __restrict__ x,
+                             const double* __restrict__ b, const double* __restrict__ K,
+                             double* __restrict__ bc_out) {
+    const int NX = nx >> 1, NY = ny >> 1, NZ = nz >> 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)NX * NY * NZ) return;
+    const int I = (int)(t % NX);
+    const long q = t / NX;
+    const int J = (int)(q % NY);
+    const int Kc = (int)(q / NY);
+    const int i = 2 * I, j = 2 * J, k = 2 * Kc;
+    const long Nx = nx, st = (long)nx * ny;
+    const long in = i + Nx * j + st * k;
+    const bool p = pin != 0;
+    double result = 0.0;
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i, j, k, in, dx2, inv_dx2);
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i + 1, j, k, in + 1, dx2, inv_dx2);
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i, j + 1, k, in + Nx, dx2, inv_dx2);
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i + 1, j + 1, k, in + 1 + Nx, dx2,
+                             inv_dx2);
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i, j, k + 1, in + st, dx2, inv_dx2);
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i + 1, j, k + 1, in + 1 + st, dx2,
+                             inv_dx2);
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i, j + 1, k + 1, in + Nx + st, dx2,
+                             inv_dx2);
+    result += residual_value(nx, ny, nz, bc, p, x, b, K, i + 1, j + 1, k + 1,
+                             in + 1 + Nx + st, dx2, inv_dx2);
+    bc_out[t] = result / 8.0;
+}
+
+// Operator value y = A x of one cell, VarCoeffLaplacian::apply
+// (src/numerics/operators/varcoeff_laplacian.cu:38-239). The pin's diagonal
+// counts only in-domain neighbours (varcoeff_laplacian.cu:110-119, 232-234).
+__device__ inline double apply_value(int nx, int ny, int nz, const BcTypes& bc, bool pin,
+                                     const double* __restrict__ K,
+                                     const double* __restrict__ x, int i, int j, int k, long idx,
+                                     double inv_dx2) {
+    const double xC = x[idx], KC = K[idx];
+    const long Nx = nx, st = (long)nx * ny;
+    if (interior_cell(nx, ny, nz, i, j, k)) {
+        const double iKC = 1.0 / KC;
+        const double Kxm = 2.0 / (iKC + 1.0 / K[idx - 1]);
+        const double Kxp = 2.0 / (iKC + 1.0 / K[idx + 1]);
+        const double Kym = 2.0 / (iKC + 1.0 / K[idx - Nx]);
+        const double Kyp = 2.0 / (iKC + 1.0 / K[idx + Nx]);
+        const double Kzm = 2.0 / (iKC + 1.0 / K[idx - st]);
+        const double Kzp = 2.0 / (iKC + 1.0 / K[idx + st]);
+        const double Ax = Kxm * (xC - x[idx - 1]) + Kxp * (xC - x[idx + 1]) +
+                          Kym * (xC - x[idx - Nx]) + Kyp * (xC - x[idx + Nx]) +
+                          Kzm * (xC - x[idx - st]) + Kzp * (xC - x[idx + st]);
+        return -Ax * inv_dx2;
+    }
+    double Ax = 0.0, aC = 0.0;
+    const int pos[3] = {i, j, k};
+    const int lim[3] = {nx, ny, nz};
+    const long stride[3] = {1, Nx, st};
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int ax = d >> 1;
+        const bool plus = d & 1;
+        const bool inside = plus ? (pos[ax] < lim[ax] - 1) : (pos[ax] > 0);
+        if (inside) {
+            const long n = plus ? idx + stride[ax] : idx - stride[ax];
+            const double Kh = 2.0 / (1.0 / KC + 1.0 / K[n]);
+            Ax += Kh * (xC - x[n]);
+            aC += Kh;
+        } else if (bc.t[d] == MF3D_BC_PERIODIC) {
+            const long n = plus ? idx - (long)(lim[ax] - 1) * stride[ax]
+                                : idx + (long)(lim[ax] - 1) * stride[ax];
+            const double Kh = 2.0 / (1.0 / KC + 1.0 / K[n]);
+            Ax += Kh * (xC - x[n]);
+        } else if (bc.t[d] == MF3D_BC_DIRICHLET) {
+            Ax += 2.0 * KC * xC;
+        }
+    }
+    if (pin && idx == 0) Ax += aC * xC;
+    return -Ax * inv_dx2;
+}
+
+__device__ inline double block_sum(double v) {
+    __shared__ double part[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = lane < kBlock / 64 ? part[lane] : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    }
+    return v;
+}
+
+// y = A x; when partials != nullptr also writes this block's share of (x, y).
+template <bool DOT>
+__global__ void __launch_bounds__(kBlock)
+    apply_kernel(int nx, int ny, int nz, double inv_dx2, BcTypes bc, int pin,
+                 const double* __restrict__ K, const double* __restrict__ x,
+                 double* __restrict__ y, double* __restrict__ partials) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)nx * ny * nz;
+    double prod = 0.0;
+    if (o < n) {
+        const int i = (int)(o % nx);
+        const long q = o / nx;
+        const int j = (int)(q % ny);
+        const int k = (int)(q / ny);
+        const double v = apply_value(nx, ny, nz, bc, pin != 0, K, x, i, j, k, o, inv_dx2);
+        y[o] = v;
+        if (DOT) prod = x[o] * v;
+    }
+    if (DOT) {
+        const double s = block_sum(prod);
+        if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    }
+}
+
+// r = b - A x via the operator (PCG initial residual and true residual,
+// pcg.cuh:113-116), with this block's share of (r, r).
+__global__ void __launch_bounds__(kBlock)
+    op_residual_kernel(int nx, int ny, int nz, double inv_dx2, BcTypes bc, int pin,
+                       const double* __restrict__ K, const double* __restrict__ x,
+                       const double* __restrict__ b, double* __restrict__ r,
+                       double* __restrict__ partials) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)nx * ny * nz;
+    double prod = 0.0;
+    if (o < n) {
+        const int i = (int)(o % nx);
+        const long q = o / nx;
+        const int j = (int)(q % ny);
+        const int k = (int)(q / ny);
+        const double Ax = apply_value(nx, ny, nz, bc, pin != 0, K, x, i, j, k, o, inv_dx2);
+        const double v = -1.0 * Ax + b[o];
+        if (r) r[o] = v;
+        prod = v * v;
+    }
+    const double s = block_sum(prod);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// restrict_kernel (src/multigrid/transfer/restrict_3d.cu:12-53)
+__global__ void __launch_bounds__(kBlock)
+    restrict_kernel(int NX, int NY, int NZ, const double* __restrict__ f,
+                    double* __restrict__ c) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)NX * NY * NZ) return;
+    const int I = (int)(t % NX);
+    const long q = t / NX;
+    const int J = (int)(q % NY);
+    const int Kk = (int)(q / NY);
+    const long Nx = 2L * NX, st = Nx * 2L * NY;
+    const long in = 2L * I + Nx * 2L * J + st * 2L * Kk;
+    double result = 0.0;
+    result += f[in];
+    result += f[in + 1];
+    result += f[in + Nx];
+    result += f[in + 1 + Nx];
+    result += f[in + st];
+    result += f[in + 1 + st];
+    result += f[in + Nx + st];
+    result += f[in + 1 + Nx + st];
+    c[t] = result / 8.0;
+}
+
+// prolong_3d_add (src/multigrid/transfer/prolong_3d.cu:30-621): x_f += x_c[i/2, j/2, k/2].
+__global__ void __launch_bounds__(kBlock)
+    prolong_add_kernel(int nx, int ny, int nz, const double* __restrict__ c,
+                       double* __restrict__ f) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long)nx * ny * nz) return;
+    const int i = (int)(o % nx);
+    const long q = o / nx;
+    const int j = (int)(q % ny);
+    const int k = (int)(q / ny);
+    const int NX = nx >> 1, NY = ny >> 1;
+    f[o] += c[(long)(i >> 1) + (long)NX * ((long)(j >> 1) + (long)NY * (long)(k >> 1))];
+}
+
+// coarsen_K_kernel (src/physics/flow/coarsen_K.cu:15-53): geometric mean.
+__global__ void __launch_bounds__(kBlock)
+    coarsen_kernel(int NX, int NY, int NZ, const double* __restrict__ Kf,
+                   double* __restrict__ Kc) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)NX * NY * NZ) return;
+    const int I = (int)(t % NX);
+    const long q = t / NX;
+    const int J = (int)(q % NY);
+    const int Kk = (int)(q / NY);
+    const long Nx = 2L * NX, st = Nx * 2L * NY;
+    const long i0 = 2L * I + Nx * 2L * J + st * 2L * Kk;
+    const double s = log(Kf[i0]) + log(Kf[i0 + 1]) + log(Kf[i0 + Nx]) + log(Kf[i0 + 1 + Nx]) +
+                     log(Kf[i0 + st]) + log(Kf[i0 + 1 + st]) + log(Kf[i0 + Nx + st]) +
+                     log(Kf[i0 + 1 + Nx + st]);
+    Kc[t] = exp(s / 8.0);
+}
+
+// build_rhs_head (src/physics/flow/rhs_head.cu:21-194): b = 0, then
+// b -= 2 h_bc K / dx^2 on each Dirichlet face, faces in the order
+// xmin, xmax, ymin, ymax, zmin, zmax.
+__global__ void __launch_bounds__(kBlock)
+    rhs_kernel(int nx, int ny, int nz, double inv_dx2, mf3d_bc bc,
+               const double* __restrict__ K, double* __restrict__ rhs) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long)nx * ny * nz) return;
+    const int i = (int)(o % nx);
+    const long q = o / nx;
+    const int j = (int)(q % ny);
+    const int k = (int)(q / ny);
+    const int pos[3] = {i, j, k};
+    const int lim[3] = {nx, ny, nz};
+    double v = 0.0;
+    for (int f = 0; f < 6; ++f) {
+        if (bc.type[f] != MF3D_BC_DIRICHLET) continue;
+        const int ax = f >> 1;
+        if (pos[ax] != ((f & 1) ? lim[ax] - 1 : 0)) continue;
+        v -= 2.0 * bc.value[f] * K[o] * inv_dx2;
+    }
+    rhs[o] = v;
+}
+
+// kernel_init_head_linear_x / kernel_fill_value (src/physics/flow/solve_head.cu:29-50)
+__global__ void __launch_bounds__(kBlock)
+    guess_kernel(int nx, int ny, int nz, int linear, double hw, double he,
+                 double* __restrict__ h) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long)nx * ny * nz) return;
+    if (linear) {
+        const int i = (int)(o % nx);
+        const double t = ((double)i + 0.5) / (double)nx;
+        h[o] = hw + (he - hw) * t;
+    } else {
+        h[o] = 0.0;
+    }
+}
+
+// Whole GSRB smoothing of a small level by one workgroup, x held in LDS.
+// Same per-cell values as gsrb_colour_kernel; replaces 2*iters launches
+// (the coarsest level runs 50 iterations, v_cycle.cu:41-45).
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallMaxCells = 4096;
+
+__global__ void __launch_bounds__(kSmallBlock)
+    gsrb_small_kernel(int nx, int ny, int nz, double dx2, BcTypes bc, int pin,
+                      double* __restrict__ x, const double* __restrict__ b,
+                      const double* __restrict__ K, int iters, int zero_init) {
+    __shared__ double xs[kSmallMaxCells];
+    const int n = nx * ny * nz;
+    for (int o = threadIdx.x; o < n; o += blockDim.x) xs[o] = zero_init ? 0.0 : x[o];
+    __syncthreads();
+    for (int it = 0; it < iters; ++it) {
+        for (int colour = 0; colour < 2; ++colour) {
+            for (int o = threadIdx.x; o < n; o += blockDim.x) {
+                const int i = o % nx, q = o / nx, j = q % ny, k = q / ny;
+                if (((i + j + k) & 1) != colour) continue;
+                xs[o] = gsrb_value<false>(nx, ny, nz, bc, pin != 0, xs, b, K, i, j, k, o, dx2);
+            }
+            __syncthreads();
+        }
+    }
+    for (int o = threadIdx.x; o < n; o += blockDim.x) x[o] = xs[o];
+}
+
+// Coarsest direct solve x = A^{-1} b, one wave per row (dense FP64 GEMV).
+__global__ void __launch_bounds__(256)
+    dense_gemv_kernel(int n, const double* __restrict__ Ainv, const double* __restrict__ b,
+                      double* __restrict__ x) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double* a = Ainv + (long)row * n;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += a[c] * b[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) x[row] = s;
+}
+
+// Columns of A^{-1} from a banded Cholesky factor of M = -A = L L^T
+// (row-major band storage, Lb[i*(w+1) + (i-j)] = L[i][j] for i-w <= j <= i).
+// One wave per column c: solves L y = e_c then L^T z = y with the band dot
+// products spread over the 64 lanes, y kept in LDS; writes row c of -M^{-1},
+// which is column c of the symmetric A^{-1}.
+__global__ void __launch_bounds__(64)
+    band_inverse_kernel(int n, int w, const double* __restrict__ Lb, double* __restrict__ Ainv) {
+    extern __shared__ double y[];
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int W1 = w + 1;
+    for (int i = lane; i < n; i += 64) y[i] = 0.0;
+    __syncthreads();
+    for (int i = c; i < n; ++i) {
+        double s = 0.0;
+        for (int t = lane; t < w; t += 64) {
+            const int j = i - 1 - t;
+            if (j >= c) s += Lb[(long)i * W1 + (i - j)] * y[j];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        const double yi = ((i == c ? 1.0 : 0.0) - s) / Lb[(long)i * W1];
+        if (lane == 0) y[i] = yi;
+        __syncthreads();
+    }
+    for (int i = n - 1; i >= c; --i) {
+        double s = 0.0;
+        for (int t = lane; t < w; t += 64) {
+            const int j = i + 1 + t;
+            if (j < n) s += Lb[(long)j * W1 + (j - i)] * y[j];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        const double zi = (y[i] - s) / Lb[(long)i * W1];
+        __syncthreads();
+        if (lane == 0) y[i] = zi;
+        __syncthreads();
+    }
+    // rows i < c of the back substitution: y[i] is 0 on entry there
+    for (int i = c - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int t = lane; t < w; t += 64) {
+            const int j = i + 1 + t;
+            if (j < n) s += Lb[(long)j * W1 + (j - i)] * y[j];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        const double zi = (0.0 - s) / Lb[(long)i * W1];
+        if (lane == 0) y[i] = zi;
+        __syncthreads();
+    }
+    double* out = Ainv + (long)c * n;
+    for (int i = lane; i < n; i += 64) out[i] = -y[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+namespace kern {
+
+int gsrb(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double* x,
+         const double* b, const double* K, int iters, bool zero_init) {
+    if (iters <= 0) return zero_init ? fill(s, x, cells(g), 0.0) : MF3D_OK;
+    const double dx2 = g.dx * g.dx;
+    const size_t n = cells(g);
+    if (n <= (size_t)kSmallMaxCells && iters > 1) {
+        gsrb_small_kernel<<<1, kSmallBlock, 0, s>>>(g.nx, g.ny, g.nz, dx2, bc, pin, x, b, K,
+                                                    iters, zero_init ? 1 : 0);
+        MF3D_HIP_TRY(hipGetLastError());
+        return MF3D_OK;
+    }
+    const size_t half = (size_t)((g.nx + 1) / 2) * g.ny * g.nz;
+    const unsigned grid = grid_for(half);
+    for (int it = 0; it < iters; ++it) {
+        for (int colour = 0; colour < 2; ++colour) {
+            if (zero_init && it == 0 && colour == 0)
+                gsrb_colour_kernel<true><<<grid, kBlock, 0, s>>>(g.nx, g.ny, g.nz, dx2, bc, pin,
+                                                                 x, b, K, colour);
+            else
+                gsrb_colour_kernel<false><<<grid, kBlock, 0, s>>>(g.nx, g.ny, g.nz, dx2, bc, pin,
+                                                                  x, b, K, colour);
+        }
+    }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int residual(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* x,
+             const double* b, const double* K, double* r) {
+    const size_t n = cells(g);
+    residual_kernel<<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, g.dx * g.dx,
+                                                   1.0 / (g.dx * g.dx), bc, pin, x, b, K, r);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int residual_restrict(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
+                      const double* x, const double* b, const double* K, double* b_coarse) {
+    const size_t nc = (size_t)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
+    residual_restrict_kernel<<<grid_for(nc), kBlock, 0, s>>>(
+        g.nx, g.ny, g.nz, g.dx * g.dx, 1.0 / (g.dx * g.dx), bc, pin, x, b, K, b_coarse);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+unsigned reduce_blocks(size_t n) { return grid_for(n); }
+
+int apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
+          const double* x, double* y, double* partials) {
+    const size_t n = cells(g);
+    const double inv_dx2 = 1.0 / (g.dx * g.dx);
+    if (partials)
+        apply_kernel<true><<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, inv_dx2, bc, pin, K,
+                                                          x, y, partials);
+    else
+        apply_kernel<false><<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, inv_dx2, bc, pin,
+                                                           K, x, y, nullptr);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int op_residual(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
+                const double* x, const double* b, double* r, double* partials) {
+    const size_t n = cells(g);
+    op_residual_kernel<<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, 1.0 / (g.dx * g.dx), bc,
+                                                      pin, K, x, b, r, partials);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int restrict_avg(hipStream_t s, const mf3d_grid& coarse, const double* f, double* c) {
+    const size_t nc = cells(coarse);
+    restrict_kernel<<<grid_for(nc), kBlock, 0, s>>>(coarse.nx, coarse.ny, coarse.nz, f, c);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int prolong_add(hipStream_t s, const mf3d_grid& fine, const double* c, double* f) {
+    prolong_add_kernel<<<grid_for(cells(fine)), kBlock, 0, s>>>(fine.nx, fine.ny, fine.nz, c, f);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int coarsen(hipStream_t s, const mf3d_grid& coarse, const double* Kf, double* Kc) {
+    coarsen_kernel<<<grid_for(cells(coarse)), kBlock, 0, s>>>(coarse.nx, coarse.ny, coarse.nz,
+                                                              Kf, Kc);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int rhs(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* K, double* out) {
+    rhs_kernel<<<grid_for(cells(g)), kBlock, 0, s>>>(g.nx, g.ny, g.nz, 1.0 / (g.dx * g.dx), bc, K,
+                                                     out);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int guess(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, int mode, double* h) {
+    // mode: MF3D_GUESS_REFERENCE (linear in x when both x faces are Dirichlet,
+    // else zero) or MF3D_GUESS_ZERO.
+    const int linear = (mode == MF3D_GUESS_REFERENCE && bc.type[0] == MF3D_BC_DIRICHLET &&
+                        bc.type[1] == MF3D_BC_DIRICHLET)
+                           ? 1
+                           : 0;
+    guess_kernel<<<grid_for(cells(g)), kBlock, 0, s>>>(g.nx, g.ny, g.nz, linear, bc.value[0],
+                                                       bc.value[1], h);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x) {
+    dense_gemv_kernel<<<(n + 3) / 4, 256, 0, s>>>(n, Ainv, b, x);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv) {
+    band_inverse_kernel<<<n, 64, (size_t)n * sizeof(double), s>>>(n, w, Lb, Ainv);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+}  // namespace kern
+}  // namespace mf3d

@@ -1,0 +1,34 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+def _gpu_available() -> bool:
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    if not _gpu_available():
+        pytest.skip("no GPU")
+    from macroflow3d_amd.runtime import HipContext
+    return HipContext(0)
+
+
+@pytest.fixture(scope="session")
+def workspace(ctx):
+    from macroflow3d_amd.physics import FlowWorkspace
+    return FlowWorkspace()

@@ -1,0 +1,322 @@
+"""GPU parity: every HIP kernel of the path against the CPU FP64 oracle,
+called through the C ABI (libmf3d_hip.so via ctypes).
+
+Tolerances: the kernels evaluate the oracle's formulas term by term in FP64;
+the GPU may fuse multiply-adds, so single operations agree to a few ulp
+(relative 1e-12 of the field's max); solves compare heads in relative L2
+(north-star bar 1e-8)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from helpers import BC_MIXES, bc_pair, grid_pair, random_K, rel_err, rel_l2
+from macroflow3d_amd import multigrid as mg
+from macroflow3d_amd import numerics as nm
+from macroflow3d_amd import physics as ph
+from macroflow3d_amd.core import PinSpec
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 8, 8), (16, 12, 10), (2, 6, 4), (32, 32, 32)]
+MIXES = list(BC_MIXES)
+OP_TOL = 1e-12
+
+
+def dev(a, ctx):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to(ctx.device)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def pin_for(name):
+    return name in ("all_neumann", "all_periodic")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("mix", MIXES)
+def test_gsrb_matches_oracle(ctx, shape, mix):
+    g, og = grid_pair(*shape, dx=0.5)
+    bc, obc = bc_pair(mix)
+    n = g.num_cells()
+    rng = np.random.default_rng(1)
+    K, x0, b = random_K(n, 2), rng.standard_normal(n), rng.standard_normal(n)
+    pin = pin_for(mix)
+    for iters in (1, 3):
+        xg = dev(x0, ctx)
+        mg.gsrb_smooth_3d(ctx, g, xg, dev(b, ctx), dev(K, ctx), iters, bc, PinSpec(pin))
+        xo = O.gsrb_smooth_3d(og, obc, pin, x0, b, K, iters)
+        assert rel_err(host(xg), xo) < OP_TOL, (mix, shape, iters)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("mix", MIXES)
+def test_residual_and_operator_match_oracle(ctx, shape, mix):
+    g, og = grid_pair(*shape, dx=2.0)
+    bc, obc = bc_pair(mix)
+    n = g.num_cells()
+    rng = np.random.default_rng(3)
+    K, x, b = random_K(n, 4), rng.standard_normal(n), rng.standard_normal(n)
+    pin = pin_for(mix)
+    r = ctx.empty(n)
+    mg.compute_residual_3d(ctx, g, dev(x, ctx), dev(b, ctx), dev(K, ctx), r, bc, PinSpec(pin))
+    assert rel_err(host(r), O.compute_residual_3d(og, obc, pin, x, b, K)) < OP_TOL
+    y = ctx.empty(n)
+    nm.VarCoeffLaplacian(g, dev(K, ctx), bc, PinSpec(pin)).apply(ctx, dev(x, ctx), y)
+    assert rel_err(host(y), O.varcoeff_apply(og, obc, pin, K, x)) < OP_TOL
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (64, 32, 16)])
+def test_transfers_and_coarsening_match_oracle(ctx, shape):
+    g, og = grid_pair(*shape)
+    c, oc = grid_pair(shape[0] // 2, shape[1] // 2, shape[2] // 2, 2.0)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(g.num_cells())
+    bcoarse = ctx.empty(c.num_cells())
+    mg.restrict_3d(ctx, g, c, dev(f, ctx), bcoarse)
+    assert np.array_equal(host(bcoarse), O.restrict_3d(oc, f))
+    xc = rng.standard_normal(c.num_cells())
+    xf = dev(f, ctx)
+    mg.prolong_3d_add(ctx, c, g, dev(xc, ctx), xf)
+    assert np.array_equal(host(xf), O.prolong_3d_add(oc, xc, f))
+    K = random_K(g.num_cells(), 6, 2.0)
+    Kc = ctx.empty(c.num_cells())
+    ph.coarsen_K(ctx, c, g, dev(K, ctx), Kc)
+    assert rel_err(host(Kc), O.coarsen_K(oc, K)) < 1e-14
+
+
+@pytest.mark.parametrize("mix", MIXES)
+def test_rhs_and_guess_match_oracle(ctx, mix):
+    g, og = grid_pair(12, 10, 8, dx=0.25)
+    bc, obc = bc_pair(mix)
+    K = random_K(g.num_cells(), 7)
+    rhs = ctx.empty(g.num_cells())
+    ph.build_rhs_head(rhs, dev(K, ctx), g, bc, ctx)
+    assert rel_err(host(rhs), O.build_rhs_head(og, obc, K)) < 1e-15 or \
+        np.array_equal(host(rhs), O.build_rhs_head(og, obc, K))
+    h = ctx.empty(g.num_cells())
+    ph.init_head_guess(h, g, bc, ctx)
+    assert np.array_equal(host(h), O.init_head_guess(og, obc))
+
+
+@pytest.mark.parametrize("mix", MIXES)
+@pytest.mark.parametrize("shape", [(9, 7, 5), (16, 16, 16)])
+def test_velocity_matches_oracle(ctx, mix, shape):
+    g, og = grid_pair(*shape, dx=0.5)
+    bc, obc = bc_pair(mix)
+    n = g.num_cells()
+    rng = np.random.default_rng(8)
+    K, h = random_K(n, 9), rng.standard_normal(n)
+    v = ph.VelocityField(g, ctx)
+    ph.compute_velocity_from_head(v, dev(h, ctx), dev(K, ctx), g, bc, ctx)
+    U, V, W = O.velocity_compact(og, obc, h, K)
+    for a, b_ in ((v.U, U), (v.V, V), (v.W, W)):
+        assert rel_err(host(a), b_) < OP_TOL
+    pv = ph.PaddedVelocityField(g, ctx)
+    ph.compute_velocity_from_head(pv, dev(h, ctx), dev(K, ctx), g, bc, ctx)
+    U, V, W = O.velocity_padded(og, obc, h, K)
+    for a, b_ in ((pv.U, U), (pv.V, V), (pv.W, W)):
+        assert rel_err(host(a), b_) < OP_TOL
+
+
+def test_dot_nrm2_and_vector_ops(ctx):
+    rng = np.random.default_rng(10)
+    for n in (1, 255, 256, 257, 100_003):
+        x, y = rng.standard_normal(n), rng.standard_normal(n)
+        red = nm.ReductionWorkspace()
+        d = nm.blas.dot_host(ctx, dev(x, ctx), dev(y, ctx), red)
+        exact = O.dot(x, y)
+        assert abs(d - exact) <= 1e-13 * (np.abs(x) @ np.abs(y))
+        assert abs(nm.blas.nrm2_host(ctx, dev(x, ctx), red) - O.nrm2(x)) <= 1e-14 * O.nrm2(x)
+        yy = dev(y, ctx)
+        nm.blas.axpy(ctx, 0.5, dev(x, ctx), yy)
+        assert rel_err(host(yy), 0.5 * x + y) < 1e-15
+        yy = dev(y, ctx)
+        nm.blas.axpby(ctx, 0.5, dev(x, ctx), -2.0, yy)
+        assert rel_err(host(yy), 0.5 * x - 2.0 * y) < 1e-15
+        yy = dev(y, ctx)
+        nm.blas.scal(ctx, yy, 3.0)
+        assert np.array_equal(host(yy), 3.0 * y)
+        nm.blas.fill(ctx, yy, 7.0)
+        assert np.all(host(yy) == 7.0)
+        nm.blas.copy(ctx, dev(x, ctx), yy)
+        assert np.array_equal(host(yy), x)
+    # empty vector: dot is 0 (dot.cu:13-18)
+    red = nm.ReductionWorkspace()
+    e = torch.empty(0, dtype=torch.float64, device=ctx.device)
+    d = red.scalar(ctx)
+    nm.blas.dot_device(ctx, e, e, d, red)
+    assert host(d)[0] == 0.0
+
+
+@pytest.mark.parametrize("coarse", [0, 1])
+@pytest.mark.parametrize("mix", ["darcy_x", "all_dirichlet", "periodic_yz", "all_neumann"])
+def test_vcycle_matches_oracle(ctx, workspace, coarse, mix):
+    g, og = grid_pair(16, 16, 16)
+    bc, obc = bc_pair(mix)
+    n = g.num_cells()
+    K = random_K(n, 11)
+    r = np.random.default_rng(12).standard_normal(n)
+    cfg = ph.HeadSolveConfig(mg_levels=3, coarse_solver=ph.CoarseSolver(coarse))
+    Kd = dev(K, ctx)
+    M = nm.MultigridPreconditioner.build(ctx, workspace, g, bc, cfg, Kd)
+    z = ctx.empty(n)
+    M.apply(ctx, dev(r, ctx), z)
+    p = O.SolveParams(mg_levels=3, coarse_solver=coarse)
+    zo = O.v_cycle_precond(og, obc, p, K, r)
+    assert rel_err(host(z), zo) < 1e-10, mix
+
+
+def _solve_pair(ctx, workspace, shape, mix, cfg, params, K, h0=None):
+    g, og = grid_pair(*shape)
+    bc, obc = bc_pair(mix)
+    n = g.num_cells()
+    h = ctx.zeros(n) if h0 is None else dev(h0, ctx)
+    res = ph.solve_head(h, dev(K, ctx), g, bc, cfg, ctx, workspace)
+    ho, ro = O.solve_head(og, obc, params, K, h0)
+    return host(h), res, ho, ro
+
+
+def test_solve_head_reference_semantics_32cubed(ctx, workspace):
+    """32^3, sigma2 = 1, lambda = 4, V(2,2), 3 levels, 50-GSRB coarsest,
+    ||r||/||r0|| < 1e-10 checked every 10 iterations (mode R)."""
+    from macroflow3d_amd import kfield
+    K = np.exp(kfield.lognormal_lnK(32, 32, 32, 1.0, 4.0, 1234)[0].numpy())
+    cfg = ph.HeadSolveConfig(mg_levels=3, cg_rtol=1e-10)
+    p = O.mode_r(mg_levels=3, cg_rtol=1e-10)
+    h, res, ho, ro = _solve_pair(ctx, workspace, (32, 32, 32), "darcy_x", cfg, p, K)
+    assert res.converged and ro.converged
+    assert res.num_iterations == ro.num_iterations
+    assert res.num_iterations % 10 == 0
+    assert rel_l2(h, ho) < 1e-10
+    assert abs(res.initial_residual - ro.initial_residual) <= 1e-12 * ro.initial_residual
+
+
+def test_solve_head_extension_semantics_32cubed(ctx, workspace):
+    """Direct coarsest solve, ||r||/||b|| <= 1e-10 every iteration (mode N),
+    zero guess; head within 1e-8 relative L2 of an oracle solve taken to 1e-13."""
+    from macroflow3d_amd import kfield
+    K = np.exp(kfield.lognormal_lnK(32, 32, 32, 1.0, 4.0, 1234)[0].numpy())
+    cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
+    p = O.mode_n(mg_levels=3)
+    h, res, ho, ro = _solve_pair(ctx, workspace, (32, 32, 32), "darcy_x", cfg, p, K)
+    assert res.converged and res.num_iterations == ro.num_iterations
+    assert res.true_residual / res.bnorm <= 1.5e-10
+    hdeep, _ = O.solve_head(O.grid(32, 32, 32), O.darcy_bc(), O.mode_n(mg_levels=3, cg_rtol=1e-13),
+                            K)
+    assert rel_l2(h, hdeep) < 1e-8
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_cg_and_mg_solvers_match_oracle(ctx, workspace, solver):
+    K = random_K(16 ** 3, 13, 0.7)
+    cfg = ph.HeadSolveConfig(solver_type=ph.HeadSolverType(solver), mg_levels=3, cg_rtol=1e-9,
+                             rtol=1e-9, mg_max_cycles=60)
+    p = O.SolveParams(solver_type=solver, mg_levels=3, cg_rtol=1e-9, rtol=1e-9, mg_max_cycles=60)
+    h, res, ho, ro = _solve_pair(ctx, workspace, (16, 16, 16), "darcy_x", cfg, p, K)
+    assert res.converged and ro.converged
+    assert res.num_iterations == ro.num_iterations
+    assert rel_l2(h, ho) < 1e-9
+
+
+@pytest.mark.parametrize("mix", ["periodic_x_dirichlet_z", "periodic_yz", "all_dirichlet"])
+def test_periodic_and_dirichlet_solves(ctx, workspace, mix):
+    """Periodic wraps and all-Dirichlet boundaries through the whole PCG
+    (the pinned singular case is covered by the V-cycle test)."""
+    K = random_K(16 * 8 * 8, 14, 0.5)
+    cfg = ph.HeadSolveConfig(mg_levels=3, cg_rtol=1e-10, coarse_solver=ph.CoarseSolver.DIRECT,
+                             stop_norm=ph.StopNorm.RHS_NORM, cg_check_every=1)
+    p = O.SolveParams(mg_levels=3, cg_rtol=1e-10, coarse_solver=1, stop_norm=1, cg_check_every=1)
+    h, res, ho, ro = _solve_pair(ctx, workspace, (16, 8, 8), mix, cfg, p, K)
+    assert res.converged and res.num_iterations == ro.num_iterations
+    assert rel_l2(h, ho) < 1e-9
+
+
+def test_provided_initial_guess_keeps_caller_head(ctx, workspace):
+    """InitGuess.PROVIDED: the solve starts from the caller's h (warm start)."""
+    K = random_K(16 ** 3, 16, 0.8)
+    g, og = grid_pair(16, 16, 16)
+    bc, obc = bc_pair("darcy_x")
+    ho, _ = O.solve_head(og, obc, O.mode_n(mg_levels=3, cg_rtol=1e-12), K)
+    h0 = ho + 1e-6 * np.random.default_rng(17).standard_normal(ho.size)
+    cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
+    cfg.init_guess = ph.InitGuess.PROVIDED
+    h = dev(h0, ctx)
+    res = ph.solve_head(h, dev(K, ctx), g, bc, cfg, ctx, workspace)
+    cfg0 = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
+    h2 = ctx.zeros(K.size)
+    res0 = ph.solve_head(h2, dev(K, ctx), g, bc, cfg0, ctx, workspace)
+    assert res.converged and res0.converged
+    assert res.num_iterations < res0.num_iterations
+    assert rel_l2(host(h), ho) < 1e-8
+
+
+def test_odd_coarsest_level_and_two_cell_levels(ctx, workspace):
+    """48^3 coarsens to 3^3 (odd coarsest) and 8x4x2 has a 2-cell dimension."""
+    for shape, levels in (((48, 48, 48), 6), ((8, 4, 2), 4)):
+        K = random_K(int(np.prod(shape)), 18, 0.5)
+        cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=levels)
+        p = O.mode_n(mg_levels=levels)
+        h, res, ho, ro = _solve_pair(ctx, workspace, shape, "darcy_x", cfg, p, K)
+        assert res.converged and res.num_iterations == ro.num_iterations, shape
+        assert rel_l2(h, ho) < 1e-9
+
+
+def test_errors_are_raised(ctx, workspace):
+    from macroflow3d_amd._lib import MF3DError
+    from macroflow3d_amd.core import BCFace, BCSpec, BCType, Grid3D
+    g = Grid3D(8, 8, 8)
+    K = ctx.zeros(10)
+    h = ctx.zeros(8 ** 3)
+    with pytest.raises(RuntimeError):
+        ph.solve_head(h, K, g, BCSpec.darcy_x(), ph.HeadSolveConfig(), ctx, workspace)
+    bad = BCSpec.darcy_x()
+    bad.ymin = BCFace(BCType.Periodic, 0.0)
+    with pytest.raises(RuntimeError, match="Periodic"):
+        ph.solve_head(h, ctx.zeros(8 ** 3), g, bad, ph.HeadSolveConfig(), ctx, workspace)
+    # odd non-coarsest level
+    g2 = Grid3D(10, 10, 10)
+    with pytest.raises(MF3DError):
+        ph.solve_head(ctx.zeros(1000), ctx.zeros(1000) + 1, g2, BCSpec.darcy_x(),
+                      ph.HeadSolveConfig(mg_levels=3), ctx, workspace)
+
+
+def test_non_convergence_is_a_flag(ctx, workspace):
+    from macroflow3d_amd.core import BCSpec
+    K = random_K(16 ** 3, 19, 1.0)
+    g, _ = grid_pair(16, 16, 16)
+    cfg = ph.HeadSolveConfig(mg_levels=3, cg_max_iter=3, cg_rtol=1e-14, cg_check_every=1)
+    res = ph.solve_head(ctx.zeros(K.size), dev(K, ctx), g, BCSpec.darcy_x(), cfg, ctx, workspace)
+    assert not res.converged and res.num_iterations == 3
+
+
+def test_large_grid_properties_256cubed(ctx):
+    """Full-size 256^3, sigma2 = 4, lambda = 8: true residual <= 1e-10 ||b||,
+    and Darcy flux conservation (equal total x-flux through every x-plane,
+    zero flux through the no-flow faces)."""
+    from macroflow3d_amd import kfield
+    from macroflow3d_amd.core import BCSpec, Grid3D
+    n = 256
+    Y, _ = kfield.lognormal_lnK(n, n, n, 4.0, 8.0, 2024, device=ctx.device)
+    K = torch.exp(Y)
+    g = Grid3D(n, n, n)
+    bc = BCSpec.darcy_x()
+    ws = ph.FlowWorkspace()
+    h = ctx.zeros(g.num_cells())
+    res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.north_star(1e-10, 6), ctx, ws)
+    assert res.converged
+    assert res.true_residual / res.bnorm <= 2e-10
+    v = ph.VelocityField(g, ctx)
+    ph.compute_velocity_from_head(v, h, K, g, bc, ctx)
+    U = v.U.view(n, n, n + 1)
+    plane_flux = U.sum(dim=(0, 1))
+    q = plane_flux.mean()
+    assert float((plane_flux - q).abs().max() / q.abs()) < 1e-4
+    V = v.V.view(n, n + 1, n)
+    assert float(V[:, 0, :].abs().max()) == 0.0 and float(V[:, n, :].abs().max()) == 0.0
+    ws.close()
