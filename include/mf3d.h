@@ -162,6 +162,14 @@ int mf3d_velocity_padded(mf3d_stream stream, const mf3d_grid* grid, const mf3d_b
 int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, const double* b,
                         const double* K, size_t n, int num_iters, const mf3d_bc* bc, int pin);
 
+/* Extension of mf3d_gsrb_smooth_3d: with a caller scratch buffer of n doubles
+ * the levels it supports (nx >= 32, ny >= 16, nz >= 8, z not periodic) run the
+ * one-pass red-black kernel (red and black of one iteration in a single
+ * sweep). zchunk: planes per workgroup (<= 0: default). Same results. */
+int mf3d_gsrb_smooth_3d_scratch(mf3d_stream stream, const mf3d_grid* grid, double* x,
+                                const double* b, const double* K, size_t n, int num_iters,
+                                const mf3d_bc* bc, int pin, double* scratch, int zchunk);
+
 /* multigrid::compute_residual_3d (residual_3d.cuh:30-32, residual_3d.cu:378-503). */
 int mf3d_compute_residual_3d(mf3d_stream stream, const mf3d_grid* grid, const double* x,
                              const double* b, const double* K, double* r, size_t n,

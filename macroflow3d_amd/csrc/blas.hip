@@ -81,9 +81,17 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kFinalBlock)
     finalize_kernel(const double* __restrict__ partials, unsigned np, int op,
                     IterState* __restrict__ st, double* __restrict__ out) {
-    double v = 0.0;
-    for (unsigned i = threadIdx.x; i < np; i += kFinalBlock) v += partials[i];
-    const double S = block_sum<kFinalBlock>(v);
+    // four independent accumulators keep several loads in flight per thread
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    unsigned i = threadIdx.x;
+    for (; i + 3 * kFinalBlock < np; i += 4 * kFinalBlock) {
+        v0 += partials[i];
+        v1 += partials[i + kFinalBlock];
+        v2 += partials[i + 2 * kFinalBlock];
+        v3 += partials[i + 3 * kFinalBlock];
+    }
+    for (; i < np; i += kFinalBlock) v0 += partials[i];
+    const double S = block_sum<kFinalBlock>((v0 + v1) + (v2 + v3));
     if (threadIdx.x != 0) return;
     switch (op) {
     case FIN_STORE: *out = S; break;

@@ -29,8 +29,12 @@ int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv);
 // ---- fast fine-level kernels (stencil_fast.hip) ----------------------------
 // True when the z-marching kernels cover this level (size and BC limits).
 bool fast_ok(const mf3d_grid& g, const BcTypes& bc);
-int gsrb_fast(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double* x,
-              const double* b, const double* K, int iters, bool zero_init);
+// iters one-pass red-black iterations; result in x, tmp = n-element scratch;
+// zero_init: x is zero on entry; xc (nullable): coarse correction added first;
+// zchunk <= 0 picks the default z-chunk.
+int gsrb_march(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double* x,
+               double* tmp, const double* b, const double* K, int iters, bool zero_init,
+               const double* xc, int zchunk);
 
 // ---- BLAS-1 and reductions (blas.hip) ---------------------------------------
 int fill(hipStream_t s, double* x, size_t n, double v);

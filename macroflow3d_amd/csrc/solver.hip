@@ -74,6 +74,8 @@ struct Level {
     double* x = nullptr;
     double* b = nullptr;
     double* K = nullptr;
+    double* t = nullptr;  // smoother scratch (one-pass red-black kernel)
+    bool fast = false;    // level served by the one-pass red-black kernel
 };
 
 inline hipStream_t as_stream(mf3d_stream s) { return reinterpret_cast<hipStream_t>(s); }
@@ -191,6 +193,7 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
             MF3D_TRY(ws->alloc(&L.b, L.n));
             MF3D_TRY(ws->alloc(&L.K, L.n));
         }
+        L.fast = false;
     }
     const size_t n = cells(g);
     ws->nvec = n;
@@ -313,6 +316,12 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     MF3D_TRY(ensure_workspace(ws, g, cfg.mg_levels));
     ws->bc = bc;
     ws->bct = bc_types(bc);
+    for (size_t l = 0; l < ws->lv.size(); ++l) {
+        Level& L = ws->lv[l];
+        L.fast = kern::fast_ok(L.g, ws->bct);
+        if (L.fast && l > 0 && !L.t) MF3D_TRY(ws->alloc(&L.t, L.n));
+        if (l == 0) L.t = ws->Ap;  // free while the V-cycle runs
+    }
     ws->pin = pin_enabled(cfg.pin_mode, bc);
     ws->pre = cfg.mg_pre_smooth;
     ws->post = cfg.mg_post_smooth;
@@ -329,11 +338,15 @@ inline const double* level_K(const mf3d_workspace* ws, size_t l) {
     return l == 0 ? ws->K0 : ws->lv[l].K;
 }
 
+// GSRB smoothing of level l; xc (nullable) is a coarse correction to
+// prolong-add into x first.
 int smooth(mf3d_workspace* ws, hipStream_t s, size_t l, double* x, const double* b, int iters,
-           bool zero_init) {
+           bool zero_init, const double* xc = nullptr) {
     const Level& L = ws->lv[l];
-    if (kern::fast_ok(L.g, ws->bct))
-        return kern::gsrb_fast(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), iters, zero_init);
+    if (L.fast)
+        return kern::gsrb_march(s, L.g, ws->bct, ws->pin, x, L.t, b, level_K(ws, l), iters,
+                                zero_init, xc, 0);
+    if (xc) MF3D_TRY(kern::prolong_add(s, L.g, xc, x));
     return kern::gsrb(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), iters, zero_init);
 }
 
@@ -353,8 +366,7 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, double* x, const double*
     MF3D_TRY(smooth(ws, s, l, x, b, ws->pre, zero_init));
     MF3D_TRY(kern::residual_restrict(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), C.b));
     MF3D_TRY(vcycle(ws, s, l + 1, C.x, C.b, true));
-    MF3D_TRY(kern::prolong_add(s, L.g, C.x, x));
-    return smooth(ws, s, l, x, b, ws->post, false);
+    return smooth(ws, s, l, x, b, ws->post, false, C.x);  // prolong_3d_add folded in
 }
 
 IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
@@ -632,7 +644,6 @@ static int check_config(const mf3d_head_solve_config* cfg) {
 int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
                     const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
                     double* h, size_t n, mf3d_head_solve_result* result) {
-    clear_stale_error();
     MF3D_CHECK_PTR(ws);
     MF3D_CHECK_PTR(result);
     std::memset(result, 0, sizeof(*result));
@@ -645,6 +656,7 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     const size_t nc = cells(*grid);
     // solve_head.cu:85-87
     if (n < nc) return fail(MF3D_ERR_SIZE_MISMATCH, "Head or K buffer too small");
+    clear_stale_error();
     hipStream_t s = as_stream(stream);
     MF3D_TRY(ensure_workspace(ws, *grid, cfg->mg_levels));
     MF3D_HIP_TRY(hipEventRecord(ws->ev[0], s));
@@ -680,41 +692,40 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
 
 int mf3d_init_head_guess(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, double* h,
                          size_t n) {
-    clear_stale_error();
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_CHECK_PTR(h);
     MF3D_CHECK_LEN(n, cells(*grid), "head");
+    clear_stale_error();
     return kern::guess(as_stream(stream), *grid, *bc, MF3D_GUESS_REFERENCE, h);
 }
 
 int mf3d_build_rhs_head(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
                         const double* K, double* rhs, size_t n) {
-    clear_stale_error();
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_CHECK_PTR(K);
     MF3D_CHECK_PTR(rhs);
     // rhs_head.cu:124-126
     if (n < cells(*grid)) return fail(MF3D_ERR_SIZE_MISMATCH, "RHS or K buffer too small");
+    clear_stale_error();
     return kern::rhs(as_stream(stream), *grid, *bc, K, rhs);
 }
 
 int mf3d_coarsen_K(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_grid* fine,
                    const double* K_fine, size_t n_fine, double* K_coarse, size_t n_coarse) {
-    clear_stale_error();
     MF3D_TRY(check_pair(coarse, fine));
     MF3D_CHECK_PTR(K_fine);
     MF3D_CHECK_PTR(K_coarse);
     MF3D_CHECK_LEN(n_fine, cells(*fine), "fine K");
     MF3D_CHECK_LEN(n_coarse, cells(*coarse), "coarse K");
+    clear_stale_error();
     return kern::coarsen(as_stream(stream), *coarse, K_fine, K_coarse);
 }
 
 int mf3d_velocity_compact(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
                           const double* h, const double* K, size_t n, double* U, size_t nU,
                           double* V, size_t nV, double* W, size_t nW) {
-    clear_stale_error();
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_CHECK_PTR(h);
@@ -727,13 +738,13 @@ int mf3d_velocity_compact(mf3d_stream stream, const mf3d_grid* grid, const mf3d_
     MF3D_CHECK_LEN(nU, (size_t)(g.nx + 1) * g.ny * g.nz, "U");
     MF3D_CHECK_LEN(nV, (size_t)g.nx * (g.ny + 1) * g.nz, "V");
     MF3D_CHECK_LEN(nW, (size_t)g.nx * g.ny * (g.nz + 1), "W");
+    clear_stale_error();
     return kern::velocity(as_stream(stream), g, *bc, h, K, U, V, W, false);
 }
 
 int mf3d_velocity_padded(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc,
                          const double* h, const double* K, size_t n, double* U, double* V,
                          double* W, size_t n_padded) {
-    clear_stale_error();
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_CHECK_PTR(h);
@@ -744,12 +755,12 @@ int mf3d_velocity_padded(mf3d_stream stream, const mf3d_grid* grid, const mf3d_b
     const mf3d_grid& g = *grid;
     MF3D_CHECK_LEN(n, cells(g), "head/K");
     MF3D_CHECK_LEN(n_padded, (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1), "padded velocity");
+    clear_stale_error();
     return kern::velocity(as_stream(stream), g, *bc, h, K, U, V, W, true);
 }
 
 int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, const double* b,
                         const double* K, size_t n, int num_iters, const mf3d_bc* bc, int pin) {
-    clear_stale_error();
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_CHECK_PTR(x);
@@ -757,16 +768,33 @@ int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, co
     MF3D_CHECK_PTR(K);
     MF3D_CHECK_LEN(n, cells(*grid), "x/b/K");
     if (num_iters < 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "num_iters < 0");
+    clear_stale_error();
+    return kern::gsrb(as_stream(stream), *grid, bc_types(*bc), pin != 0, x, b, K, num_iters,
+                      false);
+}
+
+int mf3d_gsrb_smooth_3d_scratch(mf3d_stream stream, const mf3d_grid* grid, double* x,
+                                const double* b, const double* K, size_t n, int num_iters,
+                                const mf3d_bc* bc, int pin, double* scratch, int zchunk) {
+    MF3D_TRY(validate_grid(grid));
+    MF3D_TRY(validate_bc(bc));
+    MF3D_CHECK_PTR(x);
+    MF3D_CHECK_PTR(b);
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(scratch);
+    MF3D_CHECK_LEN(n, cells(*grid), "x/b/K/scratch");
+    if (num_iters < 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "num_iters < 0");
+    clear_stale_error();
     const BcTypes bt = bc_types(*bc);
     if (kern::fast_ok(*grid, bt))
-        return kern::gsrb_fast(as_stream(stream), *grid, bt, pin != 0, x, b, K, num_iters, false);
+        return kern::gsrb_march(as_stream(stream), *grid, bt, pin != 0, x, scratch, b, K,
+                                num_iters, false, nullptr, zchunk);
     return kern::gsrb(as_stream(stream), *grid, bt, pin != 0, x, b, K, num_iters, false);
 }
 
 int mf3d_compute_residual_3d(mf3d_stream stream, const mf3d_grid* grid, const double* x,
                              const double* b, const double* K, double* r, size_t n,
                              const mf3d_bc* bc, int pin) {
-    clear_stale_error();
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_CHECK_PTR(x);
@@ -774,41 +802,42 @@ int mf3d_compute_residual_3d(mf3d_stream stream, const mf3d_grid* grid, const do
     MF3D_CHECK_PTR(K);
     MF3D_CHECK_PTR(r);
     MF3D_CHECK_LEN(n, cells(*grid), "x/b/K/r");
+    clear_stale_error();
     return kern::residual(as_stream(stream), *grid, bc_types(*bc), pin != 0, x, b, K, r);
 }
 
 int mf3d_restrict_3d(mf3d_stream stream, const mf3d_grid* fine, const mf3d_grid* coarse,
                      const double* r_fine, size_t n_fine, double* b_coarse, size_t n_coarse) {
-    clear_stale_error();
     MF3D_TRY(check_pair(coarse, fine));
     MF3D_CHECK_PTR(r_fine);
     MF3D_CHECK_PTR(b_coarse);
     MF3D_CHECK_LEN(n_fine, cells(*fine), "fine");
     MF3D_CHECK_LEN(n_coarse, cells(*coarse), "coarse");
+    clear_stale_error();
     return kern::restrict_avg(as_stream(stream), *coarse, r_fine, b_coarse);
 }
 
 int mf3d_prolong_3d_add(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_grid* fine,
                         const double* x_coarse, size_t n_coarse, double* x_fine, size_t n_fine) {
-    clear_stale_error();
     MF3D_TRY(check_pair(coarse, fine));
     MF3D_CHECK_PTR(x_coarse);
     MF3D_CHECK_PTR(x_fine);
     MF3D_CHECK_LEN(n_fine, cells(*fine), "fine");
     MF3D_CHECK_LEN(n_coarse, cells(*coarse), "coarse");
+    clear_stale_error();
     return kern::prolong_add(as_stream(stream), *fine, x_coarse, x_fine);
 }
 
 int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
                   const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
                   size_t n) {
-    clear_stale_error();
     MF3D_CHECK_PTR(ws);
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_TRY(check_config(cfg));
     MF3D_CHECK_PTR(K);
     MF3D_CHECK_LEN(n, cells(*grid), "K");
+    clear_stale_error();
     return mg_setup(ws, as_stream(stream), *grid, *bc, *cfg, K);
 }
 
@@ -818,33 +847,33 @@ int mf3d_mg_num_levels(const mf3d_workspace* ws) {
 
 int mf3d_mg_precondition(mf3d_workspace* ws, mf3d_stream stream, const double* r, double* z,
                          size_t n) {
-    clear_stale_error();
     MF3D_CHECK_PTR(ws);
     MF3D_CHECK_PTR(r);
     MF3D_CHECK_PTR(z);
     if (!ws->ready || !ws->K0)
         return fail(MF3D_ERR_INVALID_ARGUMENT, "mf3d_mg_setup has not been called");
     MF3D_CHECK_LEN(n, cells(ws->fine), "r/z");
+    clear_stale_error();
     return vcycle(ws, as_stream(stream), 0, z, r, true);
 }
 
 int mf3d_varcoeff_apply(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, int pin,
                         const double* K, const double* x, double* y, size_t n) {
-    clear_stale_error();
     MF3D_TRY(validate_grid(grid));
     MF3D_TRY(validate_bc(bc));
     MF3D_CHECK_PTR(K);
     MF3D_CHECK_PTR(x);
     MF3D_CHECK_PTR(y);
     MF3D_CHECK_LEN(n, cells(*grid), "x/y/K");
+    clear_stale_error();
     return kern::apply(as_stream(stream), *grid, bc_types(*bc), pin != 0, K, x, y, nullptr);
 }
 
 int mf3d_dot(mf3d_workspace* ws, mf3d_stream stream, const double* x, const double* y, size_t n,
              double* d_result) {
-    clear_stale_error();
     MF3D_CHECK_PTR(ws);
     MF3D_CHECK_PTR(d_result);
+    clear_stale_error();
     hipStream_t s = as_stream(stream);
     if (n == 0) return kern::fill(s, d_result, 1, 0.0);  // dot.cu:13-18
     MF3D_CHECK_PTR(x);
@@ -856,9 +885,9 @@ int mf3d_dot(mf3d_workspace* ws, mf3d_stream stream, const double* x, const doub
 
 int mf3d_nrm2(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
               double* d_result) {
-    clear_stale_error();
     MF3D_CHECK_PTR(ws);
     MF3D_CHECK_PTR(d_result);
+    clear_stale_error();
     hipStream_t s = as_stream(stream);
     if (n == 0) return kern::fill(s, d_result, 1, 0.0);
     MF3D_CHECK_PTR(x);
@@ -868,28 +897,28 @@ int mf3d_nrm2(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
 }
 
 int mf3d_axpy(mf3d_stream stream, double a, const double* x, double* y, size_t n) {
-    clear_stale_error();
     if (n && (!x || !y)) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    clear_stale_error();
     return kern::axpy(as_stream(stream), a, x, y, n);
 }
 int mf3d_axpby(mf3d_stream stream, double a, const double* x, double b, double* y, size_t n) {
-    clear_stale_error();
     if (n && (!x || !y)) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    clear_stale_error();
     return kern::axpby(as_stream(stream), a, x, b, y, n);
 }
 int mf3d_copy(mf3d_stream stream, const double* x, double* y, size_t n) {
-    clear_stale_error();
     if (n && (!x || !y)) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    clear_stale_error();
     return kern::copy(as_stream(stream), x, y, n);
 }
 int mf3d_fill(mf3d_stream stream, double* x, size_t n, double value) {
-    clear_stale_error();
     if (n && !x) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    clear_stale_error();
     return kern::fill(as_stream(stream), x, n, value);
 }
 int mf3d_scal(mf3d_stream stream, double* x, size_t n, double a) {
-    clear_stale_error();
     if (n && !x) return fail(MF3D_ERR_INVALID_ARGUMENT, "null vector");
+    clear_stale_error();
     return kern::scal(as_stream(stream), x, n, a);
 }
 

@@ -382,31 +382,70 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// Whole GSRB smoothing of a small level by one workgroup, x held in LDS.
-// Same per-cell values as gsrb_colour_kernel; replaces 2*iters launches
-// (the coarsest level runs 50 iterations, v_cycle.cu:41-45).
+// Whole GSRB smoothing of a small level by one workgroup: x and 1/K in LDS,
+// each thread's own K and b in registers. Same per-cell values as
+// gsrb_colour_kernel up to the summation order of boundary cells; replaces
+// 2*iters launches (the coarsest level runs 50 iterations, v_cycle.cu:41-45).
 constexpr int kSmallBlock = 1024;
-constexpr int kSmallMaxCells = 4096;
+constexpr int kSmallPer = 4;
+constexpr int kSmallMaxCells = kSmallBlock * kSmallPer;
 
 __global__ void __launch_bounds__(kSmallBlock)
     gsrb_small_kernel(int nx, int ny, int nz, double dx2, BcTypes bc, int pin,
                       double* __restrict__ x, const double* __restrict__ b,
                       const double* __restrict__ K, int iters, int zero_init) {
     __shared__ double xs[kSmallMaxCells];
+    __shared__ double iks[kSmallMaxCells];
     const int n = nx * ny * nz;
-    for (int o = threadIdx.x; o < n; o += blockDim.x) xs[o] = zero_init ? 0.0 : x[o];
+    double bl[kSmallPer], Kl[kSmallPer];
+#pragma unroll
+    for (int m = 0; m < kSmallPer; ++m) {
+        const int o = threadIdx.x + m * kSmallBlock;
+        bl[m] = 0.0;
+        Kl[m] = 1.0;
+        if (o < n) {
+            xs[o] = zero_init ? 0.0 : x[o];
+            Kl[m] = K[o];
+            iks[o] = 1.0 / Kl[m];
+            bl[m] = b[o];
+        }
+    }
     __syncthreads();
     for (int it = 0; it < iters; ++it) {
         for (int colour = 0; colour < 2; ++colour) {
-            for (int o = threadIdx.x; o < n; o += blockDim.x) {
+#pragma unroll
+            for (int m = 0; m < kSmallPer; ++m) {
+                const int o = threadIdx.x + m * kSmallBlock;
+                if (o >= n) continue;
                 const int i = o % nx, q = o / nx, j = q % ny, k = q / ny;
                 if (((i + j + k) & 1) != colour) continue;
-                xs[o] = gsrb_value<false>(nx, ny, nz, bc, pin != 0, xs, b, K, i, j, k, o, dx2);
+                const int pos[3] = {i, j, k};
+                const int lim[3] = {nx, ny, nz};
+                const int str[3] = {1, nx, nx * ny};
+                double result = 0.0, aC = 0.0;
+                for (int d = 0; d < 6; ++d) {
+                    const int ax = d >> 1;
+                    const bool plus = d & 1;
+                    const bool edge = plus ? pos[ax] == lim[ax] - 1 : pos[ax] == 0;
+                    double c;
+                    if (edge && bc.t[d] != MF3D_BC_PERIODIC) {
+                        c = bc.t[d] == MF3D_BC_DIRICHLET ? 2.0 * Kl[m] : 0.0;
+                    } else {
+                        const int nb = edge ? (plus ? o - (lim[ax] - 1) * str[ax]
+                                                    : o + (lim[ax] - 1) * str[ax])
+                                            : (plus ? o + str[ax] : o - str[ax]);
+                        c = 2.0 / (iks[o] + iks[nb]);
+                        result += c * xs[nb];
+                    }
+                    aC += c;
+                }
+                if (pin && o == 0) aC *= 2.0;
+                xs[o] = (result - bl[m] * dx2) / aC;
             }
             __syncthreads();
         }
     }
-    for (int o = threadIdx.x; o < n; o += blockDim.x) x[o] = xs[o];
+    for (int o = threadIdx.x; o < n; o += kSmallBlock) x[o] = xs[o];
 }
 
 // Coarsest direct solve x = A^{-1} b, one wave per row (dense FP64 GEMV).

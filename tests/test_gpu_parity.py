@@ -54,6 +54,46 @@ def test_gsrb_matches_oracle(ctx, shape, mix):
         assert rel_err(host(xg), xo) < OP_TOL, (mix, shape, iters)
 
 
+@pytest.mark.parametrize("shape", [(64, 32, 16), (96, 48, 40), (64, 64, 64)])
+@pytest.mark.parametrize("mix", MIXES)
+def test_one_pass_red_black_matches_oracle(ctx, shape, mix):
+    """The fused red-black march kernel (one sweep per iteration, ping-pong
+    scratch) against the oracle's red pass + black pass, several z-chunks and
+    odd/even iteration counts."""
+    g, og = grid_pair(*shape, dx=0.5)
+    bc, obc = bc_pair(mix)
+    n = g.num_cells()
+    rng = np.random.default_rng(21)
+    K, x0, b = random_K(n, 22, 1.5), rng.standard_normal(n), rng.standard_normal(n)
+    pin = pin_for(mix)
+    for iters, zchunk in ((1, 0), (2, 5), (3, 64), (2, 16)):
+        xg = dev(x0, ctx)
+        mg.gsrb_smooth_3d_scratch(ctx, g, xg, dev(b, ctx), dev(K, ctx), iters, bc, PinSpec(pin),
+                                  zchunk=zchunk)
+        xo = O.gsrb_smooth_3d(og, obc, pin, x0, b, K, iters)
+        assert rel_err(host(xg), xo) < OP_TOL, (mix, shape, iters, zchunk)
+
+
+@pytest.mark.parametrize("pre_post", [(2, 2), (1, 3), (3, 1), (0, 2)])
+def test_vcycle_fast_levels_match_oracle(ctx, workspace, pre_post):
+    """64^3 V-cycle: levels 64^3 and 32^3 use the one-pass kernel with the
+    prolongation folded in; odd smoothing counts exercise the ping-pong."""
+    g, og = grid_pair(64, 64, 64)
+    bc, obc = bc_pair("darcy_x")
+    n = g.num_cells()
+    K = random_K(n, 23, 1.5)
+    r = np.random.default_rng(24).standard_normal(n)
+    pre, post = pre_post
+    cfg = ph.HeadSolveConfig(mg_levels=4, mg_pre_smooth=pre, mg_post_smooth=post,
+                             coarse_solver=ph.CoarseSolver.DIRECT)
+    M = nm.MultigridPreconditioner.build(ctx, workspace, g, bc, cfg, dev(K, ctx))
+    z = ctx.empty(n)
+    M.apply(ctx, dev(r, ctx), z)
+    p = O.SolveParams(mg_levels=4, mg_pre_smooth=pre, mg_post_smooth=post, coarse_solver=1)
+    zo = O.v_cycle_precond(og, obc, p, K, r)
+    assert rel_err(host(z), zo) < 1e-10, pre_post
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("mix", MIXES)
 def test_residual_and_operator_match_oracle(ctx, shape, mix):
