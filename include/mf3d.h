@@ -87,7 +87,8 @@ typedef struct {
     int precond_precision; /* MF3D_PRECOND_* */
     int coef_format;       /* MF3D_COEF_* (FP32 preconditioner only) */
     int true_residual;     /* 1: recompute ||b - A h|| in FP64 after the solve */
-    int reserved[5];
+    int flexible;          /* 1: flexible-CG beta (r_new, z_new - z_old) / (r_old, z_old) */
+    int reserved[4];
 } mf3d_head_solve_config;
 
 /* HeadSolveResult (src/physics/flow/solve_head.cuh:136-141) + extensions. */
@@ -100,7 +101,8 @@ typedef struct {
     double true_residual; /* ||b - A h|| when requested, else -1 */
     double setup_ms;      /* device time of hierarchy/coefficient setup */
     double solve_ms;      /* device time of the iteration */
-    int reserved[4];
+    int restarts;         /* true-residual restarts of the PCG (extension) */
+    int reserved[3];
 } mf3d_head_solve_result;
 
 /* Opaque FlowWorkspace (src/physics/common/workspaces.cuh:111-167): MG
@@ -189,6 +191,14 @@ int mf3d_prolong_3d_add(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_
 int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
                   const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
                   size_t n);
+/* Tuning knobs of a workspace (extension; the autotuner sets them). */
+enum {
+    MF3D_TUNE_MARCH_MIN_CELLS_FP64 = 1, /* levels with >= this many cells use the march kernel */
+    MF3D_TUNE_MARCH_MIN_CELLS_FP32 = 2,
+    MF3D_TUNE_ZCHUNK = 3                /* planes per march workgroup (<= 0: default) */
+};
+int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
+
 /* Number of levels of the current hierarchy (0 before set-up). */
 int mf3d_mg_num_levels(const mf3d_workspace* ws);
 

@@ -42,7 +42,7 @@ class mf3d_head_solve_config(C.Structure):
                 ("cg_rtol", C.c_double), ("rtol", C.c_double), ("pin_mode", C.c_int),
                 ("init_guess", C.c_int), ("coarse_solver", C.c_int), ("stop_norm", C.c_int),
                 ("precond_precision", C.c_int), ("coef_format", C.c_int),
-                ("true_residual", C.c_int), ("reserved", C.c_int * 5)]
+                ("true_residual", C.c_int), ("flexible", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class mf3d_head_solve_result(C.Structure):
@@ -50,7 +50,7 @@ class mf3d_head_solve_result(C.Structure):
                 ("initial_residual", C.c_double), ("final_residual", C.c_double),
                 ("bnorm", C.c_double), ("true_residual", C.c_double),
                 ("setup_ms", C.c_double), ("solve_ms", C.c_double),
-                ("reserved", C.c_int * 4)]
+                ("restarts", C.c_int), ("reserved", C.c_int * 3)]
 
 
 _P = C.POINTER
@@ -84,6 +84,7 @@ SIGNATURES = {
     "mf3d_prolong_3d_add": (_i, [_vp, _g, _g, _vp, _sz, _vp, _sz]),
     "mf3d_mg_setup": (_i, [_vp, _vp, _g, _b, _cfg, _vp, _sz]),
     "mf3d_mg_num_levels": (_i, [_vp]),
+    "mf3d_workspace_set_tuning": (_i, [_vp, _i, C.c_long]),
     "mf3d_mg_precondition": (_i, [_vp, _vp, _vp, _vp, _sz]),
     "mf3d_varcoeff_apply": (_i, [_vp, _g, _b, _i, _vp, _vp, _vp, _sz]),
     "mf3d_dot": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),

@@ -121,10 +121,19 @@ __global__ void __launch_bounds__(kFinalBlock)
                 st->stop = 3;
         }
         break;
-    case FIN_RZ: // pcg.cuh:170-177
+    case FIN_RZ: // pcg.cuh:170-177 (flexible: Polak-Ribiere numerator)
         if (st->stop) break;
-        st->beta = S / st->rz;
+        st->beta = (st->flexible ? S - st->rzo : S) / st->rz;
         st->rz = S;
+        break;
+    case FIN_RZO:
+        if (st->stop) break;
+        st->rzo = S;
+        break;
+    case FIN_RESUME:
+        st->rr = S;
+        st->stop = (sqrt(S) / st->den < st->rtol) ? 1 : 0;
+        if (!st->stop && st->iter >= st->max_iter) st->stop = 3;
         break;
     case FIN_CG_RR0: // cg.cuh:47-77: rr = (r,r), tol = r0 * rtol
         st->rz = S;

@@ -105,12 +105,13 @@ struct IterState {
     double r0;     // ||r0||
     double bnorm;  // ||b||
     double rtol;   // relative tolerance
+    double rzo;    // (r_new, z_old) of the flexible variant
     int iter;      // completed iterations
     int stop;      // 0 running, 1 converged, 2 breakdown, 3 iteration cap
     int check_every;
     int max_iter;
     int stop_norm; // MF3D_STOP_*
-    int pad;
+    int flexible;  // beta = (r_new, z_new - z_old) / (r_old, z_old)
 };
 
 // Reduction finalisers (what thread 0 does with the reduced sum S).
@@ -125,7 +126,9 @@ enum Finalize : int {
     FIN_BNORM = 7,   // bnorm = sqrt(S)
     FIN_CG_RR0 = 8,  // plain CG: rz = S (= r.r), r0 = sqrt(S), den
     FIN_CG_PAP = 9,  // plain CG: pAp = S, breakdown test every check_every, alpha
-    FIN_CG_RR = 10   // plain CG: rr_new = S, ++iter, test, beta = S / rz, rz = S
+    FIN_CG_RR = 10,  // plain CG: rr_new = S, ++iter, test, beta = S / rz, rz = S
+    FIN_RZO = 11,    // rzo = S
+    FIN_RESUME = 12  // S = ||b - A x||^2: rr = S; keep "converged" only if it meets rtol
 };
 
 constexpr int kReduceBlock = 256;   // threads per first-stage block

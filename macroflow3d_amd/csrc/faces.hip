@@ -1,0 +1,822 @@
+// Face-coefficient kernels of the solve path (FP64 and mixed precision), gfx950.
+//
+// Each multigrid level stores its harmonic face conductances once per solve,
+// on the MAC face grid of the compact velocity layout:
+//   FX[i + (nx+1)(j + ny k)]  face between cells i-1 and i   (i = 0..nx)
+//   FY[i + nx(j + (ny+1) k)]  face between cells j-1 and j   (j = 0..ny)
+//   FZ[i + nx(j + ny k)]      face between cells k-1 and k   (k = 0..nz)
+// Interior faces hold 2/(1/K_a + 1/K_b) (src/multigrid/smoothers/gsrb_3d.cu:53-58);
+// boundary faces hold the bc_helpers coefficient (bc_stencil_helpers.cuh:33-267):
+// Dirichlet 2 K_cell, Neumann 0, periodic the harmonic mean across the wrap.
+// A boundary face's neighbour value is 0 unless the axis is periodic.
+//
+// With the coefficients precomputed, the smoother needs one divide per cell
+// (the reference recomputes six harmonic means, ~13 FP64 divides, per update)
+// and the operator and residual need none.
+//
+// T is the arithmetic type of x and b (double, or float for the mixed-precision
+// V-cycle); TC is the coefficient storage type (double, float, bf16, fp16).
+// Stored coefficients are multiplied by a per-level power of two `cs` (chosen
+// so that fp16 keeps its range); the kernels take bs = cs * dx^2 and
+// rs = 1 / (cs * dx^2) so every result is unchanged by the scaling.
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <type_traits>
+
+#include "faces.hpp"
+
+namespace mf3d {
+namespace {
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ---- coefficient storage conversions ---------------------------------------
+__device__ __forceinline__ double cload(const double* p, long i) { return p[i]; }
+__device__ __forceinline__ float cload(const float* p, long i) { return p[i]; }
+__device__ __forceinline__ float cload(const __hip_bfloat16* p, long i) {
+    return __bfloat162float(p[i]);
+}
+__device__ __forceinline__ float cload(const __half* p, long i) { return __half2float(p[i]); }
+__device__ __forceinline__ void cstore(double* p, long i, double v) { p[i] = v; }
+__device__ __forceinline__ void cstore(float* p, long i, double v) { p[i] = (float)v; }
+__device__ __forceinline__ void cstore(__hip_bfloat16* p, long i, double v) {
+    p[i] = __float2bfloat16((float)v);
+}
+__device__ __forceinline__ void cstore(__half* p, long i, double v) { p[i] = __float2half((float)v); }
+
+struct Dims {
+    int nx, ny, nz;
+    __device__ long fx(int i, int j, int k) const {
+        return (long)i + (long)(nx + 1) * ((long)j + (long)ny * k);
+    }
+    __device__ long fy(int i, int j, int k) const {
+        return (long)i + (long)nx * ((long)j + (long)(ny + 1) * k);
+    }
+    __device__ long fz(int i, int j, int k) const {
+        return (long)i + (long)nx * ((long)j + (long)ny * k);
+    }
+};
+
+__device__ __forceinline__ double bface(int t, double Kc) {
+    return t == MF3D_BC_DIRICHLET ? 2.0 * Kc : 0.0;
+}
+
+// ---- face coefficients from K ------------------------------------------------
+template <typename TC>
+__global__ void __launch_bounds__(kBlock)
+    faces_build_kernel(Dims d, BcTypes bc, double cs, const double* __restrict__ K,
+                       TC* __restrict__ FX, TC* __restrict__ FY, TC* __restrict__ FZ) {
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long)nx * ny * nz) return;
+    const int i = (int)(o % nx);
+    const long q = o / nx;
+    const int j = (int)(q % ny);
+    const int k = (int)(q / ny);
+    const long sy = nx, sz = (long)nx * ny;
+    const double Kc = K[o];
+    const double iKc = 1.0 / Kc;
+    // lower faces
+    double f;
+    if (i > 0) f = 2.0 / (iKc + 1.0 / K[o - 1]);
+    else f = bc.t[0] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + nx - 1]) : bface(bc.t[0], Kc);
+    cstore(FX, d.fx(i, j, k), f * cs);
+    if (j > 0) f = 2.0 / (iKc + 1.0 / K[o - sy]);
+    else f = bc.t[2] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(ny - 1) * sy])
+                                         : bface(bc.t[2], Kc);
+    cstore(FY, d.fy(i, j, k), f * cs);
+    if (k > 0) f = 2.0 / (iKc + 1.0 / K[o - sz]);
+    else f = bc.t[4] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(nz - 1) * sz])
+                                         : bface(bc.t[4], Kc);
+    cstore(FZ, d.fz(i, j, k), f * cs);
+    // upper boundary faces
+    if (i == nx - 1) {
+        f = bc.t[1] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (nx - 1)]) : bface(bc.t[1], Kc);
+        cstore(FX, d.fx(nx, j, k), f * cs);
+    }
+    if (j == ny - 1) {
+        f = bc.t[3] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(ny - 1) * sy])
+                                        : bface(bc.t[3], Kc);
+        cstore(FY, d.fy(i, ny, k), f * cs);
+    }
+    if (k == nz - 1) {
+        f = bc.t[5] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(nz - 1) * sz])
+                                        : bface(bc.t[5], Kc);
+        cstore(FZ, d.fz(i, j, nz), f * cs);
+    }
+}
+
+// Neighbour values of cell (i,j,k) in the order x+, x-, y+, y-, z+, z-; a
+// non-periodic boundary neighbour is 0.
+template <typename T>
+__device__ __forceinline__ void neighbours(const Dims& d, const BcTypes& bc, const T* x, long o,
+                                           int i, int j, int k, T (&v)[6]) {
+    const long sy = d.nx, sz = (long)d.nx * d.ny;
+    v[0] = i < d.nx - 1 ? x[o + 1] : (bc.t[1] == MF3D_BC_PERIODIC ? x[o - (d.nx - 1)] : T(0));
+    v[1] = i > 0 ? x[o - 1] : (bc.t[0] == MF3D_BC_PERIODIC ? x[o + (d.nx - 1)] : T(0));
+    v[2] = j < d.ny - 1 ? x[o + sy] : (bc.t[3] == MF3D_BC_PERIODIC ? x[o - (long)(d.ny - 1) * sy] : T(0));
+    v[3] = j > 0 ? x[o - sy] : (bc.t[2] == MF3D_BC_PERIODIC ? x[o + (long)(d.ny - 1) * sy] : T(0));
+    v[4] = k < d.nz - 1 ? x[o + sz] : (bc.t[5] == MF3D_BC_PERIODIC ? x[o - (long)(d.nz - 1) * sz] : T(0));
+    v[5] = k > 0 ? x[o - sz] : (bc.t[4] == MF3D_BC_PERIODIC ? x[o + (long)(d.nz - 1) * sz] : T(0));
+}
+
+template <typename T, typename TC>
+__device__ __forceinline__ void cell_faces(const Dims& d, const TC* FX, const TC* FY,
+                                           const TC* FZ, int i, int j, int k, T (&f)[6]) {
+    f[0] = (T)cload(FX, d.fx(i + 1, j, k));
+    f[1] = (T)cload(FX, d.fx(i, j, k));
+    f[2] = (T)cload(FY, d.fy(i, j + 1, k));
+    f[3] = (T)cload(FY, d.fy(i, j, k));
+    f[4] = (T)cload(FZ, d.fz(i, j, k + 1));
+    f[5] = (T)cload(FZ, d.fz(i, j, k));
+}
+
+// GSRB cell value (sum f x_nb - b cs dx^2) / sum f, summed x+, x-, y+, y-,
+// z+, z- as in gsrb_interior_kernel (gsrb_3d.cu:53-69); pin doubles the
+// diagonal of cell 0 (gsrb_3d.cu:360-362).
+template <typename T>
+__device__ __forceinline__ T gs_value(const T (&f)[6], const T (&v)[6], T b, T bs, bool pin0) {
+    T result = v[0] * f[0];
+    result += v[1] * f[1];
+    result += v[2] * f[2];
+    result += v[3] * f[3];
+    result += v[4] * f[4];
+    result += v[5] * f[5];
+    T aC = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
+    if (pin0) aC *= T(2);
+    return (result - b * bs) / aC;
+}
+
+// ---- one colour of GSRB on any level -----------------------------------------
+template <typename T, typename TC, bool ZERO>
+__global__ void __launch_bounds__(kBlock)
+    colour_kernel(Dims d, BcTypes bc, int pin, T bs, T* __restrict__ x,
+                  const T* __restrict__ b, const TC* __restrict__ FX,
+                  const TC* __restrict__ FY, const TC* __restrict__ FZ, int colour) {
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int nxh = (nx + 1) >> 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nxh * ny * nz) return;
+    const int ii = (int)(t % nxh);
+    const long row = t / nxh;
+    const int j = (int)(row % ny);
+    const int k = (int)(row / ny);
+    const int par = (j + k + colour) & 1;
+    const int i = 2 * ii + par;
+    const long rowbase = (long)nx * row;
+    if (ZERO) {
+        const int ip = 2 * ii + (par ^ 1);
+        if (ip < nx) x[rowbase + ip] = T(0);
+    }
+    if (i >= nx) return;
+    const long o = rowbase + i;
+    T f[6], v[6];
+    cell_faces(d, FX, FY, FZ, i, j, k, f);
+    if (ZERO) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] = T(0);
+    } else {
+        neighbours(d, bc, x, o, i, j, k, v);
+    }
+    x[o] = gs_value(f, v, b[o], bs, pin && o == 0);
+}
+
+// ---- whole smoothing of a small level in one workgroup -----------------------
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallPer = 4;
+
+template <typename T, typename TC>
+__global__ void __launch_bounds__(kSmallBlock)
+    small_kernel(Dims d, BcTypes bc, int pin, T bs, T* __restrict__ x, const T* __restrict__ b,
+                 const TC* __restrict__ FX, const TC* __restrict__ FY,
+                 const TC* __restrict__ FZ, int iters, int zero_init, const T* __restrict__ xc) {
+    __shared__ T xs[kSmallBlock * kSmallPer];
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int n = nx * ny * nz;
+    T f[kSmallPer][6], bl[kSmallPer];
+    const int NXc = nx >> 1, NYc = ny >> 1;
+#pragma unroll
+    for (int m = 0; m < kSmallPer; ++m) {
+        const int o = threadIdx.x + m * kSmallBlock;
+        bl[m] = T(0);
+        if (o < n) {
+            const int i = o % nx, q = o / nx, j = q % ny, k = q / ny;
+            T v = zero_init ? T(0) : x[o];
+            if (xc) v += xc[(i >> 1) + NXc * ((j >> 1) + NYc * (k >> 1))];
+            xs[o] = v;
+            cell_faces(d, FX, FY, FZ, i, j, k, f[m]);
+            bl[m] = b[o];
+        }
+    }
+    __syncthreads();
+    for (int it = 0; it < iters; ++it) {
+        for (int colour = 0; colour < 2; ++colour) {
+#pragma unroll
+            for (int m = 0; m < kSmallPer; ++m) {
+                const int o = threadIdx.x + m * kSmallBlock;
+                if (o >= n) continue;
+                const int i = o % nx, q = o / nx, j = q % ny, k = q / ny;
+                if (((i + j + k) & 1) != colour) continue;
+                T v[6];
+                neighbours(d, bc, xs, o, i, j, k, v);
+                xs[o] = gs_value(f[m], v, bl[m], bs, pin && o == 0);
+            }
+            __syncthreads();
+        }
+    }
+    for (int o = threadIdx.x; o < n; o += kSmallBlock) x[o] = xs[o];
+}
+
+// ---- one-pass red-black march (fine levels) ----------------------------------
+constexpr int TX = 64, TY = 16, HALO = 2;
+constexpr int OX = TX - 2 * HALO, OY = TY - 2 * HALO;
+
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+struct MarchGeom {
+    int tiles_x, tiles_y, chunks, zchunk, nblocks, per_xcd;
+};
+
+// A workgroup owns a 64 x 16 column tile and marches a z-chunk: at plane k it
+// updates the red cells of plane k on the tile minus a 1-column rim, then the
+// black cells of plane k-1 on the tile minus a 2-column rim, and writes plane
+// k-1 (the reference's red pass then black pass, gsrb_3d.cu:392-575, in one
+// sweep: red reads only old black values, black only new red values). Reads
+// come from xin, writes go to xout, so overlapping tiles never see each
+// other's updates. x of planes k and k-1 are exchanged through LDS; the
+// column's own planes and face coefficients are in registers, prefetched two
+// planes ahead into compile-time slots (the march is unrolled by 5) so no
+// prefetch is waited for before its use. Workgroups are ordered so that each
+// XCD (blockIdx % 8) takes a contiguous run of neighbouring tiles.
+template <typename T, typename TC, bool ZERO, bool PROLONG>
+__global__ void __launch_bounds__(TX * TY)
+    march_kernel(Dims d, BcTypes bc, int pin, T bs, MarchGeom geo, const T* __restrict__ xin,
+                 T* __restrict__ xout, const T* __restrict__ b, const TC* __restrict__ FX,
+                 const TC* __restrict__ FY, const TC* __restrict__ FZ,
+                 const T* __restrict__ xc) {
+    __shared__ T sx[3][TY][TX];  // ring of planes k, k-1 and the one being refilled
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int bid = blockIdx.x;
+    const int t = (bid & 7) * geo.per_xcd + (bid >> 3);
+    if (t >= geo.nblocks) return;  // padding blocks: uniform exit before any barrier
+    const int tile_x = t % geo.tiles_x;
+    const int rem = t / geo.tiles_x;
+    const int tile_y = rem % geo.tiles_y;
+    const int chunk = rem / geo.tiles_y;
+    const int z0 = chunk * geo.zchunk;
+    const int z1 = min(nz, z0 + geo.zchunk);
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    int i = tile_x * OX - HALO + tx;
+    int j = tile_y * OY - HALO + ty;
+    bool active = true;
+    if (i < 0 || i >= nx) {
+        if (bc.t[0] == MF3D_BC_PERIODIC) i = (i + nx) % nx; else active = false;
+    }
+    if (j < 0 || j >= ny) {
+        if (bc.t[2] == MF3D_BC_PERIODIC) j = (j + ny) % ny; else active = false;
+    }
+    const bool in_red = tx >= 1 && tx < TX - 1 && ty >= 1 && ty < TY - 1;
+    const bool in_out = tx >= HALO && tx < TX - HALO && ty >= HALO && ty < TY - HALO &&
+                        (tile_x * OX - HALO + tx) < nx && (tile_y * OY - HALO + ty) < ny;
+    // non-periodic domain edges: the neighbour value there is 0
+    const bool bxp = (i == nx - 1) && bc.t[1] != MF3D_BC_PERIODIC;
+    const bool bxm = (i == 0) && bc.t[0] != MF3D_BC_PERIODIC;
+    const bool byp = (j == ny - 1) && bc.t[3] != MF3D_BC_PERIODIC;
+    const bool bym = (j == 0) && bc.t[2] != MF3D_BC_PERIODIC;
+    const long plane = (long)nx * ny;
+    const long col = (long)i + (long)nx * j;
+    const int NXc = nx >> 1, NYc = ny >> 1;
+    const long ccol = (long)(i >> 1) + (long)NXc * (j >> 1);
+    const long cplane = (long)NXc * NYc;
+    const long fxcol = (long)i + (long)(nx + 1) * j, fxplane = (long)(nx + 1) * ny;
+    const long fycol = (long)i + (long)nx * j, fyplane = (long)nx * (ny + 1);
+
+    // plane p of this column: x, b and the faces x+, x-, y+, y-, z+ (z- is
+    // the z+ of plane p-1, except at p = 0)
+    auto load = [&](int p, T& xv, T& bv, T (&f)[5], T& fzm) {
+        if (active && p >= 0 && p < nz) {
+            const long o = col + plane * p;
+            bv = b[o];
+            T v = T(0);
+            if (!ZERO) v = xin[o];
+            if (PROLONG) v += xc[ccol + cplane * (p >> 1)];
+            xv = v;
+            f[0] = (T)cload(FX, fxcol + 1 + fxplane * p);
+            f[1] = (T)cload(FX, fxcol + fxplane * p);
+            f[2] = (T)cload(FY, fycol + nx + fyplane * p);
+            f[3] = (T)cload(FY, fycol + fyplane * p);
+            f[4] = (T)cload(FZ, col + plane * (p + 1));
+            fzm = (T)cload(FZ, col + plane * p);
+        } else {
+            xv = T(0);
+            bv = T(0);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) f[q] = T(0);
+            fzm = T(0);
+        }
+    };
+
+    const int kstart = z0 - 1;
+    T X[5], B[5], F[5][5], FZM[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        X[c] = T(0);
+        B[c] = T(0);
+        FZM[c] = T(0);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) F[c][q] = T(0);
+    }
+    load(kstart - 1, X[4], B[4], F[4], FZM[4]);
+    load(kstart, X[0], B[0], F[0], FZM[0]);
+    load(kstart + 1, X[1], B[1], F[1], FZM[1]);
+
+    auto step = [&](auto cc, int k) {
+        constexpr int c = decltype(cc)::value;
+        constexpr int m2 = (c + 3) % 5, m1 = (c + 4) % 5, p1 = (c + 1) % 5, p2 = (c + 2) % 5;
+        load(k + 2, X[p2], B[p2], F[p2], FZM[p2]);  // consumed two steps later
+        // slot of plane k in the 3-plane ring; plane k-1 is in slot (s + 2) % 3.
+        // The slot written here was last read two steps ago, before this
+        // step's barriers' predecessors, so two barriers per plane suffice.
+        const int s = (k - kstart) % 3;
+        sx[s][ty][tx] = X[c];
+        lds_sync();
+        // red cells of plane k on the tile minus its outer ring
+        if (k >= 0 && k < nz && in_red && active && ((i + j + k) & 1) == 0) {
+            const T f[6] = {F[c][0], F[c][1], F[c][2], F[c][3], F[c][4], FZM[c]};
+            const T v[6] = {bxp ? T(0) : sx[s][ty][tx + 1], bxm ? T(0) : sx[s][ty][tx - 1],
+                            byp ? T(0) : sx[s][ty + 1][tx], bym ? T(0) : sx[s][ty - 1][tx],
+                            k == nz - 1 ? T(0) : X[p1], k == 0 ? T(0) : X[m1]};
+            X[c] = gs_value(f, v, B[c], bs, pin && i == 0 && j == 0 && k == 0);
+            sx[s][ty][tx] = X[c];
+        }
+        lds_sync();
+        // black cells of plane p = k-1 on the output tile, then write plane p
+        const int p = k - 1;
+        if (p >= z0 && p < z1 && active && in_out) {
+            if (((i + j + p) & 1) == 1) {
+                const int q = (s + 2) % 3;
+                const T f[6] = {F[m1][0], F[m1][1], F[m1][2], F[m1][3], F[m1][4], FZM[m1]};
+                const T v[6] = {bxp ? T(0) : sx[q][ty][tx + 1], bxm ? T(0) : sx[q][ty][tx - 1],
+                                byp ? T(0) : sx[q][ty + 1][tx], bym ? T(0) : sx[q][ty - 1][tx],
+                                p == nz - 1 ? T(0) : X[c], p == 0 ? T(0) : X[m2]};
+                X[m1] = gs_value(f, v, B[m1], bs, pin && i == 0 && j == 0 && p == 0);
+            }
+            xout[col + plane * p] = X[m1];
+        }
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    for (int k = kstart; k <= z1; k += 5) {
+        step(I0{}, k);
+        if (k + 1 > z1) break;
+        step(I1{}, k + 1);
+        if (k + 2 > z1) break;
+        step(I2{}, k + 2);
+        if (k + 3 > z1) break;
+        step(I3{}, k + 3);
+        if (k + 4 > z1) break;
+        step(I4{}, k + 4);
+    }
+}
+
+// ---- residual fused with restriction -----------------------------------------
+// r = b - A x (smoother convention, pin adds aC x_C at cell 0,
+// residual_3d.cu:365-368) of the 8 children of one coarse cell, averaged in
+// the order of restrict_kernel (restrict_3d.cu:36-49).
+template <typename T, typename TC>
+__device__ __forceinline__ T residual_cell(const Dims& d, const BcTypes& bc, bool pin,
+                                           const T* x, const T* b, const TC* FX, const TC* FY,
+                                           const TC* FZ, int i, int j, int k, T rs) {
+    const long o = (long)i + (long)d.nx * ((long)j + (long)d.ny * k);
+    T f[6], v[6];
+    cell_faces(d, FX, FY, FZ, i, j, k, f);
+    neighbours(d, bc, x, o, i, j, k, v);
+    const T xc = x[o];
+    T result = f[0] * (xc - v[0]);
+    result += f[1] * (xc - v[1]);
+    result += f[2] * (xc - v[2]);
+    result += f[3] * (xc - v[3]);
+    result += f[4] * (xc - v[4]);
+    result += f[5] * (xc - v[5]);
+    if (pin && o == 0) result += (f[0] + f[1] + f[2] + f[3] + f[4] + f[5]) * xc;
+    return b[o] + result * rs;
+}
+
+template <typename T, typename TC>
+__global__ void __launch_bounds__(kBlock)
+    residual_restrict_kernel(Dims d, BcTypes bc, int pin, T rs, const T* __restrict__ x,
+                             const T* __restrict__ b, const TC* __restrict__ FX,
+                             const TC* __restrict__ FY, const TC* __restrict__ FZ,
+                             T* __restrict__ bcoarse) {
+    const int NX = d.nx >> 1, NY = d.ny >> 1, NZ = d.nz >> 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)NX * NY * NZ) return;
+    const int I = (int)(t % NX);
+    const long q = t / NX;
+    const int J = (int)(q % NY);
+    const int Kc = (int)(q / NY);
+    const int i = 2 * I, j = 2 * J, k = 2 * Kc;
+    const bool p = pin != 0;
+    T s = T(0);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i, j, k, rs);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i + 1, j, k, rs);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i, j + 1, k, rs);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i + 1, j + 1, k, rs);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i, j, k + 1, rs);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i + 1, j, k + 1, rs);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i, j + 1, k + 1, rs);
+    s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i + 1, j + 1, k + 1, rs);
+    bcoarse[t] = s / T(8);
+}
+
+// ---- prolongation for levels without the march kernel -----------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    prolong_kernel(Dims d, const T* __restrict__ c, T* __restrict__ f) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long)d.nx * d.ny * d.nz) return;
+    const int i = (int)(o % d.nx);
+    const long q = o / d.nx;
+    const int j = (int)(q % d.ny);
+    const int k = (int)(q / d.ny);
+    f[o] += c[(long)(i >> 1) + (long)(d.nx >> 1) * ((long)(j >> 1) + (long)(d.ny >> 1) * (k >> 1))];
+}
+
+// ---- FP64 operator on the finest level ----------------------------------------
+template <int BS>
+__device__ inline double block_sum(double v) {
+    __shared__ double part[BS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = lane < BS / 64 ? part[lane] : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    }
+    return v;
+}
+
+// A x = -sum f (x_C - x_nb) / dx^2 (varcoeff_laplacian.cu:38-239); the pin's
+// diagonal counts in-domain neighbours only (varcoeff_laplacian.cu:110-234).
+__device__ __forceinline__ double apply_cell(const Dims& d, const BcTypes& bc, bool pin,
+                                             const double* x, const double* FX,
+                                             const double* FY, const double* FZ, long o, int i,
+                                             int j, int k, double inv_dx2) {
+    double f[6], v[6];
+    cell_faces(d, FX, FY, FZ, i, j, k, f);
+    neighbours(d, bc, x, o, i, j, k, v);
+    const double xC = x[o];
+    double Ax = f[1] * (xC - v[1]) + f[0] * (xC - v[0]) + f[3] * (xC - v[3]) +
+                f[2] * (xC - v[2]) + f[5] * (xC - v[5]) + f[4] * (xC - v[4]);
+    if (pin && o == 0) {
+        double aC = 0.0;
+        if (i < d.nx - 1) aC += f[0];
+        if (i > 0) aC += f[1];
+        if (j < d.ny - 1) aC += f[2];
+        if (j > 0) aC += f[3];
+        if (k < d.nz - 1) aC += f[4];
+        if (k > 0) aC += f[5];
+        Ax += aC * xC;
+    }
+    return -Ax * inv_dx2;
+}
+
+// MODE 0: y = A x, partials of (x, y).  MODE 1: r = b - A x, partials of (r, r).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, const double* __restrict__ x,
+              const double* __restrict__ FX, const double* __restrict__ FY,
+              const double* __restrict__ FZ, const double* __restrict__ b,
+              double* __restrict__ y, double* __restrict__ partials) {
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)d.nx * d.ny * d.nz;
+    double prod = 0.0;
+    if (o < n) {
+        const int i = (int)(o % d.nx);
+        const long q = o / d.nx;
+        const int j = (int)(q % d.ny);
+        const int k = (int)(q / d.ny);
+        const double Ax = apply_cell(d, bc, pin != 0, x, FX, FY, FZ, o, i, j, k, inv_dx2);
+        if (MODE == 0) {
+            y[o] = Ax;
+            prod = x[o] * Ax;
+        } else {
+            const double r = -1.0 * Ax + b[o];
+            if (y) y[o] = r;
+            prod = r * r;
+        }
+    }
+    const double s = block_sum<kBlock>(prod);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) fill_t_kernel(T* __restrict__ x, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = T(0);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    sum_log_kernel(const double* __restrict__ K, long n, double* __restrict__ partials) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = i < n ? log(K[i]) : 0.0;
+    const double s = block_sum<kBlock>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// ---- precision conversions at the V-cycle boundary -----------------------------
+__global__ void __launch_bounds__(kBlock)
+    to_f32_kernel(const double* __restrict__ a, float* __restrict__ o, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (float)a[i];
+}
+__global__ void __launch_bounds__(kBlock)
+    to_f64_kernel(const float* __restrict__ a, double* __restrict__ o, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (double)a[i];
+}
+
+// coarsest direct solve in FP64 for any x/b type: x = A^{-1} b, one wave per row
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gemv_kernel(int n, const double* __restrict__ Ainv, const T* __restrict__ b,
+                T* __restrict__ x) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double* a = Ainv + (long)row * n;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += a[c] * (double)b[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) x[row] = (T)s;
+}
+
+MarchGeom geometry(const mf3d_grid& g, int zchunk) {
+    MarchGeom m;
+    m.tiles_x = (g.nx + OX - 1) / OX;
+    m.tiles_y = (g.ny + OY - 1) / OY;
+    m.zchunk = zchunk;
+    m.chunks = (g.nz + zchunk - 1) / zchunk;
+    m.nblocks = m.tiles_x * m.tiles_y * m.chunks;
+    m.per_xcd = (m.nblocks + 7) / 8;
+    return m;
+}
+
+inline Dims dims(const mf3d_grid& g) { return Dims{g.nx, g.ny, g.nz}; }
+
+template <typename TC>
+const TC* cptr(const FaceSet& f, int a) {
+    return static_cast<const TC*>(a == 0 ? f.fx : a == 1 ? f.fy : f.fz);
+}
+
+}  // namespace
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+namespace kern {
+
+size_t face_count(const mf3d_grid& g) {
+    return (size_t)(g.nx + 1) * g.ny * g.nz + (size_t)g.nx * (g.ny + 1) * g.nz +
+           (size_t)g.nx * g.ny * (g.nz + 1);
+}
+
+size_t coef_bytes(CoefType t) {
+    return t == COEF_F64 ? 8 : t == COEF_F32 ? 4 : 2;
+}
+
+int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
+                const double* K, FaceSet& f) {
+    const Dims d = dims(g);
+    const unsigned gr = grid_for(cells(g));
+    switch (f.type) {
+    case COEF_F64:
+        faces_build_kernel<double><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (double*)f.fx,
+                                                         (double*)f.fy, (double*)f.fz);
+        break;
+    case COEF_F32:
+        faces_build_kernel<float><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (float*)f.fx, (float*)f.fy,
+                                                        (float*)f.fz);
+        break;
+    case COEF_BF16:
+        faces_build_kernel<__hip_bfloat16><<<gr, kBlock, 0, s>>>(
+            d, bc, cs, K, (__hip_bfloat16*)f.fx, (__hip_bfloat16*)f.fy, (__hip_bfloat16*)f.fz);
+        break;
+    case COEF_F16:
+        faces_build_kernel<__half><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (__half*)f.fx,
+                                                         (__half*)f.fy, (__half*)f.fz);
+        break;
+    }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+bool march_ok(const mf3d_grid& g, const BcTypes& bc) {
+    if (g.nx < 32 || g.ny < 16 || g.nz < 8) return false;
+    if (bc.t[4] == MF3D_BC_PERIODIC) return false;  // the z march does not wrap
+    if ((bc.t[0] == MF3D_BC_PERIODIC && (g.nx & 1)) || (bc.t[2] == MF3D_BC_PERIODIC && (g.ny & 1)))
+        return false;  // odd periodic sizes break the two-colouring across the wrap
+    return true;
+}
+
+int march_default_zchunk(const mf3d_grid& g) {
+    const long tiles = (long)((g.nx + OX - 1) / OX) * ((g.ny + OY - 1) / OY);
+    int z = 32;
+    while (z > 4 && tiles * ((g.nz + z - 1) / z) < 1024) z >>= 1;
+    return z;
+}
+
+namespace {
+
+template <typename T, typename TC>
+int smooth_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
+             T* x, T* tmp, const T* b, const FaceSet& F, int iters, bool zero_init,
+             const T* xc, const SmoothPlan& plan) {
+    const Dims d = dims(g);
+    const TC* FX = cptr<TC>(F, 0);
+    const TC* FY = cptr<TC>(F, 1);
+    const TC* FZ = cptr<TC>(F, 2);
+    const size_t n = cells(g);
+    const T tbs = (T)bs;
+    if (iters <= 0) {
+        if (zero_init) {
+            fill_t_kernel<T><<<grid_for(n), kBlock, 0, s>>>(x, (long)n);
+        } else if (xc) {
+            prolong_kernel<T><<<grid_for(n), kBlock, 0, s>>>(d, xc, x);
+        }
+        MF3D_HIP_TRY(hipGetLastError());
+        return MF3D_OK;
+    }
+    if (plan.kind == SMOOTH_SMALL && n <= (size_t)kSmallBlock * kSmallPer) {
+        small_kernel<T, TC><<<1, kSmallBlock, 0, s>>>(d, bc, pin ? 1 : 0, tbs, x, b, FX, FY, FZ,
+                                                      iters, zero_init ? 1 : 0, xc);
+        MF3D_HIP_TRY(hipGetLastError());
+        return MF3D_OK;
+    }
+    if (plan.kind == SMOOTH_MARCH && tmp && march_ok(g, bc)) {
+        const int zc = plan.zchunk > 0 ? plan.zchunk : march_default_zchunk(g);
+        const MarchGeom geo = geometry(g, zc);
+        const unsigned nb = (unsigned)geo.per_xcd * 8;
+        // outputs alternate so the last sweep writes x; a virtual (zero) first
+        // input lets an odd count start on x, otherwise an odd count ends in tmp
+        T* out = ((iters & 1) && zero_init) ? x : tmp;
+        const T* in = x;
+        for (int it = 0; it < iters; ++it) {
+            if (it == 0 && zero_init)
+                march_kernel<T, TC, true, false><<<nb, dim3(TX, TY), 0, s>>>(
+                    d, bc, pin ? 1 : 0, tbs, geo, nullptr, out, b, FX, FY, FZ, nullptr);
+            else if (it == 0 && xc)
+                march_kernel<T, TC, false, true><<<nb, dim3(TX, TY), 0, s>>>(
+                    d, bc, pin ? 1 : 0, tbs, geo, in, out, b, FX, FY, FZ, xc);
+            else
+                march_kernel<T, TC, false, false><<<nb, dim3(TX, TY), 0, s>>>(
+                    d, bc, pin ? 1 : 0, tbs, geo, in, out, b, FX, FY, FZ, nullptr);
+            in = out;
+            out = (out == x) ? tmp : x;
+        }
+        MF3D_HIP_TRY(hipGetLastError());
+        if (in != x) MF3D_HIP_TRY(hipMemcpyAsync(x, in, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        return MF3D_OK;
+    }
+    if (xc) prolong_kernel<T><<<grid_for(n), kBlock, 0, s>>>(d, xc, x);
+    const unsigned gh = grid_for((size_t)((g.nx + 1) / 2) * g.ny * g.nz);
+    for (int it = 0; it < iters; ++it)
+        for (int colour = 0; colour < 2; ++colour) {
+            if (it == 0 && colour == 0 && zero_init)
+                colour_kernel<T, TC, true><<<gh, kBlock, 0, s>>>(d, bc, pin ? 1 : 0, tbs, x, b, FX,
+                                                                 FY, FZ, colour);
+            else
+                colour_kernel<T, TC, false><<<gh, kBlock, 0, s>>>(d, bc, pin ? 1 : 0, tbs, x, b,
+                                                                  FX, FY, FZ, colour);
+        }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+template <typename T, typename TC>
+int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double rs, const T* x,
+         const T* b, const FaceSet& F, T* bcoarse) {
+    const size_t nc = (size_t)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
+    residual_restrict_kernel<T, TC><<<grid_for(nc), kBlock, 0, s>>>(
+        dims(g), bc, pin ? 1 : 0, (T)rs, x, b, cptr<TC>(F, 0), cptr<TC>(F, 1), cptr<TC>(F, 2),
+        bcoarse);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+}  // namespace
+
+int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
+                 double* x, double* tmp, const double* b, const FaceSet& F, int iters,
+                 bool zero_init, const double* xc, const SmoothPlan& plan) {
+    if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 smoothing needs FP64 faces");
+    return smooth_t<double, double>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc, plan);
+}
+
+int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
+                 float* x, float* tmp, const float* b, const FaceSet& F, int iters,
+                 bool zero_init, const float* xc, const SmoothPlan& plan) {
+    switch (F.type) {
+    case COEF_F32:
+        return smooth_t<float, float>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc, plan);
+    case COEF_BF16:
+        return smooth_t<float, __hip_bfloat16>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init,
+                                               xc, plan);
+    case COEF_F16:
+        return smooth_t<float, __half>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc,
+                                       plan);
+    default:
+        return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");
+    }
+}
+
+int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
+                            double rs, const double* x, const double* b, const FaceSet& F,
+                            double* bcoarse) {
+    return rr_t<double, double>(s, g, bc, pin, rs, x, b, F, bcoarse);
+}
+
+int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
+                            double rs, const float* x, const float* b, const FaceSet& F,
+                            float* bcoarse) {
+    switch (F.type) {
+    case COEF_F32: return rr_t<float, float>(s, g, bc, pin, rs, x, b, F, bcoarse);
+    case COEF_BF16: return rr_t<float, __hip_bfloat16>(s, g, bc, pin, rs, x, b, F, bcoarse);
+    case COEF_F16: return rr_t<float, __half>(s, g, bc, pin, rs, x, b, F, bcoarse);
+    default: return fail(MF3D_ERR_UNSUPPORTED, "FP32 residual needs 32- or 16-bit faces");
+    }
+}
+
+int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
+             int mode, const double* x, const double* b, double* y, double* partials) {
+    if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "operator needs FP64 faces");
+    const size_t n = cells(g);
+    const double inv_dx2 = 1.0 / (g.dx * g.dx);
+    const double* FX = cptr<double>(F, 0);
+    const double* FY = cptr<double>(F, 1);
+    const double* FZ = cptr<double>(F, 2);
+    if (mode == 0)
+        op_kernel<0><<<grid_for(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
+                                                    FZ, b, y, partials);
+    else
+        op_kernel<1><<<grid_for(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
+                                                    FZ, b, y, partials);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int fill_f32(hipStream_t s, float* x, size_t n, float v) {
+    (void)v;
+    fill_t_kernel<float><<<grid_for(n), kBlock, 0, s>>>(x, (long)n);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials) {
+    sum_log_kernel<<<grid_for(n), kBlock, 0, s>>>(K, (long)n, partials);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int to_f32(hipStream_t s, const double* a, float* o, size_t n) {
+    to_f32_kernel<<<grid_for(n), kBlock, 0, s>>>(a, o, (long)n);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int to_f64(hipStream_t s, const float* a, double* o, size_t n) {
+    to_f64_kernel<<<grid_for(n), kBlock, 0, s>>>(a, o, (long)n);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x) {
+    gemv_kernel<double><<<(n + 3) / 4, 256, 0, s>>>(n, Ainv, b, x);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int gemv(hipStream_t s, int n, const double* Ainv, const float* b, float* x) {
+    gemv_kernel<float><<<(n + 3) / 4, 256, 0, s>>>(n, Ainv, b, x);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+}  // namespace kern
+}  // namespace mf3d

@@ -1,0 +1,65 @@
+// Face-coefficient fields of a multigrid level and their kernels (faces.hip).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mf3d {
+
+enum CoefType : int { COEF_F64 = 0, COEF_F32 = 1, COEF_BF16 = 2, COEF_F16 = 3 };
+
+// MAC face conductances of one level: fx (nx+1)*ny*nz, fy nx*(ny+1)*nz,
+// fz nx*ny*(nz+1) values of `type`, stored times a power-of-two scale.
+struct FaceSet {
+    CoefType type = COEF_F64;
+    void* fx = nullptr;
+    void* fy = nullptr;
+    void* fz = nullptr;
+};
+
+enum SmoothKind : int { SMOOTH_COLOUR = 0, SMOOTH_SMALL = 1, SMOOTH_MARCH = 2 };
+
+// How one level is smoothed (chosen by defaults or the autotuner).
+struct SmoothPlan {
+    int kind = SMOOTH_COLOUR;
+    int zchunk = 0;  // march kernel: planes per workgroup (<= 0: default)
+};
+
+namespace kern {
+
+size_t face_count(const mf3d_grid& g);
+size_t coef_bytes(CoefType t);
+// faces from K (cs: storage scale)
+int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs, const double* K,
+                FaceSet& f);
+bool march_ok(const mf3d_grid& g, const BcTypes& bc);
+int march_default_zchunk(const mf3d_grid& g);
+// iters GSRB iterations (red then black) with face coefficients; result in x.
+// bs = cs * dx^2. tmp: n-element scratch for the march kernel. zero_init: x is
+// zero on entry. xc (nullable): coarse correction prolonged and added first.
+int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
+                 double* x, double* tmp, const double* b, const FaceSet& F, int iters,
+                 bool zero_init, const double* xc, const SmoothPlan& plan);
+int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
+                 float* x, float* tmp, const float* b, const FaceSet& F, int iters,
+                 bool zero_init, const float* xc, const SmoothPlan& plan);
+// coarse b = restriction of (b - A x), rs = 1 / (cs * dx^2)
+int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
+                            double rs, const double* x, const double* b, const FaceSet& F,
+                            double* bcoarse);
+int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
+                            double rs, const float* x, const float* b, const FaceSet& F,
+                            float* bcoarse);
+// FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
+// mode 1: y = b - A x (y nullable), partials (y, y).
+int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
+             int mode, const double* x, const double* b, double* y, double* partials);
+int to_f32(hipStream_t s, const double* a, float* o, size_t n);
+int to_f64(hipStream_t s, const float* a, double* o, size_t n);
+int gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
+int gemv(hipStream_t s, int n, const double* Ainv, const float* b, float* x);
+int fill_f32(hipStream_t s, float* x, size_t n, float v);
+// partials of sum(log K) for the coefficient scale
+int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials);
+
+}  // namespace kern
+}  // namespace mf3d

@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "kernels.hpp"
+#include "faces.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -71,11 +71,19 @@ namespace {
 struct Level {
     mf3d_grid g{};
     size_t n = 0;
-    double* x = nullptr;
-    double* b = nullptr;
-    double* K = nullptr;
-    double* t = nullptr;  // smoother scratch (one-pass red-black kernel)
-    bool fast = false;    // level served by the one-pass red-black kernel
+    double* x = nullptr;    // FP64 V-cycle correction (levels > 0)
+    double* b = nullptr;    // FP64 V-cycle right-hand side (levels > 0)
+    double* K = nullptr;    // coarsened conductivity (levels > 0)
+    double* t = nullptr;    // FP64 march scratch (level 0 uses the PCG's Ap)
+    float* x32 = nullptr;   // FP32 V-cycle buffers
+    float* b32 = nullptr;
+    float* t32 = nullptr;
+    FaceSet f64;            // FP64 faces (operator on level 0, FP64 V-cycle)
+    FaceSet flo;            // low-precision faces of the FP32 V-cycle
+    bool has_f64 = false;
+    bool has_flo = false;
+    double cs = 1.0;        // storage scale of flo
+    SmoothPlan plan;
 };
 
 inline hipStream_t as_stream(mf3d_stream s) { return reinterpret_cast<hipStream_t>(s); }
@@ -100,26 +108,42 @@ struct mf3d_workspace {
     IterState* st = nullptr;
     double* dscalar = nullptr;
     IterState* host_st = nullptr;  // pinned
+    double* host_scalar = nullptr; // pinned
     // set-up snapshot
     mf3d_bc bc{};
     BcTypes bct{};
     bool pin = false;
     int pre = 2, post = 2, coarse_iters = 50, coarse_solver = 0;
+    int precision = MF3D_PRECOND_FP64;
+    CoefType lo_type = COEF_BF16;
     const double* K0 = nullptr;
     bool ready = false;
+    // tuning (mf3d_workspace_set_tuning / autotuner)
+    long march_min_cells[2] = {1L << 21, 1L << 21};  // FP64, FP32 V-cycle
+    int zchunk = 0;
     size_t bytes = 0;
     std::vector<void*> owned;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
-    int alloc(double** p, size_t n) {
+    int alloc_bytes(void** p, size_t nbytes) {
         void* q = nullptr;
-        if (n == 0) n = 1;
-        hipError_t e = hipMalloc(&q, n * sizeof(double));
+        if (nbytes == 0) nbytes = 8;
+        hipError_t e = hipMalloc(&q, nbytes);
         if (e != hipSuccess)
             return fail(MF3D_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
         owned.push_back(q);
-        bytes += n * sizeof(double);
-        *p = static_cast<double*>(q);
+        bytes += nbytes;
+        *p = q;
+        return MF3D_OK;
+    }
+    int alloc(double** p, size_t n) { return alloc_bytes((void**)p, n * sizeof(double)); }
+    int alloc(float** p, size_t n) { return alloc_bytes((void**)p, n * sizeof(float)); }
+    int alloc_faces(FaceSet& f, const mf3d_grid& g, CoefType t) {
+        f.type = t;
+        const size_t b = kern::coef_bytes(t);
+        MF3D_TRY(alloc_bytes(&f.fx, (size_t)(g.nx + 1) * g.ny * g.nz * b));
+        MF3D_TRY(alloc_bytes(&f.fy, (size_t)g.nx * (g.ny + 1) * g.nz * b));
+        MF3D_TRY(alloc_bytes(&f.fz, (size_t)g.nx * g.ny * (g.nz + 1) * b));
         return MF3D_OK;
     }
     void release() {
@@ -143,6 +167,7 @@ struct mf3d_workspace {
     ~mf3d_workspace() {
         release();
         if (host_st) (void)hipHostFree(host_st);
+        if (host_scalar) (void)hipHostFree(host_scalar);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -193,7 +218,6 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
             MF3D_TRY(ws->alloc(&L.b, L.n));
             MF3D_TRY(ws->alloc(&L.K, L.n));
         }
-        L.fast = false;
     }
     const size_t n = cells(g);
     ws->nvec = n;
@@ -209,6 +233,7 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     ws->st = reinterpret_cast<IterState*>(stp);
     MF3D_TRY(ws->alloc(&ws->dscalar, 8));
     if (!ws->host_st) MF3D_HIP_TRY(hipHostMalloc((void**)&ws->host_st, sizeof(IterState)));
+    if (!ws->host_scalar) MF3D_HIP_TRY(hipHostMalloc((void**)&ws->host_scalar, sizeof(double)));
     for (auto& e : ws->ev)
         if (!e) MF3D_HIP_TRY(hipEventCreate(&e));
     ws->fine = g;
@@ -225,6 +250,14 @@ int scratch_partials(mf3d_workspace* ws, size_t n) {
     ws->partials = p;
     ws->partials_cap = need;
     if (!ws->dscalar) MF3D_TRY(ws->alloc(&ws->dscalar, 8));
+    return MF3D_OK;
+}
+
+int host_scalar_sync(mf3d_workspace* ws, hipStream_t s, double* out) {
+    MF3D_HIP_TRY(hipMemcpyAsync(ws->host_scalar, ws->dscalar, sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    *out = *ws->host_scalar;
     return MF3D_OK;
 }
 
@@ -310,63 +343,143 @@ int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
     return MF3D_OK;
 }
 
-// Hierarchy coefficients for a K field (solve_head.cu:106-114).
-int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc,
-             const mf3d_head_solve_config& cfg, const double* K) {
-    MF3D_TRY(ensure_workspace(ws, g, cfg.mg_levels));
-    ws->bc = bc;
-    ws->bct = bc_types(bc);
-    for (size_t l = 0; l < ws->lv.size(); ++l) {
-        Level& L = ws->lv[l];
-        L.fast = kern::fast_ok(L.g, ws->bct);
-        if (L.fast && l > 0 && !L.t) MF3D_TRY(ws->alloc(&L.t, L.n));
-        if (l == 0) L.t = ws->Ap;  // free while the V-cycle runs
-    }
-    ws->pin = pin_enabled(cfg.pin_mode, bc);
-    ws->pre = cfg.mg_pre_smooth;
-    ws->post = cfg.mg_post_smooth;
-    ws->coarse_iters = cfg.mg_coarse_iters;
-    ws->coarse_solver = cfg.coarse_solver;
-    ws->K0 = K;
-    for (size_t l = 1; l < ws->lv.size(); ++l)
-        MF3D_TRY(kern::coarsen(s, ws->lv[l].g, l == 1 ? K : ws->lv[l - 1].K, ws->lv[l].K));
-    if (cfg.coarse_solver == MF3D_COARSE_DIRECT) MF3D_TRY(build_coarse_inverse(ws, s));
-    return MF3D_OK;
+CoefType coef_type_of(int fmt) {
+    return fmt == MF3D_COEF_FP32 ? COEF_F32 : fmt == MF3D_COEF_FP16 ? COEF_F16 : COEF_BF16;
 }
 
 inline const double* level_K(const mf3d_workspace* ws, size_t l) {
     return l == 0 ? ws->K0 : ws->lv[l].K;
 }
 
-// GSRB smoothing of level l; xc (nullable) is a coarse correction to
-// prolong-add into x first.
-int smooth(mf3d_workspace* ws, hipStream_t s, size_t l, double* x, const double* b, int iters,
-           bool zero_init, const double* xc = nullptr) {
-    const Level& L = ws->lv[l];
-    if (L.fast)
-        return kern::gsrb_march(s, L.g, ws->bct, ws->pin, x, L.t, b, level_K(ws, l), iters,
-                                zero_init, xc, 0);
-    if (xc) MF3D_TRY(kern::prolong_add(s, L.g, xc, x));
-    return kern::gsrb(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), iters, zero_init);
+// Hierarchy coefficients for a K field (solve_head.cu:106-114): coarsened K
+// (coarsen_K.cu:15-53), face coefficients per level, smoothing plans and, with
+// a DIRECT coarse solver, the dense coarsest inverse.
+int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc,
+             const mf3d_head_solve_config& cfg, const double* K) {
+    MF3D_TRY(ensure_workspace(ws, g, cfg.mg_levels));
+    ws->bc = bc;
+    ws->bct = bc_types(bc);
+    ws->pin = pin_enabled(cfg.pin_mode, bc);
+    ws->pre = cfg.mg_pre_smooth;
+    ws->post = cfg.mg_post_smooth;
+    ws->coarse_iters = cfg.mg_coarse_iters;
+    ws->coarse_solver = cfg.coarse_solver;
+    ws->precision = cfg.precond_precision;
+    ws->lo_type = coef_type_of(cfg.coef_format);
+    ws->K0 = K;
+    const bool mixed = ws->precision == MF3D_PRECOND_FP32;
+    // fp16 coefficients are stored divided by the geometric mean of K (a power
+    // of two), so both tails of a log-normal field stay inside fp16's range
+    double cs = 1.0;
+    if (mixed && ws->lo_type == COEF_F16) {
+        const size_t n0 = cells(g);
+        MF3D_TRY(kern::sum_log_partials(s, K, n0, ws->partials));
+        MF3D_TRY(kern::finalize(s, ws->partials, kern::reduce_blocks(n0), FIN_STORE, nullptr,
+                                ws->dscalar));
+        double slog = 0.0;
+        MF3D_TRY(host_scalar_sync(ws, s, &slog));
+        const double mean_log2 = slog / (double)n0 / std::log(2.0);
+        cs = std::ldexp(1.0, -(int)std::lround(mean_log2));
+    }
+    for (size_t l = 0; l < ws->lv.size(); ++l) {
+        Level& L = ws->lv[l];
+        if (l > 0) MF3D_TRY(kern::coarsen(s, L.g, level_K(ws, l - 1), L.K));
+        const bool want64 = (l == 0) || !mixed;
+        if (want64) {
+            if (!L.has_f64) {
+                MF3D_TRY(ws->alloc_faces(L.f64, L.g, COEF_F64));
+                L.has_f64 = true;
+            }
+            MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
+        }
+        const long thr = ws->march_min_cells[mixed ? 1 : 0];
+        L.plan.kind = L.n <= 4096 ? SMOOTH_SMALL
+                      : (kern::march_ok(L.g, ws->bct) && (long)L.n >= thr) ? SMOOTH_MARCH
+                                                                           : SMOOTH_COLOUR;
+        L.plan.zchunk = ws->zchunk;
+        if (mixed) {
+            if (!L.has_flo || L.flo.type != ws->lo_type) {
+                MF3D_TRY(ws->alloc_faces(L.flo, L.g, ws->lo_type));
+                L.has_flo = true;
+            }
+            L.cs = cs;
+            MF3D_TRY(kern::faces_build(s, L.g, ws->bct, cs, level_K(ws, l), L.flo));
+            if (!L.x32) MF3D_TRY(ws->alloc(&L.x32, L.n));
+            if (!L.b32) MF3D_TRY(ws->alloc(&L.b32, L.n));
+            if (L.plan.kind == SMOOTH_MARCH && !L.t32) MF3D_TRY(ws->alloc(&L.t32, L.n));
+        } else if (L.plan.kind == SMOOTH_MARCH && l > 0 && !L.t) {
+            MF3D_TRY(ws->alloc(&L.t, L.n));
+        }
+        if (l == 0) L.t = ws->Ap;  // free while the V-cycle runs
+    }
+    if (cfg.coarse_solver == MF3D_COARSE_DIRECT) MF3D_TRY(build_coarse_inverse(ws, s));
+    return MF3D_OK;
 }
 
-// v_cycle_recursive (src/multigrid/cycle/v_cycle.cu:36-74). zero_init: x is
-// zero on entry (the fill of v_cycle.cu:62 / mg_preconditioner.cu:37 is folded
-// into the first smoothing sweep).
-int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, double* x, const double* b,
-           bool zero_init) {
+// per-type views of a level's V-cycle buffers
+template <typename T> struct LevelView;
+template <> struct LevelView<double> {
+    static double* x(Level& L) { return L.x; }
+    static double* b(Level& L) { return L.b; }
+    static double* t(Level& L) { return L.t; }
+    static const FaceSet& F(const Level& L) { return L.f64; }
+    static double cs(const Level&) { return 1.0; }
+};
+template <> struct LevelView<float> {
+    static float* x(Level& L) { return L.x32; }
+    static float* b(Level& L) { return L.b32; }
+    static float* t(Level& L) { return L.t32; }
+    static const FaceSet& F(const Level& L) { return L.flo; }
+    static double cs(const Level& L) { return L.cs; }
+};
+
+// v_cycle_recursive (src/multigrid/cycle/v_cycle.cu:36-74) in precision T.
+// zero_init: x is zero on entry (the fills of v_cycle.cu:62 and
+// mg_preconditioner.cu:37 are folded into the first smoothing sweep); the
+// prolongation (v_cycle.cu:69) is folded into the first post-smoothing sweep.
+template <typename T>
+int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool zero_init) {
+    using V = LevelView<T>;
     const size_t nl = ws->lv.size();
-    const Level& L = ws->lv[l];
+    Level& L = ws->lv[l];
+    const double dx2 = L.g.dx * L.g.dx;
+    const double cs = V::cs(L);
+    const FaceSet& F = V::F(L);
     if (l == nl - 1) {
         if (ws->coarse_solver == MF3D_COARSE_DIRECT)
-            return kern::dense_gemv(s, (int)L.n, ws->coarse_inv, b, x);
-        return smooth(ws, s, l, x, b, ws->coarse_iters, zero_init);
+            return kern::gemv(s, (int)L.n, ws->coarse_inv, b, x);
+        return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F,
+                                  ws->coarse_iters, zero_init, (const T*)nullptr, L.plan);
     }
     Level& C = ws->lv[l + 1];
-    MF3D_TRY(smooth(ws, s, l, x, b, ws->pre, zero_init));
-    MF3D_TRY(kern::residual_restrict(s, L.g, ws->bct, ws->pin, x, b, level_K(ws, l), C.b));
-    MF3D_TRY(vcycle(ws, s, l + 1, C.x, C.b, true));
-    return smooth(ws, s, l, x, b, ws->post, false, C.x);  // prolong_3d_add folded in
+    MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->pre,
+                                zero_init, (const T*)nullptr, L.plan));
+    MF3D_TRY(kern::residual_restrict_faces(s, L.g, ws->bct, ws->pin, 1.0 / (cs * dx2), x, b, F,
+                                           V::b(C)));
+    MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true));
+    return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->post,
+                              false, (const T*)V::x(C), L.plan);
+}
+
+// MultigridPreconditioner::apply (mg_preconditioner.cu:16-45): z = V-cycle(b = r, x = 0),
+// in FP64 or with the FP32 V-cycle between two conversions.
+int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z) {
+    Level& L0 = ws->lv[0];
+    if (ws->precision == MF3D_PRECOND_FP64) return vcycle<double>(ws, s, 0, z, r, true);
+    MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.n));
+    MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true));
+    return kern::to_f64(s, L0.x32, z, L0.n);
+}
+
+// operator on the finest level with its FP64 faces
+int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, double* partials) {
+    Level& L0 = ws->lv[0];
+    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 0, x, nullptr, y, partials);
+}
+int residual_op(mf3d_workspace* ws, hipStream_t s, const double* x, const double* b, double* r,
+                double* partials) {
+    Level& L0 = ws->lv[0];
+    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 1, x, b, r, partials);
 }
 
 IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
@@ -376,6 +489,7 @@ IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
     st.check_every = std::max(1, cfg.cg_check_every);
     st.max_iter = std::max(0, cfg.cg_max_iter);
     st.stop_norm = cfg.stop_norm;
+    st.flexible = cfg.flexible ? 1 : 0;
     st.den = 1.0;
     if (st.max_iter == 0) st.stop = 3;
     return st;
@@ -422,40 +536,62 @@ int state_upload(mf3d_workspace* ws, hipStream_t s, const IterState& st) {
 }
 
 // pcg_solve (src/numerics/solvers/pcg.cuh:95-185) with the MG preconditioner.
+// Extensions: flexible beta (r_new, z_new - z_old) / (r_old, z_old) for an
+// inexact (FP32) preconditioner, and, once the recursive residual meets the
+// tolerance, a true-residual check: if ||b - A x|| misses it, the iteration
+// restarts from the true residual (at most 3 times).
 int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             const mf3d_head_solve_config& cfg, const double* K, double* x,
             mf3d_head_solve_result* res) {
+    (void)K;
     const size_t n = cells(g);
     const unsigned np = kern::reduce_blocks(n);
+    const bool flexible = cfg.flexible != 0;
+    // the reference's own rule (||r||/||r0||, FP64) keeps its exact semantics
+    const bool judge_true =
+        cfg.stop_norm == MF3D_STOP_RHS_NORM || ws->precision == MF3D_PRECOND_FP32;
     IterState st0 = make_state(cfg, cfg.cg_rtol);
     MF3D_TRY(state_upload(ws, s, st0));
     MF3D_TRY(kern::dot_partials(s, ws->rhs, ws->rhs, n, ws->partials));
     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_BNORM, ws->st, nullptr));
-    MF3D_TRY(kern::op_residual(s, g, ws->bct, ws->pin, K, x, ws->rhs, ws->r, ws->partials));
+    MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials));
     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_R0, ws->st, nullptr));
-    MF3D_TRY(vcycle(ws, s, 0, ws->z, ws->r, true));
-    MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
-    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ0, ws->st, nullptr));
-    MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
-    MF3D_TRY(read_state(ws, s));
-    Chunker ch;
-    if (cfg.stop_norm == MF3D_STOP_INITIAL_RESIDUAL && cfg.cg_check_every > 1)
-        ch.fixed = cfg.cg_check_every;
-    const double target_rel = cfg.cg_rtol;
-    while (!ws->host_st->stop) {
-        const double target = target_rel * ws->host_st->den;
-        const int chunk = ch.next(*ws->host_st, target);
-        for (int c = 0; c < chunk; ++c) {
-            MF3D_TRY(kern::apply(s, g, ws->bct, ws->pin, K, ws->p, ws->Ap, ws->partials));
-            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_PAP, ws->st, nullptr));
-            MF3D_TRY(kern::pcg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
-            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RR, ws->st, nullptr));
-            MF3D_TRY(vcycle(ws, s, 0, ws->z, ws->r, true));
-            MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
-            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
-            MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
-        }
+    int restarts = 0;
+    for (;;) {
+        MF3D_TRY(precondition(ws, s, ws->r, ws->z));
+        MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
+        MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ0, ws->st, nullptr));
+        MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
         MF3D_TRY(read_state(ws, s));
+        Chunker ch;
+        if (cfg.stop_norm == MF3D_STOP_INITIAL_RESIDUAL && cfg.cg_check_every > 1)
+            ch.fixed = cfg.cg_check_every;
+        while (!ws->host_st->stop) {
+            const double target = cfg.cg_rtol * ws->host_st->den;
+            const int chunk = ch.next(*ws->host_st, target);
+            for (int c = 0; c < chunk; ++c) {
+                MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials));
+                MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_PAP, ws->st, nullptr));
+                MF3D_TRY(kern::pcg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
+                MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RR, ws->st, nullptr));
+                if (flexible) {  // (r_new, z_old) before z is overwritten
+                    MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
+                    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZO, ws->st, nullptr));
+                }
+                MF3D_TRY(precondition(ws, s, ws->r, ws->z));
+                MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
+                MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
+                MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
+            }
+            MF3D_TRY(read_state(ws, s));
+        }
+        if (ws->host_st->stop != 1 || restarts >= 3 || !judge_true) break;
+        // converged on the recursive residual: judge by the true one
+        MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials));
+        MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RESUME, ws->st, nullptr));
+        MF3D_TRY(read_state(ws, s));
+        if (ws->host_st->stop == 1) break;  // the true residual meets the tolerance
+        ++restarts;
     }
     const IterState& st = *ws->host_st;
     res->num_iterations = st.iter;
@@ -463,6 +599,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     res->initial_residual = st.r0;
     res->final_residual = std::sqrt(st.rr);
     res->bnorm = st.bnorm;
+    res->restarts = restarts;
     return st.stop == 2 ? fail(MF3D_ERR_BREAKDOWN, "PCG breakdown: (p, Ap) not finite") : MF3D_OK;
 }
 
@@ -479,7 +616,7 @@ int run_cg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_BNORM, ws->st, nullptr));
     double* bneg = ws->z;
     MF3D_TRY(kern::scale_copy(s, -1.0, ws->rhs, bneg, n));
-    MF3D_TRY(kern::apply(s, g, ws->bct, ws->pin, K, x, ws->r, nullptr));
+    MF3D_TRY(apply_op(ws, s, x, ws->r, ws->partials));
     MF3D_TRY(kern::scal(s, ws->r, n, -1.0));
     MF3D_TRY(kern::axpby(s, 1.0, bneg, -1.0, ws->r, n));
     MF3D_TRY(kern::copy(s, ws->r, ws->p, n));
@@ -492,7 +629,7 @@ int run_cg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     while (!ws->host_st->stop) {
         const int chunk = ch.next(*ws->host_st, cfg.cg_rtol * ws->host_st->den);
         for (int c = 0; c < chunk; ++c) {
-            MF3D_TRY(kern::apply(s, g, ws->bct, ws->pin, K, ws->p, ws->Ap, ws->partials));
+            MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials));
             MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_PAP, ws->st, nullptr));
             MF3D_TRY(kern::cg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
             MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_RR, ws->st, nullptr));
@@ -533,7 +670,7 @@ int run_mg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     res->bnorm = bnorm;
     for (int c = 0; c < cfg.mg_max_cycles; ++c) {
         res->num_iterations = c + 1;
-        MF3D_TRY(vcycle(ws, s, 0, x, ws->rhs, false));
+        MF3D_TRY(vcycle<double>(ws, s, 0, x, ws->rhs, false));
         MF3D_TRY(kern::residual(s, g, ws->bct, ws->pin, x, ws->rhs, K, ws->r));
         MF3D_TRY(host_nrm2(ws, s, ws->r, n, &rn));
         res->final_residual = rn;
@@ -636,8 +773,10 @@ static int check_config(const mf3d_head_solve_config* cfg) {
         return fail(MF3D_ERR_INVALID_ARGUMENT, "negative iteration count or mg_levels < 1");
     if (cfg->precond_precision != MF3D_PRECOND_FP64 && cfg->precond_precision != MF3D_PRECOND_FP32)
         return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown precond_precision");
-    if (cfg->precond_precision == MF3D_PRECOND_FP32)
-        return fail(MF3D_ERR_UNSUPPORTED, "FP32 preconditioner not built in this library");
+    if (cfg->precond_precision == MF3D_PRECOND_FP32 && cfg->solver_type != MF3D_SOLVER_PCG_MG)
+        return fail(MF3D_ERR_UNSUPPORTED, "the FP32 V-cycle serves the PCG_MG solver only");
+    if (cfg->coef_format < MF3D_COEF_FP32 || cfg->coef_format > MF3D_COEF_FP16)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown coef_format");
     return MF3D_OK;
 }
 
@@ -677,8 +816,7 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     if (st != MF3D_OK) return st;
     MF3D_HIP_TRY(hipEventRecord(ws->ev[2], s));
     if (cfg->true_residual) {
-        MF3D_TRY(kern::op_residual(s, *grid, ws->bct, ws->pin, K, h, ws->rhs, nullptr,
-                                   ws->partials));
+        MF3D_TRY(residual_op(ws, s, h, ws->rhs, nullptr, ws->partials));
         MF3D_TRY(kern::finalize(s, ws->partials, kern::reduce_blocks(nc), FIN_SQRT, nullptr,
                                 ws->dscalar));
         MF3D_HIP_TRY(hipMemcpyAsync(&result->true_residual, ws->dscalar, sizeof(double),
@@ -841,6 +979,17 @@ int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
     return mg_setup(ws, as_stream(stream), *grid, *bc, *cfg, K);
 }
 
+int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
+    MF3D_CHECK_PTR(ws);
+    switch (key) {
+    case MF3D_TUNE_MARCH_MIN_CELLS_FP64: ws->march_min_cells[0] = value; break;
+    case MF3D_TUNE_MARCH_MIN_CELLS_FP32: ws->march_min_cells[1] = value; break;
+    case MF3D_TUNE_ZCHUNK: ws->zchunk = (int)value; break;
+    default: return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown tuning key");
+    }
+    return MF3D_OK;
+}
+
 int mf3d_mg_num_levels(const mf3d_workspace* ws) {
     return ws && ws->ready ? (int)ws->lv.size() : 0;
 }
@@ -854,7 +1003,7 @@ int mf3d_mg_precondition(mf3d_workspace* ws, mf3d_stream stream, const double* r
         return fail(MF3D_ERR_INVALID_ARGUMENT, "mf3d_mg_setup has not been called");
     MF3D_CHECK_LEN(n, cells(ws->fine), "r/z");
     clear_stale_error();
-    return vcycle(ws, as_stream(stream), 0, z, r, true);
+    return precondition(ws, as_stream(stream), r, z);
 }
 
 int mf3d_varcoeff_apply(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, int pin,

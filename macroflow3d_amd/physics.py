@@ -77,6 +77,7 @@ class HeadSolveConfig:
     precond_precision: PrecondPrecision = PrecondPrecision.FP64
     coef_format: CoefFormat = CoefFormat.BF16
     true_residual: bool = True
+    flexible: bool = False
 
     @staticmethod
     def from_yaml(flow: dict) -> "HeadSolveConfig":
@@ -104,6 +105,17 @@ class HeadSolveConfig:
                                init_guess=InitGuess.ZERO, coarse_solver=CoarseSolver.DIRECT,
                                stop_norm=StopNorm.RHS_NORM)
 
+    @staticmethod
+    def mixed(rtol: float = 1e-10, mg_levels: int = 6, coef: "CoefFormat" = None,
+              flexible: bool = False) -> "HeadSolveConfig":
+        """north_star() with the FP32 V-cycle and 16-bit (default BF16) face coefficients;
+        the PCG, its dot products, the operator and the coarsest solve stay FP64."""
+        c = HeadSolveConfig.north_star(rtol, mg_levels)
+        c.precond_precision = PrecondPrecision.FP32
+        c.coef_format = CoefFormat.BF16 if coef is None else coef
+        c.flexible = flexible
+        return c
+
     def c(self) -> _lib.mf3d_head_solve_config:
         o = _lib.mf3d_head_solve_config()
         _lib.lib().mf3d_default_config(C.byref(o))
@@ -124,6 +136,7 @@ class HeadSolveConfig:
         o.precond_precision = int(self.precond_precision)
         o.coef_format = int(self.coef_format)
         o.true_residual = 1 if self.true_residual else 0
+        o.flexible = 1 if self.flexible else 0
         return o
 
 
@@ -138,6 +151,7 @@ class HeadSolveResult:
     true_residual: float = -1.0
     setup_ms: float = 0.0
     solve_ms: float = 0.0
+    restarts: int = 0
 
     @property
     def relative_true_residual(self) -> float:
@@ -162,6 +176,15 @@ class FlowWorkspace:
 
     def bytes(self) -> int:
         return int(_lib.lib().mf3d_workspace_bytes(self._h))
+
+    TUNE_MARCH_MIN_CELLS_FP64 = 1
+    TUNE_MARCH_MIN_CELLS_FP32 = 2
+    TUNE_ZCHUNK = 3
+
+    def set_tuning(self, key: int, value: int) -> None:
+        """Smoother selection knobs (levels with >= value cells use the one-pass
+        red-black kernel; z-chunk of that kernel); applied at the next set-up."""
+        _lib.call("mf3d_workspace_set_tuning", self._h, int(key), int(value))
 
     def clear(self) -> None:
         self.close()
@@ -193,7 +216,7 @@ def solve_head(h: torch.Tensor, K: torch.Tensor, grid: Grid3D, bc: BCSpec,
               C.byref(cfg.c()), dptr(K, n, "K", ctx), dptr(h, n, "h", ctx), n, C.byref(res))
     return HeadSolveResult(res.num_iterations, res.initial_residual, res.final_residual,
                            bool(res.converged), res.bnorm, res.true_residual, res.setup_ms,
-                           res.solve_ms)
+                           res.solve_ms, res.restarts)
 
 
 def init_head_guess(h: torch.Tensor, grid: Grid3D, bc: BCSpec, ctx: HipContext) -> None:

@@ -86,7 +86,9 @@ def test_vcycle_fast_levels_match_oracle(ctx, workspace, pre_post):
     pre, post = pre_post
     cfg = ph.HeadSolveConfig(mg_levels=4, mg_pre_smooth=pre, mg_post_smooth=post,
                              coarse_solver=ph.CoarseSolver.DIRECT)
-    M = nm.MultigridPreconditioner.build(ctx, workspace, g, bc, cfg, dev(K, ctx))
+    ws = ph.FlowWorkspace()
+    ws.set_tuning(ph.FlowWorkspace.TUNE_MARCH_MIN_CELLS_FP64, 0)  # march on 64^3 and 32^3
+    M = nm.MultigridPreconditioner.build(ctx, ws, g, bc, cfg, dev(K, ctx))
     z = ctx.empty(n)
     M.apply(ctx, dev(r, ctx), z)
     p = O.SolveParams(mg_levels=4, mg_pre_smooth=pre, mg_post_smooth=post, coarse_solver=1)
@@ -360,3 +362,63 @@ def test_large_grid_properties_256cubed(ctx):
     V = v.V.view(n, n + 1, n)
     assert float(V[:, 0, :].abs().max()) == 0.0 and float(V[:, n, :].abs().max()) == 0.0
     ws.close()
+
+
+@pytest.mark.parametrize("coef,tol", [(ph.CoefFormat.FP32, 1e-5), (ph.CoefFormat.BF16, 2e-2),
+                                      (ph.CoefFormat.FP16, 5e-3)])
+@pytest.mark.parametrize("march", [False, True])
+def test_mixed_vcycle_close_to_fp64_oracle(ctx, coef, tol, march):
+    """FP32 V-cycle with 32/16-bit face coefficients: the preconditioned vector
+    stays within the coefficient format's precision of the FP64 oracle V-cycle."""
+    g, og = grid_pair(64, 64, 32)
+    bc, obc = bc_pair("darcy_x")
+    n = g.num_cells()
+    K = random_K(n, 31, 2.0)
+    r = np.random.default_rng(32).standard_normal(n)
+    cfg = ph.HeadSolveConfig(mg_levels=4, coarse_solver=ph.CoarseSolver.DIRECT,
+                             precond_precision=ph.PrecondPrecision.FP32, coef_format=coef)
+    ws = ph.FlowWorkspace()
+    ws.set_tuning(ph.FlowWorkspace.TUNE_MARCH_MIN_CELLS_FP32, 0 if march else 1 << 40)
+    M = nm.MultigridPreconditioner.build(ctx, ws, g, bc, cfg, dev(K, ctx))
+    z = ctx.empty(n)
+    M.apply(ctx, dev(r, ctx), z)
+    zo = O.v_cycle_precond(og, obc, O.SolveParams(mg_levels=4, coarse_solver=1), K, r)
+    assert rel_l2(host(z), zo) < tol
+
+
+@pytest.mark.parametrize("coef", [ph.CoefFormat.BF16, ph.CoefFormat.FP16, ph.CoefFormat.FP32])
+@pytest.mark.parametrize("flexible", [False, True])
+def test_mixed_precision_solve_meets_fp64_tolerance(ctx, coef, flexible):
+    """Mixed-precision MG-PCG reaches ||b - A h|| <= 1e-10 ||b|| (true residual,
+    FP64) and the head matches the FP64 oracle to 1e-8 relative L2."""
+    from macroflow3d_amd import kfield
+    n = 64
+    K = np.exp(kfield.lognormal_lnK(n, n, n, 4.0, 8.0, 41)[0].numpy())
+    g, og = grid_pair(n, n, n)
+    bc, obc = bc_pair("darcy_x")
+    ws = ph.FlowWorkspace()
+    h = ctx.zeros(g.num_cells())
+    res = ph.solve_head(h, dev(K, ctx), g, bc, ph.HeadSolveConfig.mixed(1e-10, 4, coef, flexible),
+                        ctx, ws)
+    assert res.converged and res.true_residual / res.bnorm <= 1e-10
+    hd, _ = O.solve_head(og, obc, O.mode_n(mg_levels=4, cg_rtol=1e-13), K)
+    assert rel_l2(host(h), hd) < 1e-8
+
+
+def test_fp64_march_and_colour_plans_agree(ctx):
+    """The smoother choice (one-pass march or two colour passes) does not change
+    the FP64 solve beyond rounding."""
+    from macroflow3d_amd import kfield
+    n = 64
+    K = torch.exp(kfield.lognormal_lnK(n, n, n, 2.0, 6.0, 43)[0]).to(ctx.device)
+    g = grid_pair(n, n, n)[0]
+    bc = bc_pair("darcy_x")[0]
+    out = []
+    for thr in (0, 1 << 40):
+        ws = ph.FlowWorkspace()
+        ws.set_tuning(ph.FlowWorkspace.TUNE_MARCH_MIN_CELLS_FP64, thr)
+        h = ctx.zeros(g.num_cells())
+        res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.north_star(1e-10, 4), ctx, ws)
+        out.append((host(h), res.num_iterations))
+    assert out[0][1] == out[1][1]
+    assert rel_l2(out[0][0], out[1][0]) < 1e-12
