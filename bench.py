@@ -1,23 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark: time-to-1e-10 Darcy head solve at 256^3 on one MI355X.
 
-One step = one complete head solve (hierarchy coefficients, RHS, MG-PCG to
-||r||/||b|| <= 1e-10) plus the face velocities, on a seeded log-normal K field
-(sigma2_lnK = 4, correlation length 8 cells) already resident in HBM.
+One step = one complete head solve to ||b - A h|| <= 1e-10 ||b|| (face
+coefficients of every level, RHS, MG-PCG) plus the face velocities, on a seeded
+log-normal K field (sigma2_lnK = 4, correlation length 8 cells) resident in HBM.
+
+Headline method: FP64 PCG with the FP32 V-cycle and BF16 face coefficients
+(the dots, the operator, the coarsest 8^3 solve and the true-residual check
+stay FP64). The plain FP64 MG-PCG is timed on the same field as the
+denominator, and the mixed head/velocity are compared with it.
 
 Also measured here:
-  * roofline of the dominant kernel (fine-level GSRB half-sweep): algorithmic
-    bytes per launch / average launch time from HIP events on the solve stream;
+  * roofline of the dominant kernel (the finest level's GSRB colour sweep of
+    the headline V-cycle): algorithmic bytes per launch / average launch time
+    from HIP events on the solve stream; HBM traffic per launch from the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries committed under profiles/;
   * CPU baseline: the CPU FP64 oracle (OpenMP) on the same K field, a bounded
-    sample of PCG iterations extrapolated to the GPU's iteration count.
+    sample of PCG iterations extrapolated to the FP64 solve's iteration count.
 
-Prints one JSON line.
+Prints one JSON line. `--summarize-pmc DIR COUNTER` instead prints the
+per-dispatch counter totals of a rocprofv3 --pmc run of this script.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,6 +36,8 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "time-to-1e-10 Darcy solve at 256^3, mixed+warm-start vs FP64; smoother HBM GB/s"
+# HBM traffic of the dominant kernel, from the committed FETCH_SIZE/WRITE_SIZE summary
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_hbm_traffic_headsolve_256.md")
 
 
 def parse():
@@ -34,22 +45,74 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="cells per axis")
     ap.add_argument("--sigma2", type=float, default=4.0)
     ap.add_argument("--corr", type=float, default=8.0)
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--coef", choices=["bf16", "fp16", "fp32"], default="bf16")
     ap.add_argument("--smoother-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"))
+    ap.add_argument("--summarize-trace", metavar="DIR")
     return ap.parse_args()
 
 
 def log(args, *a):
     if not args.quiet:
         print(*a, file=sys.stderr, flush=True)
+
+
+def summarize_pmc(d: str, counter: str) -> int:
+    """Per-kernel, per-dispatch counter value (KiB) of a rocprofv3 --pmc run."""
+    import glob
+    import sqlite3
+    dbs = glob.glob(os.path.join(d, "**", "*_results.db"), recursive=True)
+    if not dbs:
+        print(f"no rocprofv3 results under {d}")
+        return 1
+    cur = sqlite3.connect(dbs[0]).cursor()
+    rows = cur.execute(
+        "select kernel_name, grid_size, count(distinct dispatch_id), sum(value) "
+        "from counters_collection where counter_name = ? group by kernel_name, grid_size "
+        "order by sum(value) desc", (counter,)).fetchall()
+    for name, grid, nd, tot in rows:
+        if name.startswith("mf3d::") or name.startswith("void mf3d::"):
+            print(f"{counter}\t{tot / max(nd, 1):.1f} KiB/dispatch\tdispatches={nd}\t"
+                  f"grid={grid}\t{name[:110]}")
+    return 0
+
+
+def summarize_trace(d: str) -> int:
+    """Per-kernel, per-grid launch count and average duration of a rocprofv3
+    --kernel-trace --output-format csv run of this script."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
+    if not files:
+        print(f"no kernel_trace.csv under {d}")
+        return 1
+    acc = {}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            name = row["Kernel_Name"]
+            if not (name.startswith("mf3d::") or name.startswith("void mf3d::")):
+                continue
+            grid = 1
+            for ax in ("X", "Y", "Z"):
+                grid *= int(row.get(f"Grid_Size_{ax}", 1) or 1)
+            dur = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+            c = acc.setdefault((name, grid), [0, 0])
+            c[0] += 1
+            c[1] += dur
+    for (name, grid), (cnt, tot) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+        print(f"{tot / cnt / 1e3:10.2f} us avg\tcalls={cnt}\tgrid={grid}\ttotal={tot / 1e6:.3f} ms"
+              f"\t{name[:110]}")
+    return 0
 
 
 def cpu_baseline(args, K_host, gpu_iters):
@@ -68,24 +131,44 @@ def cpu_baseline(args, K_host, gpu_iters):
         log(args, f"cpu oracle: {its} PCG iterations in {times[-1]:.2f} s")
     per_iter = max(times[1] - times[0], 1e-9) / args.cpu_iters
     est = times[0] + (gpu_iters - 1) * per_iter
-    return {"value": est * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+    return {"value": round(est * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
             "sample": (f"CPU FP64 oracle (OpenMP, {threads} threads) on the same {args.n}^3 K: "
                        f"setup+1 and setup+{1 + args.cpu_iters} PCG iterations timed "
-                       f"({times[0]:.2f} s, {times[1]:.2f} s), extrapolated to the GPU's "
-                       f"{gpu_iters} iterations"),
-            "per_iteration_ms": per_iter * 1e3, "setup_plus_first_iteration_ms": times[0] * 1e3}
+                       f"({times[0]:.2f} s, {times[1]:.2f} s), extrapolated to the plain FP64 "
+                       f"GPU solve's {gpu_iters} iterations"),
+            "per_iteration_ms": round(per_iter * 1e3, 1),
+            "setup_plus_first_iteration_ms": round(times[0] * 1e3, 1)}
+
+
+def committed_traffic(n: int, coef: str):
+    """HBM bytes per launch of the dominant kernel from the committed PMC summary:
+    the line `traffic_bytes_per_launch n=<n> coef=<coef>: <bytes>`."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                m = re.match(r"\s*traffic_bytes_per_launch n=(\d+) coef=(\w+):\s*([0-9.eE+]+)", line)
+                if m and int(m.group(1)) == n and m.group(2) == coef:
+                    return float(m.group(3))
+    except OSError:
+        pass
+    return None
 
 
 def main():
     args = parse()
+    if args.summarize_pmc:
+        return summarize_pmc(*args.summarize_pmc)
+    if args.summarize_trace:
+        return summarize_trace(args.summarize_trace)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only: the head solve of one "
-                                   "realisation is not sharded (see DESIGN.md, sharding)",
+        print(json.dumps({"error": "this build runs on one GPU only: one realisation's head "
+                                   "solve is not sharded across GPUs",
                           "n_gpus": args.gpus}))
         return 2
     import torch
 
     from macroflow3d_amd import multigrid as mg
+    from macroflow3d_amd import numerics as nm
     from macroflow3d_amd import physics as ph
     from macroflow3d_amd.core import BCSpec, Grid3D
     from macroflow3d_amd.kfield import lognormal_lnK
@@ -104,89 +187,120 @@ def main():
     log(args, f"K field {n}^3 generated in {time.perf_counter() - t0:.2f} s, embedding "
               f"{kst['embedding']}, min eig ratio {kst['min_eig_ratio']:.2e}")
 
-    cfg = ph.HeadSolveConfig.north_star(args.rtol, args.levels)
+    # counter calibration: a plain copy of 2^27 doubles (1 GiB read, 1 GiB write)
+    ca = torch.ones(1 << 27, dtype=torch.float64, device=ctx.device)
+    cb = torch.empty_like(ca)
+    nm.blas.copy(ctx, ca, cb)
+    torch.cuda.synchronize()
+    del ca, cb
+
+    coef = {"bf16": ph.CoefFormat.BF16, "fp16": ph.CoefFormat.FP16,
+            "fp32": ph.CoefFormat.FP32}[args.coef]
+    cfg64 = ph.HeadSolveConfig.north_star(args.rtol, args.levels)
+    cfgmx = ph.HeadSolveConfig.mixed(args.rtol, args.levels, coef)
+
+    def timed(cfg, steps, warmup, h, vel, ws):
+        res = None
+        for _ in range(warmup):
+            res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+            ph.compute_velocity_from_head(vel, h, K, g, bc, ctx)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+            ph.compute_velocity_from_head(vel, h, K, g, bc, ctx)
+        torch.cuda.synchronize()
+        return res, (time.perf_counter() - t) * 1e3 / max(1, steps)
+
+    # plain FP64 MG-PCG (denominator and accuracy reference)
+    ws64 = ph.FlowWorkspace()
+    h64 = ctx.zeros(N)
+    v64 = ph.VelocityField(g, ctx)
+    r64, ms64 = timed(cfg64, args.fp64_steps, 1 if args.fp64_steps else 0, h64, v64, ws64) \
+        if args.fp64_steps else (ph.solve_head(h64, K, g, bc, cfg64, ctx, ws64), float("nan"))
+    if not args.fp64_steps:
+        ph.compute_velocity_from_head(v64, h64, K, g, bc, ctx)
+    ws64.close()
+    log(args, f"fp64: {r64.num_iterations} iterations, rel true residual "
+              f"{r64.true_residual / r64.bnorm:.3e}, {ms64:.2f} ms per solve")
+
+    # headline: mixed precision
     ws = ph.FlowWorkspace()
     h = ctx.zeros(N)
     vel = ph.VelocityField(g, ctx)
+    res, ms = timed(cfgmx, args.steps, args.warmup, h, vel, ws)
+    log(args, f"mixed ({args.coef}): {res.num_iterations} iterations, rel true residual "
+              f"{res.true_residual / res.bnorm:.3e}, setup {res.setup_ms:.2f} ms, iteration "
+              f"{res.solve_ms:.2f} ms, step {ms:.2f} ms")
 
-    def step():
-        res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
-        ph.compute_velocity_from_head(vel, h, K, g, bc, ctx)
-        return res
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
-    for _ in range(args.warmup):
-        res = step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    results = []
-    for _ in range(args.steps):
-        results.append(step())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / max(1, args.steps)
-    res = results[-1]
-    log(args, f"solve: {res.num_iterations} iterations, rel true residual "
-              f"{res.true_residual / res.bnorm:.3e}, setup {res.setup_ms:.2f} ms, "
-              f"iteration {res.solve_ms:.2f} ms, step {ms_per_step:.2f} ms")
+    head_err = rel(h, h64)
+    vel_err = max(rel(vel.U, v64.U), rel(vel.V, v64.V), rel(vel.W, v64.W))
 
-    # dominant kernel: fine-level GSRB, one launch per colour (2 per iteration)
-    x = h.clone()
-    b = ctx.empty(N)
-    ph.build_rhs_head(b, K, g, bc, ctx)
-    mg.gsrb_smooth_3d(ctx, g, x, b, K, 1, bc)
+    # dominant kernel: finest-level colour sweep of the headline V-cycle,
+    # timed in place on the workspace's own level-0 buffers
+    mg.smooth_level(ctx, ws, 0, 1)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(ctx.stream)
-    for _ in range(args.smoother_reps):
-        mg.gsrb_smooth_3d(ctx, g, x, b, K, 1, bc)
+    mg.smooth_level(ctx, ws, 0, args.smoother_reps)
     e1.record(ctx.stream)
     e1.synchronize()
-    launches = 2 * args.smoother_reps
-    t_launch_ms = e0.elapsed_time(e1) / launches
-    alg_bytes = 20.0 * N  # K 8N + opposite-colour x 4N + own-colour x 4N + own-colour b 4N
+    t_launch_ms = e0.elapsed_time(e1) / (2 * args.smoother_reps)
+    # one colour (half-sweep) of the mixed V-cycle: 3 faces x 2 B + opposite-
+    # colour x 2 B + own-colour x write 2 B + own-colour b 2 B per cell
+    alg_bytes = {"bf16": 12.0, "fp16": 12.0, "fp32": 18.0}[args.coef] * N
     achieved = alg_bytes / (t_launch_ms * 1e-3) / 1e9
-    log(args, f"smoother: {t_launch_ms * 1e3:.1f} us per half-sweep, {achieved:.0f} GB/s "
+    log(args, f"smoother: {t_launch_ms * 1e3:.1f} us per colour sweep, {achieved:.0f} GB/s "
               f"algorithmic")
+    traffic = committed_traffic(n, args.coef)
 
     out = {
         "metric": METRIC,
-        "value": round(ms_per_step, 3),
+        "value": round(ms, 3),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(ms, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": f"f64 outer / f32 V-cycle / {args.coef} coefficients",
         "data": "synthetic (seeded log-normal K, FFT circulant embedding)",
         "config": {
-            "workload": f"plain FP64 MG-PCG head solve + face velocities, {n}^3 cells, "
-                        f"sigma2_lnK={args.sigma2:g}, corr_len={args.corr:g} cells, "
-                        f"{args.levels} levels (coarsest {n >> (args.levels - 1)}^3 direct), "
-                        f"V(2,2) GSRB, ||r||/||b|| <= {args.rtol:g}",
-            "cells": N, "seed": args.seed, "precision": "fp64", "warm_start": "zero guess",
+            "workload": f"head solve + face velocities, {n}^3 cells, sigma2_lnK={args.sigma2:g}, "
+                        f"corr_len={args.corr:g} cells, {args.levels} levels (coarsest "
+                        f"{n >> (args.levels - 1)}^3 direct FP64), V(2,2) GSRB, "
+                        f"||b - A h|| <= {args.rtol:g} ||b||",
+            "cells": N, "seed": args.seed, "method": "mixed-precision MG-PCG",
+            "warm_start": "zero guess",
         },
         "iterations": res.num_iterations,
-        "converged": res.converged,
         "rel_true_residual": res.true_residual / res.bnorm,
         "setup_ms": round(res.setup_ms, 3),
         "iteration_ms": round(res.solve_ms, 3),
+        "fp64_ms": round(ms64, 3),
+        "fp64_iterations": r64.num_iterations,
+        "speedup_vs_fp64": round(ms64 / ms, 3) if ms > 0 else None,
+        "head_rel_l2_vs_fp64": head_err,
+        "velocity_rel_l2_vs_fp64": vel_err,
         "roofline": {
-            "kernel": "gsrb_colour_kernel (fine level, one colour)",
+            "kernel": f"colour_kernel<float,{args.coef}> (finest level, one colour)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": alg_bytes,
             "avg_launch_us": round(t_launch_ms * 1e3, 2),
         },
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, K.cpu().numpy(), res.num_iterations)
+        out["cpu_baseline"] = cpu_baseline(args, K.cpu().numpy(), r64.num_iterations)
     print(json.dumps(out), flush=True)
     ws.close()
     return 0

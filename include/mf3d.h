@@ -199,6 +199,12 @@ enum {
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 
+/* Extension: num_iters GSRB iterations on level `level` of the current
+ * hierarchy, in place on the workspace's own correction and right-hand side
+ * of that level (level 0: the preconditioner's input/output buffers), in the
+ * set-up's precision. For timing and testing the smoother in situ. */
+int mf3d_mg_smooth(mf3d_workspace* ws, mf3d_stream stream, int level, int num_iters);
+
 /* Number of levels of the current hierarchy (0 before set-up). */
 int mf3d_mg_num_levels(const mf3d_workspace* ws);
 

@@ -32,6 +32,20 @@ namespace {
 
 constexpr int kBlock = 256;
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+// grid of a 1-D launch remapped by xcd_block: padded to a multiple of 8 blocks
+inline unsigned xcd_grid(size_t n) { return ((grid_for(n) + 7u) / 8u) * 8u; }
+
+// Workgroups are dealt round-robin to the 8 XCDs (blockIdx % 8), each with its
+// own L2. This maps the launch so that the blocks of one XCD cover a
+// contiguous range of the logical grid (a slab of rows and planes), keeping
+// the y- and z-neighbour rows of a stencil in that XCD's L2. Returns -1 for the
+// padding blocks of a grid rounded up to a multiple of 8.
+__device__ __forceinline__ long xcd_block(long nblocks) {
+    const long per = (nblocks + 7) >> 3;
+    const long b = blockIdx.x;
+    const long t = (b & 7) * per + (b >> 3);
+    return t < nblocks ? t : -1;
+}
 
 // ---- coefficient storage conversions ---------------------------------------
 __device__ __forceinline__ double cload(const double* p, long i) { return p[i]; }
@@ -70,8 +84,11 @@ __global__ void __launch_bounds__(kBlock)
     faces_build_kernel(Dims d, BcTypes bc, double cs, const double* __restrict__ K,
                        TC* __restrict__ FX, TC* __restrict__ FY, TC* __restrict__ FZ) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
-    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= (long)nx * ny * nz) return;
+    const long n = (long)nx * ny * nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long o = blk * kBlock + threadIdx.x;
+    if (o >= n) return;
     const int i = (int)(o % nx);
     const long q = o / nx;
     const int j = (int)(q % ny);
@@ -158,8 +175,11 @@ __global__ void __launch_bounds__(kBlock)
                   const TC* __restrict__ FY, const TC* __restrict__ FZ, int colour) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const int nxh = (nx + 1) >> 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nxh * ny * nz) return;
+    const long total = (long)nxh * ny * nz;
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long t = blk * kBlock + threadIdx.x;
+    if (t >= total) return;
     const int ii = (int)(t % nxh);
     const long row = t / nxh;
     const int j = (int)(row % ny);
@@ -421,8 +441,11 @@ __global__ void __launch_bounds__(kBlock)
                              const TC* __restrict__ FY, const TC* __restrict__ FZ,
                              T* __restrict__ bcoarse) {
     const int NX = d.nx >> 1, NY = d.ny >> 1, NZ = d.nz >> 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)NX * NY * NZ) return;
+    const long total = (long)NX * NY * NZ;
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long t = blk * kBlock + threadIdx.x;
+    if (t >= total) return;
     const int I = (int)(t % NX);
     const long q = t / NX;
     const int J = (int)(q % NY);
@@ -445,8 +468,11 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     prolong_kernel(Dims d, const T* __restrict__ c, T* __restrict__ f) {
-    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= (long)d.nx * d.ny * d.nz) return;
+    const long n = (long)d.nx * d.ny * d.nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long o = blk * kBlock + threadIdx.x;
+    if (o >= n) return;
     const int i = (int)(o % d.nx);
     const long q = o / d.nx;
     const int j = (int)(q % d.ny);
@@ -503,8 +529,10 @@ __global__ void __launch_bounds__(kBlock)
               const double* __restrict__ FX, const double* __restrict__ FY,
               const double* __restrict__ FZ, const double* __restrict__ b,
               double* __restrict__ y, double* __restrict__ partials) {
-    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)d.nx * d.ny * d.nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;  // padding block: uniform exit before the block reduction
+    const long o = blk * kBlock + threadIdx.x;
     double prod = 0.0;
     if (o < n) {
         const int i = (int)(o % d.nx);
@@ -522,7 +550,7 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     const double s = block_sum<kBlock>(prod);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    if (threadIdx.x == 0) partials[blk] = s;
 }
 
 template <typename T>
@@ -604,7 +632,7 @@ size_t coef_bytes(CoefType t) {
 int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
                 const double* K, FaceSet& f) {
     const Dims d = dims(g);
-    const unsigned gr = grid_for(cells(g));
+    const unsigned gr = xcd_grid(cells(g));
     switch (f.type) {
     case COEF_F64:
         faces_build_kernel<double><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (double*)f.fx,
@@ -658,7 +686,7 @@ int smooth_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, dou
         if (zero_init) {
             fill_t_kernel<T><<<grid_for(n), kBlock, 0, s>>>(x, (long)n);
         } else if (xc) {
-            prolong_kernel<T><<<grid_for(n), kBlock, 0, s>>>(d, xc, x);
+            prolong_kernel<T><<<xcd_grid(n), kBlock, 0, s>>>(d, xc, x);
         }
         MF3D_HIP_TRY(hipGetLastError());
         return MF3D_OK;
@@ -694,8 +722,8 @@ int smooth_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, dou
         if (in != x) MF3D_HIP_TRY(hipMemcpyAsync(x, in, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         return MF3D_OK;
     }
-    if (xc) prolong_kernel<T><<<grid_for(n), kBlock, 0, s>>>(d, xc, x);
-    const unsigned gh = grid_for((size_t)((g.nx + 1) / 2) * g.ny * g.nz);
+    if (xc) prolong_kernel<T><<<xcd_grid(n), kBlock, 0, s>>>(d, xc, x);
+    const unsigned gh = xcd_grid((size_t)((g.nx + 1) / 2) * g.ny * g.nz);
     for (int it = 0; it < iters; ++it)
         for (int colour = 0; colour < 2; ++colour) {
             if (it == 0 && colour == 0 && zero_init)
@@ -713,7 +741,7 @@ template <typename T, typename TC>
 int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double rs, const T* x,
          const T* b, const FaceSet& F, T* bcoarse) {
     const size_t nc = (size_t)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
-    residual_restrict_kernel<T, TC><<<grid_for(nc), kBlock, 0, s>>>(
+    residual_restrict_kernel<T, TC><<<xcd_grid(nc), kBlock, 0, s>>>(
         dims(g), bc, pin ? 1 : 0, (T)rs, x, b, cptr<TC>(F, 0), cptr<TC>(F, 1), cptr<TC>(F, 2),
         bcoarse);
     MF3D_HIP_TRY(hipGetLastError());
@@ -772,10 +800,10 @@ int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, con
     const double* FY = cptr<double>(F, 1);
     const double* FZ = cptr<double>(F, 2);
     if (mode == 0)
-        op_kernel<0><<<grid_for(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
+        op_kernel<0><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
                                                     FZ, b, y, partials);
     else
-        op_kernel<1><<<grid_for(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
+        op_kernel<1><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
                                                     FZ, b, y, partials);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;

@@ -119,7 +119,7 @@ struct mf3d_workspace {
     const double* K0 = nullptr;
     bool ready = false;
     // tuning (mf3d_workspace_set_tuning / autotuner)
-    long march_min_cells[2] = {1L << 21, 1L << 21};  // FP64, FP32 V-cycle
+    long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
     int zchunk = 0;
     size_t bytes = 0;
     std::vector<void*> owned;
@@ -988,6 +988,26 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     default: return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
     return MF3D_OK;
+}
+
+int mf3d_mg_smooth(mf3d_workspace* ws, mf3d_stream stream, int level, int num_iters) {
+    MF3D_CHECK_PTR(ws);
+    if (!ws->ready || !ws->K0)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "mf3d_mg_setup has not been called");
+    if (level < 0 || level >= (int)ws->lv.size())
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "level out of range");
+    if (num_iters < 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "num_iters < 0");
+    clear_stale_error();
+    hipStream_t s = as_stream(stream);
+    Level& L = ws->lv[level];
+    const double dx2 = L.g.dx * L.g.dx;
+    if (ws->precision == MF3D_PRECOND_FP32)
+        return kern::smooth_faces(s, L.g, ws->bct, ws->pin, L.cs * dx2, L.x32, L.t32, L.b32,
+                                  L.flo, num_iters, false, (const float*)nullptr, L.plan);
+    double* x = level == 0 ? ws->z : L.x;
+    const double* b = level == 0 ? ws->r : L.b;
+    return kern::smooth_faces(s, L.g, ws->bct, ws->pin, dx2, x, L.t, b, L.f64, num_iters, false,
+                              (const double*)nullptr, L.plan);
 }
 
 int mf3d_mg_num_levels(const mf3d_workspace* ws) {

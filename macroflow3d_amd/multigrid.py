@@ -113,3 +113,9 @@ def v_cycle(ctx: HipContext, workspace, r: torch.Tensor, z: torch.Tensor) -> Non
     n = r.numel()
     _lib.call("mf3d_mg_precondition", workspace.handle, ctx.handle, dptr(r, n, "r", ctx),
               dptr(z, n, "z", ctx), n)
+
+
+def smooth_level(ctx: HipContext, workspace, level: int, num_iters: int) -> None:
+    """num_iters GSRB iterations on one level of the workspace's hierarchy, in
+    place on that level's own buffers (timing and in-situ tests)."""
+    _lib.call("mf3d_mg_smooth", workspace.handle, ctx.handle, int(level), int(num_iters))
