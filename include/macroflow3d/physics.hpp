@@ -1,0 +1,351 @@
+// C++ host surface of libmf3d_hip: the flow physics.
+//
+//   ScalarField / KField / HeadField        src/physics/common/fields.cuh:37-92
+//   VelocityField                            src/physics/common/fields.cuh:108-212
+//   PaddedVelocityField                      src/physics/common/fields.cuh:248-319
+//   FlowWorkspace                            src/physics/common/workspaces.cuh:111-167
+//   HeadSolverType/HeadSolveConfig/Result    src/physics/flow/solve_head.cuh:52-141
+//   solve_head, init_head_guess              src/physics/flow/solve_head.cuh:158-174
+//   build_rhs_head                           src/physics/flow/rhs_head.cuh:50-51
+//   coarsen_K                                src/physics/flow/coarsen_K.cuh:46-47
+//   compute_velocity_from_head (x2), compute_norm2/sum/mean, check_no_nans
+//                                            src/physics/flow/velocity_from_head.cuh:54-107
+//
+// HeadSolveConfig keeps the reference fields and defaults; the extension
+// fields after them default to the reference behaviour.
+#pragma once
+
+#include <cstdio>
+
+#include "numerics.hpp"
+
+namespace macroflow3d {
+
+// ---- fields -------------------------------------------------------------------
+struct ScalarField {
+    int nx = 0, ny = 0, nz = 0;
+    real dx = 1.0, dy = 1.0, dz = 1.0;
+    DeviceBuffer<real> data;
+
+    ScalarField() = default;
+    explicit ScalarField(const Grid3D& g)
+        : nx(g.nx), ny(g.ny), nz(g.nz), dx(g.dx), dy(g.dy), dz(g.dz), data(g.num_cells()) {}
+    ScalarField(int nx_, int ny_, int nz_, real dx_ = 1.0, real dy_ = 1.0, real dz_ = 1.0)
+        : nx(nx_), ny(ny_), nz(nz_), dx(dx_), dy(dy_), dz(dz_), data((size_t)nx_ * ny_ * nz_) {}
+
+    void resize(const Grid3D& g) {
+        nx = g.nx, ny = g.ny, nz = g.nz, dx = g.dx, dy = g.dy, dz = g.dz;
+        data.resize(g.num_cells());
+    }
+    void resize(int nx_, int ny_, int nz_) {
+        nx = nx_, ny = ny_, nz = nz_;
+        data.resize((size_t)nx_ * ny_ * nz_);
+    }
+    size_t size() const { return (size_t)nx * ny * nz; }
+    bool empty() const { return data.size() == 0; }
+    DeviceSpan<real> span() { return data.span(); }
+    DeviceSpan<const real> span() const { return data.span(); }
+    real* device_ptr() { return data.data(); }
+    const real* device_ptr() const { return data.data(); }
+    Grid3D grid() const { return Grid3D(nx, ny, nz, dx, dy, dz); }
+    size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)nx * (j + (size_t)ny * k); }
+};
+using KField = ScalarField;
+using HeadField = ScalarField;
+
+// Compact MAC faces: U (nx+1)*ny*nz, V nx*(ny+1)*nz, W nx*ny*(nz+1).
+struct VelocityField {
+    int nx = 0, ny = 0, nz = 0;
+    real dx = 1.0, dy = 1.0, dz = 1.0;
+    DeviceBuffer<real> U, V, W;
+
+    VelocityField() = default;
+    explicit VelocityField(const Grid3D& g) { resize(g); }
+    VelocityField(int nx_, int ny_, int nz_, real dx_ = 1.0, real dy_ = 1.0, real dz_ = 1.0)
+        : nx(nx_), ny(ny_), nz(nz_), dx(dx_), dy(dy_), dz(dz_) {
+        allocate();
+    }
+    void allocate() {
+        if (nx > 0 && ny > 0 && nz > 0) {
+            U.resize(size_U());
+            V.resize(size_V());
+            W.resize(size_W());
+        }
+    }
+    void resize(const Grid3D& g) {
+        nx = g.nx, ny = g.ny, nz = g.nz, dx = g.dx, dy = g.dy, dz = g.dz;
+        allocate();
+    }
+    void resize(int nx_, int ny_, int nz_) {
+        nx = nx_, ny = ny_, nz = nz_;
+        allocate();
+    }
+    size_t size_U() const { return (size_t)(nx + 1) * ny * nz; }
+    size_t size_V() const { return (size_t)nx * (ny + 1) * nz; }
+    size_t size_W() const { return (size_t)nx * ny * (nz + 1); }
+    size_t total_size() const { return size_U() + size_V() + size_W(); }
+    bool empty() const { return U.size() == 0; }
+    DeviceSpan<real> U_span() { return U.span(); }
+    DeviceSpan<real> V_span() { return V.span(); }
+    DeviceSpan<real> W_span() { return W.span(); }
+    DeviceSpan<const real> U_span() const { return U.span(); }
+    DeviceSpan<const real> V_span() const { return V.span(); }
+    DeviceSpan<const real> W_span() const { return W.span(); }
+    int stride_U_j() const { return nx + 1; }
+    int stride_U_k() const { return (nx + 1) * ny; }
+    int stride_V_j() const { return nx; }
+    int stride_V_k() const { return nx * (ny + 1); }
+    int stride_W_j() const { return nx; }
+    int stride_W_k() const { return nx * ny; }
+    size_t idx_U(int i, int j, int k) const { return i + (size_t)j * stride_U_j() + (size_t)k * stride_U_k(); }
+    size_t idx_V(int i, int j, int k) const { return i + (size_t)j * stride_V_j() + (size_t)k * stride_V_k(); }
+    size_t idx_W(int i, int j, int k) const { return i + (size_t)j * stride_W_j() + (size_t)k * stride_W_k(); }
+};
+
+// Padded faces: each component (nx+1)(ny+1)(nz+1), index i + (nx+1)(j + (ny+1) k),
+// zero-filled once at allocation (padding entries stay zero).
+struct PaddedVelocityField {
+    int nx = 0, ny = 0, nz = 0;
+    real dx = 1.0, dy = 1.0, dz = 1.0;
+    DeviceBuffer<real> U, V, W;
+
+    PaddedVelocityField() = default;
+    explicit PaddedVelocityField(const Grid3D& g) { resize(g); }
+    void allocate() {
+        if (nx > 0 && ny > 0 && nz > 0) {
+            const size_t s = field_size();
+            for (DeviceBuffer<real>* b : {&U, &V, &W}) {
+                b->resize(s);
+                hip_check(hipMemset(b->data(), 0, s * sizeof(real)), "PaddedVelocityField");
+            }
+        }
+    }
+    void resize(const Grid3D& g) {
+        nx = g.nx, ny = g.ny, nz = g.nz, dx = g.dx, dy = g.dy, dz = g.dz;
+        allocate();
+    }
+    size_t field_size() const { return (size_t)(nx + 1) * (ny + 1) * (nz + 1); }
+    size_t total_size() const { return 3 * field_size(); }
+    bool empty() const { return U.size() == 0; }
+    size_t idx(int i, int j, int k) const {
+        return (size_t)i + (size_t)(nx + 1) * ((size_t)j + (size_t)(ny + 1) * k);
+    }
+    DeviceSpan<real> U_span() { return U.span(); }
+    DeviceSpan<real> V_span() { return V.span(); }
+    DeviceSpan<real> W_span() { return W.span(); }
+    DeviceSpan<const real> U_span() const { return U.span(); }
+    DeviceSpan<const real> V_span() const { return V.span(); }
+    DeviceSpan<const real> W_span() const { return W.span(); }
+};
+
+// ---- workspace ------------------------------------------------------------------
+// Holds the MG hierarchy and the Krylov vectors of solve_head (all inside
+// mg_hierarchy's device workspace, sized lazily for the grid it last saw).
+struct FlowWorkspace {
+    multigrid::MGHierarchy mg_hierarchy;
+    size_t device_bytes() const { return mg_hierarchy.device_bytes(); }
+    void set_tuning(int key, long value) { mg_hierarchy.set_tuning(key, value); }
+};
+
+namespace physics {
+
+enum class HeadSolverType { CG, MG, PCG_MG };
+
+// extensions (value 0 = reference behaviour)
+enum class InitGuess { Reference, Zero, Provided };
+enum class CoarseSolver { GSRB, Direct };
+enum class StopNorm { InitialResidual, RhsNorm };
+enum class PrecondPrecision { FP64, FP32 };
+enum class CoefFormat { FP32, BF16, FP16 };
+
+struct HeadSolveConfig {
+    HeadSolverType solver_type = HeadSolverType::PCG_MG;
+    int mg_levels = 4;
+    int mg_pre_smooth = 2;
+    int mg_post_smooth = 2;
+    int mg_max_cycles = 20;
+    int mg_coarse_iters = 50;
+    int cg_max_iter = 1000;
+    int cg_check_every = 10;
+    real cg_rtol = 1e-8;
+    real rtol = 1e-6;
+    io::PinConfig pin;
+    // extensions
+    InitGuess init_guess = InitGuess::Reference;
+    CoarseSolver coarse_solver = CoarseSolver::GSRB;
+    StopNorm stop_norm = StopNorm::InitialResidual;
+    PrecondPrecision precond_precision = PrecondPrecision::FP64;
+    CoefFormat coef_format = CoefFormat::BF16;
+    bool true_residual = true;
+    bool flexible = false;
+
+    // direct coarsest solve, ||b - A h|| <= rtol ||b|| tested every iteration, zero guess
+    static HeadSolveConfig north_star(real rtol = 1e-10, int levels = 6) {
+        HeadSolveConfig c;
+        c.mg_levels = levels;
+        c.cg_rtol = rtol;
+        c.cg_check_every = 1;
+        c.init_guess = InitGuess::Zero;
+        c.coarse_solver = CoarseSolver::Direct;
+        c.stop_norm = StopNorm::RhsNorm;
+        return c;
+    }
+    // north_star with the FP32 V-cycle and 16-bit face coefficients
+    static HeadSolveConfig mixed(real rtol = 1e-10, int levels = 6,
+                                 CoefFormat coef = CoefFormat::BF16) {
+        HeadSolveConfig c = north_star(rtol, levels);
+        c.precond_precision = PrecondPrecision::FP32;
+        c.coef_format = coef;
+        return c;
+    }
+
+    mf3d_head_solve_config c() const {
+        mf3d_head_solve_config o;
+        mf3d_default_config(&o);
+        o.solver_type = (int)solver_type;
+        o.mg_levels = mg_levels;
+        o.mg_pre_smooth = mg_pre_smooth;
+        o.mg_post_smooth = mg_post_smooth;
+        o.mg_max_cycles = mg_max_cycles;
+        o.mg_coarse_iters = mg_coarse_iters;
+        o.cg_max_iter = cg_max_iter;
+        o.cg_check_every = cg_check_every;
+        o.cg_rtol = cg_rtol;
+        o.rtol = rtol;
+        o.pin_mode = (int)pin.mode;
+        o.init_guess = (int)init_guess;
+        o.coarse_solver = (int)coarse_solver;
+        o.stop_norm = (int)stop_norm;
+        o.precond_precision = (int)precond_precision;
+        o.coef_format = (int)coef_format;
+        o.true_residual = true_residual ? 1 : 0;
+        o.flexible = flexible ? 1 : 0;
+        return o;
+    }
+};
+
+struct HeadSolveResult {
+    int num_iterations = 0;
+    real initial_residual = 0.0;
+    real final_residual = 0.0;
+    bool converged = false;
+    // extensions
+    real bnorm = 0.0;
+    real true_residual = -1.0;
+    real setup_ms = 0.0;
+    real solve_ms = 0.0;
+    int restarts = 0;
+};
+
+// Solves -div(K grad h) = 0 with the boundary conditions; h is overwritten
+// by the initial guess unless cfg.init_guess is Provided. Non-convergence is
+// reported in the result, not thrown.
+inline HeadSolveResult solve_head(DeviceSpan<real> h, DeviceSpan<const real> K,
+                                  const Grid3D& grid, const BCSpec& bc,
+                                  const HeadSolveConfig& cfg, CudaContext& ctx,
+                                  FlowWorkspace& workspace) {
+    bc.validate();
+    if (K.size() < h.size()) throw std::runtime_error("solve_head: K shorter than h");
+    const mf3d_grid g = to_c(grid);
+    const mf3d_bc b = to_c(bc);
+    const mf3d_head_solve_config c = cfg.c();
+    mf3d_head_solve_result r{};
+    mf3d_check(mf3d_solve_head(workspace.mg_hierarchy.handle(), ctx.c_stream(), &g, &b, &c,
+                               K.data(), h.data(), h.size(), &r),
+               "solve_head");
+    HeadSolveResult o;
+    o.num_iterations = r.num_iterations;
+    o.initial_residual = r.initial_residual;
+    o.final_residual = r.final_residual;
+    o.converged = r.converged != 0;
+    o.bnorm = r.bnorm;
+    o.true_residual = r.true_residual;
+    o.setup_ms = r.setup_ms;
+    o.solve_ms = r.solve_ms;
+    o.restarts = r.restarts;
+    return o;
+}
+
+inline void init_head_guess(DeviceSpan<real> h, const Grid3D& grid, const BCSpec& bc,
+                            const CudaContext& ctx) {
+    const mf3d_grid g = to_c(grid);
+    const mf3d_bc b = to_c(bc);
+    mf3d_check(mf3d_init_head_guess(ctx.c_stream(), &g, &b, h.data(), h.size()),
+               "init_head_guess");
+}
+
+inline void build_rhs_head(DeviceSpan<real> rhs, DeviceSpan<const real> K, const Grid3D& grid,
+                           const BCSpec& bc, const CudaContext& ctx) {
+    const mf3d_grid g = to_c(grid);
+    const mf3d_bc b = to_c(bc);
+    if (K.size() < rhs.size()) throw std::runtime_error("build_rhs_head: K shorter than rhs");
+    mf3d_check(mf3d_build_rhs_head(ctx.c_stream(), &g, &b, K.data(), rhs.data(), rhs.size()),
+               "build_rhs_head");
+}
+
+inline void coarsen_K(CudaContext& ctx, const Grid3D& coarse_grid, const Grid3D& fine_grid,
+                      DeviceSpan<const real> K_fine, DeviceSpan<real> K_coarse) {
+    const mf3d_grid c = to_c(coarse_grid), f = to_c(fine_grid);
+    mf3d_check(mf3d_coarsen_K(ctx.c_stream(), &c, &f, K_fine.data(), K_fine.size(),
+                              K_coarse.data(), K_coarse.size()),
+               "coarsen_K");
+}
+
+inline void compute_velocity_from_head(VelocityField& vel, const HeadField& head, const KField& K,
+                                       const Grid3D& grid, const BCSpec& bc, CudaContext& ctx) {
+    const mf3d_grid g = to_c(grid);
+    const mf3d_bc b = to_c(bc);
+    if (vel.nx != grid.nx || vel.ny != grid.ny || vel.nz != grid.nz) vel.resize(grid);
+    mf3d_check(mf3d_velocity_compact(ctx.c_stream(), &g, &b, head.device_ptr(), K.device_ptr(),
+                                     head.data.size(), vel.U.data(), vel.U.size(), vel.V.data(),
+                                     vel.V.size(), vel.W.data(), vel.W.size()),
+               "compute_velocity_from_head");
+}
+
+inline void compute_velocity_from_head(PaddedVelocityField& vel, const HeadField& head,
+                                       const KField& K, const Grid3D& grid, const BCSpec& bc,
+                                       CudaContext& ctx) {
+    const mf3d_grid g = to_c(grid);
+    const mf3d_bc b = to_c(bc);
+    if (vel.nx != grid.nx || vel.ny != grid.ny || vel.nz != grid.nz) vel.resize(grid);
+    mf3d_check(mf3d_velocity_padded(ctx.c_stream(), &g, &b, head.device_ptr(), K.device_ptr(),
+                                    head.data.size(), vel.U.data(), vel.V.data(), vel.W.data(),
+                                    vel.field_size()),
+               "compute_velocity_from_head");
+}
+
+// ---- checksum utilities (synchronous) ------------------------------------------
+inline real compute_norm2(DeviceSpan<const real> data, CudaContext& ctx) {
+    if (data.empty()) return 0.0;
+    blas::ReductionWorkspace red;
+    return blas::nrm2_host(ctx, data, red);
+}
+
+inline real compute_sum(DeviceSpan<const real> data, CudaContext& ctx) {
+    if (data.empty()) return 0.0;
+    DeviceBuffer<real> ones(data.size());
+    blas::fill(ctx, ones.span(), 1.0);
+    blas::ReductionWorkspace red;
+    return blas::dot_host(ctx, data, ones.span(), red);
+}
+
+inline real compute_mean(DeviceSpan<const real> data, CudaContext& ctx) {
+    return data.empty() ? 0.0 : compute_sum(data, ctx) / (real)data.size();
+}
+
+inline bool check_no_nans(DeviceSpan<const real> data, CudaContext& ctx) {
+    return !std::isnan(compute_norm2(data, ctx));
+}
+
+inline void print_velocity_checksums(const VelocityField& vel, CudaContext& ctx) {
+    const char* names[3] = {"U", "V", "W"};
+    DeviceSpan<const real> spans[3] = {vel.U_span(), vel.V_span(), vel.W_span()};
+    for (int c = 0; c < 3; ++c) {
+        const real nrm = compute_norm2(spans[c], ctx);
+        std::printf("[velocity] %s: sum=%.17g norm2=%.17g nan=%s\n", names[c],
+                    compute_sum(spans[c], ctx), nrm, std::isnan(nrm) ? "yes" : "no");
+    }
+}
+
+}  // namespace physics
+}  // namespace macroflow3d

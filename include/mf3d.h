@@ -195,7 +195,8 @@ int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
 enum {
     MF3D_TUNE_MARCH_MIN_CELLS_FP64 = 1, /* levels with >= this many cells use the march kernel */
     MF3D_TUNE_MARCH_MIN_CELLS_FP32 = 2,
-    MF3D_TUNE_ZCHUNK = 3                /* planes per march workgroup (<= 0: default) */
+    MF3D_TUNE_ZCHUNK = 3,               /* planes per march workgroup (<= 0: default) */
+    MF3D_TUNE_SPLIT = 4                 /* 1 (default): colour-split FP32 V-cycle vectors */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

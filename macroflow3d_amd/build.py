@@ -45,6 +45,22 @@ def _compile(src: str) -> str:
     return obj
 
 
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_solve.cpp")
+DROPIN_BIN = os.path.join(BUILD, "dropin_solve")
+
+
+def build_dropin() -> str | None:
+    """The C++ drop-in driver (tests/cpp/dropin_solve.cpp) over include/macroflow3d."""
+    if not os.path.exists(DROPIN_SRC):
+        return None
+    cmd = [hipcc(), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           DROPIN_SRC, "-L" + HERE, "-lmf3d_hip", "-Wl,-rpath,$ORIGIN/..", "-o", DROPIN_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on dropin_solve.cpp:\n{r.stdout}\n{r.stderr}")
+    return DROPIN_BIN
+
+
 def build(jobs: int = 8, verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -55,6 +71,7 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_dropin()
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -204,6 +204,128 @@ __global__ void __launch_bounds__(kBlock)
     x[o] = gs_value(f, v, b[o], bs, pin && o == 0);
 }
 
+// ---- colour-split storage of the FP32 V-cycle vectors --------------------------
+// With even nx, ny, nz a level's x and b may be stored colour by colour: the
+// cells of colour c = (i+j+k) & 1 at c*N/2 + (i >> 1) + (nx/2)(j + ny k). A
+// half-sweep then reads only the opposite colour's x and its own colour's b
+// and writes only its own colour, all with unit stride (12 B per cell with
+// 16-bit faces instead of the 18 B a natural-order half-sweep moves).
+// Every neighbour of a cell has the other colour: the y and z neighbours sit at
+// the same (i >> 1) one row or plane away, the x neighbours at ((i +- 1) >> 1)
+// of the same row. Face coefficients stay in the natural MAC order.
+struct Split {
+    int h;       // nx / 2
+    long half;   // N / 2
+    __device__ long at(const Dims& d, int i, int j, int k) const {
+        return (long)((i + j + k) & 1) * half + (long)(i >> 1) + (long)h * ((long)j + (long)d.ny * k);
+    }
+};
+__device__ __forceinline__ Split split_of(const Dims& d) {
+    return Split{d.nx >> 1, (long)(d.nx >> 1) * d.ny * d.nz};
+}
+
+// neighbours (x+, x-, y+, y-, z+, z-) of cell (i,j,k) from the opposite
+// colour's array O; r0 = (j + ny k) * nx/2 is the cell's row offset
+template <typename T>
+__device__ __forceinline__ void neighbours_split(const Dims& d, const BcTypes& bc, const T* O,
+                                                 long r0, int h, int i, int j, int k, T (&v)[6]) {
+    const int m = i >> 1;
+    const long sy = h, sz = (long)h * d.ny;
+    v[0] = i < d.nx - 1 ? O[r0 + ((i + 1) >> 1)] : (bc.t[1] == MF3D_BC_PERIODIC ? O[r0] : T(0));
+    v[1] = i > 0 ? O[r0 + ((i - 1) >> 1)] : (bc.t[0] == MF3D_BC_PERIODIC ? O[r0 + h - 1] : T(0));
+    v[2] = j < d.ny - 1 ? O[r0 + sy + m]
+                        : (bc.t[3] == MF3D_BC_PERIODIC ? O[r0 - (long)(d.ny - 1) * sy + m] : T(0));
+    v[3] = j > 0 ? O[r0 - sy + m]
+                 : (bc.t[2] == MF3D_BC_PERIODIC ? O[r0 + (long)(d.ny - 1) * sy + m] : T(0));
+    v[4] = k < d.nz - 1 ? O[r0 + sz + m]
+                        : (bc.t[5] == MF3D_BC_PERIODIC ? O[r0 - (long)(d.nz - 1) * sz + m] : T(0));
+    v[5] = k > 0 ? O[r0 - sz + m]
+                 : (bc.t[4] == MF3D_BC_PERIODIC ? O[r0 + (long)(d.nz - 1) * sz + m] : T(0));
+}
+
+// index of the coarse parent of fine cell (i,j,k) in a natural or split coarse array
+template <bool CS>
+__device__ __forceinline__ long parent_index(const Dims& dc, int i, int j, int k) {
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    if (CS) return split_of(dc).at(dc, I, J, K);
+    return (long)I + (long)dc.nx * ((long)J + (long)dc.ny * K);
+}
+
+// wrapped neighbour coordinates (x+, x-, y+, y-, z+, z-); false for a
+// non-periodic boundary neighbour
+__device__ __forceinline__ bool nb_coord(const Dims& d, const BcTypes& bc, int q, int& i, int& j,
+                                         int& k) {
+    switch (q) {
+    case 0: if (++i == d.nx) { if (bc.t[1] != MF3D_BC_PERIODIC) return false; i = 0; } break;
+    case 1: if (--i < 0) { if (bc.t[0] != MF3D_BC_PERIODIC) return false; i = d.nx - 1; } break;
+    case 2: if (++j == d.ny) { if (bc.t[3] != MF3D_BC_PERIODIC) return false; j = 0; } break;
+    case 3: if (--j < 0) { if (bc.t[2] != MF3D_BC_PERIODIC) return false; j = d.ny - 1; } break;
+    case 4: if (++k == d.nz) { if (bc.t[5] != MF3D_BC_PERIODIC) return false; k = 0; } break;
+    default: if (--k < 0) { if (bc.t[4] != MF3D_BC_PERIODIC) return false; k = d.nz - 1; } break;
+    }
+    return true;
+}
+
+// One colour of GSRB on colour-split x and b. MODE 0: plain. MODE 1: x is zero
+// on entry (no neighbour reads). MODE 2: the coarse correction xc is
+// prolonged (piecewise constant) into the neighbours as they are read; used
+// for the first colour of post-smoothing, whose own-colour values GS never
+// reads, and the other colour is rewritten by the next half-sweep.
+template <typename TC, int MODE, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    colour_split_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
+                        const float* __restrict__ b, const TC* __restrict__ FX,
+                        const TC* __restrict__ FY, const TC* __restrict__ FZ, int colour,
+                        const float* __restrict__ xc) {
+    const Split sp = split_of(d);
+    const long blk = xcd_block((sp.half + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long t = blk * kBlock + threadIdx.x;
+    if (t >= sp.half) return;
+    const int m = (int)(t % sp.h);
+    const long row = t / sp.h;
+    const int j = (int)(row % d.ny);
+    const int k = (int)(row / d.ny);
+    const int i = 2 * m + ((j + k + colour) & 1);
+    float f[6], v[6];
+    cell_faces(d, FX, FY, FZ, i, j, k, f);
+    if (MODE == 1) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] = 0.0f;
+    } else {
+        const float* O = x + (long)(colour ^ 1) * sp.half;
+        neighbours_split(d, bc, O, row * sp.h, sp.h, i, j, k, v);
+        if (MODE == 2) {
+            const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                int a = i, bb = j, c = k;
+                if (nb_coord(d, bc, q, a, bb, c)) v[q] += xc[parent_index<CS>(dc, a, bb, c)];
+            }
+        }
+    }
+    const long o = (long)colour * sp.half + t;
+    x[o] = gs_value(f, v, b[o], bs, pin && colour == 0 && t == 0);
+}
+
+// prolong-add between any two orders (used when a level has no post-smoothing)
+template <bool FS, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    prolong_order_kernel(Dims d, const float* __restrict__ c, float* __restrict__ f) {
+    const long n = (long)d.nx * d.ny * d.nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long o = blk * kBlock + threadIdx.x;
+    if (o >= n) return;
+    const int i = (int)(o % d.nx);
+    const long q = o / d.nx;
+    const int j = (int)(q % d.ny);
+    const int k = (int)(q / d.ny);
+    const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
+    const long fo = FS ? split_of(d).at(d, i, j, k) : o;
+    f[fo] += c[parent_index<CS>(dc, i, j, k)];
+}
+
 // ---- whole smoothing of a small level in one workgroup -----------------------
 constexpr int kSmallBlock = 1024;
 constexpr int kSmallPer = 4;
@@ -464,6 +586,62 @@ __global__ void __launch_bounds__(kBlock)
     bcoarse[t] = s / T(8);
 }
 
+// residual of one cell of a colour-split level (same sums as residual_cell)
+template <typename TC>
+__device__ __forceinline__ float residual_cell_split(const Dims& d, const BcTypes& bc, bool pin,
+                                                     const Split& sp, const float* x,
+                                                     const float* b, const TC* FX, const TC* FY,
+                                                     const TC* FZ, int i, int j, int k, float rs) {
+    const int c = (i + j + k) & 1;
+    const long r0 = (long)sp.h * ((long)j + (long)d.ny * k);
+    const long o = (long)c * sp.half + r0 + (i >> 1);
+    float f[6], v[6];
+    cell_faces(d, FX, FY, FZ, i, j, k, f);
+    neighbours_split(d, bc, x + (long)(c ^ 1) * sp.half, r0, sp.h, i, j, k, v);
+    const float xc = x[o];
+    float result = f[0] * (xc - v[0]);
+    result += f[1] * (xc - v[1]);
+    result += f[2] * (xc - v[2]);
+    result += f[3] * (xc - v[3]);
+    result += f[4] * (xc - v[4]);
+    result += f[5] * (xc - v[5]);
+    if (pin && (i | j | k) == 0) result += (f[0] + f[1] + f[2] + f[3] + f[4] + f[5]) * xc;
+    return b[o] + result * rs;
+}
+
+// residual + restriction of a colour-split fine level into a coarse level of
+// either order
+template <typename TC, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    residual_restrict_split_kernel(Dims d, BcTypes bc, int pin, float rs,
+                                   const float* __restrict__ x, const float* __restrict__ b,
+                                   const TC* __restrict__ FX, const TC* __restrict__ FY,
+                                   const TC* __restrict__ FZ, float* __restrict__ bcoarse) {
+    const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
+    const long total = (long)dc.nx * dc.ny * dc.nz;
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long t = blk * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const int I = (int)(t % dc.nx);
+    const long q = t / dc.nx;
+    const int J = (int)(q % dc.ny);
+    const int Kc = (int)(q / dc.ny);
+    const int i = 2 * I, j = 2 * J, k = 2 * Kc;
+    const bool p = pin != 0;
+    const Split sp = split_of(d);
+    float s = 0.0f;
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j + 1, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j + 1, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j, k + 1, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j, k + 1, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j + 1, k + 1, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j + 1, k + 1, rs);
+    bcoarse[CS ? split_of(dc).at(dc, I, J, Kc) : t] = s / 8.0f;
+}
+
 // ---- prolongation for levels without the march kernel -----------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -577,6 +755,34 @@ __global__ void __launch_bounds__(kBlock)
     to_f64_kernel(const float* __restrict__ a, double* __restrict__ o, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) o[i] = (double)a[i];
+}
+
+// FP64 natural -> FP32 colour-split, and back
+__global__ void __launch_bounds__(kBlock)
+    to_f32_split_kernel(Dims d, const double* __restrict__ a, float* __restrict__ o) {
+    const long n = (long)d.nx * d.ny * d.nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long e = blk * kBlock + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e % d.nx);
+    const long q = e / d.nx;
+    const int j = (int)(q % d.ny);
+    const int k = (int)(q / d.ny);
+    o[split_of(d).at(d, i, j, k)] = (float)a[e];
+}
+__global__ void __launch_bounds__(kBlock)
+    to_f64_split_kernel(Dims d, const float* __restrict__ a, double* __restrict__ o) {
+    const long n = (long)d.nx * d.ny * d.nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long e = blk * kBlock + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e % d.nx);
+    const long q = e / d.nx;
+    const int j = (int)(q % d.ny);
+    const int k = (int)(q / d.ny);
+    o[e] = (double)a[split_of(d).at(d, i, j, k)];
 }
 
 // coarsest direct solve in FP64 for any x/b type: x = A^{-1} b, one wave per row
@@ -752,14 +958,104 @@ int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double 
 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
-                 bool zero_init, const double* xc, const SmoothPlan& plan) {
+                 bool zero_init, const double* xc, const SmoothPlan& plan, int order,
+                 int corder) {
     if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 smoothing needs FP64 faces");
+    if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
+        return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels are stored in natural order");
     return smooth_t<double, double>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc, plan);
+}
+
+namespace {
+
+template <typename TC>
+int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
+                 float* x, const float* b, const FaceSet& F, int iters, bool zero_init,
+                 const float* xc, bool csplit) {
+    const Dims d = dims(g);
+    const TC* FX = cptr<TC>(F, 0);
+    const TC* FY = cptr<TC>(F, 1);
+    const TC* FZ = cptr<TC>(F, 2);
+    const size_t n = cells(g);
+    const float fbs = (float)bs;
+    const int p = pin ? 1 : 0;
+    if (zero_init && xc) {  // not used by the V-cycle; keep the meaning x = 0 + P xc
+        fill_t_kernel<float><<<grid_for(n), kBlock, 0, s>>>(x, (long)n);
+        zero_init = false;
+    }
+    if (iters <= 0) {
+        if (zero_init) {
+            fill_t_kernel<float><<<grid_for(n), kBlock, 0, s>>>(x, (long)n);
+        } else if (xc) {
+            if (csplit) prolong_order_kernel<true, true><<<xcd_grid(n), kBlock, 0, s>>>(d, xc, x);
+            else prolong_order_kernel<true, false><<<xcd_grid(n), kBlock, 0, s>>>(d, xc, x);
+        }
+        MF3D_HIP_TRY(hipGetLastError());
+        return MF3D_OK;
+    }
+    const unsigned gh = xcd_grid(n / 2);
+    for (int it = 0; it < iters; ++it)
+        for (int colour = 0; colour < 2; ++colour) {
+            if (it == 0 && colour == 0 && zero_init)
+                colour_split_kernel<TC, 1, false><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
+                                                                        FY, FZ, colour, nullptr);
+            else if (it == 0 && colour == 0 && xc && csplit)
+                colour_split_kernel<TC, 2, true><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
+                                                                       FY, FZ, colour, xc);
+            else if (it == 0 && colour == 0 && xc)
+                colour_split_kernel<TC, 2, false><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
+                                                                        FY, FZ, colour, xc);
+            else
+                colour_split_kernel<TC, 0, false><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
+                                                                        FY, FZ, colour, nullptr);
+        }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+template <typename TC>
+int rr_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double rs,
+             const float* x, const float* b, const FaceSet& F, float* bcoarse, bool csplit) {
+    const size_t nc = (size_t)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
+    const Dims d = dims(g);
+    if (csplit)
+        residual_restrict_split_kernel<TC, true><<<xcd_grid(nc), kBlock, 0, s>>>(
+            d, bc, pin ? 1 : 0, (float)rs, x, b, cptr<TC>(F, 0), cptr<TC>(F, 1), cptr<TC>(F, 2),
+            bcoarse);
+    else
+        residual_restrict_split_kernel<TC, false><<<xcd_grid(nc), kBlock, 0, s>>>(
+            d, bc, pin ? 1 : 0, (float)rs, x, b, cptr<TC>(F, 0), cptr<TC>(F, 1), cptr<TC>(F, 2),
+            bcoarse);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+}  // namespace
+
+bool split_ok(const mf3d_grid& g) {
+    return g.nx >= 2 && g.ny >= 2 && g.nz >= 2 && !(g.nx & 1) && !(g.ny & 1) && !(g.nz & 1);
 }
 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
-                 bool zero_init, const float* xc, const SmoothPlan& plan) {
+                 bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder) {
+    if (order == ORDER_SPLIT) {
+        if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
+        const bool cs = corder == ORDER_SPLIT;
+        switch (F.type) {
+        case COEF_F32:
+            return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs);
+        case COEF_BF16:
+            return smooth_split<__hip_bfloat16>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc,
+                                                cs);
+        case COEF_F16:
+            return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs);
+        default:
+            return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");
+        }
+    }
+    if (corder == ORDER_SPLIT)
+        return fail(MF3D_ERR_UNSUPPORTED, "a natural-order level cannot take a split coarse level");
     switch (F.type) {
     case COEF_F32:
         return smooth_t<float, float>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc, plan);
@@ -776,13 +1072,27 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
 
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const double* x, const double* b, const FaceSet& F,
-                            double* bcoarse) {
+                            double* bcoarse, int order, int corder) {
+    if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
+        return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels are stored in natural order");
     return rr_t<double, double>(s, g, bc, pin, rs, x, b, F, bcoarse);
 }
 
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const float* x, const float* b, const FaceSet& F,
-                            float* bcoarse) {
+                            float* bcoarse, int order, int corder) {
+    if (order == ORDER_SPLIT) {
+        if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
+        const bool cs = corder == ORDER_SPLIT;
+        switch (F.type) {
+        case COEF_F32: return rr_split<float>(s, g, bc, pin, rs, x, b, F, bcoarse, cs);
+        case COEF_BF16: return rr_split<__hip_bfloat16>(s, g, bc, pin, rs, x, b, F, bcoarse, cs);
+        case COEF_F16: return rr_split<__half>(s, g, bc, pin, rs, x, b, F, bcoarse, cs);
+        default: return fail(MF3D_ERR_UNSUPPORTED, "FP32 residual needs 32- or 16-bit faces");
+        }
+    }
+    if (corder == ORDER_SPLIT)
+        return fail(MF3D_ERR_UNSUPPORTED, "a natural-order level cannot take a split coarse level");
     switch (F.type) {
     case COEF_F32: return rr_t<float, float>(s, g, bc, pin, rs, x, b, F, bcoarse);
     case COEF_BF16: return rr_t<float, __hip_bfloat16>(s, g, bc, pin, rs, x, b, F, bcoarse);
@@ -830,6 +1140,22 @@ int to_f32(hipStream_t s, const double* a, float* o, size_t n) {
 
 int to_f64(hipStream_t s, const float* a, double* o, size_t n) {
     to_f64_kernel<<<grid_for(n), kBlock, 0, s>>>(a, o, (long)n);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int to_f32(hipStream_t s, const double* a, float* o, const mf3d_grid& g, int order) {
+    if (order != ORDER_SPLIT) return to_f32(s, a, o, cells(g));
+    if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
+    to_f32_split_kernel<<<xcd_grid(cells(g)), kBlock, 0, s>>>(dims(g), a, o);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int to_f64(hipStream_t s, const float* a, double* o, const mf3d_grid& g, int order) {
+    if (order != ORDER_SPLIT) return to_f64(s, a, o, cells(g));
+    if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
+    to_f64_split_kernel<<<xcd_grid(cells(g)), kBlock, 0, s>>>(dims(g), a, o);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

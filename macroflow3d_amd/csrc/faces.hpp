@@ -16,6 +16,11 @@ struct FaceSet {
     void* fz = nullptr;
 };
 
+// Storage order of a level's FP32 V-cycle vectors x and b. NATURAL:
+// i + nx (j + ny k). SPLIT (even nx, ny, nz): the cells of colour
+// c = (i+j+k) & 1 at c*N/2 + (i >> 1) + (nx/2)(j + ny k).
+enum Order : int { ORDER_NATURAL = 0, ORDER_SPLIT = 1 };
+
 enum SmoothKind : int { SMOOTH_COLOUR = 0, SMOOTH_SMALL = 1, SMOOTH_MARCH = 2 };
 
 // How one level is smoothed (chosen by defaults or the autotuner).
@@ -38,23 +43,33 @@ int march_default_zchunk(const mf3d_grid& g);
 // zero on entry. xc (nullable): coarse correction prolonged and added first.
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
-                 bool zero_init, const double* xc, const SmoothPlan& plan);
+                 bool zero_init, const double* xc, const SmoothPlan& plan,
+                 int order = ORDER_NATURAL, int corder = ORDER_NATURAL);
+// order: storage order of x/b, corder: of xc (FP64 levels are natural only);
+// a split level's coarse level may be natural, not the reverse.
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
-                 bool zero_init, const float* xc, const SmoothPlan& plan);
+                 bool zero_init, const float* xc, const SmoothPlan& plan,
+                 int order = ORDER_NATURAL, int corder = ORDER_NATURAL);
+bool split_ok(const mf3d_grid& g);
 // coarse b = restriction of (b - A x), rs = 1 / (cs * dx^2)
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const double* x, const double* b, const FaceSet& F,
-                            double* bcoarse);
+                            double* bcoarse, int order = ORDER_NATURAL,
+                            int corder = ORDER_NATURAL);
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const float* x, const float* b, const FaceSet& F,
-                            float* bcoarse);
+                            float* bcoarse, int order = ORDER_NATURAL,
+                            int corder = ORDER_NATURAL);
 // FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
 // mode 1: y = b - A x (y nullable), partials (y, y).
 int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
              int mode, const double* x, const double* b, double* y, double* partials);
 int to_f32(hipStream_t s, const double* a, float* o, size_t n);
 int to_f64(hipStream_t s, const float* a, double* o, size_t n);
+// conversions between a natural FP64 vector and an FP32 vector of `order`
+int to_f32(hipStream_t s, const double* a, float* o, const mf3d_grid& g, int order);
+int to_f64(hipStream_t s, const float* a, double* o, const mf3d_grid& g, int order);
 int gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
 int gemv(hipStream_t s, int n, const double* Ainv, const float* b, float* x);
 int fill_f32(hipStream_t s, float* x, size_t n, float v);

@@ -83,6 +83,7 @@ struct Level {
     bool has_f64 = false;
     bool has_flo = false;
     double cs = 1.0;        // storage scale of flo
+    int order = ORDER_NATURAL;  // storage order of x32/b32
     SmoothPlan plan;
 };
 
@@ -121,6 +122,7 @@ struct mf3d_workspace {
     // tuning (mf3d_workspace_set_tuning / autotuner)
     long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
     int zchunk = 0;
+    bool split = true;  // colour-split FP32 levels where the sizes allow
     size_t bytes = 0;
     std::vector<void*> owned;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -412,6 +414,15 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
         }
         if (l == 0) L.t = ws->Ap;  // free while the V-cycle runs
     }
+    // colour-split FP32 vectors on a prefix of colour-swept levels (never the
+    // coarsest, whose direct or small-level solve works in natural order)
+    bool prefix = mixed && ws->split;
+    for (size_t l = 0; l < ws->lv.size(); ++l) {
+        Level& L = ws->lv[l];
+        prefix = prefix && l + 1 < ws->lv.size() && L.plan.kind == SMOOTH_COLOUR &&
+                 kern::split_ok(L.g);
+        L.order = prefix ? ORDER_SPLIT : ORDER_NATURAL;
+    }
     if (cfg.coarse_solver == MF3D_COARSE_DIRECT) MF3D_TRY(build_coarse_inverse(ws, s));
     return MF3D_OK;
 }
@@ -424,6 +435,7 @@ template <> struct LevelView<double> {
     static double* t(Level& L) { return L.t; }
     static const FaceSet& F(const Level& L) { return L.f64; }
     static double cs(const Level&) { return 1.0; }
+    static int order(const Level&) { return ORDER_NATURAL; }
 };
 template <> struct LevelView<float> {
     static float* x(Level& L) { return L.x32; }
@@ -431,6 +443,7 @@ template <> struct LevelView<float> {
     static float* t(Level& L) { return L.t32; }
     static const FaceSet& F(const Level& L) { return L.flo; }
     static double cs(const Level& L) { return L.cs; }
+    static int order(const Level& L) { return L.order; }
 };
 
 // v_cycle_recursive (src/multigrid/cycle/v_cycle.cu:36-74) in precision T.
@@ -452,13 +465,14 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
                                   ws->coarse_iters, zero_init, (const T*)nullptr, L.plan);
     }
     Level& C = ws->lv[l + 1];
+    const int fo = V::order(L), co = V::order(C);
     MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->pre,
-                                zero_init, (const T*)nullptr, L.plan));
+                                zero_init, (const T*)nullptr, L.plan, fo, co));
     MF3D_TRY(kern::residual_restrict_faces(s, L.g, ws->bct, ws->pin, 1.0 / (cs * dx2), x, b, F,
-                                           V::b(C)));
+                                           V::b(C), fo, co));
     MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true));
     return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->post,
-                              false, (const T*)V::x(C), L.plan);
+                              false, (const T*)V::x(C), L.plan, fo, co);
 }
 
 // MultigridPreconditioner::apply (mg_preconditioner.cu:16-45): z = V-cycle(b = r, x = 0),
@@ -466,9 +480,9 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
 int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z) {
     Level& L0 = ws->lv[0];
     if (ws->precision == MF3D_PRECOND_FP64) return vcycle<double>(ws, s, 0, z, r, true);
-    MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.n));
+    MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.g, L0.order));
     MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true));
-    return kern::to_f64(s, L0.x32, z, L0.n);
+    return kern::to_f64(s, L0.x32, z, L0.g, L0.order);
 }
 
 // operator on the finest level with its FP64 faces
@@ -985,6 +999,7 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     case MF3D_TUNE_MARCH_MIN_CELLS_FP64: ws->march_min_cells[0] = value; break;
     case MF3D_TUNE_MARCH_MIN_CELLS_FP32: ws->march_min_cells[1] = value; break;
     case MF3D_TUNE_ZCHUNK: ws->zchunk = (int)value; break;
+    case MF3D_TUNE_SPLIT: ws->split = value != 0; break;
     default: return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
     return MF3D_OK;
@@ -1003,7 +1018,8 @@ int mf3d_mg_smooth(mf3d_workspace* ws, mf3d_stream stream, int level, int num_it
     const double dx2 = L.g.dx * L.g.dx;
     if (ws->precision == MF3D_PRECOND_FP32)
         return kern::smooth_faces(s, L.g, ws->bct, ws->pin, L.cs * dx2, L.x32, L.t32, L.b32,
-                                  L.flo, num_iters, false, (const float*)nullptr, L.plan);
+                                  L.flo, num_iters, false, (const float*)nullptr, L.plan,
+                                  L.order);
     double* x = level == 0 ? ws->z : L.x;
     const double* b = level == 0 ? ws->r : L.b;
     return kern::smooth_faces(s, L.g, ws->bct, ws->pin, dx2, x, L.t, b, L.f64, num_iters, false,
