@@ -288,7 +288,7 @@ def main():
         "head_rel_l2_vs_fp64": head_err,
         "velocity_rel_l2_vs_fp64": vel_err,
         "roofline": {
-            "kernel": f"colour_split_kernel<{args.coef}> (finest level, one colour)",
+            "kernel": f"colour_split4_kernel<{args.coef}> (finest level, one colour)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

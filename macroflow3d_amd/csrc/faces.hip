@@ -204,15 +204,21 @@ __global__ void __launch_bounds__(kBlock)
     x[o] = gs_value(f, v, b[o], bs, pin && o == 0);
 }
 
-// ---- colour-split storage of the FP32 V-cycle vectors --------------------------
+// ---- colour-split storage of the FP32 V-cycle levels -----------------------------
 // With even nx, ny, nz a level's x and b may be stored colour by colour: the
 // cells of colour c = (i+j+k) & 1 at c*N/2 + (i >> 1) + (nx/2)(j + ny k). A
 // half-sweep then reads only the opposite colour's x and its own colour's b
-// and writes only its own colour, all with unit stride (12 B per cell with
-// 16-bit faces instead of the 18 B a natural-order half-sweep moves).
+// and writes only its own colour, all with unit stride.
 // Every neighbour of a cell has the other colour: the y and z neighbours sit at
 // the same (i >> 1) one row or plane away, the x neighbours at ((i +- 1) >> 1)
-// of the same row. Face coefficients stay in the natural MAC order.
+// of the same row.
+// The face coefficients of such a level are stored the same way, each face
+// with the cell on its upper side: LX/LY/LZ[split(i,j,k)] hold the faces
+// between (i-1,j,k)|(i,j,k), (i,j-1,k)|(i,j,k), (i,j,k-1)|(i,j,k) (the MAC
+// entries FX(i,j,k), FY(i,j,k), FZ(i,j,k)); the faces on the upper domain
+// boundary, owned by no cell, go to UX[j + ny k], UY[i + nx k], UZ[i + nx j]
+// (FX(nx,j,k), FY(i,ny,k), FZ(i,j,nz)). A cell's lower faces are then at its
+// own index, its upper faces at its +x/+y/+z neighbours' indices.
 struct Split {
     int h;       // nx / 2
     long half;   // N / 2
@@ -266,16 +272,56 @@ __device__ __forceinline__ bool nb_coord(const Dims& d, const BcTypes& bc, int q
     return true;
 }
 
-// One colour of GSRB on colour-split x and b. MODE 0: plain. MODE 1: x is zero
-// on entry (no neighbour reads). MODE 2: the coarse correction xc is
-// prolonged (piecewise constant) into the neighbours as they are read; used
-// for the first colour of post-smoothing, whose own-colour values GS never
-// reads, and the other colour is rewritten by the next half-sweep.
+template <typename TC>
+struct SFaces {
+    const TC* lx;
+    const TC* ly;
+    const TC* lz;
+    const TC* ux;
+    const TC* uy;
+    const TC* uz;
+};
+
+__device__ __forceinline__ float cvt(float v) { return v; }
+__device__ __forceinline__ float cvt(__hip_bfloat16 v) { return __bfloat162float(v); }
+__device__ __forceinline__ float cvt(__half v) { return __half2float(v); }
+
+// faces (x+, x-, y+, y-, z+, z-) of cell (i,j,k) of a colour-split level
+template <typename TC>
+__device__ __forceinline__ void cell_faces_split(const Dims& d, const Split& sp,
+                                                 const SFaces<TC>& F, int i, int j, int k,
+                                                 float (&f)[6]) {
+    const long own = sp.at(d, i, j, k);
+    f[1] = cvt(F.lx[own]);
+    f[3] = cvt(F.ly[own]);
+    f[5] = cvt(F.lz[own]);
+    f[0] = i < d.nx - 1 ? cvt(F.lx[sp.at(d, i + 1, j, k)]) : cvt(F.ux[j + (long)d.ny * k]);
+    f[2] = j < d.ny - 1 ? cvt(F.ly[sp.at(d, i, j + 1, k)]) : cvt(F.uy[i + (long)d.nx * k]);
+    f[4] = k < d.nz - 1 ? cvt(F.lz[sp.at(d, i, j, k + 1)]) : cvt(F.uz[i + (long)d.nx * j]);
+}
+
+// adds the parent coarse value to each in-domain neighbour value (MODE 2)
+template <bool CS>
+__device__ __forceinline__ void add_parents(const Dims& d, const BcTypes& bc, const float* xc,
+                                            int i, int j, int k, float (&v)[6]) {
+    const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        int a = i, bb = j, c = k;
+        if (nb_coord(d, bc, q, a, bb, c)) v[q] += xc[parent_index<CS>(dc, a, bb, c)];
+    }
+}
+
+// One colour of GSRB on a colour-split level, one cell per thread (any even
+// sizes). MODE 0: plain. MODE 1: x is zero on entry (no neighbour reads).
+// MODE 2: the coarse correction xc is prolonged (piecewise constant) into the
+// neighbours as they are read; used for the first colour of post-smoothing,
+// whose own-colour values GS never reads (the other colour is rewritten by
+// the next half-sweep).
 template <typename TC, int MODE, bool CS>
 __global__ void __launch_bounds__(kBlock)
     colour_split_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
-                        const float* __restrict__ b, const TC* __restrict__ FX,
-                        const TC* __restrict__ FY, const TC* __restrict__ FZ, int colour,
+                        const float* __restrict__ b, SFaces<TC> F, int colour,
                         const float* __restrict__ xc) {
     const Split sp = split_of(d);
     const long blk = xcd_block((sp.half + kBlock - 1) / kBlock);
@@ -288,24 +334,139 @@ __global__ void __launch_bounds__(kBlock)
     const int k = (int)(row / d.ny);
     const int i = 2 * m + ((j + k + colour) & 1);
     float f[6], v[6];
-    cell_faces(d, FX, FY, FZ, i, j, k, f);
+    cell_faces_split(d, sp, F, i, j, k, f);
     if (MODE == 1) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) v[q] = 0.0f;
     } else {
         const float* O = x + (long)(colour ^ 1) * sp.half;
         neighbours_split(d, bc, O, row * sp.h, sp.h, i, j, k, v);
-        if (MODE == 2) {
-            const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                int a = i, bb = j, c = k;
-                if (nb_coord(d, bc, q, a, bb, c)) v[q] += xc[parent_index<CS>(dc, a, bb, c)];
-            }
-        }
+        if (MODE == 2) add_parents<CS>(d, bc, xc, i, j, k, v);
     }
     const long o = (long)colour * sp.half + t;
     x[o] = gs_value(f, v, b[o], bs, pin && colour == 0 && t == 0);
+}
+
+// 4 consecutive same-colour cells of a row per thread (nx a multiple of 8):
+// x, b and the faces move as 16-byte / 8-byte vectors; the x-neighbour one
+// place outside the thread's 4 comes from the adjacent lane.
+template <typename TC>
+struct alignas(4 * sizeof(TC)) Pack4 {
+    TC v[4];
+};
+
+template <typename TC>
+__device__ __forceinline__ void load4(const TC* p, float (&o)[4]) {
+    const Pack4<TC> q = *reinterpret_cast<const Pack4<TC>*>(p);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) o[e] = cvt(q.v[e]);
+}
+
+template <typename TC, int MODE, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    colour_split4_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
+                         const float* __restrict__ b, SFaces<TC> F, int colour,
+                         const float* __restrict__ xc) {
+    const Split sp = split_of(d);
+    const int h4 = sp.h >> 2;
+    const long total = sp.half >> 2;
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;  // whole workgroup
+    const long t = blk * kBlock + threadIdx.x;
+    const bool valid = t < total;
+    const long tt = valid ? t : total - 1;  // lanes past the end mirror the last one
+    const int lane = threadIdx.x & 63;
+    const int q = (int)(tt % h4);
+    const long row = tt / h4;
+    const int j = (int)(row % d.ny);
+    const int k = (int)(row / d.ny);
+    const int p = (j + k + colour) & 1;
+    const long r0 = row * sp.h;
+    const long base = r0 + 4L * q;
+    const long own = (long)colour * sp.half + base;
+    const long oth = (long)(colour ^ 1) * sp.half + base;
+    const long sz = (long)sp.h * d.ny;
+    const bool xlo = q == 0, xhi = q == h4 - 1;  // thread holds i = 0 / i = nx - 1
+
+    // faces: lower ones at the own index, upper ones at the +x/+y/+z neighbours
+    float fxm[4], fym[4], fzm[4], fxp[4], fyp[4], fzp[4];
+    load4(F.lx + own, fxm);
+    load4(F.ly + own, fym);
+    load4(F.lz + own, fzm);
+    if (j < d.ny - 1) {
+        load4(F.ly + oth + sp.h, fyp);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) fyp[e] = cvt(F.uy[2 * (4 * q + e) + p + (long)d.nx * k]);
+    }
+    if (k < d.nz - 1) {
+        load4(F.lz + oth + sz, fzp);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) fzp[e] = cvt(F.uz[2 * (4 * q + e) + p + (long)d.nx * j]);
+    }
+    {
+        float lo[4];
+        load4(F.lx + oth, lo);  // lower x faces of the other colour, same places
+        float nxt = __shfl_down(lo[0], 1, 64);
+        if (lane == 63 && !xhi) nxt = cvt(F.lx[oth + 4]);
+        if (p == 0) {  // i = 2m: the +x neighbour is other-colour place m
+#pragma unroll
+            for (int e = 0; e < 4; ++e) fxp[e] = lo[e];
+        } else {       // i = 2m + 1: place m + 1
+            fxp[0] = lo[1];
+            fxp[1] = lo[2];
+            fxp[2] = lo[3];
+            fxp[3] = xhi ? cvt(F.ux[row]) : nxt;
+        }
+    }
+
+    float vxp[4], vxm[4], vyp[4], vym[4], vzp[4], vzm[4];
+    if (MODE == 1) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) vxp[e] = vxm[e] = vyp[e] = vym[e] = vzp[e] = vzm[e] = 0.0f;
+    } else {
+        const float* O = x + (long)(colour ^ 1) * sp.half;
+        float a[4];
+        load4(O + base, a);
+        float prv = __shfl_up(a[3], 1, 64);
+        float nxt = __shfl_down(a[0], 1, 64);
+        if (lane == 0 && !xlo) prv = O[base - 1];
+        if (lane == 63 && !xhi) nxt = O[base + 4];
+        if (p == 0) {  // x+ at place m, x- at m - 1
+            vxp[0] = a[0], vxp[1] = a[1], vxp[2] = a[2], vxp[3] = a[3];
+            vxm[1] = a[0], vxm[2] = a[1], vxm[3] = a[2];
+            vxm[0] = xlo ? (bc.t[0] == MF3D_BC_PERIODIC ? O[r0 + sp.h - 1] : 0.0f) : prv;
+        } else {       // x- at place m, x+ at m + 1
+            vxm[0] = a[0], vxm[1] = a[1], vxm[2] = a[2], vxm[3] = a[3];
+            vxp[0] = a[1], vxp[1] = a[2], vxp[2] = a[3];
+            vxp[3] = xhi ? (bc.t[1] == MF3D_BC_PERIODIC ? O[r0] : 0.0f) : nxt;
+        }
+        const long sy = sp.h;
+        if (j < d.ny - 1) load4(O + base + sy, vyp);
+        else if (bc.t[3] == MF3D_BC_PERIODIC) load4(O + base - (long)(d.ny - 1) * sy, vyp);
+        else vyp[0] = vyp[1] = vyp[2] = vyp[3] = 0.0f;
+        if (j > 0) load4(O + base - sy, vym);
+        else if (bc.t[2] == MF3D_BC_PERIODIC) load4(O + base + (long)(d.ny - 1) * sy, vym);
+        else vym[0] = vym[1] = vym[2] = vym[3] = 0.0f;
+        if (k < d.nz - 1) load4(O + base + sz, vzp);
+        else if (bc.t[5] == MF3D_BC_PERIODIC) load4(O + base - (long)(d.nz - 1) * sz, vzp);
+        else vzp[0] = vzp[1] = vzp[2] = vzp[3] = 0.0f;
+        if (k > 0) load4(O + base - sz, vzm);
+        else if (bc.t[4] == MF3D_BC_PERIODIC) load4(O + base + (long)(d.nz - 1) * sz, vzm);
+        else vzm[0] = vzm[1] = vzm[2] = vzm[3] = 0.0f;
+    }
+    float bb[4];
+    load4(b + own, bb);
+    float res[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        float f[6] = {fxp[e], fxm[e], fyp[e], fym[e], fzp[e], fzm[e]};
+        float v[6] = {vxp[e], vxm[e], vyp[e], vym[e], vzp[e], vzm[e]};
+        if (MODE == 2) add_parents<CS>(d, bc, xc, 2 * (4 * q + e) + p, j, k, v);
+        res[e] = gs_value(f, v, bb[e], bs, pin && colour == 0 && row == 0 && q == 0 && e == 0);
+    }
+    if (valid) *reinterpret_cast<float4*>(x + own) = make_float4(res[0], res[1], res[2], res[3]);
 }
 
 // prolong-add between any two orders (used when a level has no post-smoothing)
@@ -590,13 +751,13 @@ __global__ void __launch_bounds__(kBlock)
 template <typename TC>
 __device__ __forceinline__ float residual_cell_split(const Dims& d, const BcTypes& bc, bool pin,
                                                      const Split& sp, const float* x,
-                                                     const float* b, const TC* FX, const TC* FY,
-                                                     const TC* FZ, int i, int j, int k, float rs) {
+                                                     const float* b, const SFaces<TC>& F, int i,
+                                                     int j, int k, float rs) {
     const int c = (i + j + k) & 1;
     const long r0 = (long)sp.h * ((long)j + (long)d.ny * k);
     const long o = (long)c * sp.half + r0 + (i >> 1);
     float f[6], v[6];
-    cell_faces(d, FX, FY, FZ, i, j, k, f);
+    cell_faces_split(d, sp, F, i, j, k, f);
     neighbours_split(d, bc, x + (long)(c ^ 1) * sp.half, r0, sp.h, i, j, k, v);
     const float xc = x[o];
     float result = f[0] * (xc - v[0]);
@@ -615,8 +776,7 @@ template <typename TC, bool CS>
 __global__ void __launch_bounds__(kBlock)
     residual_restrict_split_kernel(Dims d, BcTypes bc, int pin, float rs,
                                    const float* __restrict__ x, const float* __restrict__ b,
-                                   const TC* __restrict__ FX, const TC* __restrict__ FY,
-                                   const TC* __restrict__ FZ, float* __restrict__ bcoarse) {
+                                   SFaces<TC> F, float* __restrict__ bcoarse) {
     const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
     const long total = (long)dc.nx * dc.ny * dc.nz;
     const long blk = xcd_block((total + kBlock - 1) / kBlock);
@@ -631,15 +791,63 @@ __global__ void __launch_bounds__(kBlock)
     const bool p = pin != 0;
     const Split sp = split_of(d);
     float s = 0.0f;
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j + 1, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j + 1, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j, k + 1, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j, k + 1, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i, j + 1, k + 1, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, FX, FY, FZ, i + 1, j + 1, k + 1, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j, k + 1, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j, k + 1, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k + 1, rs);
+    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k + 1, rs);
     bcoarse[CS ? split_of(dc).at(dc, I, J, Kc) : t] = s / 8.0f;
+}
+
+// face coefficients of a colour-split level from K (values as faces_build_kernel)
+template <typename TC>
+__global__ void __launch_bounds__(kBlock)
+    faces_build_split_kernel(Dims d, BcTypes bc, double cs, const double* __restrict__ K,
+                             TC* __restrict__ LX, TC* __restrict__ LY, TC* __restrict__ LZ,
+                             TC* __restrict__ UX, TC* __restrict__ UY, TC* __restrict__ UZ) {
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long n = (long)nx * ny * nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    const long o = blk * kBlock + threadIdx.x;
+    if (o >= n) return;
+    const int i = (int)(o % nx);
+    const long q = o / nx;
+    const int j = (int)(q % ny);
+    const int k = (int)(q / ny);
+    const long sy = nx, sz = (long)nx * ny;
+    const long so = split_of(d).at(d, i, j, k);
+    const double Kc = K[o];
+    const double iKc = 1.0 / Kc;
+    double f;
+    if (i > 0) f = 2.0 / (iKc + 1.0 / K[o - 1]);
+    else f = bc.t[0] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + nx - 1]) : bface(bc.t[0], Kc);
+    cstore(LX, so, f * cs);
+    if (j > 0) f = 2.0 / (iKc + 1.0 / K[o - sy]);
+    else f = bc.t[2] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(ny - 1) * sy])
+                                         : bface(bc.t[2], Kc);
+    cstore(LY, so, f * cs);
+    if (k > 0) f = 2.0 / (iKc + 1.0 / K[o - sz]);
+    else f = bc.t[4] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(nz - 1) * sz])
+                                         : bface(bc.t[4], Kc);
+    cstore(LZ, so, f * cs);
+    if (i == nx - 1) {
+        f = bc.t[1] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (nx - 1)]) : bface(bc.t[1], Kc);
+        cstore(UX, j + (long)ny * k, f * cs);
+    }
+    if (j == ny - 1) {
+        f = bc.t[3] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(ny - 1) * sy])
+                                        : bface(bc.t[3], Kc);
+        cstore(UY, i + (long)nx * k, f * cs);
+    }
+    if (k == nz - 1) {
+        f = bc.t[5] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(nz - 1) * sz])
+                                        : bface(bc.t[5], Kc);
+        cstore(UZ, i + (long)nx * j, f * cs);
+    }
 }
 
 // ---- prolongation for levels without the march kernel -----------------------
@@ -835,10 +1043,31 @@ size_t coef_bytes(CoefType t) {
     return t == COEF_F64 ? 8 : t == COEF_F32 ? 4 : 2;
 }
 
+namespace {
+template <typename TC>
+void launch_faces_split(hipStream_t s, const Dims& d, const BcTypes& bc, double cs,
+                        const double* K, FaceSet& f, unsigned gr) {
+    faces_build_split_kernel<TC><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (TC*)f.fx, (TC*)f.fy,
+                                                      (TC*)f.fz, (TC*)f.ux, (TC*)f.uy,
+                                                      (TC*)f.uz);
+}
+}  // namespace
+
 int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
                 const double* K, FaceSet& f) {
     const Dims d = dims(g);
     const unsigned gr = xcd_grid(cells(g));
+    if (f.split) {
+        if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "split faces need even sizes");
+        switch (f.type) {
+        case COEF_F32: launch_faces_split<float>(s, d, bc, cs, K, f, gr); break;
+        case COEF_BF16: launch_faces_split<__hip_bfloat16>(s, d, bc, cs, K, f, gr); break;
+        case COEF_F16: launch_faces_split<__half>(s, d, bc, cs, K, f, gr); break;
+        default: return fail(MF3D_ERR_UNSUPPORTED, "split faces are 32- or 16-bit");
+        }
+        MF3D_HIP_TRY(hipGetLastError());
+        return MF3D_OK;
+    }
     switch (f.type) {
     case COEF_F64:
         faces_build_kernel<double><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (double*)f.fx,
@@ -969,13 +1198,30 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
 namespace {
 
 template <typename TC>
+SFaces<TC> sfaces(const FaceSet& F) {
+    return SFaces<TC>{(const TC*)F.fx, (const TC*)F.fy, (const TC*)F.fz,
+                      (const TC*)F.ux, (const TC*)F.uy, (const TC*)F.uz};
+}
+
+template <typename TC, int MODE, bool CS>
+void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin, float bs,
+                  float* x, const float* b, const SFaces<TC>& F, int colour, const float* xc) {
+    const Dims d = dims(g);
+    const size_t n = cells(g);
+    if ((g.nx & 7) == 0)
+        colour_split4_kernel<TC, MODE, CS><<<xcd_grid(n / 8), kBlock, 0, s>>>(d, bc, pin, bs, x, b,
+                                                                            F, colour, xc);
+    else
+        colour_split_kernel<TC, MODE, CS><<<xcd_grid(n / 2), kBlock, 0, s>>>(d, bc, pin, bs, x, b,
+                                                                           F, colour, xc);
+}
+
+template <typename TC>
 int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, const float* b, const FaceSet& F, int iters, bool zero_init,
                  const float* xc, bool csplit) {
     const Dims d = dims(g);
-    const TC* FX = cptr<TC>(F, 0);
-    const TC* FY = cptr<TC>(F, 1);
-    const TC* FZ = cptr<TC>(F, 2);
+    const SFaces<TC> SF = sfaces<TC>(F);
     const size_t n = cells(g);
     const float fbs = (float)bs;
     const int p = pin ? 1 : 0;
@@ -993,21 +1239,16 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
         MF3D_HIP_TRY(hipGetLastError());
         return MF3D_OK;
     }
-    const unsigned gh = xcd_grid(n / 2);
     for (int it = 0; it < iters; ++it)
         for (int colour = 0; colour < 2; ++colour) {
             if (it == 0 && colour == 0 && zero_init)
-                colour_split_kernel<TC, 1, false><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
-                                                                        FY, FZ, colour, nullptr);
+                launch_split<TC, 1, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr);
             else if (it == 0 && colour == 0 && xc && csplit)
-                colour_split_kernel<TC, 2, true><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
-                                                                       FY, FZ, colour, xc);
+                launch_split<TC, 2, true>(s, g, bc, p, fbs, x, b, SF, colour, xc);
             else if (it == 0 && colour == 0 && xc)
-                colour_split_kernel<TC, 2, false><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
-                                                                        FY, FZ, colour, xc);
+                launch_split<TC, 2, false>(s, g, bc, p, fbs, x, b, SF, colour, xc);
             else
-                colour_split_kernel<TC, 0, false><<<gh, kBlock, 0, s>>>(d, bc, p, fbs, x, b, FX,
-                                                                        FY, FZ, colour, nullptr);
+                launch_split<TC, 0, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr);
         }
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
@@ -1020,12 +1261,10 @@ int rr_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, dou
     const Dims d = dims(g);
     if (csplit)
         residual_restrict_split_kernel<TC, true><<<xcd_grid(nc), kBlock, 0, s>>>(
-            d, bc, pin ? 1 : 0, (float)rs, x, b, cptr<TC>(F, 0), cptr<TC>(F, 1), cptr<TC>(F, 2),
-            bcoarse);
+            d, bc, pin ? 1 : 0, (float)rs, x, b, sfaces<TC>(F), bcoarse);
     else
         residual_restrict_split_kernel<TC, false><<<xcd_grid(nc), kBlock, 0, s>>>(
-            d, bc, pin ? 1 : 0, (float)rs, x, b, cptr<TC>(F, 0), cptr<TC>(F, 1), cptr<TC>(F, 2),
-            bcoarse);
+            d, bc, pin ? 1 : 0, (float)rs, x, b, sfaces<TC>(F), bcoarse);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -1039,6 +1278,8 @@ bool split_ok(const mf3d_grid& g) {
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder) {
+    if ((order == ORDER_SPLIT) != F.split)
+        return fail(MF3D_ERR_UNSUPPORTED, "vector order and face layout differ");
     if (order == ORDER_SPLIT) {
         if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
         const bool cs = corder == ORDER_SPLIT;
@@ -1081,6 +1322,8 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const float* x, const float* b, const FaceSet& F,
                             float* bcoarse, int order, int corder) {
+    if ((order == ORDER_SPLIT) != F.split)
+        return fail(MF3D_ERR_UNSUPPORTED, "vector order and face layout differ");
     if (order == ORDER_SPLIT) {
         if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
         const bool cs = corder == ORDER_SPLIT;

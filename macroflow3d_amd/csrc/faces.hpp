@@ -7,13 +7,20 @@ namespace mf3d {
 
 enum CoefType : int { COEF_F64 = 0, COEF_F32 = 1, COEF_BF16 = 2, COEF_F16 = 3 };
 
-// MAC face conductances of one level: fx (nx+1)*ny*nz, fy nx*(ny+1)*nz,
-// fz nx*ny*(nz+1) values of `type`, stored times a power-of-two scale.
+// Face conductances of one level, values of `type` stored times a
+// power-of-two scale. Natural (MAC) layout: fx (nx+1)*ny*nz, fy nx*(ny+1)*nz,
+// fz nx*ny*(nz+1). Split layout (colour-split levels): fx/fy/fz hold N values,
+// the lower faces of each cell at the cell's colour-split index, and ux
+// (ny*nz), uy (nx*nz), uz (nx*ny) the faces on the upper domain boundary.
 struct FaceSet {
     CoefType type = COEF_F64;
+    bool split = false;
     void* fx = nullptr;
     void* fy = nullptr;
     void* fz = nullptr;
+    void* ux = nullptr;
+    void* uy = nullptr;
+    void* uz = nullptr;
 };
 
 // Storage order of a level's FP32 V-cycle vectors x and b. NATURAL:

@@ -140,9 +140,21 @@ struct mf3d_workspace {
     }
     int alloc(double** p, size_t n) { return alloc_bytes((void**)p, n * sizeof(double)); }
     int alloc(float** p, size_t n) { return alloc_bytes((void**)p, n * sizeof(float)); }
-    int alloc_faces(FaceSet& f, const mf3d_grid& g, CoefType t) {
+    int alloc_faces(FaceSet& f, const mf3d_grid& g, CoefType t, bool split = false) {
         f.type = t;
+        f.split = split;
         const size_t b = kern::coef_bytes(t);
+        if (split) {
+            const size_t n = cells(g);
+            MF3D_TRY(alloc_bytes(&f.fx, n * b));
+            MF3D_TRY(alloc_bytes(&f.fy, n * b));
+            MF3D_TRY(alloc_bytes(&f.fz, n * b));
+            MF3D_TRY(alloc_bytes(&f.ux, (size_t)g.ny * g.nz * b));
+            MF3D_TRY(alloc_bytes(&f.uy, (size_t)g.nx * g.nz * b));
+            MF3D_TRY(alloc_bytes(&f.uz, (size_t)g.nx * g.ny * b));
+            return MF3D_OK;
+        }
+        f.ux = f.uy = f.uz = nullptr;
         MF3D_TRY(alloc_bytes(&f.fx, (size_t)(g.nx + 1) * g.ny * g.nz * b));
         MF3D_TRY(alloc_bytes(&f.fy, (size_t)g.nx * (g.ny + 1) * g.nz * b));
         MF3D_TRY(alloc_bytes(&f.fz, (size_t)g.nx * g.ny * (g.nz + 1) * b));
@@ -399,29 +411,28 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
                       : (kern::march_ok(L.g, ws->bct) && (long)L.n >= thr) ? SMOOTH_MARCH
                                                                            : SMOOTH_COLOUR;
         L.plan.zchunk = ws->zchunk;
-        if (mixed) {
-            if (!L.has_flo || L.flo.type != ws->lo_type) {
-                MF3D_TRY(ws->alloc_faces(L.flo, L.g, ws->lo_type));
-                L.has_flo = true;
-            }
-            L.cs = cs;
-            MF3D_TRY(kern::faces_build(s, L.g, ws->bct, cs, level_K(ws, l), L.flo));
-            if (!L.x32) MF3D_TRY(ws->alloc(&L.x32, L.n));
-            if (!L.b32) MF3D_TRY(ws->alloc(&L.b32, L.n));
-            if (L.plan.kind == SMOOTH_MARCH && !L.t32) MF3D_TRY(ws->alloc(&L.t32, L.n));
-        } else if (L.plan.kind == SMOOTH_MARCH && l > 0 && !L.t) {
-            MF3D_TRY(ws->alloc(&L.t, L.n));
-        }
+        if (!mixed && L.plan.kind == SMOOTH_MARCH && l > 0 && !L.t) MF3D_TRY(ws->alloc(&L.t, L.n));
         if (l == 0) L.t = ws->Ap;  // free while the V-cycle runs
     }
-    // colour-split FP32 vectors on a prefix of colour-swept levels (never the
-    // coarsest, whose direct or small-level solve works in natural order)
+    // colour-split FP32 vectors and faces on a prefix of colour-swept levels
+    // (never the coarsest, whose direct or small-level solve is natural order)
     bool prefix = mixed && ws->split;
     for (size_t l = 0; l < ws->lv.size(); ++l) {
         Level& L = ws->lv[l];
         prefix = prefix && l + 1 < ws->lv.size() && L.plan.kind == SMOOTH_COLOUR &&
                  kern::split_ok(L.g);
         L.order = prefix ? ORDER_SPLIT : ORDER_NATURAL;
+        if (!mixed) continue;
+        const bool split = L.order == ORDER_SPLIT;
+        if (!L.has_flo || L.flo.type != ws->lo_type || L.flo.split != split) {
+            MF3D_TRY(ws->alloc_faces(L.flo, L.g, ws->lo_type, split));
+            L.has_flo = true;
+        }
+        L.cs = cs;
+        MF3D_TRY(kern::faces_build(s, L.g, ws->bct, cs, level_K(ws, l), L.flo));
+        if (!L.x32) MF3D_TRY(ws->alloc(&L.x32, L.n));
+        if (!L.b32) MF3D_TRY(ws->alloc(&L.b32, L.n));
+        if (L.plan.kind == SMOOTH_MARCH && !L.t32) MF3D_TRY(ws->alloc(&L.t32, L.n));
     }
     if (cfg.coarse_solver == MF3D_COARSE_DIRECT) MF3D_TRY(build_coarse_inverse(ws, s));
     return MF3D_OK;

@@ -429,12 +429,15 @@ def test_fp64_march_and_colour_plans_agree(ctx):
 @pytest.mark.parametrize("shape,levels,pre_post", [((32, 16, 24), 4, (2, 2)),
                                                    ((64, 32, 32), 4, (1, 3)),
                                                    ((32, 32, 32), 3, (2, 0)),
-                                                   ((64, 64, 32), 4, (2, 0))])
+                                                   ((64, 64, 32), 4, (2, 0)),
+                                                   ((20, 16, 16), 2, (2, 2)),
+                                                   ((36, 16, 12), 3, (1, 2))])
 @pytest.mark.parametrize("coef", [ph.CoefFormat.BF16, ph.CoefFormat.FP16])
 def test_colour_split_vcycle_equals_natural_order(ctx, mix, shape, levels, pre_post, coef):
-    """Storing the FP32 V-cycle vectors colour by colour (and folding the
-    prolongation into the first post-smoothing half-sweep) gives the same
-    preconditioned vector as the natural-order kernels."""
+    """Storing the FP32 V-cycle vectors and faces colour by colour (and folding
+    the prolongation into the first post-smoothing half-sweep) gives the same
+    preconditioned vector as the natural-order kernels; nx a multiple of 8 runs
+    the 4-cells-per-thread kernel, other even nx the one-cell kernel."""
     g, _ = grid_pair(*shape)
     bc, _ = bc_pair(mix)
     n = g.num_cells()
