@@ -238,6 +238,45 @@ int mf3d_fill(mf3d_stream stream, double* x, size_t n, double value);
 /* blas::scal (scal.cu:7-27): x = a x */
 int mf3d_scal(mf3d_stream stream, double* x, size_t n, double a);
 
+/* ---- learned warm start (extension, no counterpart in MacroFlow3D) ----
+ * A 3-D convolutional encoder-decoder (3 levels, 16/32/64 channels, 3^3
+ * kernels, BF16 weights/activations, FP32 accumulation on MFMA) mapping ln K
+ * to an initial head h0 = h_lin + J y for the Darcy set-up with Dirichlet x
+ * faces (h_lin linear between them, J = (h_W - h_E) / nx); levels 1 and 2
+ * are 4 and 16 times coarser, so sizes must be divisible by 16. Parameters are FP32 masters: per layer (conv_a..conv_g)
+ * the weights [cout][27][cin] (tap = dx + 3 dy + 9 dz over offsets -1..1,
+ * +1) followed by the bias [cout]. */
+typedef struct mf3d_wsnet mf3d_wsnet;
+long mf3d_wsnet_param_count(void);
+int mf3d_wsnet_create(mf3d_wsnet** net);  /* parameters zero: set them before use */
+int mf3d_wsnet_destroy(mf3d_wsnet* net);
+int mf3d_wsnet_set_params(mf3d_wsnet* net, const float* host_params, long n); /* resets Adam */
+int mf3d_wsnet_get_params(const mf3d_wsnet* net, float* host_params, long n);
+int mf3d_wsnet_get_grads(const mf3d_wsnet* net, float* host_grads, long n);
+/* h0 (n >= cells) from K (k_is_log = 0) or ln K (k_is_log = 1). */
+int mf3d_wsnet_predict(mf3d_wsnet* net, mf3d_stream stream, const mf3d_grid* grid,
+                       const mf3d_bc* bc, const double* K, int k_is_log, double* h0, size_t n);
+/* Loss mean((y - (h - h_lin)/J)^2) for the solved head h and the gradient of
+ * every parameter (read with mf3d_wsnet_get_grads). Synchronises if loss != NULL. */
+int mf3d_wsnet_gradients(mf3d_wsnet* net, mf3d_stream stream, const mf3d_grid* grid,
+                         const mf3d_bc* bc, const double* K, int k_is_log, const double* h,
+                         size_t n, double* loss);
+/* mf3d_wsnet_gradients followed by one Adam step (beta 0.9/0.999, eps 1e-8). */
+int mf3d_wsnet_train_step(mf3d_wsnet* net, mf3d_stream stream, const mf3d_grid* grid,
+                          const mf3d_bc* bc, const double* K, int k_is_log, const double* h,
+                          size_t n, double lr, double* loss);
+/* Single-layer kernels (BF16 channels-last tensors [cells][channels]; zero
+ * padding; synchronous). Forward: Y = relu?(conv(X, w) + bias + R), cin a
+ * multiple of 8, cout 16/32/64. Data gradient: dX = conv^T(dY, w) * (M > 0).
+ * Weight gradient: gw [cout][27][cin], gb [cout] (FP32). */
+int mf3d_conv3d_forward(mf3d_stream stream, const mf3d_grid* grid, int cin, int cout,
+                        const void* X, const float* w, const float* bias, const void* R,
+                        int relu, void* Y);
+int mf3d_conv3d_backward_data(mf3d_stream stream, const mf3d_grid* grid, int cin, int cout,
+                              const void* dY, const float* w, const void* M, void* dX);
+int mf3d_conv3d_backward_weights(mf3d_stream stream, const mf3d_grid* grid, int cin, int cout,
+                                 const void* X, const void* dY, float* gw, float* gb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,21 @@ SIGNATURES = {
     "mf3d_copy": (_i, [_vp, _vp, _vp, _sz]),
     "mf3d_fill": (_i, [_vp, _vp, _sz, _d]),
     "mf3d_scal": (_i, [_vp, _vp, _sz, _d]),
+    # learned warm start
+    "mf3d_wsnet_param_count": (C.c_long, []),
+    "mf3d_wsnet_create": (_i, [_P(_vp)]),
+    "mf3d_wsnet_destroy": (_i, [_vp]),
+    "mf3d_wsnet_set_params": (_i, [_vp, _vp, C.c_long]),
+    "mf3d_wsnet_get_params": (_i, [_vp, _vp, C.c_long]),
+    "mf3d_wsnet_get_grads": (_i, [_vp, _vp, C.c_long]),
+    "mf3d_wsnet_predict": (_i, [_vp, _vp, _P(mf3d_grid), _P(mf3d_bc), _vp, _i, _vp, _sz]),
+    "mf3d_wsnet_gradients": (_i, [_vp, _vp, _P(mf3d_grid), _P(mf3d_bc), _vp, _i, _vp, _sz,
+                                  _P(_d)]),
+    "mf3d_wsnet_train_step": (_i, [_vp, _vp, _P(mf3d_grid), _P(mf3d_bc), _vp, _i, _vp, _sz, _d,
+                                   _P(_d)]),
+    "mf3d_conv3d_forward": (_i, [_vp, _P(mf3d_grid), _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "mf3d_conv3d_backward_data": (_i, [_vp, _P(mf3d_grid), _i, _i, _vp, _vp, _vp, _vp]),
+    "mf3d_conv3d_backward_weights": (_i, [_vp, _P(mf3d_grid), _i, _i, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

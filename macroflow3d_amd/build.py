@@ -18,7 +18,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmf3d_hip.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["stencil.hip", "stencil_fast.hip", "faces.hip", "blas.hip", "velocity.hip", "solver.hip"]
+SOURCES = ["stencil.hip", "stencil_fast.hip", "faces.hip", "blas.hip", "velocity.hip", "solver.hip",
+           "cnn.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -52,14 +52,16 @@ class HipContext:
 CudaContext = HipContext
 
 
-def dptr(t: torch.Tensor, need: int, name: str, ctx: HipContext | None = None) -> int:
-    """Checks a device array argument and returns its address."""
+def dptr(t: torch.Tensor, need: int, name: str, ctx: HipContext | None = None,
+         dtype: torch.dtype = torch.float64) -> int:
+    """Checks a device array argument (dtype, contiguity, device, length) and
+    returns its address."""
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name}: expected a torch tensor")
     if not t.is_cuda:
         raise ValueError(f"{name}: must be a device tensor")
-    if t.dtype != torch.float64:
-        raise TypeError(f"{name}: must be float64, got {t.dtype}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name}: must be {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError(f"{name}: must be contiguous")
     if ctx is not None and t.device != ctx.device:

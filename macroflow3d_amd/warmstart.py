@@ -1,0 +1,280 @@
+"""Learned warm start for the head solve (no counterpart in MacroFlow3D).
+
+A 3-D convolutional encoder-decoder (3 levels, 16/32/64 channels, 3^3
+kernels) maps ln K to an initial head for the Darcy set-up with Dirichlet x
+faces: h0 = h_lin + J y(ln K), h_lin linear between the x faces and
+J = (h_W - h_E) / nx, so one network serves any grid size. Forward, backward
+and the Adam step run in libmf3d_hip.so (BF16 MFMA convolutions with FP32
+accumulation, FP32 master weights); this module holds the parameters, makes
+training data with the plain FP64 MG-PCG solve and drives the training.
+
+The solve keeps its own guarantees: the prediction is only the initial
+iterate (``InitGuess.PROVIDED``), PCG still runs to ``||b - A h|| <= rtol ||b||``
+judged on the true residual, and a zero guess remains the fallback.
+
+Command line (the GPU runs; writes FP32 parameters as .npy):
+    python -m macroflow3d_amd.warmstart train --out weights.npy [--train 2000 --held 200]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import physics as ph
+from .core import BCSpec, Grid3D
+from .kfield import lognormal_lnK
+from .runtime import HipContext, dptr
+
+LAYERS = [(2, 16), (16, 32), (32, 64), (64, 64), (64, 32), (32, 16), (16, 1)]
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                               "warmstart_128.npy")
+
+
+def param_count() -> int:
+    return int(_lib.lib().mf3d_wsnet_param_count())
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights (std sqrt(2 / (27 cin))), zero biases; the output layer
+    scaled by 0.1 so an untrained network predicts close to the linear head."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for li, (ci, co) in enumerate(LAYERS):
+        std = np.sqrt(2.0 / (27 * ci)) * (0.1 if li == len(LAYERS) - 1 else 1.0)
+        parts.append((rng.standard_normal(co * 27 * ci) * std).astype(np.float32))
+        parts.append(np.zeros(co, dtype=np.float32))
+    return np.concatenate(parts)
+
+
+class WarmStartNet:
+    """Device copy of the network (parameters, Adam state, activations)."""
+
+    def __init__(self, params: np.ndarray | None = None, seed: int = 0):
+        h = C.c_void_p()
+        _lib.call("mf3d_wsnet_create", C.byref(h))
+        self._h = h
+        self.n = param_count()
+        self.set_params(init_params(seed) if params is None else params)
+
+    @classmethod
+    def load(cls, path: str = DEFAULT_WEIGHTS) -> "WarmStartNet":
+        return cls(np.load(path, allow_pickle=False))
+
+    def save(self, path: str) -> None:
+        np.save(path, self.get_params())
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n:
+            raise ValueError(f"expected {self.n} parameters, got {p.size}")
+        _lib.call("mf3d_wsnet_set_params", self._h, p.ctypes.data, self.n)
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n, dtype=np.float32)
+        _lib.call("mf3d_wsnet_get_params", self._h, p.ctypes.data, self.n)
+        return p
+
+    def get_grads(self) -> np.ndarray:
+        g = np.empty(self.n, dtype=np.float32)
+        _lib.call("mf3d_wsnet_get_grads", self._h, g.ctypes.data, self.n)
+        return g
+
+    def predict(self, ctx: HipContext, K: torch.Tensor, grid: Grid3D, bc: BCSpec,
+                h0: torch.Tensor, k_is_log: bool = False) -> None:
+        """h0 <- network prediction (enqueued on ctx's stream)."""
+        n = grid.num_cells()
+        _lib.call("mf3d_wsnet_predict", self._h, ctx.handle, C.byref(grid.c()),
+                  C.byref(bc.c()), dptr(K, n, "K", ctx), 1 if k_is_log else 0,
+                  dptr(h0, n, "h0", ctx), n)
+
+    def gradients(self, ctx: HipContext, K: torch.Tensor, h: torch.Tensor, grid: Grid3D,
+                  bc: BCSpec, k_is_log: bool = False) -> float:
+        n = grid.num_cells()
+        loss = C.c_double()
+        _lib.call("mf3d_wsnet_gradients", self._h, ctx.handle, C.byref(grid.c()),
+                  C.byref(bc.c()), dptr(K, n, "K", ctx), 1 if k_is_log else 0,
+                  dptr(h, n, "h", ctx), n, C.byref(loss))
+        return loss.value
+
+    def train_step(self, ctx: HipContext, K: torch.Tensor, h: torch.Tensor, grid: Grid3D,
+                   bc: BCSpec, lr: float, k_is_log: bool = False, want_loss: bool = True) -> float:
+        n = grid.num_cells()
+        loss = C.c_double()
+        _lib.call("mf3d_wsnet_train_step", self._h, ctx.handle, C.byref(grid.c()),
+                  C.byref(bc.c()), dptr(K, n, "K", ctx), 1 if k_is_log else 0,
+                  dptr(h, n, "h", ctx), n, float(lr), C.byref(loss) if want_loss else None)
+        return loss.value if want_loss else float("nan")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.lib().mf3d_wsnet_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- training data ------------------------------------------------------------
+def field_params(seed: int, count: int, sigma2=(2.0, 4.0), corr=(4.0, 12.0)):
+    """(sigma2, correlation length, field seed) per realisation, from a seeded
+    host generator; field seed = seed + r."""
+    rng = np.random.default_rng(seed)
+    s2 = rng.uniform(*sigma2, count)
+    lam = rng.uniform(*corr, count)
+    return [(float(s2[r]), float(lam[r]), seed + r) for r in range(count)]
+
+
+def make_samples(ctx: HipContext, n: int, params, levels: int = 5, rtol: float = 1e-10,
+                 log=None):
+    """lnK and the plain FP64 MG-PCG head for each (sigma2, corr, seed); device tensors."""
+    g = Grid3D(n, n, n)
+    bc = BCSpec.darcy_x()
+    cfg = ph.HeadSolveConfig.north_star(rtol, levels)
+    ws = ph.FlowWorkspace()
+    out = []
+    t0 = time.perf_counter()
+    for r, (s2, lam, seed) in enumerate(params):
+        lnK, _ = lognormal_lnK(n, n, n, s2, lam, seed, device=ctx.device)
+        h = ctx.zeros(g.num_cells())
+        res = ph.solve_head(h, torch.exp(lnK), g, bc, cfg, ctx, ws)
+        if not res.converged:
+            raise RuntimeError(f"training solve {r} did not converge")
+        out.append((lnK, h))
+        if log and (r + 1) % 100 == 0:
+            log(f"samples {r + 1}/{len(params)} ({time.perf_counter() - t0:.1f} s)")
+    ws.close()
+    return out
+
+
+def train(ctx: HipContext, net: WarmStartNet, samples, n: int, epochs: int, lr: float,
+          lr_final: float, log=None, seed: int = 0, time_budget: float = float("inf")):
+    """Adam over the samples (one field per step, shuffled each epoch, learning
+    rate decayed geometrically from lr to lr_final). Stops after the epoch in
+    which time_budget seconds have passed. Returns mean sampled loss per epoch."""
+    t_start = time.perf_counter()
+    g = Grid3D(n, n, n)
+    bc = BCSpec.darcy_x()
+    rng = np.random.default_rng(seed)
+    total = epochs * len(samples)
+    hist = []
+    step = 0
+    for ep in range(epochs):
+        order = rng.permutation(len(samples))
+        acc = 0.0
+        t0 = time.perf_counter()
+        for q, r in enumerate(order):
+            rate = lr * (lr_final / lr) ** (step / max(1, total - 1))
+            want = (q % 50 == 0)
+            loss = net.train_step(ctx, samples[r][0], samples[r][1], g, bc, rate, k_is_log=True,
+                                  want_loss=want)
+            if want:
+                acc += loss
+            step += 1
+            if log and want and q % 500 == 0:
+                log(f"  step {q}/{len(samples)} loss {loss:.4e} lr {rate:.2e}")
+        hist.append(acc / max(1, (len(samples) + 49) // 50))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: sampled loss {hist[-1]:.4e} "
+                f"({time.perf_counter() - t0:.1f} s)")
+        if time.perf_counter() - t_start > time_budget:
+            break
+    return hist
+
+
+def compare_guesses(ctx: HipContext, net: WarmStartNet, samples, n: int, levels: int = 5,
+                    rtol: float = 1e-10, precond=ph.PrecondPrecision.FP64):
+    """PCG iterations and times to ||b - A h|| <= rtol ||b|| from the zero, the
+    linear and the predicted initial head, per sample."""
+    g = Grid3D(n, n, n)
+    bc = BCSpec.darcy_x()
+    ws = ph.FlowWorkspace()
+    rows = []
+    for lnK, _ in samples:
+        K = torch.exp(lnK)
+        row = {}
+        for name in ("zero", "linear", "predicted"):
+            cfg = ph.HeadSolveConfig.north_star(rtol, levels)
+            cfg.precond_precision = precond
+            h = ctx.zeros(g.num_cells())
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name == "linear":
+                cfg.init_guess = ph.InitGuess.REFERENCE
+            elif name == "predicted":
+                net.predict(ctx, lnK, g, bc, h, k_is_log=True)
+                cfg.init_guess = ph.InitGuess.PROVIDED
+            res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+            ms = (time.perf_counter() - t0) * 1e3
+            row[name] = {"iterations": res.num_iterations, "converged": res.converged,
+                         "rel_true_residual": res.true_residual / res.bnorm, "ms": ms,
+                         "initial_rel_residual": res.initial_residual / res.bnorm}
+        rows.append(row)
+    ws.close()
+    return rows
+
+
+def _main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", required=True)
+    t.add_argument("--n", type=int, default=128)
+    t.add_argument("--train", type=int, default=2000)
+    t.add_argument("--held", type=int, default=200)
+    t.add_argument("--epochs", type=int, default=4)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--lr-final", type=float, default=1e-4)
+    t.add_argument("--seed", type=int, default=1000)
+    t.add_argument("--report", default=None, help="JSON summary path")
+    t.add_argument("--time-budget", type=float, default=float("inf"),
+                   help="stop after the epoch that passes this many seconds of training")
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    ctx = HipContext(0)
+    ptrain = field_params(a.seed, a.train)
+    pheld = field_params(a.seed + 10 ** 6, a.held)
+    log(f"making {a.train} training + {a.held} held-out fields at {a.n}^3")
+    tr = make_samples(ctx, a.n, ptrain, log=log)
+    net = WarmStartNet(seed=a.seed)
+    hist = train(ctx, net, tr, a.n, a.epochs, a.lr, a.lr_final, log=log, seed=a.seed,
+                 time_budget=a.time_budget)
+    net.save(a.out)
+    del tr
+    held = make_samples(ctx, a.n, pheld, log=log)
+    g = Grid3D(a.n, a.n, a.n)
+    bc = BCSpec.darcy_x()
+    held_loss = float(np.mean([net.gradients(ctx, lnK, h, g, bc, k_is_log=True)
+                               for lnK, h in held]))
+    rows = compare_guesses(ctx, net, held, a.n)
+    summ = {k: {"mean_iterations": float(np.mean([r[k]["iterations"] for r in rows])),
+                "mean_ms": float(np.mean([r[k]["ms"] for r in rows])),
+                "all_converged": all(r[k]["converged"] for r in rows),
+                "max_rel_true_residual": max(r[k]["rel_true_residual"] for r in rows)}
+            for k in ("zero", "linear", "predicted")}
+    rep = {"n": a.n, "train": a.train, "held": a.held, "epochs": a.epochs, "lr": a.lr,
+           "lr_final": a.lr_final, "seed": a.seed, "loss_per_epoch": hist,
+           "held_loss": held_loss, "held_out": summ}
+    log(json.dumps(rep))
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f, indent=1)
+    net.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(_main())
