@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "mf3d.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|size_t|const char\*)\s+(mf3d_\w+)\(", text,
+    return sorted(set(re.findall(r"^\s*(?:int|long|void|size_t|const char\*)\s+(mf3d_\w+)\(", text,
                                  re.M)))
 
 
