@@ -131,6 +131,24 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
                     const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
                     double* h, size_t n, mf3d_head_solve_result* result);
 
+/* Extension (no counterpart in MacroFlow3D; replaces the sequential solve
+ * loop of EnsembleRunner.cu:259-311): `count` independent head solves, one
+ * workspace per realisation (distinct, reused across calls), grids of any
+ * sizes, shared BCs and configuration. Every kernel launch of the iteration
+ * covers all realisations; one PCG iteration is captured once into a hipGraph
+ * and replayed between host checks (flags & MF3D_BATCH_NO_GRAPH: plain
+ * launches). Requires PCG_MG with the FP32 V-cycle, the direct coarsest solve,
+ * pre/post smoothing >= 1, flexible = 0, and nx % 8 == 0 on every
+ * colour-swept level. Each realisation stops on its own tolerance; results[r]
+ * holds its iterations and residuals (setup_ms/solve_ms: the whole batch).
+ * Synchronises. */
+enum { MF3D_BATCH_NO_GRAPH = 1 };
+int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream stream,
+                            const mf3d_grid* grids, const mf3d_bc* bc,
+                            const mf3d_head_solve_config* cfg, const double* const* K,
+                            double* const* h, const size_t* n, int flags,
+                            mf3d_head_solve_result* results);
+
 /* physics::init_head_guess (solve_head.cuh:173-174, solve_head.cu:56-73). */
 int mf3d_init_head_guess(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, double* h,
                          size_t n);

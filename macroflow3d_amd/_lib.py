@@ -95,6 +95,9 @@ SIGNATURES = {
     "mf3d_copy": (_i, [_vp, _vp, _vp, _sz]),
     "mf3d_fill": (_i, [_vp, _vp, _sz, _d]),
     "mf3d_scal": (_i, [_vp, _vp, _sz, _d]),
+    "mf3d_solve_head_batched": (_i, [_i, _P(_vp), _vp, _P(mf3d_grid), _P(mf3d_bc), _cfg,
+                                     _P(_vp), _P(_vp), _P(_sz), _i,
+                                     _P(mf3d_head_solve_result)]),
     # learned warm start
     "mf3d_wsnet_param_count": (C.c_long, []),
     "mf3d_wsnet_create": (_i, [_P(_vp)]),

@@ -78,9 +78,8 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ void __launch_bounds__(kFinalBlock)
-    finalize_kernel(const double* __restrict__ partials, unsigned np, int op,
-                    IterState* __restrict__ st, double* __restrict__ out) {
+__device__ void finalize_body(const double* __restrict__ partials, unsigned np, int op,
+                              IterState* __restrict__ st, double* __restrict__ out) {
     // four independent accumulators keep several loads in flight per thread
     double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
     unsigned i = threadIdx.x;
@@ -168,6 +167,72 @@ __global__ void __launch_bounds__(kFinalBlock)
         break;
     default: break;
     }
+}
+
+__global__ void __launch_bounds__(kFinalBlock)
+    finalize_kernel(const double* __restrict__ partials, unsigned np, int op,
+                    IterState* __restrict__ st, double* __restrict__ out) {
+    finalize_body(partials, np, op, st, out);
+}
+
+// ---- batched (one launch for many realisations, descriptors BatchVec) --------
+// second stage: one block per realisation; guard 0: always, 1: only running
+// realisations (stop == 0), 2: only converged ones (stop == 1)
+__global__ void __launch_bounds__(kFinalBlock)
+    finalize_batch_kernel(const BatchVec* __restrict__ Vs, int op, int guard) {
+    const BatchVec& V = Vs[blockIdx.x];
+    const int stop = V.st->stop;
+    if ((guard == 1 && stop != 0) || (guard == 2 && stop != 1)) return;
+    finalize_body(V.partials, V.np, op, V.st, V.out);
+}
+
+// MODE 0: (r, z), skipped once stopped; MODE 1: (rhs, rhs)
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    dot_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m) {
+    const int e = batch_entry(start, m, blockIdx.x);
+    const BatchVec& V = Vs[e];
+    if (MODE == 0 && V.st->stop) return;
+    const long blk = (long)blockIdx.x - start[e];
+    const long n = (long)V.nx * V.ny * V.nz;
+    const long i = blk * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (i < n) v = MODE == 0 ? V.r[i] * V.z[i] : V.rhs[i] * V.rhs[i];
+    const double s = block_sum<kBlock>(v);
+    if (threadIdx.x == 0) V.partials[blk] = s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    xr_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m) {
+    const int e = batch_entry(start, m, blockIdx.x);
+    const BatchVec& V = Vs[e];
+    if (V.st->stop) return;
+    const double alpha = V.st->alpha;
+    const long blk = (long)blockIdx.x - start[e];
+    const long n = (long)V.nx * V.ny * V.nz;
+    const long i = blk * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        V.x[i] = alpha * V.p[i] + V.x[i];
+        const double ri = -alpha * V.Ap[i] + V.r[i];
+        V.r[i] = ri;
+        v = ri * ri;
+    }
+    const double s = block_sum<kBlock>(v);
+    if (threadIdx.x == 0) V.partials[blk] = s;
+}
+
+// MODE 0: p = z + beta p; MODE 1: p = z (start of a cycle)
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    p_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m) {
+    const int e = batch_entry(start, m, blockIdx.x);
+    const BatchVec& V = Vs[e];
+    if (V.st->stop) return;
+    const double beta = V.st->beta;
+    const long n = (long)V.nx * V.ny * V.nz;
+    const long i = ((long)blockIdx.x - start[e]) * kBlock + threadIdx.x;
+    if (i < n) V.p[i] = MODE == 0 ? 1.0 * V.z[i] + beta * V.p[i] : V.z[i];
 }
 
 // PCG x/r update (pcg.cuh:149-153 as two axpy) fused with (r, r).
@@ -301,6 +366,30 @@ int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const doub
 }
 int cg_update_p(hipStream_t s, const IterState* st, const double* r, double* p, size_t n) {
     cg_update_p_kernel<<<grid_for(n), kBlock, 0, s>>>(st, r, p, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+
+int batch_finalize(hipStream_t s, const BatchVec* V, int m, Finalize op, int guard) {
+    finalize_batch_kernel<<<m, kFinalBlock, 0, s>>>(V, (int)op, guard);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int batch_dot(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks) {
+    if (mode == 0) dot_batch_kernel<0><<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
+    else dot_batch_kernel<1><<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int batch_update_xr(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks) {
+    xr_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int batch_update_p(hipStream_t s, int mode, const BatchVec* V, const int* start, int m,
+                   long nblocks) {
+    if (mode == 0) p_batch_kernel<0><<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
+    else p_batch_kernel<1><<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }

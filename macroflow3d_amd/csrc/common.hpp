@@ -134,4 +134,62 @@ enum Finalize : int {
 constexpr int kReduceBlock = 256;   // threads per first-stage block
 constexpr int kFinalBlock = 1024;   // threads of the second stage
 
+// ---------------------------------------------------------------------------
+// Batched solves: one launch covers many realisations. Entry e of a launch
+// owns the blocks [start[e], start[e+1]) (start in device memory, m + 1
+// values); each block finds its entry by binary search and works on that
+// entry's descriptor. A realisation whose state says stop skips guarded work.
+// ---------------------------------------------------------------------------
+struct BatchLevel {  // one FP32 V-cycle level of one realisation
+    int nx, ny, nz;
+    BcTypes bc;
+    int pin;
+    float bs, rs;          // cs dx^2 and 1 / (cs dx^2)
+    float* x;
+    float* b;
+    float* bcoarse;        // right-hand side of the next level (restriction target)
+    const float* xc;       // correction of the next level (prolongation source)
+    const void* fx;        // faces (natural MAC or colour-split, type of the solve)
+    const void* fy;
+    const void* fz;
+    const void* ux;
+    const void* uy;
+    const void* uz;
+    const double* Ainv;    // coarsest level: dense inverse
+    int nc;                // coarsest level: cells
+    const IterState* st;
+};
+
+struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
+    int nx, ny, nz;
+    BcTypes bc;
+    int pin;
+    double inv_dx2;
+    const double* fx;
+    const double* fy;
+    const double* fz;
+    double* x;
+    double* r;
+    double* z;
+    double* p;
+    double* Ap;
+    double* rhs;
+    float* b32;
+    float* x32;
+    double* partials;
+    unsigned np;
+    IterState* st;
+    double* out;  // scalar result of a FIN_STORE / FIN_SQRT second stage
+};
+
+__device__ __forceinline__ int batch_entry(const int* __restrict__ start, int m, long blk) {
+    int lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 }  // namespace mf3d

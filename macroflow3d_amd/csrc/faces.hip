@@ -363,15 +363,13 @@ __device__ __forceinline__ void load4(const TC* p, float (&o)[4]) {
 }
 
 template <typename TC, int MODE, bool CS>
-__global__ void __launch_bounds__(kBlock)
-    colour_split4_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
-                         const float* __restrict__ b, SFaces<TC> F, int colour,
-                         const float* __restrict__ xc) {
+__device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, int pin, float bs,
+                                            float* __restrict__ x, const float* __restrict__ b,
+                                            const SFaces<TC>& F, int colour,
+                                            const float* __restrict__ xc, long blk) {
     const Split sp = split_of(d);
     const int h4 = sp.h >> 2;
     const long total = sp.half >> 2;
-    const long blk = xcd_block((total + kBlock - 1) / kBlock);
-    if (blk < 0) return;  // whole workgroup
     const long t = blk * kBlock + threadIdx.x;
     const bool valid = t < total;
     const long tt = valid ? t : total - 1;  // lanes past the end mirror the last one
@@ -469,6 +467,17 @@ __global__ void __launch_bounds__(kBlock)
     if (valid) *reinterpret_cast<float4*>(x + own) = make_float4(res[0], res[1], res[2], res[3]);
 }
 
+template <typename TC, int MODE, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    colour_split4_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
+                         const float* __restrict__ b, SFaces<TC> F, int colour,
+                         const float* __restrict__ xc) {
+    const long total = ((long)(d.nx >> 1) * d.ny * d.nz) >> 2;
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;  // whole workgroup
+    split4_body<TC, MODE, CS>(d, bc, pin, bs, x, b, F, colour, xc, blk);
+}
+
 // prolong-add between any two orders (used when a level has no post-smoothing)
 template <bool FS, bool CS>
 __global__ void __launch_bounds__(kBlock)
@@ -492,11 +501,11 @@ constexpr int kSmallBlock = 1024;
 constexpr int kSmallPer = 4;
 
 template <typename T, typename TC>
-__global__ void __launch_bounds__(kSmallBlock)
-    small_kernel(Dims d, BcTypes bc, int pin, T bs, T* __restrict__ x, const T* __restrict__ b,
-                 const TC* __restrict__ FX, const TC* __restrict__ FY,
-                 const TC* __restrict__ FZ, int iters, int zero_init, const T* __restrict__ xc) {
-    __shared__ T xs[kSmallBlock * kSmallPer];
+__device__ __forceinline__ void small_body(T* xs, const Dims& d, const BcTypes& bc, int pin, T bs,
+                                           T* __restrict__ x, const T* __restrict__ b,
+                                           const TC* __restrict__ FX, const TC* __restrict__ FY,
+                                           const TC* __restrict__ FZ, int iters, int zero_init,
+                                           const T* __restrict__ xc) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const int n = nx * ny * nz;
     T f[kSmallPer][6], bl[kSmallPer];
@@ -531,6 +540,15 @@ __global__ void __launch_bounds__(kSmallBlock)
         }
     }
     for (int o = threadIdx.x; o < n; o += kSmallBlock) x[o] = xs[o];
+}
+
+template <typename T, typename TC>
+__global__ void __launch_bounds__(kSmallBlock)
+    small_kernel(Dims d, BcTypes bc, int pin, T bs, T* __restrict__ x, const T* __restrict__ b,
+                 const TC* __restrict__ FX, const TC* __restrict__ FY,
+                 const TC* __restrict__ FZ, int iters, int zero_init, const T* __restrict__ xc) {
+    __shared__ T xs[kSmallBlock * kSmallPer];
+    small_body<T, TC>(xs, d, bc, pin, bs, x, b, FX, FY, FZ, iters, zero_init, xc);
 }
 
 // ---- one-pass red-black march (fine levels) ----------------------------------
@@ -718,15 +736,13 @@ __device__ __forceinline__ T residual_cell(const Dims& d, const BcTypes& bc, boo
 }
 
 template <typename T, typename TC>
-__global__ void __launch_bounds__(kBlock)
-    residual_restrict_kernel(Dims d, BcTypes bc, int pin, T rs, const T* __restrict__ x,
-                             const T* __restrict__ b, const TC* __restrict__ FX,
-                             const TC* __restrict__ FY, const TC* __restrict__ FZ,
-                             T* __restrict__ bcoarse) {
+__device__ __forceinline__ void rr_body(const Dims& d, const BcTypes& bc, int pin, T rs,
+                                        const T* __restrict__ x, const T* __restrict__ b,
+                                        const TC* __restrict__ FX, const TC* __restrict__ FY,
+                                        const TC* __restrict__ FZ, T* __restrict__ bcoarse,
+                                        long blk) {
     const int NX = d.nx >> 1, NY = d.ny >> 1, NZ = d.nz >> 1;
     const long total = (long)NX * NY * NZ;
-    const long blk = xcd_block((total + kBlock - 1) / kBlock);
-    if (blk < 0) return;
     const long t = blk * kBlock + threadIdx.x;
     if (t >= total) return;
     const int I = (int)(t % NX);
@@ -745,6 +761,18 @@ __global__ void __launch_bounds__(kBlock)
     s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i, j + 1, k + 1, rs);
     s += residual_cell(d, bc, p, x, b, FX, FY, FZ, i + 1, j + 1, k + 1, rs);
     bcoarse[t] = s / T(8);
+}
+
+template <typename T, typename TC>
+__global__ void __launch_bounds__(kBlock)
+    residual_restrict_kernel(Dims d, BcTypes bc, int pin, T rs, const T* __restrict__ x,
+                             const T* __restrict__ b, const TC* __restrict__ FX,
+                             const TC* __restrict__ FY, const TC* __restrict__ FZ,
+                             T* __restrict__ bcoarse) {
+    const long total = (long)(d.nx >> 1) * (d.ny >> 1) * (d.nz >> 1);
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    rr_body<T, TC>(d, bc, pin, rs, x, b, FX, FY, FZ, bcoarse, blk);
 }
 
 // residual of one cell of a colour-split level (same sums as residual_cell)
@@ -773,14 +801,12 @@ __device__ __forceinline__ float residual_cell_split(const Dims& d, const BcType
 // residual + restriction of a colour-split fine level into a coarse level of
 // either order
 template <typename TC, bool CS>
-__global__ void __launch_bounds__(kBlock)
-    residual_restrict_split_kernel(Dims d, BcTypes bc, int pin, float rs,
-                                   const float* __restrict__ x, const float* __restrict__ b,
-                                   SFaces<TC> F, float* __restrict__ bcoarse) {
+__device__ __forceinline__ void rr_split_body(const Dims& d, const BcTypes& bc, int pin, float rs,
+                                              const float* __restrict__ x,
+                                              const float* __restrict__ b, const SFaces<TC>& F,
+                                              float* __restrict__ bcoarse, long blk) {
     const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
     const long total = (long)dc.nx * dc.ny * dc.nz;
-    const long blk = xcd_block((total + kBlock - 1) / kBlock);
-    if (blk < 0) return;
     const long t = blk * kBlock + threadIdx.x;
     if (t >= total) return;
     const int I = (int)(t % dc.nx);
@@ -800,6 +826,17 @@ __global__ void __launch_bounds__(kBlock)
     s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k + 1, rs);
     s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k + 1, rs);
     bcoarse[CS ? split_of(dc).at(dc, I, J, Kc) : t] = s / 8.0f;
+}
+
+template <typename TC, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    residual_restrict_split_kernel(Dims d, BcTypes bc, int pin, float rs,
+                                   const float* __restrict__ x, const float* __restrict__ b,
+                                   SFaces<TC> F, float* __restrict__ bcoarse) {
+    const long total = (long)(d.nx >> 1) * (d.ny >> 1) * (d.nz >> 1);
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    rr_split_body<TC, CS>(d, bc, pin, rs, x, b, F, bcoarse, blk);
 }
 
 // face coefficients of a colour-split level from K (values as faces_build_kernel)
@@ -910,14 +947,14 @@ __device__ __forceinline__ double apply_cell(const Dims& d, const BcTypes& bc, b
 
 // MODE 0: y = A x, partials of (x, y).  MODE 1: r = b - A x, partials of (r, r).
 template <int MODE>
-__global__ void __launch_bounds__(kBlock)
-    op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, const double* __restrict__ x,
-              const double* __restrict__ FX, const double* __restrict__ FY,
-              const double* __restrict__ FZ, const double* __restrict__ b,
-              double* __restrict__ y, double* __restrict__ partials) {
+__device__ __forceinline__ void op_body(const Dims& d, const BcTypes& bc, int pin, double inv_dx2,
+                                        const double* __restrict__ x,
+                                        const double* __restrict__ FX,
+                                        const double* __restrict__ FY,
+                                        const double* __restrict__ FZ,
+                                        const double* __restrict__ b, double* __restrict__ y,
+                                        double* __restrict__ partials, long blk) {
     const long n = (long)d.nx * d.ny * d.nz;
-    const long blk = xcd_block((n + kBlock - 1) / kBlock);
-    if (blk < 0) return;  // padding block: uniform exit before the block reduction
     const long o = blk * kBlock + threadIdx.x;
     double prod = 0.0;
     if (o < n) {
@@ -937,6 +974,18 @@ __global__ void __launch_bounds__(kBlock)
     }
     const double s = block_sum<kBlock>(prod);
     if (threadIdx.x == 0) partials[blk] = s;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, const double* __restrict__ x,
+              const double* __restrict__ FX, const double* __restrict__ FY,
+              const double* __restrict__ FZ, const double* __restrict__ b,
+              double* __restrict__ y, double* __restrict__ partials) {
+    const long n = (long)d.nx * d.ny * d.nz;
+    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    if (blk < 0) return;  // padding block: uniform exit before the block reduction
+    op_body<MODE>(d, bc, pin, inv_dx2, x, FX, FY, FZ, b, y, partials, blk);
 }
 
 template <typename T>
@@ -966,11 +1015,9 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // FP64 natural -> FP32 colour-split, and back
-__global__ void __launch_bounds__(kBlock)
-    to_f32_split_kernel(Dims d, const double* __restrict__ a, float* __restrict__ o) {
+__device__ __forceinline__ void to_f32_split_body(const Dims& d, const double* __restrict__ a,
+                                                  float* __restrict__ o, long blk) {
     const long n = (long)d.nx * d.ny * d.nz;
-    const long blk = xcd_block((n + kBlock - 1) / kBlock);
-    if (blk < 0) return;
     const long e = blk * kBlock + threadIdx.x;
     if (e >= n) return;
     const int i = (int)(e % d.nx);
@@ -979,11 +1026,9 @@ __global__ void __launch_bounds__(kBlock)
     const int k = (int)(q / d.ny);
     o[split_of(d).at(d, i, j, k)] = (float)a[e];
 }
-__global__ void __launch_bounds__(kBlock)
-    to_f64_split_kernel(Dims d, const float* __restrict__ a, double* __restrict__ o) {
+__device__ __forceinline__ void to_f64_split_body(const Dims& d, const float* __restrict__ a,
+                                                  double* __restrict__ o, long blk) {
     const long n = (long)d.nx * d.ny * d.nz;
-    const long blk = xcd_block((n + kBlock - 1) / kBlock);
-    if (blk < 0) return;
     const long e = blk * kBlock + threadIdx.x;
     if (e >= n) return;
     const int i = (int)(e % d.nx);
@@ -992,13 +1037,24 @@ __global__ void __launch_bounds__(kBlock)
     const int k = (int)(q / d.ny);
     o[e] = (double)a[split_of(d).at(d, i, j, k)];
 }
+__global__ void __launch_bounds__(kBlock)
+    to_f32_split_kernel(Dims d, const double* __restrict__ a, float* __restrict__ o) {
+    const long blk = xcd_block(((long)d.nx * d.ny * d.nz + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    to_f32_split_body(d, a, o, blk);
+}
+__global__ void __launch_bounds__(kBlock)
+    to_f64_split_kernel(Dims d, const float* __restrict__ a, double* __restrict__ o) {
+    const long blk = xcd_block(((long)d.nx * d.ny * d.nz + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    to_f64_split_body(d, a, o, blk);
+}
 
 // coarsest direct solve in FP64 for any x/b type: x = A^{-1} b, one wave per row
 template <typename T>
-__global__ void __launch_bounds__(256)
-    gemv_kernel(int n, const double* __restrict__ Ainv, const T* __restrict__ b,
-                T* __restrict__ x) {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void gemv_body(int n, const double* __restrict__ Ainv,
+                                          const T* __restrict__ b, T* __restrict__ x, long blk) {
+    const int row = (int)blk * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     const double* a = Ainv + (long)row * n;
@@ -1007,6 +1063,121 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) x[row] = (T)s;
+}
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gemv_kernel(int n, const double* __restrict__ Ainv, const T* __restrict__ b,
+                T* __restrict__ x) {
+    gemv_body<T>(n, Ainv, b, x, blockIdx.x);
+}
+
+// ---- batched V-cycle launches -------------------------------------------------
+template <typename TC>
+__device__ __forceinline__ SFaces<TC> batch_sfaces(const BatchLevel& L) {
+    return SFaces<TC>{(const TC*)L.fx, (const TC*)L.fy, (const TC*)L.fz,
+                      (const TC*)L.ux, (const TC*)L.uy, (const TC*)L.uz};
+}
+
+template <typename TC, int MODE, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    split4_batch_kernel(const BatchLevel* __restrict__ Ls, const int* __restrict__ start, int m,
+                        long nblocks, int colour) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchLevel& L = Ls[e];
+    if (L.st->stop) return;
+    split4_body<TC, MODE, CS>(Dims{L.nx, L.ny, L.nz}, L.bc, L.pin, L.bs, L.x, L.b,
+                              batch_sfaces<TC>(L), colour, L.xc, blk - start[e]);
+}
+
+template <typename TC, bool CS>
+__global__ void __launch_bounds__(kBlock)
+    rr_split_batch_kernel(const BatchLevel* __restrict__ Ls, const int* __restrict__ start, int m,
+                          long nblocks) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchLevel& L = Ls[e];
+    if (L.st->stop) return;
+    rr_split_body<TC, CS>(Dims{L.nx, L.ny, L.nz}, L.bc, L.pin, L.rs, L.x, L.b,
+                          batch_sfaces<TC>(L), L.bcoarse, blk - start[e]);
+}
+
+// small (single-workgroup) levels: one workgroup per entry
+template <typename TC>
+__global__ void __launch_bounds__(kSmallBlock)
+    small_batch_kernel(const BatchLevel* __restrict__ Ls, int iters, int zero_init, int use_xc) {
+    __shared__ float xs[kSmallBlock * kSmallPer];
+    const BatchLevel& L = Ls[blockIdx.x];
+    if (L.st->stop) return;
+    small_body<float, TC>(xs, Dims{L.nx, L.ny, L.nz}, L.bc, L.pin, L.bs, L.x, L.b,
+                          (const TC*)L.fx, (const TC*)L.fy, (const TC*)L.fz, iters, zero_init,
+                          use_xc ? L.xc : nullptr);
+}
+
+template <typename TC>
+__global__ void __launch_bounds__(kBlock)
+    rr_batch_kernel(const BatchLevel* __restrict__ Ls, const int* __restrict__ start, int m,
+                    long nblocks) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchLevel& L = Ls[e];
+    if (L.st->stop) return;
+    rr_body<float, TC>(Dims{L.nx, L.ny, L.nz}, L.bc, L.pin, L.rs, L.x, L.b, (const TC*)L.fx,
+                       (const TC*)L.fy, (const TC*)L.fz, L.bcoarse, blk - start[e]);
+}
+
+__global__ void __launch_bounds__(256)
+    gemv_batch_kernel(const BatchLevel* __restrict__ Ls, const int* __restrict__ start, int m) {
+    const int e = batch_entry(start, m, blockIdx.x);
+    const BatchLevel& L = Ls[e];
+    if (L.st->stop) return;
+    gemv_body<float>(L.nc, L.Ainv, L.b, L.x, (long)blockIdx.x - start[e]);
+}
+
+// finest level: FP64 <-> FP32 colour-split conversions, FP64 operator
+__global__ void __launch_bounds__(kBlock)
+    tof32_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
+                       long nblocks) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchVec& V = Vs[e];
+    if (V.st->stop) return;
+    to_f32_split_body(Dims{V.nx, V.ny, V.nz}, V.r, V.b32, blk - start[e]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    tof64_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
+                       long nblocks) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchVec& V = Vs[e];
+    if (V.st->stop) return;
+    to_f64_split_body(Dims{V.nx, V.ny, V.nz}, V.x32, V.z, blk - start[e]);
+}
+
+// MODE 0: Ap = A p with (p, Ap) partials, skipped once stopped. MODE 1:
+// r = rhs - A x with (r, r) partials, always (the true-residual check).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    op_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
+                    long nblocks) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchVec& V = Vs[e];
+    if (MODE == 0 && V.st->stop) return;
+    const Dims d{V.nx, V.ny, V.nz};
+    if (MODE == 0)
+        op_body<0>(d, V.bc, V.pin, V.inv_dx2, V.p, V.fx, V.fy, V.fz, nullptr, V.Ap, V.partials,
+                   blk - start[e]);
+    else
+        op_body<1>(d, V.bc, V.pin, V.inv_dx2, V.x, V.fx, V.fy, V.fz, V.rhs, V.r, V.partials,
+                   blk - start[e]);
 }
 
 MarchGeom geometry(const mf3d_grid& g, int zchunk) {
@@ -1411,6 +1582,117 @@ int gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x) {
 
 int gemv(hipStream_t s, int n, const double* Ainv, const float* b, float* x) {
     gemv_kernel<float><<<(n + 3) / 4, 256, 0, s>>>(n, Ainv, b, x);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+// ---- batched launchers ----------------------------------------------------------
+namespace {
+template <typename TC>
+void launch_split4_batch(hipStream_t s, int mode, bool cs, const BatchLevel* L, const int* start,
+                         int m, long nblocks, int colour) {
+    const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
+    if (mode == 1)
+        split4_batch_kernel<TC, 1, false><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+    else if (mode == 2 && cs)
+        split4_batch_kernel<TC, 2, true><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+    else if (mode == 2)
+        split4_batch_kernel<TC, 2, false><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+    else
+        split4_batch_kernel<TC, 0, false><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+}
+template <typename TC>
+void launch_rr_split_batch(hipStream_t s, bool cs, const BatchLevel* L, const int* start, int m,
+                           long nblocks) {
+    const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
+    if (cs) rr_split_batch_kernel<TC, true><<<g, kBlock, 0, s>>>(L, start, m, nblocks);
+    else rr_split_batch_kernel<TC, false><<<g, kBlock, 0, s>>>(L, start, m, nblocks);
+}
+}  // namespace
+
+long batch_blocks(long threads) { return (threads + kBlock - 1) / kBlock; }
+
+int batch_sweep(hipStream_t s, CoefType t, int mode, bool cs, const BatchLevel* L,
+                const int* start, int m, long nblocks, int colour) {
+    switch (t) {
+    case COEF_F32: launch_split4_batch<float>(s, mode, cs, L, start, m, nblocks, colour); break;
+    case COEF_BF16:
+        launch_split4_batch<__hip_bfloat16>(s, mode, cs, L, start, m, nblocks, colour);
+        break;
+    case COEF_F16: launch_split4_batch<__half>(s, mode, cs, L, start, m, nblocks, colour); break;
+    default: return fail(MF3D_ERR_UNSUPPORTED, "batched sweep: coefficient type");
+    }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_residual_restrict_split(hipStream_t s, CoefType t, bool cs, const BatchLevel* L,
+                                  const int* start, int m, long nblocks) {
+    switch (t) {
+    case COEF_F32: launch_rr_split_batch<float>(s, cs, L, start, m, nblocks); break;
+    case COEF_BF16: launch_rr_split_batch<__hip_bfloat16>(s, cs, L, start, m, nblocks); break;
+    case COEF_F16: launch_rr_split_batch<__half>(s, cs, L, start, m, nblocks); break;
+    default: return fail(MF3D_ERR_UNSUPPORTED, "batched residual: coefficient type");
+    }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_small(hipStream_t s, CoefType t, const BatchLevel* L, int m, int iters, bool zero_init,
+                bool use_xc) {
+    switch (t) {
+    case COEF_F32:
+        small_batch_kernel<float><<<m, kSmallBlock, 0, s>>>(L, iters, zero_init, use_xc);
+        break;
+    case COEF_BF16:
+        small_batch_kernel<__hip_bfloat16><<<m, kSmallBlock, 0, s>>>(L, iters, zero_init, use_xc);
+        break;
+    case COEF_F16:
+        small_batch_kernel<__half><<<m, kSmallBlock, 0, s>>>(L, iters, zero_init, use_xc);
+        break;
+    default: return fail(MF3D_ERR_UNSUPPORTED, "batched small level: coefficient type");
+    }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_residual_restrict(hipStream_t s, CoefType t, const BatchLevel* L, const int* start,
+                            int m, long nblocks) {
+    const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
+    switch (t) {
+    case COEF_F32: rr_batch_kernel<float><<<g, kBlock, 0, s>>>(L, start, m, nblocks); break;
+    case COEF_BF16:
+        rr_batch_kernel<__hip_bfloat16><<<g, kBlock, 0, s>>>(L, start, m, nblocks);
+        break;
+    case COEF_F16: rr_batch_kernel<__half><<<g, kBlock, 0, s>>>(L, start, m, nblocks); break;
+    default: return fail(MF3D_ERR_UNSUPPORTED, "batched residual: coefficient type");
+    }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_gemv(hipStream_t s, const BatchLevel* L, const int* start, int m, long nblocks) {
+    gemv_batch_kernel<<<(unsigned)nblocks, 256, 0, s>>>(L, start, m);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_to_f32(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks) {
+    tof32_batch_kernel<<<(unsigned)(((nblocks + 7) / 8) * 8), kBlock, 0, s>>>(V, start, m, nblocks);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_to_f64(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks) {
+    tof64_batch_kernel<<<(unsigned)(((nblocks + 7) / 8) * 8), kBlock, 0, s>>>(V, start, m, nblocks);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks) {
+    const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
+    if (mode == 0) op_batch_kernel<0><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
+    else op_batch_kernel<1><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

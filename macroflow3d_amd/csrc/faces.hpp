@@ -83,5 +83,30 @@ int fill_f32(hipStream_t s, float* x, size_t n, float v);
 // partials of sum(log K) for the coefficient scale
 int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials);
 
+// ---- batched launches (descriptors and block starts in device memory) --------
+// kBlock-thread blocks needed for `threads` threads
+long batch_blocks(long threads);
+// one colour of GSRB on colour-split levels (nx % 8 == 0), mode as
+// colour_split4_kernel: 0 plain, 1 zero start, 2 prolongation folded in (cs:
+// coarse level colour-split); nblocks = sum of batch_blocks(n / 8)
+int batch_sweep(hipStream_t s, CoefType t, int mode, bool cs, const BatchLevel* L,
+                const int* start, int m, long nblocks, int colour);
+// residual + restriction of colour-split levels (nblocks: sum of batch_blocks(n / 8))
+int batch_residual_restrict_split(hipStream_t s, CoefType t, bool cs, const BatchLevel* L,
+                                  const int* start, int m, long nblocks);
+// whole smoothing of small natural-order levels, one workgroup per entry
+int batch_small(hipStream_t s, CoefType t, const BatchLevel* L, int m, int iters, bool zero_init,
+                bool use_xc);
+// residual + restriction of natural-order levels (nblocks: sum of batch_blocks(n / 8))
+int batch_residual_restrict(hipStream_t s, CoefType t, const BatchLevel* L, const int* start,
+                            int m, long nblocks);
+// coarsest direct solve (nblocks: sum of ceil(nc / 4))
+int batch_gemv(hipStream_t s, const BatchLevel* L, const int* start, int m, long nblocks);
+// finest level (nblocks: sum of batch_blocks(n)): r -> b32 split, x32 split -> z,
+// mode 0: Ap = A p (+ (p, Ap) partials), mode 1: r = rhs - A x (+ (r, r) partials)
+int batch_to_f32(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
+int batch_to_f64(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
+int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
+
 }  // namespace kern
 }  // namespace mf3d

@@ -60,6 +60,19 @@ int cg_update_p(hipStream_t s, const IterState* st, const double* r, double* p, 
 // y = a * x (used to negate vectors for the CG path)
 int scale_copy(hipStream_t s, double a, const double* x, double* y, size_t n);
 
+// ---- batched Krylov steps (blas.hip; BatchVec descriptors, nblocks = sum of
+// reduce_blocks(n) over the realisations, one partial per block) -------------
+// second stage per realisation (one block each); guard 0: all, 1: only
+// running realisations (stop == 0), 2: only converged ones (stop == 1)
+int batch_finalize(hipStream_t s, const BatchVec* V, int m, Finalize op, int guard);
+// mode 0: (r, z) partials (guarded), mode 1: (rhs, rhs) partials
+int batch_dot(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
+// x += alpha p, r -= alpha Ap with (r, r) partials (guarded)
+int batch_update_xr(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
+// mode 0: p = z + beta p, mode 1: p = z (guarded)
+int batch_update_p(hipStream_t s, int mode, const BatchVec* V, const int* start, int m,
+                   long nblocks);
+
 // ---- velocity (velocity.hip) -----------------------------------------------
 int velocity(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* h,
              const double* K, double* U, double* V, double* W, bool padded);

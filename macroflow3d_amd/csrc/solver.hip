@@ -713,6 +713,296 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// ---------------------------------------------------------------------------
+// batched solves: many realisations (own workspaces, any grid sizes) advanced
+// together, every kernel launch covering all of them; one PCG iteration is
+// captured once into a hipGraph and replayed between host checks.
+// ---------------------------------------------------------------------------
+struct BLaunch {
+    int m = 0;
+    long nblocks = 0;
+    void* desc = nullptr;  // device BatchLevel[m] or BatchVec[m]
+    int* start = nullptr;  // device, m + 1 block offsets
+};
+
+struct BDepth {
+    BLaunch s4_cs, s4_cn;  // colour-split levels whose next level is split / natural
+    BLaunch small, small_rr;  // small natural-order levels (non-coarsest)
+    BLaunch direct;         // coarsest direct solves
+};
+
+struct BPlan {
+    std::vector<void*> mem;
+    BLaunch vec;  // finest level, blocks of reduce_blocks(n)
+    std::vector<BDepth> depth;
+    CoefType type = COEF_BF16;
+    int pre = 2, post = 2;
+    ~BPlan() {
+        for (void* q : mem) (void)hipFree(q);
+    }
+};
+
+template <typename D>
+int upload_launch(BPlan& P, BLaunch& L, const std::vector<D>& d, const std::vector<long>& blocks) {
+    L.m = (int)d.size();
+    if (L.m == 0) return MF3D_OK;
+    std::vector<int> start(d.size() + 1, 0);
+    for (size_t e = 0; e < d.size(); ++e) {
+        if (start[e] + blocks[e] > (long)INT32_MAX)
+            return fail(MF3D_ERR_UNSUPPORTED, "batched launch larger than 2^31 blocks");
+        start[e + 1] = start[e] + (int)blocks[e];
+    }
+    L.nblocks = start.back();
+    void* q = nullptr;
+    MF3D_HIP_TRY(hipMalloc(&q, d.size() * sizeof(D)));
+    P.mem.push_back(q);
+    MF3D_HIP_TRY(hipMemcpy(q, d.data(), d.size() * sizeof(D), hipMemcpyHostToDevice));
+    L.desc = q;
+    MF3D_HIP_TRY(hipMalloc(&q, start.size() * sizeof(int)));
+    P.mem.push_back(q);
+    MF3D_HIP_TRY(hipMemcpy(q, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+    L.start = static_cast<int*>(q);
+    return MF3D_OK;
+}
+
+// Launch plan of a batch whose workspaces are set up (mixed precision, direct
+// coarsest solve). Levels must be colour-split with nx % 8 == 0, small, or the
+// coarsest.
+int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h) {
+    P.type = ws[0]->lo_type;
+    P.pre = ws[0]->pre;
+    P.post = ws[0]->post;
+    size_t maxl = 0;
+    for (int r = 0; r < count; ++r) maxl = std::max(maxl, ws[r]->lv.size());
+    P.depth.resize(maxl);
+    std::vector<BatchVec> vd;
+    std::vector<long> vb;
+    struct G {
+        std::vector<BatchLevel> d;
+        std::vector<long> b;
+    };
+    std::vector<G> s4c(maxl), s4n(maxl), sm(maxl), smr(maxl), dir(maxl);
+    for (int r = 0; r < count; ++r) {
+        mf3d_workspace* w = ws[r];
+        Level& L0 = w->lv[0];
+        if (L0.order != ORDER_SPLIT || (L0.g.nx & 7))
+            return fail(MF3D_ERR_UNSUPPORTED,
+                        "batched solve: the finest level must be colour-split with nx % 8 == 0");
+        BatchVec v{};
+        v.nx = L0.g.nx, v.ny = L0.g.ny, v.nz = L0.g.nz;
+        v.bc = w->bct;
+        v.pin = w->pin ? 1 : 0;
+        v.inv_dx2 = 1.0 / (L0.g.dx * L0.g.dx);
+        v.fx = (const double*)L0.f64.fx;
+        v.fy = (const double*)L0.f64.fy;
+        v.fz = (const double*)L0.f64.fz;
+        v.x = h[r];
+        v.r = w->r, v.z = w->z, v.p = w->p, v.Ap = w->Ap, v.rhs = w->rhs;
+        v.b32 = L0.b32, v.x32 = L0.x32;
+        v.partials = w->partials;
+        v.np = kern::reduce_blocks(L0.n);
+        v.st = w->st;
+        v.out = w->dscalar;
+        vd.push_back(v);
+        vb.push_back((long)kern::reduce_blocks(L0.n));
+        const size_t nl = w->lv.size();
+        for (size_t l = 0; l < nl; ++l) {
+            Level& L = w->lv[l];
+            const double dx2 = L.g.dx * L.g.dx;
+            BatchLevel b{};
+            b.nx = L.g.nx, b.ny = L.g.ny, b.nz = L.g.nz;
+            b.bc = w->bct;
+            b.pin = w->pin ? 1 : 0;
+            b.bs = (float)(L.cs * dx2);
+            b.rs = (float)(1.0 / (L.cs * dx2));
+            b.x = L.x32, b.b = L.b32;
+            b.fx = L.flo.fx, b.fy = L.flo.fy, b.fz = L.flo.fz;
+            b.ux = L.flo.ux, b.uy = L.flo.uy, b.uz = L.flo.uz;
+            b.st = w->st;
+            if (l + 1 == nl) {
+                b.Ainv = w->coarse_inv;
+                b.nc = (int)L.n;
+                dir[l].d.push_back(b);
+                dir[l].b.push_back((long)((L.n + 3) / 4));
+                continue;
+            }
+            Level& C = w->lv[l + 1];
+            b.bcoarse = C.b32;
+            b.xc = C.x32;
+            const long nb8 = kern::batch_blocks((long)(L.n / 8));
+            if (L.order == ORDER_SPLIT) {
+                if (L.g.nx & 7)
+                    return fail(MF3D_ERR_UNSUPPORTED,
+                                "batched solve: colour-split levels need nx % 8 == 0");
+                G& grp = C.order == ORDER_SPLIT ? s4c[l] : s4n[l];
+                grp.d.push_back(b);
+                grp.b.push_back(nb8);
+            } else if (L.plan.kind == SMOOTH_SMALL) {
+                sm[l].d.push_back(b);
+                sm[l].b.push_back(1);
+                smr[l].d.push_back(b);
+                smr[l].b.push_back(nb8);
+            } else {
+                return fail(MF3D_ERR_UNSUPPORTED, "batched solve: level kind not batched");
+            }
+        }
+    }
+    MF3D_TRY(upload_launch(P, P.vec, vd, vb));
+    for (size_t l = 0; l < maxl; ++l) {
+        MF3D_TRY(upload_launch(P, P.depth[l].s4_cs, s4c[l].d, s4c[l].b));
+        MF3D_TRY(upload_launch(P, P.depth[l].s4_cn, s4n[l].d, s4n[l].b));
+        MF3D_TRY(upload_launch(P, P.depth[l].small, sm[l].d, sm[l].b));
+        MF3D_TRY(upload_launch(P, P.depth[l].small_rr, smr[l].d, smr[l].b));
+        MF3D_TRY(upload_launch(P, P.depth[l].direct, dir[l].d, dir[l].b));
+    }
+    return MF3D_OK;
+}
+
+#define BL(L) static_cast<const BatchLevel*>((L).desc), (L).start, (L).m, (L).nblocks
+#define BV(L) static_cast<const BatchVec*>((L).desc), (L).start, (L).m, (L).nblocks
+
+// V-cycle from x = 0 at depth d for every realisation that has that level
+int vcycle_batch(const BPlan& P, hipStream_t s, size_t d) {
+    const BDepth& D = P.depth[d];
+    for (int q = 0; q < 2; ++q) {
+        const BLaunch& L = q == 0 ? D.s4_cs : D.s4_cn;
+        if (!L.m) continue;
+        for (int it = 0; it < P.pre; ++it)
+            for (int c = 0; c < 2; ++c)
+                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? 1 : 0, q == 0, BL(L), c));
+        MF3D_TRY(kern::batch_residual_restrict_split(s, P.type, q == 0, BL(L)));
+    }
+    if (D.small.m) {
+        MF3D_TRY(kern::batch_small(s, P.type, static_cast<const BatchLevel*>(D.small.desc),
+                                   D.small.m, P.pre, true, false));
+        MF3D_TRY(kern::batch_residual_restrict(s, P.type, BL(D.small_rr)));
+    }
+    if (D.direct.m) {
+        const BLaunch& L = D.direct;
+        MF3D_TRY(kern::batch_gemv(s, static_cast<const BatchLevel*>(L.desc), L.start, L.m,
+                                  L.nblocks));
+    }
+    if (d + 1 < P.depth.size()) MF3D_TRY(vcycle_batch(P, s, d + 1));
+    for (int q = 0; q < 2; ++q) {
+        const BLaunch& L = q == 0 ? D.s4_cs : D.s4_cn;
+        if (!L.m) continue;
+        for (int it = 0; it < P.post; ++it)
+            for (int c = 0; c < 2; ++c)
+                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? 2 : 0, q == 0, BL(L), c));
+    }
+    if (D.small.m)
+        MF3D_TRY(kern::batch_small(s, P.type, static_cast<const BatchLevel*>(D.small.desc),
+                                   D.small.m, P.post, false, true));
+    return MF3D_OK;
+}
+
+int precondition_batch(const BPlan& P, hipStream_t s) {
+    MF3D_TRY(kern::batch_to_f32(s, BV(P.vec)));
+    MF3D_TRY(vcycle_batch(P, s, 0));
+    return kern::batch_to_f64(s, BV(P.vec));
+}
+
+// one PCG iteration for every running realisation (pcg.cuh:139-180)
+int pcg_iteration_batch(const BPlan& P, hipStream_t s) {
+    const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
+    const int m = P.vec.m;
+    MF3D_TRY(kern::batch_op(s, 0, BV(P.vec)));
+    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_PAP, 1));
+    MF3D_TRY(kern::batch_update_xr(s, BV(P.vec)));
+    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RR, 1));
+    MF3D_TRY(precondition_batch(P, s));
+    MF3D_TRY(kern::batch_dot(s, 0, BV(P.vec)));
+    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RZ, 1));
+    return kern::batch_update_p(s, 0, BV(P.vec));
+}
+
+int read_states(mf3d_workspace* const* ws, int count, hipStream_t s) {
+    for (int r = 0; r < count; ++r)
+        MF3D_HIP_TRY(hipMemcpyAsync(ws[r]->host_st, ws[r]->st, sizeof(IterState),
+                                    hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    return MF3D_OK;
+}
+
+struct GraphHolder {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~GraphHolder() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+int capture_iteration(const BPlan& P, hipStream_t s, GraphHolder& G) {
+    MF3D_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int st = pcg_iteration_batch(P, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (st != MF3D_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(MF3D_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                       hipGetErrorString(e));
+    G.g = g;
+    MF3D_HIP_TRY(hipGraphInstantiate(&G.x, g, nullptr, nullptr, 0));
+    return MF3D_OK;
+}
+
+// PCG of run_pcg for all realisations (FP32 preconditioner: convergence is
+// judged on the true residual, with up to 3 restarts)
+int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
+                  const mf3d_head_solve_config& cfg, const BPlan& P, bool use_graph,
+                  int* restarts_out) {
+    const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
+    const int m = P.vec.m;
+    const IterState st0 = make_state(cfg, cfg.cg_rtol);
+    for (int r = 0; r < count; ++r) {
+        *ws[r]->host_st = st0;
+        MF3D_HIP_TRY(hipMemcpyAsync(ws[r]->st, ws[r]->host_st, sizeof(IterState),
+                                    hipMemcpyHostToDevice, s));
+    }
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    MF3D_TRY(kern::batch_dot(s, 1, BV(P.vec)));
+    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_BNORM, 0));
+    MF3D_TRY(kern::batch_op(s, 1, BV(P.vec)));
+    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_R0, 0));
+    GraphHolder G;
+    if (use_graph) MF3D_TRY(capture_iteration(P, s, G));
+    int restarts = 0;
+    for (;;) {
+        // start of a cycle for the running realisations: z = M r, rz, p = z
+        MF3D_TRY(precondition_batch(P, s));
+        MF3D_TRY(kern::batch_dot(s, 0, BV(P.vec)));
+        MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RZ0, 1));
+        MF3D_TRY(kern::batch_update_p(s, 1, BV(P.vec)));
+        MF3D_TRY(read_states(ws, count, s));
+        for (;;) {
+            bool running = false;
+            for (int r = 0; r < count; ++r) running = running || !ws[r]->host_st->stop;
+            if (!running) break;
+            for (int c = 0; c < 4; ++c) {
+                if (use_graph) MF3D_HIP_TRY(hipGraphLaunch(G.x, s));
+                else MF3D_TRY(pcg_iteration_batch(P, s));
+            }
+            MF3D_TRY(read_states(ws, count, s));
+        }
+        // converged on the recursive residual: judge by the true one
+        MF3D_TRY(kern::batch_op(s, 1, BV(P.vec)));
+        MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RESUME, 2));
+        MF3D_TRY(read_states(ws, count, s));
+        bool again = false;
+        for (int r = 0; r < count; ++r) again = again || !ws[r]->host_st->stop;
+        if (!again || restarts >= 3) break;
+        ++restarts;
+    }
+    *restarts_out = restarts;
+    return MF3D_OK;
+}
+
+#undef BL
+#undef BV
+
 }  // namespace
 
 // ===========================================================================
@@ -851,6 +1141,98 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     result->setup_ms = elapsed(ws->ev[0], ws->ev[1]);
     result->solve_ms = elapsed(ws->ev[1], ws->ev[2]);
     return MF3D_OK;
+}
+
+int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream stream,
+                            const mf3d_grid* grids, const mf3d_bc* bc,
+                            const mf3d_head_solve_config* cfg, const double* const* K,
+                            double* const* h, const size_t* n, int flags,
+                            mf3d_head_solve_result* results) {
+    if (count <= 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "count must be positive");
+    MF3D_CHECK_PTR(ws);
+    MF3D_CHECK_PTR(grids);
+    MF3D_CHECK_PTR(K);
+    MF3D_CHECK_PTR(h);
+    MF3D_CHECK_PTR(n);
+    MF3D_CHECK_PTR(results);
+    MF3D_TRY(validate_bc(bc));
+    MF3D_TRY(check_config(cfg));
+    if (cfg->solver_type != MF3D_SOLVER_PCG_MG || cfg->precond_precision != MF3D_PRECOND_FP32 ||
+        cfg->coarse_solver != MF3D_COARSE_DIRECT || cfg->flexible || cfg->mg_pre_smooth < 1 ||
+        cfg->mg_post_smooth < 1)
+        return fail(MF3D_ERR_UNSUPPORTED,
+                    "batched solve: PCG_MG with the FP32 V-cycle, direct coarsest solve, "
+                    "pre/post smoothing >= 1 and the standard beta");
+    for (int r = 0; r < count; ++r) {
+        MF3D_CHECK_PTR(ws[r]);
+        MF3D_CHECK_PTR(K[r]);
+        MF3D_CHECK_PTR(h[r]);
+        MF3D_TRY(validate_grid(&grids[r]));
+        if (n[r] < cells(grids[r])) return fail(MF3D_ERR_SIZE_MISMATCH, "Head or K buffer too small");
+        for (int q = 0; q < r; ++q)
+            if (ws[q] == ws[r]) return fail(MF3D_ERR_INVALID_ARGUMENT, "workspaces must differ");
+        std::memset(&results[r], 0, sizeof(results[r]));
+        results[r].true_residual = -1.0;
+    }
+    clear_stale_error();
+    hipStream_t caller = as_stream(stream);
+    hipStream_t s = nullptr;
+    MF3D_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{s};
+    hipEvent_t ev[4];
+    for (auto& e : ev) MF3D_HIP_TRY(hipEventCreate(&e));
+    struct EventGuard {
+        hipEvent_t* e;
+        ~EventGuard() {
+            for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } eguard{ev};
+    MF3D_HIP_TRY(hipEventRecord(ev[3], caller));
+    MF3D_HIP_TRY(hipStreamWaitEvent(s, ev[3], 0));
+    MF3D_HIP_TRY(hipEventRecord(ev[0], s));
+    for (int r = 0; r < count; ++r) {
+        mf3d_workspace* w = ws[r];
+        MF3D_TRY(ensure_workspace(w, grids[r], cfg->mg_levels));
+        MF3D_TRY(mg_setup(w, s, grids[r], *bc, *cfg, K[r]));
+        MF3D_TRY(kern::rhs(s, grids[r], *bc, K[r], w->rhs));
+        if (cfg->init_guess != MF3D_GUESS_PROVIDED)
+            MF3D_TRY(kern::guess(s, grids[r], *bc, cfg->init_guess, h[r]));
+    }
+    BPlan P;
+    MF3D_TRY(build_plan(P, ws, count, h));
+    MF3D_HIP_TRY(hipEventRecord(ev[1], s));
+    int restarts = 0;
+    MF3D_TRY(run_pcg_batch(ws, count, s, *cfg, P, !(flags & MF3D_BATCH_NO_GRAPH), &restarts));
+    MF3D_HIP_TRY(hipEventRecord(ev[2], s));
+    if (cfg->true_residual) {
+        const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
+        MF3D_TRY(kern::batch_op(s, 1, V, P.vec.start, P.vec.m, P.vec.nblocks));
+        MF3D_TRY(kern::batch_finalize(s, V, P.vec.m, FIN_SQRT, 0));
+        for (int r = 0; r < count; ++r)
+            MF3D_HIP_TRY(hipMemcpyAsync(&results[r].true_residual, ws[r]->dscalar, sizeof(double),
+                                        hipMemcpyDeviceToHost, s));
+    }
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    const float setup = elapsed(ev[0], ev[1]), solve = elapsed(ev[1], ev[2]);
+    int status = MF3D_OK;
+    for (int r = 0; r < count; ++r) {
+        const IterState& st = *ws[r]->host_st;
+        results[r].num_iterations = st.iter;
+        results[r].converged = st.stop == 1;
+        results[r].initial_residual = st.r0;
+        results[r].final_residual = std::sqrt(st.rr);
+        results[r].bnorm = st.bnorm;
+        results[r].restarts = restarts;
+        results[r].setup_ms = setup;
+        results[r].solve_ms = solve;
+        if (st.stop == 2) status = fail(MF3D_ERR_BREAKDOWN, "PCG breakdown: (p, Ap) not finite");
+    }
+    MF3D_HIP_TRY(hipEventRecord(ev[3], s));
+    MF3D_HIP_TRY(hipStreamWaitEvent(caller, ev[3], 0));
+    return status;
 }
 
 int mf3d_init_head_guess(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc* bc, double* h,

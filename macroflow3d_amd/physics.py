@@ -220,6 +220,30 @@ def solve_head(h: torch.Tensor, K: torch.Tensor, grid: Grid3D, bc: BCSpec,
                            res.solve_ms, res.restarts)
 
 
+def solve_head_batched(hs, Ks, grids, bc: BCSpec, cfg: HeadSolveConfig, ctx: HipContext,
+                       workspaces, use_graph: bool = True) -> list:
+    """Many independent head solves in one batched run (extension; replaces the
+    sequential loop of EnsembleRunner.cu:259-311). One FlowWorkspace per
+    realisation; grids may differ; BCs and configuration are shared. Needs the
+    FP32 V-cycle with the direct coarsest solve (``HeadSolveConfig.mixed``).
+    Synchronises; returns one HeadSolveResult per realisation."""
+    bc.validate()
+    m = len(hs)
+    if not (len(Ks) == len(grids) == len(workspaces) == m) or m == 0:
+        raise ValueError("hs, Ks, grids and workspaces must have the same nonzero length")
+    G = (_lib.mf3d_grid * m)(*[g.c() for g in grids])
+    W = (C.c_void_p * m)(*[w.handle.value for w in workspaces])
+    KP = (C.c_void_p * m)(*[dptr(K, g.num_cells(), "K", ctx) for K, g in zip(Ks, grids)])
+    HP = (C.c_void_p * m)(*[dptr(h, g.num_cells(), "h", ctx) for h, g in zip(hs, grids)])
+    N = (C.c_size_t * m)(*[g.num_cells() for g in grids])
+    R = (_lib.mf3d_head_solve_result * m)()
+    _lib.call("mf3d_solve_head_batched", m, W, ctx.handle, G, C.byref(bc.c()), C.byref(cfg.c()),
+              KP, HP, N, 0 if use_graph else 1, R)
+    return [HeadSolveResult(r.num_iterations, r.initial_residual, r.final_residual,
+                            bool(r.converged), r.bnorm, r.true_residual, r.setup_ms, r.solve_ms,
+                            r.restarts) for r in R]
+
+
 def init_head_guess(h: torch.Tensor, grid: Grid3D, bc: BCSpec, ctx: HipContext) -> None:
     """physics::init_head_guess (solve_head.cu:56-73)."""
     _lib.call("mf3d_init_head_guess", ctx.handle, C.byref(grid.c()), C.byref(bc.c()),
