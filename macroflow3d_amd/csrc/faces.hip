@@ -370,7 +370,7 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     const Split sp = split_of(d);
     const int h4 = sp.h >> 2;
     const long total = sp.half >> 2;
-    const long t = blk * kBlock + threadIdx.x;
+    const long t = blk * (long)blockDim.x + threadIdx.x;
     const bool valid = t < total;
     const long tt = valid ? t : total - 1;  // lanes past the end mirror the last one
     const int lane = threadIdx.x & 63;
@@ -467,13 +467,16 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     if (valid) *reinterpret_cast<float4*>(x + own) = make_float4(res[0], res[1], res[2], res[3]);
 }
 
+// any block size up to kMaxSweepBlock (a tuning choice: results do not depend on it)
+constexpr int kMaxSweepBlock = 512;
+
 template <typename TC, int MODE, bool CS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kMaxSweepBlock)
     colour_split4_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
                          const float* __restrict__ b, SFaces<TC> F, int colour,
                          const float* __restrict__ xc) {
     const long total = ((long)(d.nx >> 1) * d.ny * d.nz) >> 2;
-    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    const long blk = xcd_block((total + blockDim.x - 1) / blockDim.x);
     if (blk < 0) return;  // whole workgroup
     split4_body<TC, MODE, CS>(d, bc, pin, bs, x, b, F, colour, xc, blk);
 }
@@ -1079,7 +1082,7 @@ __device__ __forceinline__ SFaces<TC> batch_sfaces(const BatchLevel& L) {
 }
 
 template <typename TC, int MODE, bool CS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kMaxSweepBlock)
     split4_batch_kernel(const BatchLevel* __restrict__ Ls, const int* __restrict__ start, int m,
                         long nblocks, int colour) {
     const long blk = xcd_block(nblocks);
@@ -1376,21 +1379,24 @@ SFaces<TC> sfaces(const FaceSet& F) {
 
 template <typename TC, int MODE, bool CS>
 void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin, float bs,
-                  float* x, const float* b, const SFaces<TC>& F, int colour, const float* xc) {
+                  float* x, const float* b, const SFaces<TC>& F, int colour, const float* xc,
+                  int tb) {
     const Dims d = dims(g);
     const size_t n = cells(g);
-    if ((g.nx & 7) == 0)
-        colour_split4_kernel<TC, MODE, CS><<<xcd_grid(n / 8), kBlock, 0, s>>>(d, bc, pin, bs, x, b,
-                                                                            F, colour, xc);
-    else
+    if ((g.nx & 7) == 0) {
+        const size_t nb = (n / 8 + tb - 1) / tb;
+        colour_split4_kernel<TC, MODE, CS><<<(unsigned)((nb + 7) / 8 * 8), tb, 0, s>>>(
+            d, bc, pin, bs, x, b, F, colour, xc);
+    } else {
         colour_split_kernel<TC, MODE, CS><<<xcd_grid(n / 2), kBlock, 0, s>>>(d, bc, pin, bs, x, b,
                                                                            F, colour, xc);
+    }
 }
 
 template <typename TC>
 int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, const float* b, const FaceSet& F, int iters, bool zero_init,
-                 const float* xc, bool csplit) {
+                 const float* xc, bool csplit, int tb) {
     const Dims d = dims(g);
     const SFaces<TC> SF = sfaces<TC>(F);
     const size_t n = cells(g);
@@ -1413,13 +1419,13 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
     for (int it = 0; it < iters; ++it)
         for (int colour = 0; colour < 2; ++colour) {
             if (it == 0 && colour == 0 && zero_init)
-                launch_split<TC, 1, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr);
+                launch_split<TC, 1, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb);
             else if (it == 0 && colour == 0 && xc && csplit)
-                launch_split<TC, 2, true>(s, g, bc, p, fbs, x, b, SF, colour, xc);
+                launch_split<TC, 2, true>(s, g, bc, p, fbs, x, b, SF, colour, xc, tb);
             else if (it == 0 && colour == 0 && xc)
-                launch_split<TC, 2, false>(s, g, bc, p, fbs, x, b, SF, colour, xc);
+                launch_split<TC, 2, false>(s, g, bc, p, fbs, x, b, SF, colour, xc, tb);
             else
-                launch_split<TC, 0, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr);
+                launch_split<TC, 0, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb);
         }
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
@@ -1454,14 +1460,17 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
     if (order == ORDER_SPLIT) {
         if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
         const bool cs = corder == ORDER_SPLIT;
+        const int tb = plan.block;
+        if (tb < 64 || tb > kMaxSweepBlock || (tb & 63))
+            return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");
         switch (F.type) {
         case COEF_F32:
-            return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs);
+            return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb);
         case COEF_BF16:
             return smooth_split<__hip_bfloat16>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc,
-                                                cs);
+                                                cs, tb);
         case COEF_F16:
-            return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs);
+            return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb);
         default:
             return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");
         }
@@ -1590,16 +1599,16 @@ int gemv(hipStream_t s, int n, const double* Ainv, const float* b, float* x) {
 namespace {
 template <typename TC>
 void launch_split4_batch(hipStream_t s, int mode, bool cs, const BatchLevel* L, const int* start,
-                         int m, long nblocks, int colour) {
+                         int m, long nblocks, int colour, int tb) {
     const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
     if (mode == 1)
-        split4_batch_kernel<TC, 1, false><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+        split4_batch_kernel<TC, 1, false><<<g, tb, 0, s>>>(L, start, m, nblocks, colour);
     else if (mode == 2 && cs)
-        split4_batch_kernel<TC, 2, true><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+        split4_batch_kernel<TC, 2, true><<<g, tb, 0, s>>>(L, start, m, nblocks, colour);
     else if (mode == 2)
-        split4_batch_kernel<TC, 2, false><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+        split4_batch_kernel<TC, 2, false><<<g, tb, 0, s>>>(L, start, m, nblocks, colour);
     else
-        split4_batch_kernel<TC, 0, false><<<g, kBlock, 0, s>>>(L, start, m, nblocks, colour);
+        split4_batch_kernel<TC, 0, false><<<g, tb, 0, s>>>(L, start, m, nblocks, colour);
 }
 template <typename TC>
 void launch_rr_split_batch(hipStream_t s, bool cs, const BatchLevel* L, const int* start, int m,
@@ -1613,13 +1622,15 @@ void launch_rr_split_batch(hipStream_t s, bool cs, const BatchLevel* L, const in
 long batch_blocks(long threads) { return (threads + kBlock - 1) / kBlock; }
 
 int batch_sweep(hipStream_t s, CoefType t, int mode, bool cs, const BatchLevel* L,
-                const int* start, int m, long nblocks, int colour) {
+                const int* start, int m, long nblocks, int colour, int tb) {
+    if (tb < 64 || tb > kMaxSweepBlock || (tb & 63))
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");
     switch (t) {
-    case COEF_F32: launch_split4_batch<float>(s, mode, cs, L, start, m, nblocks, colour); break;
+    case COEF_F32: launch_split4_batch<float>(s, mode, cs, L, start, m, nblocks, colour, tb); break;
     case COEF_BF16:
-        launch_split4_batch<__hip_bfloat16>(s, mode, cs, L, start, m, nblocks, colour);
+        launch_split4_batch<__hip_bfloat16>(s, mode, cs, L, start, m, nblocks, colour, tb);
         break;
-    case COEF_F16: launch_split4_batch<__half>(s, mode, cs, L, start, m, nblocks, colour); break;
+    case COEF_F16: launch_split4_batch<__half>(s, mode, cs, L, start, m, nblocks, colour, tb); break;
     default: return fail(MF3D_ERR_UNSUPPORTED, "batched sweep: coefficient type");
     }
     MF3D_HIP_TRY(hipGetLastError());

@@ -33,7 +33,8 @@ enum SmoothKind : int { SMOOTH_COLOUR = 0, SMOOTH_SMALL = 1, SMOOTH_MARCH = 2 };
 // How one level is smoothed (chosen by defaults or the autotuner).
 struct SmoothPlan {
     int kind = SMOOTH_COLOUR;
-    int zchunk = 0;  // march kernel: planes per workgroup (<= 0: default)
+    int zchunk = 0;    // march kernel: planes per workgroup (<= 0: default)
+    int block = 256;   // colour-split sweeps: threads per workgroup (64..512)
 };
 
 namespace kern {
@@ -88,9 +89,10 @@ int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials)
 long batch_blocks(long threads);
 // one colour of GSRB on colour-split levels (nx % 8 == 0), mode as
 // colour_split4_kernel: 0 plain, 1 zero start, 2 prolongation folded in (cs:
-// coarse level colour-split); nblocks = sum of batch_blocks(n / 8)
+// coarse level colour-split); tb threads per workgroup, nblocks = sum of
+// ceil(n / 8 / tb)
 int batch_sweep(hipStream_t s, CoefType t, int mode, bool cs, const BatchLevel* L,
-                const int* start, int m, long nblocks, int colour);
+                const int* start, int m, long nblocks, int colour, int tb);
 // residual + restriction of colour-split levels (nblocks: sum of batch_blocks(n / 8))
 int batch_residual_restrict_split(hipStream_t s, CoefType t, bool cs, const BatchLevel* L,
                                   const int* start, int m, long nblocks);

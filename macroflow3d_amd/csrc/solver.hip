@@ -123,6 +123,7 @@ struct mf3d_workspace {
     long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
     int zchunk = 0;
     bool split = true;  // colour-split FP32 levels where the sizes allow
+    int sweep_block = 256;  // threads per workgroup of the colour-split sweeps
     size_t bytes = 0;
     std::vector<void*> owned;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -411,6 +412,7 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
                       : (kern::march_ok(L.g, ws->bct) && (long)L.n >= thr) ? SMOOTH_MARCH
                                                                            : SMOOTH_COLOUR;
         L.plan.zchunk = ws->zchunk;
+        L.plan.block = ws->sweep_block;
         if (!mixed && L.plan.kind == SMOOTH_MARCH && l > 0 && !L.t) MF3D_TRY(ws->alloc(&L.t, L.n));
         if (l == 0) L.t = ws->Ap;  // free while the V-cycle runs
     }
@@ -723,6 +725,8 @@ struct BLaunch {
     long nblocks = 0;
     void* desc = nullptr;  // device BatchLevel[m] or BatchVec[m]
     int* start = nullptr;  // device, m + 1 block offsets
+    long snblocks = 0;     // colour-split sweeps: blocks of sweep_block threads
+    int* sstart = nullptr;
 };
 
 struct BDepth {
@@ -737,31 +741,40 @@ struct BPlan {
     std::vector<BDepth> depth;
     CoefType type = COEF_BF16;
     int pre = 2, post = 2;
+    int tb = 256;  // sweep threads per workgroup
     ~BPlan() {
         for (void* q : mem) (void)hipFree(q);
     }
 };
 
-template <typename D>
-int upload_launch(BPlan& P, BLaunch& L, const std::vector<D>& d, const std::vector<long>& blocks) {
-    L.m = (int)d.size();
-    if (L.m == 0) return MF3D_OK;
-    std::vector<int> start(d.size() + 1, 0);
-    for (size_t e = 0; e < d.size(); ++e) {
+int upload_starts(BPlan& P, const std::vector<long>& blocks, int** dst, long* total) {
+    std::vector<int> start(blocks.size() + 1, 0);
+    for (size_t e = 0; e < blocks.size(); ++e) {
         if (start[e] + blocks[e] > (long)INT32_MAX)
             return fail(MF3D_ERR_UNSUPPORTED, "batched launch larger than 2^31 blocks");
         start[e + 1] = start[e] + (int)blocks[e];
     }
-    L.nblocks = start.back();
+    *total = start.back();
+    void* q = nullptr;
+    MF3D_HIP_TRY(hipMalloc(&q, start.size() * sizeof(int)));
+    P.mem.push_back(q);
+    MF3D_HIP_TRY(hipMemcpy(q, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+    *dst = static_cast<int*>(q);
+    return MF3D_OK;
+}
+
+template <typename D>
+int upload_launch(BPlan& P, BLaunch& L, const std::vector<D>& d, const std::vector<long>& blocks,
+                  const std::vector<long>* sweep_blocks = nullptr) {
+    L.m = (int)d.size();
+    if (L.m == 0) return MF3D_OK;
     void* q = nullptr;
     MF3D_HIP_TRY(hipMalloc(&q, d.size() * sizeof(D)));
     P.mem.push_back(q);
     MF3D_HIP_TRY(hipMemcpy(q, d.data(), d.size() * sizeof(D), hipMemcpyHostToDevice));
     L.desc = q;
-    MF3D_HIP_TRY(hipMalloc(&q, start.size() * sizeof(int)));
-    P.mem.push_back(q);
-    MF3D_HIP_TRY(hipMemcpy(q, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
-    L.start = static_cast<int*>(q);
+    MF3D_TRY(upload_starts(P, blocks, &L.start, &L.nblocks));
+    if (sweep_blocks) MF3D_TRY(upload_starts(P, *sweep_blocks, &L.sstart, &L.snblocks));
     return MF3D_OK;
 }
 
@@ -772,6 +785,7 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
     P.type = ws[0]->lo_type;
     P.pre = ws[0]->pre;
     P.post = ws[0]->post;
+    P.tb = ws[0]->sweep_block;
     size_t maxl = 0;
     for (int r = 0; r < count; ++r) maxl = std::max(maxl, ws[r]->lv.size());
     P.depth.resize(maxl);
@@ -779,7 +793,7 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
     std::vector<long> vb;
     struct G {
         std::vector<BatchLevel> d;
-        std::vector<long> b;
+        std::vector<long> b, sb;  // blocks of 256 threads, sweep blocks
     };
     std::vector<G> s4c(maxl), s4n(maxl), sm(maxl), smr(maxl), dir(maxl);
     for (int r = 0; r < count; ++r) {
@@ -837,6 +851,7 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
                 G& grp = C.order == ORDER_SPLIT ? s4c[l] : s4n[l];
                 grp.d.push_back(b);
                 grp.b.push_back(nb8);
+                grp.sb.push_back((long)((L.n / 8 + P.tb - 1) / P.tb));
             } else if (L.plan.kind == SMOOTH_SMALL) {
                 sm[l].d.push_back(b);
                 sm[l].b.push_back(1);
@@ -849,8 +864,8 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
     }
     MF3D_TRY(upload_launch(P, P.vec, vd, vb));
     for (size_t l = 0; l < maxl; ++l) {
-        MF3D_TRY(upload_launch(P, P.depth[l].s4_cs, s4c[l].d, s4c[l].b));
-        MF3D_TRY(upload_launch(P, P.depth[l].s4_cn, s4n[l].d, s4n[l].b));
+        MF3D_TRY(upload_launch(P, P.depth[l].s4_cs, s4c[l].d, s4c[l].b, &s4c[l].sb));
+        MF3D_TRY(upload_launch(P, P.depth[l].s4_cn, s4n[l].d, s4n[l].b, &s4n[l].sb));
         MF3D_TRY(upload_launch(P, P.depth[l].small, sm[l].d, sm[l].b));
         MF3D_TRY(upload_launch(P, P.depth[l].small_rr, smr[l].d, smr[l].b));
         MF3D_TRY(upload_launch(P, P.depth[l].direct, dir[l].d, dir[l].b));
@@ -859,6 +874,7 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
 }
 
 #define BL(L) static_cast<const BatchLevel*>((L).desc), (L).start, (L).m, (L).nblocks
+#define BS(L) static_cast<const BatchLevel*>((L).desc), (L).sstart, (L).m, (L).snblocks
 #define BV(L) static_cast<const BatchVec*>((L).desc), (L).start, (L).m, (L).nblocks
 
 // V-cycle from x = 0 at depth d for every realisation that has that level
@@ -869,7 +885,8 @@ int vcycle_batch(const BPlan& P, hipStream_t s, size_t d) {
         if (!L.m) continue;
         for (int it = 0; it < P.pre; ++it)
             for (int c = 0; c < 2; ++c)
-                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? 1 : 0, q == 0, BL(L), c));
+                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? 1 : 0, q == 0, BS(L), c,
+                                           P.tb));
         MF3D_TRY(kern::batch_residual_restrict_split(s, P.type, q == 0, BL(L)));
     }
     if (D.small.m) {
@@ -888,7 +905,8 @@ int vcycle_batch(const BPlan& P, hipStream_t s, size_t d) {
         if (!L.m) continue;
         for (int it = 0; it < P.post; ++it)
             for (int c = 0; c < 2; ++c)
-                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? 2 : 0, q == 0, BL(L), c));
+                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? 2 : 0, q == 0, BS(L), c,
+                                           P.tb));
     }
     if (D.small.m)
         MF3D_TRY(kern::batch_small(s, P.type, static_cast<const BatchLevel*>(D.small.desc),
@@ -1001,6 +1019,7 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
 }
 
 #undef BL
+#undef BS
 #undef BV
 
 }  // namespace
@@ -1393,6 +1412,11 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     case MF3D_TUNE_MARCH_MIN_CELLS_FP32: ws->march_min_cells[1] = value; break;
     case MF3D_TUNE_ZCHUNK: ws->zchunk = (int)value; break;
     case MF3D_TUNE_SPLIT: ws->split = value != 0; break;
+    case MF3D_TUNE_SWEEP_BLOCK:
+        if (value < 64 || value > 512 || (value & 63))
+            return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");
+        ws->sweep_block = (int)value;
+        break;
     default: return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
     return MF3D_OK;

@@ -181,6 +181,7 @@ class FlowWorkspace:
     TUNE_MARCH_MIN_CELLS_FP32 = 2
     TUNE_ZCHUNK = 3
     TUNE_SPLIT = 4
+    TUNE_SWEEP_BLOCK = 5
 
     def set_tuning(self, key: int, value: int) -> None:
         """Smoother selection knobs (levels with >= value cells use the one-pass

@@ -477,3 +477,27 @@ def test_colour_split_solve_matches_natural_order(ctx):
         ws.close()
     assert out[0][1] == out[1][1]
     assert rel_l2(out[0][0], out[1][0]) < 1e-10
+
+
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_sweep_block_size_does_not_change_results(ctx, block):
+    """The colour-split sweep's workgroup size is a tuning choice only: the
+    preconditioned vector is bitwise the same as with the default 256."""
+    g, _ = grid_pair(64, 32, 48)
+    bc, _ = bc_pair("periodic_yz")
+    n = g.num_cells()
+    K = random_K(n, 61, 1.5)
+    r = np.random.default_rng(62).standard_normal(n)
+    cfg = ph.HeadSolveConfig(mg_levels=4, coarse_solver=ph.CoarseSolver.DIRECT,
+                             precond_precision=ph.PrecondPrecision.FP32,
+                             coef_format=ph.CoefFormat.BF16)
+    out = []
+    for tb in (256, block):
+        ws = ph.FlowWorkspace()
+        ws.set_tuning(ph.FlowWorkspace.TUNE_SWEEP_BLOCK, tb)
+        M = nm.MultigridPreconditioner.build(ctx, ws, g, bc, cfg, dev(K, ctx))
+        z = ctx.empty(n)
+        M.apply(ctx, dev(r, ctx), z)
+        out.append(host(z))
+        ws.close()
+    assert np.array_equal(out[0], out[1])
