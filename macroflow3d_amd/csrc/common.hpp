@@ -165,9 +165,8 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
     BcTypes bc;
     int pin;
     double inv_dx2;
-    const double* fx;
-    const double* fy;
-    const double* fz;
+    const double* K;
+    const double* iK;
     double* x;
     double* r;
     double* z;

@@ -69,15 +69,21 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
                             double rs, const float* x, const float* b, const FaceSet& F,
                             float* bcoarse, int order = ORDER_NATURAL,
                             int corder = ORDER_NATURAL);
-// FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
-// mode 1: y = b - A x (y nullable), partials (y, y).
-int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
-             int mode, const double* x, const double* b, double* y, double* partials);
+// FP64 operator from K and iK = 1/K (faces formed on the fly, same values as
+// faces_build). mode 0: y = A x, partials (x, y); mode 1: y = b - A x (y
+// nullable), partials (y, y).
+int op_k(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
+         const double* iK, int mode, const double* x, const double* b, double* y,
+         double* partials);
+// iK = 1 / K
+int inverse(hipStream_t s, const double* K, double* iK, size_t n);
 int to_f32(hipStream_t s, const double* a, float* o, size_t n);
 int to_f64(hipStream_t s, const float* a, double* o, size_t n);
 // conversions between a natural FP64 vector and an FP32 vector of `order`
 int to_f32(hipStream_t s, const double* a, float* o, const mf3d_grid& g, int order);
-int to_f64(hipStream_t s, const float* a, double* o, const mf3d_grid& g, int order);
+// with r (nullable): also the (r, o) first-stage partials, as dot_partials
+int to_f64(hipStream_t s, const float* a, double* o, const mf3d_grid& g, int order,
+           const double* r = nullptr, double* partials = nullptr);
 int gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
 int gemv(hipStream_t s, int n, const double* Ainv, const float* b, float* x);
 int fill_f32(hipStream_t s, float* x, size_t n, float v);
@@ -107,6 +113,7 @@ int batch_gemv(hipStream_t s, const BatchLevel* L, const int* start, int m, long
 // finest level (nblocks: sum of batch_blocks(n)): r -> b32 split, x32 split -> z,
 // mode 0: Ap = A p (+ (p, Ap) partials), mode 1: r = rhs - A x (+ (r, r) partials)
 int batch_to_f32(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
+// x32 split -> z with the (r, z) partials (replaces a batch_dot mode 0)
 int batch_to_f64(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
 int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
 

@@ -118,6 +118,7 @@ struct mf3d_workspace {
     int precision = MF3D_PRECOND_FP64;
     CoefType lo_type = COEF_BF16;
     const double* K0 = nullptr;
+    double* iK = nullptr;  // 1 / K on the finest level (FP64 operator)
     bool ready = false;
     // tuning (mf3d_workspace_set_tuning / autotuner)
     long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
@@ -171,6 +172,7 @@ struct mf3d_workspace {
         coarse_cap = 0;
         nvec = 0;
         rhs = r = z = p = Ap = nullptr;
+        iK = nullptr;
         partials = nullptr;
         partials_cap = 0;
         st = nullptr;
@@ -383,6 +385,8 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     ws->lo_type = coef_type_of(cfg.coef_format);
     ws->K0 = K;
     const bool mixed = ws->precision == MF3D_PRECOND_FP32;
+    if (!ws->iK) MF3D_TRY(ws->alloc(&ws->iK, cells(g)));
+    MF3D_TRY(kern::inverse(s, K, ws->iK, cells(g)));
     // fp16 coefficients are stored divided by the geometric mean of K (a power
     // of two), so both tails of a log-normal field stay inside fp16's range
     double cs = 1.0;
@@ -399,7 +403,7 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     for (size_t l = 0; l < ws->lv.size(); ++l) {
         Level& L = ws->lv[l];
         if (l > 0) MF3D_TRY(kern::coarsen(s, L.g, level_K(ws, l - 1), L.K));
-        const bool want64 = (l == 0) || !mixed;
+        const bool want64 = !mixed;  // the FP64 V-cycle (the operator works from K and 1/K)
         if (want64) {
             if (!L.has_f64) {
                 MF3D_TRY(ws->alloc_faces(L.f64, L.g, COEF_F64));
@@ -490,23 +494,28 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
 
 // MultigridPreconditioner::apply (mg_preconditioner.cu:16-45): z = V-cycle(b = r, x = 0),
 // in FP64 or with the FP32 V-cycle between two conversions.
-int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z) {
+// rz_partials (nullable): also leave the first-stage partials of (r, z) there.
+int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
+                 double* rz_partials = nullptr) {
     Level& L0 = ws->lv[0];
-    if (ws->precision == MF3D_PRECOND_FP64) return vcycle<double>(ws, s, 0, z, r, true);
+    if (ws->precision == MF3D_PRECOND_FP64) {
+        MF3D_TRY(vcycle<double>(ws, s, 0, z, r, true));
+        return rz_partials ? kern::dot_partials(s, r, z, L0.n, rz_partials) : MF3D_OK;
+    }
     MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.g, L0.order));
     MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true));
-    return kern::to_f64(s, L0.x32, z, L0.g, L0.order);
+    return kern::to_f64(s, L0.x32, z, L0.g, L0.order, rz_partials ? r : nullptr, rz_partials);
 }
 
 // operator on the finest level with its FP64 faces
 int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, double* partials) {
     Level& L0 = ws->lv[0];
-    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 0, x, nullptr, y, partials);
+    return kern::op_k(s, L0.g, ws->bct, ws->pin, ws->K0, ws->iK, 0, x, nullptr, y, partials);
 }
 int residual_op(mf3d_workspace* ws, hipStream_t s, const double* x, const double* b, double* r,
                 double* partials) {
     Level& L0 = ws->lv[0];
-    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 1, x, b, r, partials);
+    return kern::op_k(s, L0.g, ws->bct, ws->pin, ws->K0, ws->iK, 1, x, b, r, partials);
 }
 
 IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
@@ -585,8 +594,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_R0, ws->st, nullptr));
     int restarts = 0;
     for (;;) {
-        MF3D_TRY(precondition(ws, s, ws->r, ws->z));
-        MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
+        MF3D_TRY(precondition(ws, s, ws->r, ws->z, ws->partials));
         MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ0, ws->st, nullptr));
         MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
         MF3D_TRY(read_state(ws, s));
@@ -605,8 +613,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                     MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
                     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZO, ws->st, nullptr));
                 }
-                MF3D_TRY(precondition(ws, s, ws->r, ws->z));
-                MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
+                MF3D_TRY(precondition(ws, s, ws->r, ws->z, ws->partials));
                 MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
                 MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
             }
@@ -807,9 +814,8 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
         v.bc = w->bct;
         v.pin = w->pin ? 1 : 0;
         v.inv_dx2 = 1.0 / (L0.g.dx * L0.g.dx);
-        v.fx = (const double*)L0.f64.fx;
-        v.fy = (const double*)L0.f64.fy;
-        v.fz = (const double*)L0.f64.fz;
+        v.K = w->K0;
+        v.iK = w->iK;
         v.x = h[r];
         v.r = w->r, v.z = w->z, v.p = w->p, v.Ap = w->Ap, v.rhs = w->rhs;
         v.b32 = L0.b32, v.x32 = L0.x32;
@@ -928,8 +934,7 @@ int pcg_iteration_batch(const BPlan& P, hipStream_t s) {
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_PAP, 1));
     MF3D_TRY(kern::batch_update_xr(s, BV(P.vec)));
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RR, 1));
-    MF3D_TRY(precondition_batch(P, s));
-    MF3D_TRY(kern::batch_dot(s, 0, BV(P.vec)));
+    MF3D_TRY(precondition_batch(P, s));  // also the (r, z) partials
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RZ, 1));
     return kern::batch_update_p(s, 0, BV(P.vec));
 }
@@ -990,8 +995,7 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
     int restarts = 0;
     for (;;) {
         // start of a cycle for the running realisations: z = M r, rz, p = z
-        MF3D_TRY(precondition_batch(P, s));
-        MF3D_TRY(kern::batch_dot(s, 0, BV(P.vec)));
+        MF3D_TRY(precondition_batch(P, s));  // also the (r, z) partials
         MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RZ0, 1));
         MF3D_TRY(kern::batch_update_p(s, 1, BV(P.vec)));
         MF3D_TRY(read_states(ws, count, s));
