@@ -149,6 +149,7 @@ struct BatchLevel {  // one FP32 V-cycle level of one realisation
     float* b;
     float* bcoarse;        // right-hand side of the next level (restriction target)
     const float* xc;       // correction of the next level (prolongation source)
+    const double* r64;     // finest level: FP64 right-hand side converted into b
     const void* fx;        // faces (natural MAC or colour-split, type of the solve)
     const void* fy;
     const void* fz;

@@ -362,11 +362,15 @@ __device__ __forceinline__ void load4(const TC* p, float (&o)[4]) {
     for (int e = 0; e < 4; ++e) o[e] = cvt(q.v[e]);
 }
 
+// MODE 3: as MODE 1, with b taken from the FP64 natural-order r64 (both
+// colours of b written as FP32 on the way: the conversion at the entry of the
+// FP32 V-cycle folded into its first half-sweep).
 template <typename TC, int MODE, bool CS>
 __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, int pin, float bs,
                                             float* __restrict__ x, const float* __restrict__ b,
                                             const SFaces<TC>& F, int colour,
-                                            const float* __restrict__ xc, long blk) {
+                                            const float* __restrict__ xc,
+                                            const double* __restrict__ r64, long blk) {
     const Split sp = split_of(d);
     const int h4 = sp.h >> 2;
     const long total = sp.half >> 2;
@@ -420,7 +424,7 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     }
 
     float vxp[4], vxm[4], vyp[4], vym[4], vzp[4], vzm[4];
-    if (MODE == 1) {
+    if (MODE == 1 || MODE == 3) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) vxp[e] = vxm[e] = vyp[e] = vym[e] = vzp[e] = vzm[e] = 0.0f;
     } else {
@@ -455,7 +459,30 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
         else vzm[0] = vzm[1] = vzm[2] = vzm[3] = 0.0f;
     }
     float bb[4];
-    load4(b + own, bb);
+    if (MODE == 3) {
+        // the thread's 8 natural cells of the row: own colour at 2e + p, the other at 2e + 1 - p
+        const double2* rp = reinterpret_cast<const double2*>(r64 + row * d.nx + 8L * q);
+        double rv[8];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double2 v2 = rp[e];
+            rv[2 * e] = v2.x;
+            rv[2 * e + 1] = v2.y;
+        }
+        float bo[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            bb[e] = (float)rv[2 * e + p];
+            bo[e] = (float)rv[2 * e + 1 - p];
+        }
+        if (valid) {
+            float* bw = const_cast<float*>(b);
+            *reinterpret_cast<float4*>(bw + own) = make_float4(bb[0], bb[1], bb[2], bb[3]);
+            *reinterpret_cast<float4*>(bw + oth) = make_float4(bo[0], bo[1], bo[2], bo[3]);
+        }
+    } else {
+        load4(b + own, bb);
+    }
     float res[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -474,11 +501,11 @@ template <typename TC, int MODE, bool CS>
 __global__ void __launch_bounds__(kMaxSweepBlock)
     colour_split4_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
                          const float* __restrict__ b, SFaces<TC> F, int colour,
-                         const float* __restrict__ xc) {
+                         const float* __restrict__ xc, const double* __restrict__ r64) {
     const long total = ((long)(d.nx >> 1) * d.ny * d.nz) >> 2;
     const long blk = xcd_block((total + blockDim.x - 1) / blockDim.x);
     if (blk < 0) return;  // whole workgroup
-    split4_body<TC, MODE, CS>(d, bc, pin, bs, x, b, F, colour, xc, blk);
+    split4_body<TC, MODE, CS>(d, bc, pin, bs, x, b, F, colour, xc, r64, blk);
 }
 
 // prolong-add between any two orders (used when a level has no post-smoothing)
@@ -1138,7 +1165,7 @@ __global__ void __launch_bounds__(kMaxSweepBlock)
     const BatchLevel& L = Ls[e];
     if (L.st->stop) return;
     split4_body<TC, MODE, CS>(Dims{L.nx, L.ny, L.nz}, L.bc, L.pin, L.bs, L.x, L.b,
-                              batch_sfaces<TC>(L), colour, L.xc, blk - start[e]);
+                              batch_sfaces<TC>(L), colour, L.xc, L.r64, blk - start[e]);
 }
 
 template <typename TC, bool CS>
@@ -1410,8 +1437,9 @@ int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan, int order,
-                 int corder) {
+                 int corder, const double* r64) {
     if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 smoothing needs FP64 faces");
+    if (r64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels read b directly");
     if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
         return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels are stored in natural order");
     return smooth_t<double, double>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc, plan);
@@ -1428,13 +1456,13 @@ SFaces<TC> sfaces(const FaceSet& F) {
 template <typename TC, int MODE, bool CS>
 void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin, float bs,
                   float* x, const float* b, const SFaces<TC>& F, int colour, const float* xc,
-                  int tb) {
+                  int tb, const double* r64 = nullptr) {
     const Dims d = dims(g);
     const size_t n = cells(g);
     if ((g.nx & 7) == 0) {
         const size_t nb = (n / 8 + tb - 1) / tb;
         colour_split4_kernel<TC, MODE, CS><<<(unsigned)((nb + 7) / 8 * 8), tb, 0, s>>>(
-            d, bc, pin, bs, x, b, F, colour, xc);
+            d, bc, pin, bs, x, b, F, colour, xc, r64);
     } else {
         colour_split_kernel<TC, MODE, CS><<<xcd_grid(n / 2), kBlock, 0, s>>>(d, bc, pin, bs, x, b,
                                                                            F, colour, xc);
@@ -1444,7 +1472,7 @@ void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin,
 template <typename TC>
 int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, const float* b, const FaceSet& F, int iters, bool zero_init,
-                 const float* xc, bool csplit, int tb) {
+                 const float* xc, bool csplit, int tb, const double* r64) {
     const Dims d = dims(g);
     const SFaces<TC> SF = sfaces<TC>(F);
     const size_t n = cells(g);
@@ -1466,7 +1494,9 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
     }
     for (int it = 0; it < iters; ++it)
         for (int colour = 0; colour < 2; ++colour) {
-            if (it == 0 && colour == 0 && zero_init)
+            if (it == 0 && colour == 0 && zero_init && r64)
+                launch_split<TC, 3, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb, r64);
+            else if (it == 0 && colour == 0 && zero_init)
                 launch_split<TC, 1, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb);
             else if (it == 0 && colour == 0 && xc && csplit)
                 launch_split<TC, 2, true>(s, g, bc, p, fbs, x, b, SF, colour, xc, tb);
@@ -1502,7 +1532,10 @@ bool split_ok(const mf3d_grid& g) {
 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
-                 bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder) {
+                 bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder,
+                 const double* r64) {
+    if (r64 && !(order == ORDER_SPLIT && (g.nx & 7) == 0 && zero_init && iters > 0 && !xc))
+        return fail(MF3D_ERR_UNSUPPORTED, "b from FP64 needs a zero-start split4 sweep");
     if ((order == ORDER_SPLIT) != F.split)
         return fail(MF3D_ERR_UNSUPPORTED, "vector order and face layout differ");
     if (order == ORDER_SPLIT) {
@@ -1513,12 +1546,14 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");
         switch (F.type) {
         case COEF_F32:
-            return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb);
+            return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
+                                      r64);
         case COEF_BF16:
             return smooth_split<__hip_bfloat16>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc,
-                                                cs, tb);
+                                                cs, tb, r64);
         case COEF_F16:
-            return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb);
+            return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
+                                       r64);
         default:
             return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");
         }
@@ -1661,7 +1696,9 @@ template <typename TC>
 void launch_split4_batch(hipStream_t s, int mode, bool cs, const BatchLevel* L, const int* start,
                          int m, long nblocks, int colour, int tb) {
     const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
-    if (mode == 1)
+    if (mode == 3)
+        split4_batch_kernel<TC, 3, false><<<g, tb, 0, s>>>(L, start, m, nblocks, colour);
+    else if (mode == 1)
         split4_batch_kernel<TC, 1, false><<<g, tb, 0, s>>>(L, start, m, nblocks, colour);
     else if (mode == 2 && cs)
         split4_batch_kernel<TC, 2, true><<<g, tb, 0, s>>>(L, start, m, nblocks, colour);

@@ -52,13 +52,17 @@ int march_default_zchunk(const mf3d_grid& g);
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan,
-                 int order = ORDER_NATURAL, int corder = ORDER_NATURAL);
+                 int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
+                 const double* r64 = nullptr);
 // order: storage order of x/b, corder: of xc (FP64 levels are natural only);
 // a split level's coarse level may be natural, not the reverse.
+// r64 (nullable): b is the FP32 copy of this FP64 natural-order vector, made
+// by the first half-sweep (colour-split level, nx % 8 == 0, zero_init, no xc)
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan,
-                 int order = ORDER_NATURAL, int corder = ORDER_NATURAL);
+                 int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
+                 const double* r64 = nullptr);
 bool split_ok(const mf3d_grid& g);
 // coarse b = restriction of (b - A x), rs = 1 / (cs * dx^2)
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
@@ -94,7 +98,8 @@ int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials)
 // kBlock-thread blocks needed for `threads` threads
 long batch_blocks(long threads);
 // one colour of GSRB on colour-split levels (nx % 8 == 0), mode as
-// colour_split4_kernel: 0 plain, 1 zero start, 2 prolongation folded in (cs:
+// colour_split4_kernel: 0 plain, 1 zero start, 3 zero start with b from the
+// descriptor's r64, 2 prolongation folded in (cs:
 // coarse level colour-split); tb threads per workgroup, nblocks = sum of
 // ceil(n / 8 / tb)
 int batch_sweep(hipStream_t s, CoefType t, int mode, bool cs, const BatchLevel* L,

@@ -468,7 +468,8 @@ template <> struct LevelView<float> {
 // mg_preconditioner.cu:37 are folded into the first smoothing sweep); the
 // prolongation (v_cycle.cu:69) is folded into the first post-smoothing sweep.
 template <typename T>
-int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool zero_init) {
+int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool zero_init,
+           const double* r64 = nullptr) {
     using V = LevelView<T>;
     const size_t nl = ws->lv.size();
     Level& L = ws->lv[l];
@@ -484,7 +485,7 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
     Level& C = ws->lv[l + 1];
     const int fo = V::order(L), co = V::order(C);
     MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->pre,
-                                zero_init, (const T*)nullptr, L.plan, fo, co));
+                                zero_init, (const T*)nullptr, L.plan, fo, co, r64));
     MF3D_TRY(kern::residual_restrict_faces(s, L.g, ws->bct, ws->pin, 1.0 / (cs * dx2), x, b, F,
                                            V::b(C), fo, co));
     MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true));
@@ -502,8 +503,11 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
         MF3D_TRY(vcycle<double>(ws, s, 0, z, r, true));
         return rz_partials ? kern::dot_partials(s, r, z, L0.n, rz_partials) : MF3D_OK;
     }
-    MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.g, L0.order));
-    MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true));
+    // a colour-split finest level takes r directly in its first half-sweep
+    const bool fold = ws->lv.size() > 1 && L0.order == ORDER_SPLIT && !(L0.g.nx & 7) &&
+                      ws->pre > 0 && L0.plan.kind == SMOOTH_COLOUR;
+    if (!fold) MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.g, L0.order));
+    MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true, fold ? r : nullptr));
     return kern::to_f64(s, L0.x32, z, L0.g, L0.order, rz_partials ? r : nullptr, rz_partials);
 }
 
@@ -836,6 +840,7 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
             b.bs = (float)(L.cs * dx2);
             b.rs = (float)(1.0 / (L.cs * dx2));
             b.x = L.x32, b.b = L.b32;
+            b.r64 = l == 0 ? w->r : nullptr;
             b.fx = L.flo.fx, b.fy = L.flo.fy, b.fz = L.flo.fz;
             b.ux = L.flo.ux, b.uy = L.flo.uy, b.uz = L.flo.uz;
             b.st = w->st;
@@ -891,8 +896,8 @@ int vcycle_batch(const BPlan& P, hipStream_t s, size_t d) {
         if (!L.m) continue;
         for (int it = 0; it < P.pre; ++it)
             for (int c = 0; c < 2; ++c)
-                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? 1 : 0, q == 0, BS(L), c,
-                                           P.tb));
+                MF3D_TRY(kern::batch_sweep(s, P.type, (it == 0 && c == 0) ? (d == 0 ? 3 : 1) : 0,
+                                           q == 0, BS(L), c, P.tb));
         MF3D_TRY(kern::batch_residual_restrict_split(s, P.type, q == 0, BL(L)));
     }
     if (D.small.m) {
@@ -920,8 +925,8 @@ int vcycle_batch(const BPlan& P, hipStream_t s, size_t d) {
     return MF3D_OK;
 }
 
+// the finest levels' first half-sweep reads r (FP64) and fills b32
 int precondition_batch(const BPlan& P, hipStream_t s) {
-    MF3D_TRY(kern::batch_to_f32(s, BV(P.vec)));
     MF3D_TRY(vcycle_batch(P, s, 0));
     return kern::batch_to_f64(s, BV(P.vec));
 }
