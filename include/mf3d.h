@@ -261,9 +261,11 @@ int mf3d_scal(mf3d_stream stream, double* x, size_t n, double a);
 /* ---- learned warm start (extension, no counterpart in MacroFlow3D) ----
  * A 3-D convolutional encoder-decoder (3 levels, 16/32/64 channels, 3^3
  * kernels, BF16 weights/activations, FP32 accumulation on MFMA) mapping ln K
- * to an initial head h0 = h_lin + J y for the Darcy set-up with Dirichlet x
- * faces (h_lin linear between them, J = (h_W - h_E) / nx); levels 1 and 2
- * are 4 and 16 times coarser, so sizes must be divisible by 16. Parameters are FP32 masters: per layer (conv_a..conv_g)
+ * to an initial head h0 = h_lin + J trilinear(y) for the Darcy set-up with
+ * Dirichlet x faces (h_lin linear between them, J = (h_W - h_E) / nx). It
+ * runs on the grid coarsened 4x per axis (inputs: block means of ln K, K and
+ * 1/K); its levels are 4, 8 and 16 times coarser than the grid, so sizes must
+ * be divisible by 16. Parameters are FP32 masters: per layer (conv_a..conv_g)
  * the weights [cout][27][cin] (tap = dx + 3 dy + 9 dz over offsets -1..1,
  * +1) followed by the bias [cout]. */
 typedef struct mf3d_wsnet mf3d_wsnet;

@@ -2,20 +2,24 @@
 // that maps ln K to an initial head, trained and run with BF16 MFMA kernels
 // (FP32 accumulation) on gfx950. No counterpart in MacroFlow3D.
 //
-// Network (3 levels, 16/32/64 channels, 3^3 kernels, zero padding):
-//   x0  = [lnK / 2, 2 (i + 1/2) / nx - 1]   (2 channels, stored as 16, level 0)
-//   e0  = relu(conv_a(x0))          2 -> 16  level 0
-//   e1  = relu(conv_b(pool(e0)))   16 -> 32  level 1
-//   e2  = relu(conv_c(pool(e1)))   32 -> 64  level 2
-//   e2b = relu(conv_d(e2))         64 -> 64  level 2
-//   d1  = relu(conv_e(up(e2b)) + e1)   64 -> 32  level 1
-//   d0  = relu(conv_f(up(d1)) + e0)    32 -> 16  level 0
-//   y   = conv_g(d0)               16 -> 1   level 0  (FP32)
-//   h0  = h_lin + J y,  h_lin the linear head between the x faces, J = (h_W - h_E) / nx
-// pool = mean of the 4x4x4 children, up = copy to the 64 children (levels
-// 1 and 2 are 4 and 16 times coarser, so the coarsest convolutions see about
-// +-40 cells of the fine grid). Training
-// minimises mean((y - (h - h_lin) / J)^2) with Adam on FP32 master weights.
+// The network runs on the grid coarsened 4x per axis (block statistics of K
+// in, a smooth correction out): MG-PCG removes high-frequency error in one
+// V-cycle, so only the smooth part of a guess shortens the solve, and the
+// coarse grid makes inference 64x cheaper and the receptive field 4x wider.
+// Network (3 levels at 4x, 8x and 16x coarsening, 16/32/64 channels, 3^3
+// kernels, zero padding), on the coarse grid G (cell V = a 4x4x4 block):
+//   x0  = [<lnK>/2, log<K>/2, -log<1/K>/2, 2 (I + 1/2)/NX - 1]  (4 channels, stored as 16)
+//   e0  = relu(conv_a(x0))          4 -> 16  G
+//   e1  = relu(conv_b(pool(e0)))   16 -> 32  G/2
+//   e2  = relu(conv_c(pool(e1)))   32 -> 64  G/4
+//   e2b = relu(conv_d(e2))         64 -> 64  G/4
+//   d1  = relu(conv_e(up(e2b)) + e1)   64 -> 32  G/2
+//   d0  = relu(conv_f(up(d1)) + e0)    32 -> 16  G
+//   y   = conv_g(d0)               16 -> 1   G  (FP32)
+//   h0  = h_lin + J trilinear(y),  h_lin linear between the x faces, J = (h_W - h_E) / nx
+// (<.> = mean over the block, pool = mean of 2x2x2 children, up = copy to the
+// 8 children). Training minimises mean over G of (y - <(h - h_lin) / J>)^2
+// with Adam on FP32 master weights.
 //
 // Activations are BF16, channels-last: a[v * C + c], v = i + nx (j + ny k),
 // C a multiple of 16. Convolution = implicit GEMM: for 16 output channels x
@@ -261,7 +265,8 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- pooling / upsampling (channels-last), factor kF per axis ------------------
-constexpr int kF = 4;
+constexpr int kS = 4;  // fine cells per network cell, per axis
+constexpr int kF = 2;
 constexpr float kInvF3 = 1.0f / (kF * kF * kF);
 
 __device__ __forceinline__ long parent_of(const Geo& g, long v) {
@@ -331,48 +336,110 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- input, output, loss -------------------------------------------------------
-// x0[v] = [lnK/2, 2 (i + 1/2)/nx - 1, 0 x 14]; from K (log taken here) or ln K
+// x0 on the coarse grid from the fine K (or ln K) of the kS^3 block of each cell
 __global__ void __launch_bounds__(256)
     input_kernel(Geo g, const double* __restrict__ K, int is_log, __hip_bfloat16* __restrict__ x0) {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= g.n()) return;
-    const int i = (int)(v % g.nx);
-    const double l = is_log ? K[v] : log(K[v]);
-    __hip_bfloat16* o = x0 + v * 16;
-    o[0] = f2bf((float)(0.5 * l));
-    o[1] = f2bf((float)(2.0 * (i + 0.5) / g.nx - 1.0));
+    const int NX = g.nx / kS, NY = g.ny / kS, NZ = g.nz / kS;
+    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V >= (long)NX * NY * NZ) return;
+    const int I = (int)(V % NX);
+    const long q = V / NX;
+    const int J = (int)(q % NY), Kc = (int)(q / NY);
+    double sl = 0.0, sk = 0.0, si = 0.0;
+    for (int dz = 0; dz < kS; ++dz)
+        for (int dy = 0; dy < kS; ++dy)
+            for (int dx = 0; dx < kS; ++dx) {
+                const long o = (long)(kS * I + dx) +
+                               (long)g.nx * ((kS * J + dy) + (long)g.ny * (kS * Kc + dz));
+                const double l = is_log ? K[o] : log(K[o]);
+                const double k = is_log ? exp(K[o]) : K[o];
+                sl += l;
+                sk += k;
+                si += 1.0 / k;
+            }
+    const double w = 1.0 / (kS * kS * kS);
+    __hip_bfloat16* o = x0 + V * 16;
+    o[0] = f2bf((float)(0.5 * sl * w));
+    o[1] = f2bf((float)(0.5 * log(sk * w)));
+    o[2] = f2bf((float)(-0.5 * log(si * w)));
+    o[3] = f2bf((float)(2.0 * (I + 0.5) / NX - 1.0));
 #pragma unroll
-    for (int c = 2; c < 16; ++c) o[c] = f2bf(0.0f);
+    for (int c = 4; c < 16; ++c) o[c] = f2bf(0.0f);
 }
 
-// h0 = h_lin + J y (FP64 head)
+// linear interpolation weights of fine index i on a coarse axis of n cells
+// (cell centres at (I + 1/2) kS; constant beyond the outer centres)
+__device__ __forceinline__ void lin_axis(int i, int n, int& i0, int& i1, float& w1) {
+    const float u = (i + 0.5f) / kS - 0.5f;
+    if (u <= 0.0f) {
+        i0 = i1 = 0;
+        w1 = 0.0f;
+    } else if (u >= n - 1) {
+        i0 = i1 = n - 1;
+        w1 = 0.0f;
+    } else {
+        i0 = (int)u;
+        i1 = i0 + 1;
+        w1 = u - i0;
+    }
+}
+
+// h0 = h_lin + J trilinear(y) on the fine grid g (y: coarse, ych channels, channel 0)
 __global__ void __launch_bounds__(256)
     output_kernel(Geo g, double hw, double he, const float* __restrict__ y, int ych,
                   double* __restrict__ h) {
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.n()) return;
     const int i = (int)(v % g.nx);
+    const long q = v / g.nx;
+    const int j = (int)(q % g.ny), k = (int)(q / g.ny);
+    const int NX = g.nx / kS, NY = g.ny / kS, NZ = g.nz / kS;
+    int x0, x1, y0, y1, z0, z1;
+    float wx, wy, wz;
+    lin_axis(i, NX, x0, x1, wx);
+    lin_axis(j, NY, y0, y1, wy);
+    lin_axis(k, NZ, z0, z1, wz);
+    auto Y = [&](int a, int b, int c) {
+        return y[((long)a + (long)NX * (b + (long)NY * c)) * ych];
+    };
+    const float c00 = Y(x0, y0, z0) * (1.f - wx) + Y(x1, y0, z0) * wx;
+    const float c10 = Y(x0, y1, z0) * (1.f - wx) + Y(x1, y1, z0) * wx;
+    const float c01 = Y(x0, y0, z1) * (1.f - wx) + Y(x1, y0, z1) * wx;
+    const float c11 = Y(x0, y1, z1) * (1.f - wx) + Y(x1, y1, z1) * wx;
+    const float c0 = c00 * (1.f - wy) + c10 * wy, c1 = c01 * (1.f - wy) + c11 * wy;
+    const float yi = c0 * (1.f - wz) + c1 * wz;
     const double J = (hw - he) / g.nx;
-    h[v] = hw + (he - hw) * (i + 0.5) / g.nx + J * (double)y[v * ych];
+    h[v] = hw + (he - hw) * (i + 0.5) / g.nx + J * (double)yi;
 }
 
-// target t = (h - h_lin) / J; dOut[v][0] = 2 (y - t) / N (channels 1..ch-1 zero);
-// loss partials of (y - t)^2 per block
+// per coarse cell V: target t = mean over its block of (h - h_lin) / J;
+// dOut[V][0] = 2 (y - t) / N_coarse (channels 1..dch-1 zero); loss partials
 __global__ void __launch_bounds__(256)
     loss_kernel(Geo g, double hw, double he, const float* __restrict__ y, int ych,
                 const double* __restrict__ h, __hip_bfloat16* __restrict__ dOut, int dch,
                 float* __restrict__ partials) {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NX = g.nx / kS, NY = g.ny / kS, NZ = g.nz / kS;
+    const long nc = (long)NX * NY * NZ;
+    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
     float e2 = 0.0f;
-    if (v < g.n()) {
-        const int i = (int)(v % g.nx);
-        const double J = (hw - he) / g.nx;
-        const double hl = hw + (he - hw) * (i + 0.5) / g.nx;
-        const float t = (float)((h[v] - hl) / J);
-        const float d = y[v * ych] - t;
+    if (V < nc) {
+        const int I = (int)(V % NX);
+        const long q = V / NX;
+        const int J = (int)(q % NY), Kc = (int)(q / NY);
+        const double Jg = (hw - he) / g.nx;
+        double st = 0.0;
+        for (int dz = 0; dz < kS; ++dz)
+            for (int dy = 0; dy < kS; ++dy)
+                for (int dx = 0; dx < kS; ++dx) {
+                    const int i = kS * I + dx;
+                    const long o = (long)i + (long)g.nx * ((kS * J + dy) + (long)g.ny * (kS * Kc + dz));
+                    st += (h[o] - (hw + (he - hw) * (i + 0.5) / g.nx)) / Jg;
+                }
+        const float t = (float)(st / (kS * kS * kS));
+        const float d = y[V * ych] - t;
         e2 = d * d;
-        dOut[v * dch] = f2bf(2.0f * d / (float)g.n());
-        for (int c = 1; c < dch; ++c) dOut[v * dch + c] = f2bf(0.0f);
+        dOut[V * dch] = f2bf(2.0f * d / (float)nc);
+        for (int c = 1; c < dch; ++c) dOut[V * dch + c] = f2bf(0.0f);
     }
     __shared__ float part[256];
     part[threadIdx.x] = e2;
@@ -450,7 +517,7 @@ struct LayerDef {
     int cin, cout, level;
 };
 // conv_a .. conv_g
-constexpr LayerDef kLayers[7] = {{2, 16, 0},  {16, 32, 1}, {32, 64, 2}, {64, 64, 2},
+constexpr LayerDef kLayers[7] = {{4, 16, 0},  {16, 32, 1}, {32, 64, 2}, {64, 64, 2},
                                  {64, 32, 1}, {32, 16, 0}, {16, 1, 0}};
 constexpr int kNumLayers = 7;
 
@@ -521,7 +588,8 @@ void free_acts(mf3d_wsnet* net) {
 
 int check_grid(const mf3d_grid* g) {
     MF3D_TRY(validate_grid(g));
-    if ((g->nx % (kF * kF)) || (g->ny % (kF * kF)) || (g->nz % (kF * kF)))
+    constexpr int q = kS * kF * kF;
+    if ((g->nx % q) || (g->ny % q) || (g->nz % q))
         return fail(MF3D_ERR_UNSUPPORTED, "warm-start network needs sizes divisible by 16");
     return MF3D_OK;
 }
@@ -651,14 +719,16 @@ mf3d_grid coarse(const mf3d_grid& g, int f) {
     return c;
 }
 
-int forward(mf3d_wsnet* net, hipStream_t s, const mf3d_grid& g, const double* K, int is_log) {
+// forward pass for the fine grid gf (the network runs on gf / kS)
+int forward(mf3d_wsnet* net, hipStream_t s, const mf3d_grid& gf, const double* K, int is_log) {
+    const mf3d_grid g = coarse(gf, kS);
     MF3D_TRY(ensure_acts(net, g));
     if (!net->packed) MF3D_TRY(pack(net, s));
     const mf3d_grid g1 = coarse(g, kF), g2 = coarse(g, kF * kF);
     const Geo G0{g.nx, g.ny, g.nz}, G1{g1.nx, g1.ny, g1.nz};
     const long n0 = (long)cells(g), n1 = (long)cells(g1), n2 = (long)cells(g2);
     float* P = net->params;
-    input_kernel<<<blocks(n0), 256, 0, s>>>(G0, K, is_log, net->x0);
+    input_kernel<<<blocks(n0), 256, 0, s>>>(Geo{gf.nx, gf.ny, gf.nz}, K, is_log, net->x0);
     MF3D_TRY(conv(s, g, 16, 16, net->kpad[0], 16, net->x0, net->W[0], P + net->boff[0], nullptr,
                   nullptr, true, false, net->e0, false));
     pool_kernel<<<blocks(n1 * 16), 256, 0, s>>>(G0, 16, net->e0, net->p1);
@@ -682,15 +752,17 @@ int forward(mf3d_wsnet* net, hipStream_t s, const mf3d_grid& g, const double* K,
 }
 
 // loss and all parameter gradients (net->grads) for target head h
-int backward(mf3d_wsnet* net, hipStream_t s, const mf3d_grid& g, double hw, double he,
+int backward(mf3d_wsnet* net, hipStream_t s, const mf3d_grid& gf, double hw, double he,
              const double* h, double* loss) {
+    const mf3d_grid g = coarse(gf, kS);
     const mf3d_grid g1 = coarse(g, kF), g2 = coarse(g, kF * kF);
     const Geo G0{g.nx, g.ny, g.nz}, G1{g1.nx, g1.ny, g1.nz};
     const long n0 = (long)cells(g), n1 = (long)cells(g1), n2 = (long)cells(g2);
     const unsigned lb = blocks(n0);
     float* GR = net->grads;
     // dY of conv_g (16 channels, channel 0 live) -> g0c
-    loss_kernel<<<lb, 256, 0, s>>>(G0, hw, he, net->y, 16, h, net->g0c, 16, net->lpart);
+    loss_kernel<<<lb, 256, 0, s>>>(Geo{gf.nx, gf.ny, gf.nz}, hw, he, net->y, 16, h, net->g0c, 16,
+                                   net->lpart);
     auto grads_of = [&](int l, const mf3d_grid& gg, int cin_ch, int cout_ch,
                         const __hip_bfloat16* dY, const __hip_bfloat16* X) -> int {
         const LayerDef& L = kLayers[l];

@@ -2,8 +2,11 @@
 
 A 3-D convolutional encoder-decoder (3 levels, 16/32/64 channels, 3^3
 kernels) maps ln K to an initial head for the Darcy set-up with Dirichlet x
-faces: h0 = h_lin + J y(ln K), h_lin linear between the x faces and
-J = (h_W - h_E) / nx, so one network serves any grid size. Forward, backward
+faces: h0 = h_lin + J trilinear(y(ln K)), h_lin linear between the x faces and
+J = (h_W - h_E) / nx, so one network serves any grid size. The network sees
+block statistics of K on the grid coarsened 4x per axis and predicts the
+smooth part of the head there: MG-PCG removes high-frequency error in one
+V-cycle, so only the smooth part of a guess shortens the solve. Forward, backward
 and the Adam step run in libmf3d_hip.so (BF16 MFMA convolutions with FP32
 accumulation, FP32 master weights); this module holds the parameters, makes
 training data with the plain FP64 MG-PCG solve and drives the training.
@@ -33,7 +36,7 @@ from .core import BCSpec, Grid3D
 from .kfield import lognormal_lnK
 from .runtime import HipContext, dptr
 
-LAYERS = [(2, 16), (16, 32), (32, 64), (64, 64), (64, 32), (32, 16), (16, 1)]
+LAYERS = [(4, 16), (16, 32), (32, 64), (64, 64), (64, 32), (32, 16), (16, 1)]
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                "warmstart_128.npy")
 

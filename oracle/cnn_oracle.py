@@ -17,7 +17,8 @@ from __future__ import annotations
 
 import numpy as np
 
-LAYERS = [(2, 16), (16, 32), (32, 64), (64, 64), (64, 32), (32, 16), (16, 1)]
+LAYERS = [(4, 16), (16, 32), (32, 64), (64, 64), (64, 32), (32, 16), (16, 1)]
+S = 4  # fine cells per network cell, per axis
 TAPS = [(t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1) for t in range(27)]
 
 
@@ -88,7 +89,7 @@ def conv_bwd_weights(x, dy):
     return gw, d2.sum(axis=0)
 
 
-F = 4  # pooling factor per axis between levels
+F = 2  # pooling factor per axis between network levels
 
 
 def pool(x):
@@ -109,26 +110,58 @@ def up_bwd(dy):
     return dy.reshape(nz // F, F, ny // F, F, nx // F, F, c).sum(axis=(1, 3, 5))
 
 
-def inputs(lnK, nx):
-    """x0: [lnK / 2, 2 (i + 1/2)/nx - 1] as (nz, ny, nx, 2)."""
-    nz, ny, _ = lnK.shape
-    xc = 2.0 * (np.arange(nx) + 0.5) / nx - 1.0
-    return np.stack([0.5 * lnK, np.broadcast_to(xc, lnK.shape)], axis=-1)
+def block_mean(a):
+    """Mean over the S^3 fine blocks of a fine (nz, ny, nx) array."""
+    nz, ny, nx = a.shape
+    return a.reshape(nz // S, S, ny // S, S, nx // S, S).mean(axis=(1, 3, 5))
+
+
+def inputs(lnK):
+    """x0 on the coarse grid: [<lnK>/2, log<K>/2, -log<1/K>/2, 2 (I + 1/2)/NX - 1]."""
+    K = np.exp(lnK)
+    a = 0.5 * block_mean(lnK)
+    b = 0.5 * np.log(block_mean(K))
+    c = -0.5 * np.log(block_mean(1.0 / K))
+    NX = a.shape[2]
+    xc = 2.0 * (np.arange(NX) + 0.5) / NX - 1.0
+    return np.stack([a, b, c, np.broadcast_to(xc, a.shape)], axis=-1)
 
 
 def target(h, hw, he):
-    """(h - h_lin) / J, J = (hw - he) / nx."""
+    """Block mean of (h - h_lin) / J, J = (hw - he) / nx (coarse grid)."""
     nx = h.shape[2]
     hl = hw + (he - hw) * (np.arange(nx) + 0.5) / nx
-    return (h - hl) / ((hw - he) / nx)
+    return block_mean((h - hl) / ((hw - he) / nx))
+
+
+def _lin_axis(n_fine, n):
+    u = (np.arange(n_fine) + 0.5) / S - 0.5
+    i0 = np.clip(np.floor(u), 0, n - 1).astype(int)
+    i1 = np.minimum(i0 + 1, n - 1)
+    w = np.where((u <= 0) | (u >= n - 1), 0.0, u - np.floor(u))
+    return i0, i1, w
+
+
+def output(y, hw, he, fine_shape):
+    """h0 = h_lin + J trilinear(y) on the fine grid."""
+    nz, ny, nx = fine_shape
+    Z, Y, X = y.shape
+    x0, x1, wx = _lin_axis(nx, X)
+    y0, y1, wy = _lin_axis(ny, Y)
+    z0, z1, wz = _lin_axis(nz, Z)
+    a = y[:, :, x0] * (1 - wx) + y[:, :, x1] * wx
+    b = a[:, y0, :] * (1 - wy)[:, None] + a[:, y1, :] * wy[:, None]
+    c = b[z0] * (1 - wz)[:, None, None] + b[z1] * wz[:, None, None]
+    J = (hw - he) / nx
+    return hw + (he - hw) * (np.arange(nx) + 0.5) / nx + J * c
 
 
 def forward(p, lnK, rb=False):
-    """Network output y (nz, ny, nx) and the activations, for flat params p."""
+    """Network output y on the coarse grid and the activations, for flat params
+    p and the fine (nz, ny, nx) ln K."""
     r = bf16 if rb else (lambda a: a)
     L = [(r(w), b) for w, b in unflatten(np.asarray(p, dtype=np.float64))]
-    nx = lnK.shape[2]
-    A = {"x0": r(inputs(lnK, nx))}
+    A = {"x0": r(inputs(lnK))}
     relu = lambda a: np.maximum(a, 0.0)  # noqa: E731
     A["e0"] = r(relu(conv(A["x0"], *L[0])))
     A["p1"] = r(pool(A["e0"]))
@@ -146,7 +179,8 @@ def forward(p, lnK, rb=False):
 
 
 def loss_and_grads(p, lnK, h, hw, he, rb=False):
-    """mean((y - t)^2) and the flat gradient, t = (h - h_lin)/J."""
+    """mean over the coarse grid of (y - t)^2 and the flat gradient, t the
+    block mean of (h - h_lin)/J."""
     r = bf16 if rb else (lambda a: a)
     y, A = forward(p, lnK, rb)
     L = A["L"]

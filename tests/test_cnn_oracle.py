@@ -58,7 +58,7 @@ def test_backward_routines_are_adjoints():
     gw, gb = CO.conv_bwd_weights(x, dy)
     assert np.isclose(lhs, float((w * gw).sum()), rtol=1e-12)
     assert np.allclose(gb, dy.reshape(-1, 7).sum(axis=0))
-    x = rng.standard_normal((8, 4, 12, 5))
+    x = rng.standard_normal((4, 2, 6, 5))
     p = rng.standard_normal((2, 1, 3, 5))
     assert np.isclose(float((CO.pool(x) * p).sum()), float((x * CO.pool_bwd(p)).sum()))
     assert np.isclose(float((CO.up(p) * x).sum()), float((p * CO.up_bwd(x)).sum()))
@@ -74,7 +74,7 @@ def test_bf16_rounding_matches_torch():
 
 def test_network_gradient_matches_finite_differences():
     rng = np.random.default_rng(5)
-    n = 16
+    n = 32
     lnK = rng.standard_normal((n, n, n))
     h = rng.standard_normal((n, n, n)) * 0.01 + np.linspace(1, 0, n)
     p = CO.init_params(7)
@@ -95,9 +95,32 @@ def test_network_gradient_matches_finite_differences():
         assert abs(fd - g[q]) <= 1e-5 * max(1e-3, abs(fd)) + 1e-9, (q, fd, g[q])
 
 
-@pytest.mark.parametrize("rb", [False, True])
-def test_target_roundtrip(rb):
-    h = np.random.default_rng(6).standard_normal((4, 4, 8))
-    t = CO.target(h, 2.0, -1.0)
-    hl = 2.0 + (-3.0) * (np.arange(8) + 0.5) / 8
-    assert np.allclose(hl + t * (3.0 / 8), h)
+def test_target_and_output_closed_forms():
+    """A head linear in x plus a constant offset c gives the target c / J on
+    every block, and the trilinear output of a constant y is h_lin + J y; a y
+    linear in the coarse x index is reproduced exactly between the outer centres."""
+    nx = 16
+    hl = 2.0 + (-3.0) * (np.arange(nx) + 0.5) / nx
+    h = np.broadcast_to(hl, (8, 4, nx)) + 0.3
+    J = 3.0 / nx
+    assert np.allclose(CO.target(h, 2.0, -1.0), 0.3 / J)
+    y = np.full((2, 1, 4), 1.7)
+    assert np.allclose(CO.output(y, 2.0, -1.0, (8, 4, nx)), hl + J * 1.7)
+    y = np.broadcast_to(np.arange(4.0), (2, 1, 4))
+    out = CO.output(y, 2.0, -1.0, (8, 4, nx))
+    u = (np.arange(nx) + 0.5) / 4 - 0.5
+    inner = (u >= 0) & (u <= 3)
+    assert np.allclose(((out - hl) / J)[..., inner], u[inner])
+
+
+def test_inputs_block_statistics():
+    rng = np.random.default_rng(8)
+    lnK = rng.standard_normal((8, 8, 8))
+    x0 = CO.inputs(lnK)
+    blk = lnK[:4, :4, 4:8]
+    K = np.exp(blk)
+    assert np.isclose(x0[0, 0, 1, 0], 0.5 * blk.mean())
+    assert np.isclose(x0[0, 0, 1, 1], 0.5 * np.log(K.mean()))
+    assert np.isclose(x0[0, 0, 1, 2], -0.5 * np.log((1 / K).mean()))
+    assert x0[0, 0, 1, 1] >= x0[0, 0, 1, 0] >= x0[0, 0, 1, 2]  # arithmetic >= geometric >= harmonic
+    assert np.isclose(x0[0, 0, 1, 3], 2 * 1.5 / 2 - 1)
