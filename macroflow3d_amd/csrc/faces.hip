@@ -483,12 +483,41 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     } else {
         load4(b + own, bb);
     }
+    if (MODE == 2) {
+        // prolongation: each in-domain neighbour gains its coarse parent's value.
+        // Own cells i = 8q + 2e + p have parents I = 4q + e; their y/z neighbours
+        // too (in the parent rows of j +- 1, k +- 1); x neighbours (i +- 1) >> 1.
+        const int NXc = d.nx >> 1, NYc = d.ny >> 1, NZc = d.nz >> 1;
+        const int J0 = j >> 1, K0 = k >> 1;
+        const long hc = NXc >> 1, HC = hc * NYc * NZc;
+        auto cv = [&](int I, int J, int K) -> float {
+            if (CS) return xc[(long)((I + J + K) & 1) * HC + (I >> 1) + hc * ((long)J + (long)NYc * K)];
+            return xc[(long)I + (long)NXc * ((long)J + (long)NYc * K)];
+        };
+        const bool yp = j + 1 < d.ny || bc.t[3] == MF3D_BC_PERIODIC;
+        const bool ym = j > 0 || bc.t[2] == MF3D_BC_PERIODIC;
+        const bool zp = k + 1 < d.nz || bc.t[5] == MF3D_BC_PERIODIC;
+        const bool zm = k > 0 || bc.t[4] == MF3D_BC_PERIODIC;
+        const int Jp = j + 1 < d.ny ? (j + 1) >> 1 : 0, Jm = j > 0 ? (j - 1) >> 1 : NYc - 1;
+        const int Kp = k + 1 < d.nz ? (k + 1) >> 1 : 0, Km = k > 0 ? (k - 1) >> 1 : NZc - 1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int I = 4 * q + e, i = 2 * I + p;
+            if (yp) vyp[e] += cv(I, Jp, K0);
+            if (ym) vym[e] += cv(I, Jm, K0);
+            if (zp) vzp[e] += cv(I, J0, Kp);
+            if (zm) vzm[e] += cv(I, J0, Km);
+            if (i + 1 < d.nx) vxp[e] += cv((i + 1) >> 1, J0, K0);
+            else if (bc.t[1] == MF3D_BC_PERIODIC) vxp[e] += cv(0, J0, K0);
+            if (i > 0) vxm[e] += cv((i - 1) >> 1, J0, K0);
+            else if (bc.t[0] == MF3D_BC_PERIODIC) vxm[e] += cv(NXc - 1, J0, K0);
+        }
+    }
     float res[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         float f[6] = {fxp[e], fxm[e], fyp[e], fym[e], fzp[e], fzm[e]};
         float v[6] = {vxp[e], vxm[e], vyp[e], vym[e], vzp[e], vzm[e]};
-        if (MODE == 2) add_parents<CS>(d, bc, xc, 2 * (4 * q + e) + p, j, k, v);
         res[e] = gs_value(f, v, bb[e], bs, pin && colour == 0 && row == 0 && q == 0 && e == 0);
     }
     if (valid) *reinterpret_cast<float4*>(x + own) = make_float4(res[0], res[1], res[2], res[3]);
