@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"))
     ap.add_argument("--summarize-trace", metavar="DIR")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="headline from the linear head instead of the learned warm start")
+    ap.add_argument("--weights", default=None, help="warm-start parameters (.npy)")
     return ap.parse_args()
 
 
@@ -124,7 +127,7 @@ def cpu_baseline(args, K_host, gpu_iters):
     b = O.darcy_bc()
     times = []
     for its in (1, 1 + args.cpu_iters):
-        p = O.mode_n(mg_levels=args.levels, cg_rtol=args.rtol, cg_max_iter=its)
+        p = O.mode_n(mg_levels=args.levels, cg_rtol=args.rtol, cg_max_iter=its, init_guess=0)
         t0 = time.perf_counter()
         O.solve_head(g, b, p, K_host)
         times.append(time.perf_counter() - t0)
@@ -196,19 +199,34 @@ def main():
 
     coef = {"bf16": ph.CoefFormat.BF16, "fp16": ph.CoefFormat.FP16,
             "fp32": ph.CoefFormat.FP32}[args.coef]
+    # plain FP64 MG-PCG starts, like the pipeline, from the linear head
+    # (init_head_guess before solve_head, EnsembleRunner.cu:286-290)
     cfg64 = ph.HeadSolveConfig.north_star(args.rtol, args.levels)
-    cfgmx = ph.HeadSolveConfig.mixed(args.rtol, args.levels, coef)
+    cfg64.init_guess = ph.InitGuess.REFERENCE
+    cfglin = ph.HeadSolveConfig.mixed(args.rtol, args.levels, coef)
+    cfglin.init_guess = ph.InitGuess.REFERENCE
+    cfgws = ph.HeadSolveConfig.mixed(args.rtol, args.levels, coef)
+    cfgws.init_guess = ph.InitGuess.PROVIDED
+    net = None
+    if not args.no_warm_start:
+        from macroflow3d_amd import warmstart as wsm
+        net = wsm.WarmStartNet.load(args.weights)
 
-    def timed(cfg, steps, warmup, h, vel, ws):
+    def step(cfg, h, vel, ws, warm):
+        if warm:
+            net.predict(ctx, K, g, bc, h)
+        res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+        ph.compute_velocity_from_head(vel, h, K, g, bc, ctx)
+        return res
+
+    def timed(cfg, steps, warmup, h, vel, ws, warm=False):
         res = None
         for _ in range(warmup):
-            res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
-            ph.compute_velocity_from_head(vel, h, K, g, bc, ctx)
+            res = step(cfg, h, vel, ws, warm)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
-            res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
-            ph.compute_velocity_from_head(vel, h, K, g, bc, ctx)
+            res = step(cfg, h, vel, ws, warm)
         torch.cuda.synchronize()
         return res, (time.perf_counter() - t) * 1e3 / max(1, steps)
 
@@ -216,22 +234,30 @@ def main():
     ws64 = ph.FlowWorkspace()
     h64 = ctx.zeros(N)
     v64 = ph.VelocityField(g, ctx)
-    r64, ms64 = timed(cfg64, args.fp64_steps, 1 if args.fp64_steps else 0, h64, v64, ws64) \
-        if args.fp64_steps else (ph.solve_head(h64, K, g, bc, cfg64, ctx, ws64), float("nan"))
-    if not args.fp64_steps:
-        ph.compute_velocity_from_head(v64, h64, K, g, bc, ctx)
+    if args.fp64_steps:
+        r64, ms64 = timed(cfg64, args.fp64_steps, 1, h64, v64, ws64)
+    else:
+        r64, ms64 = step(cfg64, h64, v64, ws64, False), float("nan")
     ws64.close()
     log(args, f"fp64: {r64.num_iterations} iterations, rel true residual "
               f"{r64.true_residual / r64.bnorm:.3e}, {ms64:.2f} ms per solve")
 
-    # headline: mixed precision
+    # mixed precision from the linear head
     ws = ph.FlowWorkspace()
     h = ctx.zeros(N)
     vel = ph.VelocityField(g, ctx)
-    res, ms = timed(cfgmx, args.steps, args.warmup, h, vel, ws)
-    log(args, f"mixed ({args.coef}): {res.num_iterations} iterations, rel true residual "
-              f"{res.true_residual / res.bnorm:.3e}, setup {res.setup_ms:.2f} ms, iteration "
-              f"{res.solve_ms:.2f} ms, step {ms:.2f} ms")
+    rlin, mslin = timed(cfglin, args.steps, args.warmup, h, vel, ws)
+    log(args, f"mixed ({args.coef}), linear guess: {rlin.num_iterations} iterations, "
+              f"{mslin:.2f} ms per step")
+    # headline: mixed precision from the learned warm start
+    if net is not None:
+        res, ms = timed(cfgws, args.steps, args.warmup, h, vel, ws, warm=True)
+        log(args, f"mixed ({args.coef}) + warm start: {res.num_iterations} iterations, rel true "
+                  f"residual {res.true_residual / res.bnorm:.3e}, setup {res.setup_ms:.2f} ms, "
+                  f"iteration {res.solve_ms:.2f} ms, step {ms:.2f} ms")
+    else:
+        res, ms = rlin, mslin
+        step(cfglin, h, vel, ws, False)
 
     def rel(a, b):
         return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
@@ -276,13 +302,17 @@ def main():
                         f"{n >> (args.levels - 1)}^3 direct FP64), V(2,2) GSRB, "
                         f"||b - A h|| <= {args.rtol:g} ||b||",
             "cells": N, "seed": args.seed, "method": "mixed-precision MG-PCG",
-            "warm_start": "zero guess",
+            "warm_start": ("learned (3-D conv encoder-decoder, BF16 MFMA, inference inside "
+                           "the timed step)" if net is not None else "linear head"),
         },
         "iterations": res.num_iterations,
+        "mixed_linear_guess_ms": round(mslin, 3),
+        "mixed_linear_guess_iterations": rlin.num_iterations,
         "rel_true_residual": res.true_residual / res.bnorm,
         "setup_ms": round(res.setup_ms, 3),
         "iteration_ms": round(res.solve_ms, 3),
         "fp64_ms": round(ms64, 3),
+        "fp64_guess": "linear head",
         "fp64_iterations": r64.num_iterations,
         "speedup_vs_fp64": round(ms64 / ms, 3) if ms > 0 else None,
         "head_rel_l2_vs_fp64": head_err,

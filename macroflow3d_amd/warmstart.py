@@ -68,8 +68,9 @@ class WarmStartNet:
         self.set_params(init_params(seed) if params is None else params)
 
     @classmethod
-    def load(cls, path: str = DEFAULT_WEIGHTS) -> "WarmStartNet":
-        return cls(np.load(path, allow_pickle=False))
+    def load(cls, path: str | None = None) -> "WarmStartNet":
+        """Network with the parameters of path (default: the shipped 128^3 training)."""
+        return cls(np.load(path or DEFAULT_WEIGHTS, allow_pickle=False))
 
     def save(self, path: str) -> None:
         np.save(path, self.get_params())

@@ -162,18 +162,20 @@ def test_training_reduces_loss(ctx):
 @gpu
 @pytest.mark.skipif(not os.path.exists(W.DEFAULT_WEIGHTS), reason="no trained weights")
 def test_trained_network_beats_zero_and_linear_guess(ctx):
-    """Held-out 128^3 fields (sigma2 2-4, correlation 4-12 cells, seeds not used
-    in training): PCG from the predicted head reaches ||b - A h|| <= 1e-10 ||b||
-    in fewer iterations than from the zero or the linear head."""
+    """24 held-out 128^3 fields (sigma2 2-4, correlation 4-12 cells, seeds not
+    used in training): PCG from the predicted head reaches ||b - A h|| <= 1e-10
+    ||b|| on every field, in fewer iterations on average than from the zero or
+    the linear head, and never in more than the linear head needs plus one."""
     net = W.WarmStartNet.load()
-    params = W.field_params(77_000_000, 4)
+    params = W.field_params(77_000_000, 24)
     print("warm start held-out 128^3", flush=True)
     samples = W.make_samples(ctx, 128, params)
     rows = W.compare_guesses(ctx, net, samples, 128)
     for r in rows:
         for k in ("zero", "linear", "predicted"):
             assert r[k]["converged"] and r[k]["rel_true_residual"] <= 1e-10
-        assert r["predicted"]["initial_rel_residual"] < r["linear"]["initial_rel_residual"]
+        assert r["predicted"]["iterations"] <= r["linear"]["iterations"] + 1
     it = {k: np.mean([r[k]["iterations"] for r in rows]) for k in ("zero", "linear", "predicted")}
-    assert it["predicted"] < it["linear"] and it["predicted"] < it["zero"], it
+    print(f"mean PCG iterations: {it}", flush=True)
+    assert it["predicted"] < it["linear"] < it["zero"], it
     net.close()
