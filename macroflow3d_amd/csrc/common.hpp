@@ -166,8 +166,9 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
     BcTypes bc;
     int pin;
     double inv_dx2;
-    const double* K;
-    const double* iK;
+    const double* fx;  // FP64 faces of the finest level
+    const double* fy;
+    const double* fz;
     double* x;
     double* r;
     double* z;
@@ -180,6 +181,12 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
     unsigned np;
     IterState* st;
     double* out;  // scalar result of a FIN_STORE / FIN_SQRT second stage
+};
+
+struct BandDesc {  // banded Cholesky factor of one coarsest operator and its inverse
+    int n, w;
+    const double* Lb;
+    double* Ainv;
 };
 
 __device__ __forceinline__ int batch_entry(const int* __restrict__ start, int m, long blk) {

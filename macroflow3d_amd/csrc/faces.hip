@@ -979,41 +979,14 @@ __device__ inline double block_sum(double v) {
     return v;
 }
 
-// Faces (x+, x-, y+, y-, z+, z-) of cell o from K and iK = 1/K with the same
-// operations as faces_build_kernel, hence the same values: interior and
-// periodic 2 / (1/K_a + 1/K_b), Dirichlet 2 K_C, Neumann 0. The FP64 operator
-// reads 16 B per cell this way instead of the 24 B of stored FP64 faces.
-__device__ __forceinline__ void cell_faces_ik(const Dims& d, const BcTypes& bc, const double* K,
-                                              const double* iK, long o, int i, int j, int k,
-                                              double (&f)[6]) {
-    const long sy = d.nx, sz = (long)d.nx * d.ny;
-    const double a = iK[o];
-    if (i < d.nx - 1) f[0] = 2.0 / (iK[o + 1] + a);
-    else f[0] = bc.t[1] == MF3D_BC_PERIODIC ? 2.0 / (a + iK[o - (d.nx - 1)]) : bface(bc.t[1], K[o]);
-    if (i > 0) f[1] = 2.0 / (a + iK[o - 1]);
-    else f[1] = bc.t[0] == MF3D_BC_PERIODIC ? 2.0 / (a + iK[o + d.nx - 1]) : bface(bc.t[0], K[o]);
-    if (j < d.ny - 1) f[2] = 2.0 / (iK[o + sy] + a);
-    else f[2] = bc.t[3] == MF3D_BC_PERIODIC ? 2.0 / (a + iK[o - (long)(d.ny - 1) * sy])
-                                            : bface(bc.t[3], K[o]);
-    if (j > 0) f[3] = 2.0 / (a + iK[o - sy]);
-    else f[3] = bc.t[2] == MF3D_BC_PERIODIC ? 2.0 / (a + iK[o + (long)(d.ny - 1) * sy])
-                                            : bface(bc.t[2], K[o]);
-    if (k < d.nz - 1) f[4] = 2.0 / (iK[o + sz] + a);
-    else f[4] = bc.t[5] == MF3D_BC_PERIODIC ? 2.0 / (a + iK[o - (long)(d.nz - 1) * sz])
-                                            : bface(bc.t[5], K[o]);
-    if (k > 0) f[5] = 2.0 / (a + iK[o - sz]);
-    else f[5] = bc.t[4] == MF3D_BC_PERIODIC ? 2.0 / (a + iK[o + (long)(d.nz - 1) * sz])
-                                            : bface(bc.t[4], K[o]);
-}
-
 // A x = -sum f (x_C - x_nb) / dx^2 (varcoeff_laplacian.cu:38-239); the pin's
 // diagonal counts in-domain neighbours only (varcoeff_laplacian.cu:110-234).
 __device__ __forceinline__ double apply_cell(const Dims& d, const BcTypes& bc, bool pin,
-                                             const double* x, const double* K,
-                                             const double* iK, long o, int i, int j, int k,
-                                             double inv_dx2) {
+                                             const double* x, const double* FX,
+                                             const double* FY, const double* FZ, long o, int i,
+                                             int j, int k, double inv_dx2) {
     double f[6], v[6];
-    cell_faces_ik(d, bc, K, iK, o, i, j, k, f);
+    cell_faces(d, FX, FY, FZ, i, j, k, f);
     neighbours(d, bc, x, o, i, j, k, v);
     const double xC = x[o];
     double Ax = f[1] * (xC - v[1]) + f[0] * (xC - v[0]) + f[3] * (xC - v[3]) +
@@ -1035,8 +1008,9 @@ __device__ __forceinline__ double apply_cell(const Dims& d, const BcTypes& bc, b
 template <int MODE>
 __device__ __forceinline__ void op_body(const Dims& d, const BcTypes& bc, int pin, double inv_dx2,
                                         const double* __restrict__ x,
-                                        const double* __restrict__ K,
-                                        const double* __restrict__ iK,
+                                        const double* __restrict__ FX,
+                                        const double* __restrict__ FY,
+                                        const double* __restrict__ FZ,
                                         const double* __restrict__ b, double* __restrict__ y,
                                         double* __restrict__ partials, long blk) {
     const long n = (long)d.nx * d.ny * d.nz;
@@ -1047,7 +1021,7 @@ __device__ __forceinline__ void op_body(const Dims& d, const BcTypes& bc, int pi
         const long q = o / d.nx;
         const int j = (int)(q % d.ny);
         const int k = (int)(q / d.ny);
-        const double Ax = apply_cell(d, bc, pin != 0, x, K, iK, o, i, j, k, inv_dx2);
+        const double Ax = apply_cell(d, bc, pin != 0, x, FX, FY, FZ, o, i, j, k, inv_dx2);
         if (MODE == 0) {
             y[o] = Ax;
             prod = x[o] * Ax;
@@ -1064,19 +1038,13 @@ __device__ __forceinline__ void op_body(const Dims& d, const BcTypes& bc, int pi
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
     op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, const double* __restrict__ x,
-              const double* __restrict__ K, const double* __restrict__ iK,
-              const double* __restrict__ b, double* __restrict__ y,
-              double* __restrict__ partials) {
+              const double* __restrict__ FX, const double* __restrict__ FY,
+              const double* __restrict__ FZ, const double* __restrict__ b,
+              double* __restrict__ y, double* __restrict__ partials) {
     const long n = (long)d.nx * d.ny * d.nz;
     const long blk = xcd_block((n + kBlock - 1) / kBlock);
     if (blk < 0) return;  // padding block: uniform exit before the block reduction
-    op_body<MODE>(d, bc, pin, inv_dx2, x, K, iK, b, y, partials, blk);
-}
-
-__global__ void __launch_bounds__(kBlock)
-    inverse_kernel(const double* __restrict__ K, double* __restrict__ iK, long n) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) iK[i] = 1.0 / K[i];
+    op_body<MODE>(d, bc, pin, inv_dx2, x, FX, FY, FZ, b, y, partials, blk);
 }
 
 template <typename T>
@@ -1280,10 +1248,10 @@ __global__ void __launch_bounds__(kBlock)
     if (MODE == 0 && V.st->stop) return;
     const Dims d{V.nx, V.ny, V.nz};
     if (MODE == 0)
-        op_body<0>(d, V.bc, V.pin, V.inv_dx2, V.p, V.K, V.iK, nullptr, V.Ap, V.partials,
+        op_body<0>(d, V.bc, V.pin, V.inv_dx2, V.p, V.fx, V.fy, V.fz, nullptr, V.Ap, V.partials,
                    blk - start[e]);
     else
-        op_body<1>(d, V.bc, V.pin, V.inv_dx2, V.x, V.K, V.iK, V.rhs, V.r, V.partials,
+        op_body<1>(d, V.bc, V.pin, V.inv_dx2, V.x, V.fx, V.fy, V.fz, V.rhs, V.r, V.partials,
                    blk - start[e]);
 }
 
@@ -1636,23 +1604,20 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
     }
 }
 
-int op_k(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
-         const double* iK, int mode, const double* x, const double* b, double* y,
-         double* partials) {
+int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
+             int mode, const double* x, const double* b, double* y, double* partials) {
+    if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "operator needs FP64 faces");
     const size_t n = cells(g);
     const double inv_dx2 = 1.0 / (g.dx * g.dx);
+    const double* FX = cptr<double>(F, 0);
+    const double* FY = cptr<double>(F, 1);
+    const double* FZ = cptr<double>(F, 2);
     if (mode == 0)
-        op_kernel<0><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, K, iK, b,
-                                                    y, partials);
+        op_kernel<0><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
+                                                    FZ, b, y, partials);
     else
-        op_kernel<1><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, K, iK, b,
-                                                    y, partials);
-    MF3D_HIP_TRY(hipGetLastError());
-    return MF3D_OK;
-}
-
-int inverse(hipStream_t s, const double* K, double* iK, size_t n) {
-    inverse_kernel<<<grid_for(n), kBlock, 0, s>>>(K, iK, (long)n);
+        op_kernel<1><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
+                                                    FZ, b, y, partials);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

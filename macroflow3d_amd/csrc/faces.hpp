@@ -73,14 +73,10 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
                             double rs, const float* x, const float* b, const FaceSet& F,
                             float* bcoarse, int order = ORDER_NATURAL,
                             int corder = ORDER_NATURAL);
-// FP64 operator from K and iK = 1/K (faces formed on the fly, same values as
-// faces_build). mode 0: y = A x, partials (x, y); mode 1: y = b - A x (y
-// nullable), partials (y, y).
-int op_k(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
-         const double* iK, int mode, const double* x, const double* b, double* y,
-         double* partials);
-// iK = 1 / K
-int inverse(hipStream_t s, const double* K, double* iK, size_t n);
+// FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
+// mode 1: y = b - A x (y nullable), partials (y, y).
+int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
+             int mode, const double* x, const double* b, double* y, double* partials);
 int to_f32(hipStream_t s, const double* a, float* o, size_t n);
 int to_f64(hipStream_t s, const float* a, double* o, size_t n);
 // conversions between a natural FP64 vector and an FP32 vector of `order`

@@ -25,6 +25,9 @@ int rhs(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* K, d
 int guess(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, int mode, double* h);
 int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
 int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv);
+// band_inverse for m factors in one launch (nblocks = sum of n, nmax = max n)
+int band_inverse_batch(hipStream_t s, const BandDesc* D, const int* start, int m, int nblocks,
+                       int nmax);
 
 // ---- fast fine-level kernels (stencil_fast.hip) ----------------------------
 // True when the z-marching kernels cover this level (size and BC limits).

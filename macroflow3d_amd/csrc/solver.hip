@@ -118,7 +118,6 @@ struct mf3d_workspace {
     int precision = MF3D_PRECOND_FP64;
     CoefType lo_type = COEF_BF16;
     const double* K0 = nullptr;
-    double* iK = nullptr;  // 1 / K on the finest level (FP64 operator)
     bool ready = false;
     // tuning (mf3d_workspace_set_tuning / autotuner)
     long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
@@ -172,7 +171,6 @@ struct mf3d_workspace {
         coarse_cap = 0;
         nvec = 0;
         rhs = r = z = p = Ap = nullptr;
-        iK = nullptr;
         partials = nullptr;
         partials_cap = 0;
         st = nullptr;
@@ -283,14 +281,11 @@ int host_scalar_sync(mf3d_workspace* ws, hipStream_t s, double* out) {
 // residual operator, pin included), banded Cholesky on the host, explicit
 // inverse on the device (band_inverse_kernel).
 // ---------------------------------------------------------------------------
-int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
-    Level& L = ws->lv.back();
+// Banded Cholesky factor of M = -A (coarsest level) from its host K: returns
+// the band width w and fills band (row-major, band[i*(w+1) + (i-j)] = L[i][j]).
+int coarse_band(const mf3d_workspace* ws, const double* Kc, std::vector<double>& band, int* wout) {
+    const Level& L = ws->lv.back();
     const int n = (int)L.n;
-    if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
-    std::vector<double> Kc(n);
-    const double* Kdev = ws->lv.size() == 1 ? ws->K0 : L.K;
-    MF3D_HIP_TRY(hipMemcpyAsync(Kc.data(), Kdev, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    MF3D_HIP_TRY(hipStreamSynchronize(s));
     const mf3d_grid& g = L.g;
     const double dx2 = g.dx * g.dx;
     // band width from the neighbour offsets actually present
@@ -304,8 +299,8 @@ int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
                 double asum = 0.0;
                 for (int d = 0; d < 6; ++d) {
                     double coef;
-                    const long nb =
-                        neighbour_coef(g.nx, g.ny, g.nz, ws->bct, Kc.data(), i, j, k, d, Kc[c], coef);
+                    const long nb = neighbour_coef(g.nx, g.ny, g.nz, ws->bct, Kc, i, j, k, d, Kc[c],
+                                                   coef);
                     asum += coef;
                     if (nb >= 0) {
                         rows[c].push_back({(int)nb, -coef / dx2});
@@ -316,7 +311,7 @@ int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
                 diag[c] = asum / dx2;
             }
     const int W1 = w + 1;
-    std::vector<double> band((size_t)n * W1, 0.0);  // band[i*W1 + (i-j)] = M[i][j], j <= i
+    band.assign((size_t)n * W1, 0.0);  // band[i*W1 + (i-j)] = M[i][j], j <= i
     for (int c = 0; c < n; ++c) {
         band[(size_t)c * W1] += diag[c];
         for (auto& e : rows[c]) {
@@ -345,12 +340,39 @@ int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
             band[(size_t)i * W1 + (i - j)] = sacc / Ljj;
         }
     }
-    const size_t need = (size_t)n * n + (size_t)n * W1;
+    *wout = w;
+    return MF3D_OK;
+}
+
+const double* coarsest_K(const mf3d_workspace* ws) {
+    return ws->lv.size() == 1 ? ws->K0 : ws->lv.back().K;
+}
+
+// room for the n*n inverse followed by the n*(w+1) band
+int reserve_coarse(mf3d_workspace* ws, int n, int w) {
+    const size_t need = (size_t)n * n + (size_t)n * (w + 1);
     if (!ws->coarse_inv || ws->coarse_cap < need) {
         MF3D_TRY(ws->alloc(&ws->coarse_inv, need));
         ws->coarse_cap = need;
     }
     ws->coarse_n = n;
+    return MF3D_OK;
+}
+
+// Dense coarsest inverse: coarsest K to the host, banded Cholesky there, the
+// columns of the inverse on the GPU (band_inverse_kernel).
+int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
+    Level& L = ws->lv.back();
+    const int n = (int)L.n;
+    if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
+    std::vector<double> Kc(n);
+    MF3D_HIP_TRY(hipMemcpyAsync(Kc.data(), coarsest_K(ws), n * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<double> band;
+    int w = 0;
+    MF3D_TRY(coarse_band(ws, Kc.data(), band, &w));
+    MF3D_TRY(reserve_coarse(ws, n, w));
     double* dband = ws->coarse_inv + (size_t)n * n;
     MF3D_HIP_TRY(hipMemcpyAsync(dband, band.data(), band.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s));
@@ -372,7 +394,7 @@ inline const double* level_K(const mf3d_workspace* ws, size_t l) {
 // (coarsen_K.cu:15-53), face coefficients per level, smoothing plans and, with
 // a DIRECT coarse solver, the dense coarsest inverse.
 int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc,
-             const mf3d_head_solve_config& cfg, const double* K) {
+             const mf3d_head_solve_config& cfg, const double* K, bool defer_coarse = false) {
     MF3D_TRY(ensure_workspace(ws, g, cfg.mg_levels));
     ws->bc = bc;
     ws->bct = bc_types(bc);
@@ -385,8 +407,6 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     ws->lo_type = coef_type_of(cfg.coef_format);
     ws->K0 = K;
     const bool mixed = ws->precision == MF3D_PRECOND_FP32;
-    if (!ws->iK) MF3D_TRY(ws->alloc(&ws->iK, cells(g)));
-    MF3D_TRY(kern::inverse(s, K, ws->iK, cells(g)));
     // fp16 coefficients are stored divided by the geometric mean of K (a power
     // of two), so both tails of a log-normal field stay inside fp16's range
     double cs = 1.0;
@@ -403,7 +423,7 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     for (size_t l = 0; l < ws->lv.size(); ++l) {
         Level& L = ws->lv[l];
         if (l > 0) MF3D_TRY(kern::coarsen(s, L.g, level_K(ws, l - 1), L.K));
-        const bool want64 = !mixed;  // the FP64 V-cycle (the operator works from K and 1/K)
+        const bool want64 = (l == 0) || !mixed;  // the FP64 operator, the FP64 V-cycle
         if (want64) {
             if (!L.has_f64) {
                 MF3D_TRY(ws->alloc_faces(L.f64, L.g, COEF_F64));
@@ -440,8 +460,67 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
         if (!L.b32) MF3D_TRY(ws->alloc(&L.b32, L.n));
         if (L.plan.kind == SMOOTH_MARCH && !L.t32) MF3D_TRY(ws->alloc(&L.t32, L.n));
     }
-    if (cfg.coarse_solver == MF3D_COARSE_DIRECT) MF3D_TRY(build_coarse_inverse(ws, s));
+    if (cfg.coarse_solver == MF3D_COARSE_DIRECT && !defer_coarse)
+        MF3D_TRY(build_coarse_inverse(ws, s));
     return MF3D_OK;
+}
+
+// build_coarse_inverse for many workspaces with one host round trip and one
+// band-inverse launch; the pinned host staging (keep_host) and the device
+// descriptors (keep_dev) must live until s completes
+int build_coarse_inverses(mf3d_workspace* const* ws, int count, hipStream_t s,
+                          std::vector<void*>& keep_host, std::vector<void*>& keep_dev) {
+    std::vector<size_t> off(count + 1, 0);
+    for (int r = 0; r < count; ++r) {
+        const size_t n = ws[r]->lv.back().n;
+        if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
+        off[r + 1] = off[r] + n;
+    }
+    double* kh = nullptr;
+    MF3D_HIP_TRY(hipHostMalloc((void**)&kh, off[count] * sizeof(double), hipHostMallocDefault));
+    keep_host.push_back(kh);
+    for (int r = 0; r < count; ++r)
+        MF3D_HIP_TRY(hipMemcpyAsync(kh + off[r], coarsest_K(ws[r]),
+                                    (off[r + 1] - off[r]) * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<std::vector<double>> bands(count);
+    std::vector<int> wv(count);
+    size_t tot = 0;
+    for (int r = 0; r < count; ++r) {
+        MF3D_TRY(coarse_band(ws[r], kh + off[r], bands[r], &wv[r]));
+        tot += bands[r].size();
+    }
+    double* bh = nullptr;
+    MF3D_HIP_TRY(hipHostMalloc((void**)&bh, tot * sizeof(double) + sizeof(BandDesc) * count +
+                                                sizeof(int) * (count + 1),
+                               hipHostMallocDefault));
+    keep_host.push_back(bh);
+    BandDesc* dh = reinterpret_cast<BandDesc*>(bh + tot);
+    int* sh = reinterpret_cast<int*>(dh + count);
+    size_t bo = 0;
+    int nmax = 0;
+    sh[0] = 0;
+    for (int r = 0; r < count; ++r) {
+        const int n = (int)ws[r]->lv.back().n;
+        MF3D_TRY(reserve_coarse(ws[r], n, wv[r]));
+        double* dband = ws[r]->coarse_inv + (size_t)n * n;
+        std::memcpy(bh + bo, bands[r].data(), bands[r].size() * sizeof(double));
+        MF3D_HIP_TRY(hipMemcpyAsync(dband, bh + bo, bands[r].size() * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+        bo += bands[r].size();
+        dh[r] = BandDesc{n, wv[r], dband, ws[r]->coarse_inv};
+        sh[r + 1] = sh[r] + n;
+        nmax = std::max(nmax, n);
+    }
+    void* dd = nullptr;
+    MF3D_HIP_TRY(hipMalloc(&dd, sizeof(BandDesc) * count + sizeof(int) * (count + 1)));
+    keep_dev.push_back(dd);
+    MF3D_HIP_TRY(hipMemcpyAsync(dd, dh, sizeof(BandDesc) * count + sizeof(int) * (count + 1),
+                                hipMemcpyHostToDevice, s));
+    const BandDesc* D = static_cast<const BandDesc*>(dd);
+    const int* st = reinterpret_cast<const int*>(D + count);
+    return kern::band_inverse_batch(s, D, st, count, sh[count], nmax);
 }
 
 // per-type views of a level's V-cycle buffers
@@ -514,12 +593,12 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
 // operator on the finest level with its FP64 faces
 int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, double* partials) {
     Level& L0 = ws->lv[0];
-    return kern::op_k(s, L0.g, ws->bct, ws->pin, ws->K0, ws->iK, 0, x, nullptr, y, partials);
+    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 0, x, nullptr, y, partials);
 }
 int residual_op(mf3d_workspace* ws, hipStream_t s, const double* x, const double* b, double* r,
                 double* partials) {
     Level& L0 = ws->lv[0];
-    return kern::op_k(s, L0.g, ws->bct, ws->pin, ws->K0, ws->iK, 1, x, b, r, partials);
+    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 1, x, b, r, partials);
 }
 
 IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
@@ -818,8 +897,9 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
         v.bc = w->bct;
         v.pin = w->pin ? 1 : 0;
         v.inv_dx2 = 1.0 / (L0.g.dx * L0.g.dx);
-        v.K = w->K0;
-        v.iK = w->iK;
+        v.fx = (const double*)L0.f64.fx;
+        v.fy = (const double*)L0.f64.fy;
+        v.fz = (const double*)L0.f64.fz;
         v.x = h[r];
         v.r = w->r, v.z = w->z, v.p = w->p, v.Ap = w->Ap, v.rhs = w->rhs;
         v.b32 = L0.b32, v.x32 = L0.x32;
@@ -1224,11 +1304,19 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
     for (int r = 0; r < count; ++r) {
         mf3d_workspace* w = ws[r];
         MF3D_TRY(ensure_workspace(w, grids[r], cfg->mg_levels));
-        MF3D_TRY(mg_setup(w, s, grids[r], *bc, *cfg, K[r]));
+        MF3D_TRY(mg_setup(w, s, grids[r], *bc, *cfg, K[r], true));
         MF3D_TRY(kern::rhs(s, grids[r], *bc, K[r], w->rhs));
         if (cfg->init_guess != MF3D_GUESS_PROVIDED)
             MF3D_TRY(kern::guess(s, grids[r], *bc, cfg->init_guess, h[r]));
     }
+    struct Keep {
+        std::vector<void*> host, dev;
+        ~Keep() {
+            for (void* q : host) (void)hipHostFree(q);
+            for (void* q : dev) (void)hipFree(q);
+        }
+    } keep;
+    MF3D_TRY(build_coarse_inverses(ws, count, s, keep.host, keep.dev));
     BPlan P;
     MF3D_TRY(build_plan(P, ws, count, h));
     MF3D_HIP_TRY(hipEventRecord(ev[1], s));

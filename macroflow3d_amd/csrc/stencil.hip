@@ -468,10 +468,9 @@ __global__ void __launch_bounds__(256)
 // One wave per column c: solves L y = e_c then L^T z = y with the band dot
 // products spread over the 64 lanes, y kept in LDS; writes row c of -M^{-1},
 // which is column c of the symmetric A^{-1}.
-__global__ void __launch_bounds__(64)
-    band_inverse_kernel(int n, int w, const double* __restrict__ Lb, double* __restrict__ Ainv) {
+__device__ void band_inverse_body(int n, int w, const double* __restrict__ Lb,
+                                  double* __restrict__ Ainv, int c) {
     extern __shared__ double y[];
-    const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int W1 = w + 1;
     for (int i = lane; i < n; i += 64) y[i] = 0.0;
@@ -516,6 +515,20 @@ __global__ void __launch_bounds__(64)
     }
     double* out = Ainv + (long)c * n;
     for (int i = lane; i < n; i += 64) out[i] = -y[i];
+}
+
+__global__ void __launch_bounds__(64)
+    band_inverse_kernel(int n, int w, const double* __restrict__ Lb, double* __restrict__ Ainv) {
+    band_inverse_body(n, w, Lb, Ainv, blockIdx.x);
+}
+
+// many factors at once: entry e owns the blocks [start[e], start[e+1]), one per column
+__global__ void __launch_bounds__(64)
+    band_inverse_batch_kernel(const BandDesc* __restrict__ D, const int* __restrict__ start,
+                              int m) {
+    const int e = batch_entry(start, m, blockIdx.x);
+    const BandDesc& B = D[e];
+    band_inverse_body(B.n, B.w, B.Lb, B.Ainv, (int)blockIdx.x - start[e]);
 }
 
 }  // namespace
@@ -643,6 +656,13 @@ int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double
 
 int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv) {
     band_inverse_kernel<<<n, 64, (size_t)n * sizeof(double), s>>>(n, w, Lb, Ainv);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int band_inverse_batch(hipStream_t s, const BandDesc* D, const int* start, int m, int nblocks,
+                       int nmax) {
+    band_inverse_batch_kernel<<<nblocks, 64, (size_t)nmax * sizeof(double), s>>>(D, start, m);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

@@ -130,6 +130,7 @@ def _main(argv=None) -> int:
             f"{a.per_size} x {n}^3" for n in (48, 64, 96)),
         "cells": int(sum(g.num_cells() for g in grids)),
         "batched_ms": round(t_batch * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
+        "batched_setup_ms": round(res[0].setup_ms, 2), "batched_iteration_ms": round(res[0].solve_ms, 2),
         "solves_per_s_batched": round(len(reals) / t_batch, 1),
         "solves_per_s_sequential": round(len(reals) / t_seq, 1),
         "speedup": round(t_seq / t_batch, 2),
