@@ -7,6 +7,9 @@
 // the chunk length.
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -1301,6 +1304,13 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
     MF3D_HIP_TRY(hipEventRecord(ev[3], caller));
     MF3D_HIP_TRY(hipStreamWaitEvent(s, ev[3], 0));
     MF3D_HIP_TRY(hipEventRecord(ev[0], s));
+    // host-side phase times on stderr when MF3D_BATCH_TIMING is set
+    const bool timing = std::getenv("MF3D_BATCH_TIMING") != nullptr;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    const auto t0 = now();
     for (int r = 0; r < count; ++r) {
         mf3d_workspace* w = ws[r];
         MF3D_TRY(ensure_workspace(w, grids[r], cfg->mg_levels));
@@ -1316,12 +1326,22 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
             for (void* q : dev) (void)hipFree(q);
         }
     } keep;
+    const auto t1 = now();
     MF3D_TRY(build_coarse_inverses(ws, count, s, keep.host, keep.dev));
+    const auto t2 = now();
     BPlan P;
     MF3D_TRY(build_plan(P, ws, count, h));
+    const auto t3 = now();
     MF3D_HIP_TRY(hipEventRecord(ev[1], s));
     int restarts = 0;
     MF3D_TRY(run_pcg_batch(ws, count, s, *cfg, P, !(flags & MF3D_BATCH_NO_GRAPH), &restarts));
+    if (timing) {
+        MF3D_HIP_TRY(hipStreamSynchronize(s));
+        std::fprintf(stderr,
+                     "[batch] enqueue setup %.2f ms, coarse inverses %.2f ms, plan %.2f ms, "
+                     "iterations %.2f ms (host clock)\n",
+                     ms(t0, t1), ms(t1, t2), ms(t2, t3), ms(t3, now()));
+    }
     MF3D_HIP_TRY(hipEventRecord(ev[2], s));
     if (cfg->true_residual) {
         const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);

@@ -161,9 +161,27 @@ def make_samples(ctx: HipContext, n: int, params, levels: int = 5, rtol: float =
     return out
 
 
+def augment(t: torch.Tensor, n: int, code: int) -> torch.Tensor:
+    """One of the 8 symmetries of the Darcy set-up on an n^3 field (x fastest):
+    bit 0 mirrors y, bit 1 mirrors z, bit 2 swaps y and z (the x faces carry
+    the head drop, the y and z faces are alike)."""
+    if code == 0:
+        return t
+    v = t.view(n, n, n)  # (z, y, x)
+    if code & 1:
+        v = torch.flip(v, dims=[1])
+    if code & 2:
+        v = torch.flip(v, dims=[0])
+    if code & 4:
+        v = v.transpose(0, 1)
+    return v.contiguous().view(-1)
+
+
 def train(ctx: HipContext, net: WarmStartNet, samples, n: int, epochs: int, lr: float,
-          lr_final: float, log=None, seed: int = 0, time_budget: float = float("inf")):
-    """Adam over the samples (one field per step, shuffled each epoch, learning
+          lr_final: float, log=None, seed: int = 0, time_budget: float = float("inf"),
+          augment_data: bool = True):
+    """Adam over the samples (one field per step, shuffled each epoch, each
+    step under a random symmetry of the set-up when augment_data, learning
     rate decayed geometrically from lr to lr_final). Stops after the epoch in
     which time_budget seconds have passed. Returns mean sampled loss per epoch."""
     t_start = time.perf_counter()
@@ -180,8 +198,10 @@ def train(ctx: HipContext, net: WarmStartNet, samples, n: int, epochs: int, lr: 
         for q, r in enumerate(order):
             rate = lr * (lr_final / lr) ** (step / max(1, total - 1))
             want = (q % 50 == 0)
-            loss = net.train_step(ctx, samples[r][0], samples[r][1], g, bc, rate, k_is_log=True,
-                                  want_loss=want)
+            code = int(rng.integers(8)) if augment_data else 0
+            lnK = augment(samples[r][0], n, code)
+            h = augment(samples[r][1], n, code)
+            loss = net.train_step(ctx, lnK, h, g, bc, rate, k_is_log=True, want_loss=want)
             if want:
                 acc += loss
             step += 1
