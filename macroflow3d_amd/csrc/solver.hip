@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "faces.hpp"
@@ -487,13 +488,31 @@ int build_coarse_inverses(mf3d_workspace* const* ws, int count, hipStream_t s,
                                     (off[r + 1] - off[r]) * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
     MF3D_HIP_TRY(hipStreamSynchronize(s));
+    const auto t_sync = std::chrono::steady_clock::now();
     std::vector<std::vector<double>> bands(count);
     std::vector<int> wv(count);
+    std::vector<int> fst(count, MF3D_OK);
+    // independent factorizations: spread over host threads
+    const int nth = std::max(1, std::min(count, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t)
+        pool.emplace_back([&, t] {
+            for (int r = t; r < count; r += nth)
+                fst[r] = coarse_band(ws[r], kh + off[r], bands[r], &wv[r]);
+        });
+    for (auto& th : pool) th.join();
     size_t tot = 0;
     for (int r = 0; r < count; ++r) {
-        MF3D_TRY(coarse_band(ws[r], kh + off[r], bands[r], &wv[r]));
+        if (fst[r] != MF3D_OK)  // the worker's message stayed on its thread
+            return fail(fst[r], "coarsest factorization of realisation " + std::to_string(r) +
+                                   " failed (operator not definite?)");
         tot += bands[r].size();
     }
+    if (std::getenv("MF3D_BATCH_TIMING"))
+        std::fprintf(stderr, "[batch] coarse factorizations %.2f ms on %d threads\n",
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                               t_sync).count(),
+                     nth);
     double* bh = nullptr;
     MF3D_HIP_TRY(hipHostMalloc((void**)&bh, tot * sizeof(double) + sizeof(BandDesc) * count +
                                                 sizeof(int) * (count + 1),
