@@ -89,7 +89,8 @@ def _main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--per-size", type=int, default=32)
     ap.add_argument("--seed", type=int, default=4000)
-    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--levels", type=int, default=6,
+                    help="hierarchy depth (6: coarsest 3^3 or 2^3 for the uneven batch)")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true")
