@@ -10,12 +10,16 @@
 //   coarsen_K                                src/physics/flow/coarsen_K.cuh:46-47
 //   compute_velocity_from_head (x2), compute_norm2/sum/mean, check_no_nans
 //                                            src/physics/flow/velocity_from_head.cuh:54-107
+//   solve_head_batched, WarmStartNet         extensions without a counterpart
 //
 // HeadSolveConfig keeps the reference fields and defaults; the extension
 // fields after them default to the reference behaviour.
 #pragma once
 
 #include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
 
 #include "numerics.hpp"
 
@@ -237,6 +241,20 @@ struct HeadSolveResult {
     int restarts = 0;
 };
 
+inline HeadSolveResult from_c(const mf3d_head_solve_result& r) {
+    HeadSolveResult o;
+    o.num_iterations = r.num_iterations;
+    o.initial_residual = r.initial_residual;
+    o.final_residual = r.final_residual;
+    o.converged = r.converged != 0;
+    o.bnorm = r.bnorm;
+    o.true_residual = r.true_residual;
+    o.setup_ms = r.setup_ms;
+    o.solve_ms = r.solve_ms;
+    o.restarts = r.restarts;
+    return o;
+}
+
 // Solves -div(K grad h) = 0 with the boundary conditions; h is overwritten
 // by the initial guess unless cfg.init_guess is Provided. Non-convergence is
 // reported in the result, not thrown.
@@ -253,17 +271,7 @@ inline HeadSolveResult solve_head(DeviceSpan<real> h, DeviceSpan<const real> K,
     mf3d_check(mf3d_solve_head(workspace.mg_hierarchy.handle(), ctx.c_stream(), &g, &b, &c,
                                K.data(), h.data(), h.size(), &r),
                "solve_head");
-    HeadSolveResult o;
-    o.num_iterations = r.num_iterations;
-    o.initial_residual = r.initial_residual;
-    o.final_residual = r.final_residual;
-    o.converged = r.converged != 0;
-    o.bnorm = r.bnorm;
-    o.true_residual = r.true_residual;
-    o.setup_ms = r.setup_ms;
-    o.solve_ms = r.solve_ms;
-    o.restarts = r.restarts;
-    return o;
+    return from_c(r);
 }
 
 inline void init_head_guess(DeviceSpan<real> h, const Grid3D& grid, const BCSpec& bc,
@@ -346,6 +354,134 @@ inline void print_velocity_checksums(const VelocityField& vel, CudaContext& ctx)
                     compute_sum(spans[c], ctx), nrm, std::isnan(nrm) ? "yes" : "no");
     }
 }
+
+// ---- extensions without a counterpart in MacroFlow3D ---------------------------
+
+// Independent head solves of many realisations at once (the realisation loop
+// of EnsembleRunner.cu:259-311 solved together): one workspace per
+// realisation, grids of any sizes, shared BCs and configuration. Each
+// realisation gets the head and iteration count of its own solve_head call.
+// Needs PCG_MG with the FP32 V-cycle (HeadSolveConfig::mixed) and nx % 8 == 0
+// on every colour-swept level; other configurations throw (status
+// MF3D_ERR_UNSUPPORTED) and the caller loops over solve_head instead.
+inline std::vector<HeadSolveResult> solve_head_batched(
+    const std::vector<DeviceSpan<real>>& h, const std::vector<DeviceSpan<const real>>& K,
+    const std::vector<Grid3D>& grids, const BCSpec& bc, const HeadSolveConfig& cfg,
+    CudaContext& ctx, const std::vector<FlowWorkspace*>& workspaces, bool use_graph = true) {
+    bc.validate();
+    const size_t R = h.size();
+    if (K.size() != R || grids.size() != R || workspaces.size() != R)
+        throw std::runtime_error("solve_head_batched: h, K, grids and workspaces differ in count");
+    std::vector<mf3d_workspace*> ws(R);
+    std::vector<mf3d_grid> g(R);
+    std::vector<const double*> kp(R);
+    std::vector<double*> hp(R);
+    std::vector<size_t> n(R);
+    for (size_t r = 0; r < R; ++r) {
+        if (!workspaces[r]) throw std::runtime_error("solve_head_batched: null workspace");
+        if (K[r].size() < h[r].size())
+            throw std::runtime_error("solve_head_batched: K shorter than h");
+        ws[r] = workspaces[r]->mg_hierarchy.handle();
+        g[r] = to_c(grids[r]);
+        kp[r] = K[r].data();
+        hp[r] = h[r].data();
+        n[r] = h[r].size();
+    }
+    const mf3d_bc b = to_c(bc);
+    const mf3d_head_solve_config c = cfg.c();
+    std::vector<mf3d_head_solve_result> res(R);
+    mf3d_check(mf3d_solve_head_batched((int)R, ws.data(), ctx.c_stream(), g.data(), &b, &c,
+                                       kp.data(), hp.data(), n.data(),
+                                       use_graph ? 0 : MF3D_BATCH_NO_GRAPH, res.data()),
+               "solve_head_batched");
+    std::vector<HeadSolveResult> out(R);
+    for (size_t r = 0; r < R; ++r) out[r] = from_c(res[r]);
+    return out;
+}
+
+// Learned initial head for the Darcy set-up (Dirichlet x faces): a 3-D
+// convolutional encoder-decoder on the grid coarsened 4x per axis predicts
+// h0 = h_lin + J trilinear(y); solve from it with InitGuess::Provided. Grid
+// sizes must be divisible by 16. Parameters: the FP32 vector of
+// macroflow3d_amd/data/warmstart_128.npy (load_npy) or any vector of
+// param_count() values.
+class WarmStartNet {
+public:
+    WarmStartNet() { mf3d_check(mf3d_wsnet_create(&net_), "mf3d_wsnet_create"); }
+    explicit WarmStartNet(const std::string& npy_path) : WarmStartNet() { load_npy(npy_path); }
+    ~WarmStartNet() {
+        if (net_) mf3d_wsnet_destroy(net_);
+    }
+    WarmStartNet(const WarmStartNet&) = delete;
+    WarmStartNet& operator=(const WarmStartNet&) = delete;
+
+    static long param_count() { return mf3d_wsnet_param_count(); }
+
+    void set_params(const std::vector<float>& p) {
+        mf3d_check(mf3d_wsnet_set_params(net_, p.data(), (long)p.size()), "wsnet_set_params");
+    }
+    std::vector<float> params() const {
+        std::vector<float> p((size_t)param_count());
+        mf3d_check(mf3d_wsnet_get_params(net_, p.data(), (long)p.size()), "wsnet_get_params");
+        return p;
+    }
+
+    // Reads a 1-D little-endian float32 .npy file (format versions 1-3).
+    void load_npy(const std::string& path) {
+        FILE* f = std::fopen(path.c_str(), "rb");
+        if (!f) throw std::runtime_error("WarmStartNet: cannot open " + path);
+        std::vector<float> p;
+        std::string err;
+        unsigned char pre[10];
+        if (std::fread(pre, 1, 10, f) != 10 || std::string((const char*)pre + 1, 5) != "NUMPY" ||
+            pre[0] != 0x93) {
+            err = "not a .npy file";
+        } else {
+            size_t hlen = (size_t)pre[8] | ((size_t)pre[9] << 8);
+            if (pre[6] >= 2) {  // 4-byte header length
+                unsigned char ext[2];
+                if (std::fread(ext, 1, 2, f) != 2) err = "truncated header";
+                hlen |= ((size_t)ext[0] << 16) | ((size_t)ext[1] << 24);
+            }
+            std::string hdr(hlen, '\0');
+            if (err.empty() && std::fread(&hdr[0], 1, hlen, f) != hlen) err = "truncated header";
+            if (err.empty() && (hdr.find("'<f4'") == std::string::npos ||
+                                hdr.find("'fortran_order': False") == std::string::npos))
+                err = "expected a C-order float32 array";
+            const size_t sp = hdr.find("'shape': (");
+            if (err.empty() && sp == std::string::npos) err = "no shape";
+            if (err.empty()) {
+                const long count = std::strtol(hdr.c_str() + sp + 10, nullptr, 10);
+                if (count != param_count()) {
+                    err = "holds " + std::to_string(count) + " values, the network has " +
+                          std::to_string(param_count());
+                } else {
+                    p.resize((size_t)count);
+                    if (std::fread(p.data(), sizeof(float), p.size(), f) != p.size())
+                        err = "truncated data";
+                }
+            }
+        }
+        std::fclose(f);
+        if (!err.empty()) throw std::runtime_error("WarmStartNet: " + path + ": " + err);
+        set_params(p);
+    }
+
+    // h0 <- prediction from K (or ln K with k_is_log); enqueued on ctx's stream.
+    void predict(CudaContext& ctx, const Grid3D& grid, const BCSpec& bc, DeviceSpan<const real> K,
+                 DeviceSpan<real> h0, bool k_is_log = false) {
+        if (K.size() < grid.num_cells() || h0.size() < grid.num_cells())
+            throw std::runtime_error("WarmStartNet::predict: buffers shorter than the grid");
+        const mf3d_grid g = to_c(grid);
+        const mf3d_bc b = to_c(bc);
+        mf3d_check(mf3d_wsnet_predict(net_, ctx.c_stream(), &g, &b, K.data(), k_is_log ? 1 : 0,
+                                      h0.data(), h0.size()),
+                   "wsnet_predict");
+    }
+
+private:
+    mf3d_wsnet* net_ = nullptr;
+};
 
 }  // namespace physics
 }  // namespace macroflow3d

@@ -3,12 +3,15 @@
 // compute_velocity_from_head, then the checksums) and of a caller that builds
 // its own PCG from VarCoeffLaplacian + MultigridPreconditioner (pcg.cuh).
 //
-// Usage: dropin_solve <n> <out_prefix>
+// Usage: dropin_solve <n> <out_prefix> [warm_start.npy]
 // K = exp(0.8 sin(0.31 i + 0.17 j) cos(0.23 k - 0.11 i)) on an n^3 grid, h = 1
 // at x = 0, h = 0 at x = L, no-flow elsewhere. Writes <out_prefix>.K.bin,
 // .h_ref.bin (reference semantics), .h_ns.bin (direct coarsest, ||r||/||b||),
 // .h_pcg.bin (generic pcg_solve), .U.bin/.V.bin/.W.bin (velocity of h_ns),
 // .Up.bin (padded U of h_ns) as raw float64, and prints one JSON line.
+// Extensions: a mixed-precision solve, the same solve batched with a second
+// realisation (K^2) and compared bit for bit with its single solve, and, given
+// the network file, the learned initial head (.h0.bin) and a solve from it.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -105,7 +108,46 @@ int main(int argc, char** argv) {
         write_bin(out + ".h_pcg.bin", to_host(x.span(), ctx));
         write_bin(out + ".K.bin", Kh);
 
-        std::printf("{\"n\": %d, \"ref_iters\": %d, \"ref_converged\": %d, \"ref_r0\": %.17g, "
+        // extensions: mixed precision, batched, learned warm start
+        const physics::HeadSolveConfig mx = physics::HeadSolveConfig::mixed(1e-10, 3);
+        HeadField hm(grid);
+        const physics::HeadSolveResult r_mx =
+            physics::solve_head(hm.span(), K.span(), grid, bc, mx, ctx, ws);
+        std::vector<double> K2h(N);
+        for (size_t c = 0; c < N; ++c) K2h[c] = Kh[c] * Kh[c];
+        KField K2(grid);
+        hip_check(hipMemcpy(K2.device_ptr(), K2h.data(), N * sizeof(double), hipMemcpyHostToDevice),
+                  "upload K2");
+        HeadField h2(grid);
+        const physics::HeadSolveResult r_2 =
+            physics::solve_head(h2.span(), K2.span(), grid, bc, mx, ctx, ws);
+        FlowWorkspace wa, wb;
+        HeadField ha(grid), hb(grid);
+        const std::vector<physics::HeadSolveResult> rb = physics::solve_head_batched(
+            {ha.span(), hb.span()}, {K.span(), K2.span()}, {grid, grid}, bc, mx, ctx, {&wa, &wb});
+        const bool batched_equal = to_host(ha.span(), ctx) == to_host(hm.span(), ctx) &&
+                                   to_host(hb.span(), ctx) == to_host(h2.span(), ctx) &&
+                                   rb[0].num_iterations == r_mx.num_iterations &&
+                                   rb[1].num_iterations == r_2.num_iterations;
+        int ws_iters = -1, ws_converged = 0;
+        if (argc > 3) {
+            physics::WarmStartNet net(argv[3]);
+            HeadField hw(grid);
+            net.predict(ctx, grid, bc, K.span(), hw.span());
+            write_bin(out + ".h0.bin", to_host(hw.span(), ctx));
+            physics::HeadSolveConfig wc = mx;
+            wc.init_guess = physics::InitGuess::Provided;
+            const physics::HeadSolveResult r_w =
+                physics::solve_head(hw.span(), K.span(), grid, bc, wc, ctx, ws);
+            ws_iters = r_w.num_iterations;
+            ws_converged = r_w.converged && r_w.true_residual <= 1e-10 * r_w.bnorm;
+        }
+
+        std::printf("{\"mixed_iters\": %d, \"mixed_converged\": %d, \"batched_equal\": %d, "
+                    "\"warm_iters\": %d, \"warm_converged\": %d, ",
+                    r_mx.num_iterations, (int)(r_mx.converged && r_mx.true_residual <= 1e-10 * r_mx.bnorm),
+                    (int)batched_equal, ws_iters, ws_converged);
+        std::printf("\"n\": %d, \"ref_iters\": %d, \"ref_converged\": %d, \"ref_r0\": %.17g, "
                     "\"ref_rfinal\": %.17g, \"ns_iters\": %d, \"ns_converged\": %d, "
                     "\"ns_true_residual\": %.17g, \"ns_bnorm\": %.17g, \"pcg_iters\": %d, "
                     "\"pcg_converged\": %d, \"pcg_r0\": %.17g, \"U_sum\": %.17g, "

@@ -4,7 +4,8 @@ pipeline calls the head solve, checked against the CPU oracle.
 CPU: every header compiles on its own. GPU: tests/cpp/dropin_solve.cpp (built
 by macroflow3d_amd.build) runs solve_head with the reference and extension
 semantics, both velocity layouts, and the generic pcg_solve over
-VarCoeffLaplacian + MultigridPreconditioner; results are compared with the
+VarCoeffLaplacian + MultigridPreconditioner, and the extensions (mixed
+precision, solve_head_batched, WarmStartNet); results are compared with the
 oracle on the same K."""
 from __future__ import annotations
 
@@ -17,6 +18,8 @@ import pytest
 
 import oracle as O
 from macroflow3d_amd import build as B
+from macroflow3d_amd import warmstart as WS
+from oracle import cnn_oracle as CO
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = ["core.hpp", "multigrid.hpp", "numerics.hpp", "physics.hpp"]
@@ -43,8 +46,8 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     n = 32
     prefix = str(tmp_path / "run")
     print(f"dropin_solve n={n}", flush=True)
-    r = subprocess.run([B.DROPIN_BIN, str(n), prefix], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([B.DROPIN_BIN, str(n), prefix, WS.DEFAULT_WEIGHTS], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])
     rd = lambda s: np.fromfile(f"{prefix}.{s}.bin", dtype=np.float64)  # noqa: E731
@@ -72,3 +75,13 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     assert np.array_equal(Up[:n, :n, :], rd("U").reshape(n, n, n + 1))
     assert np.abs(info["U_sum"] - U.sum()) <= 1e-10 * abs(U.sum()) and info["U_finite"]
     assert info["levels"] == 3
+    # extensions: mixed precision reaches the FP64 residual bar, the batched
+    # solve reproduces the single solves bit for bit, the C++ network loader
+    # and prediction match the oracle network, and PCG converges from it
+    assert info["mixed_converged"] and info["batched_equal"]
+    p = np.load(WS.DEFAULT_WEIGHTS).astype(np.float64)
+    lnK = np.log(K).reshape(n, n, n)
+    y_o, _ = CO.forward(p, lnK, rb=True)
+    ref = CO.output(y_o, 1.0, 0.0, (n, n, n))
+    assert np.abs(rd("h0").reshape(n, n, n) - ref).max() < 2e-4 / n
+    assert info["warm_converged"] and info["warm_iters"] > 0
