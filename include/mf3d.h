@@ -215,8 +215,10 @@ enum {
     MF3D_TUNE_MARCH_MIN_CELLS_FP32 = 2,
     MF3D_TUNE_ZCHUNK = 3,               /* planes per march workgroup (<= 0: default) */
     MF3D_TUNE_SPLIT = 4,                /* 1 (default): colour-split FP32 V-cycle vectors */
-    MF3D_TUNE_SWEEP_BLOCK = 5           /* threads per workgroup of the colour-split sweeps
+    MF3D_TUNE_SWEEP_BLOCK = 5,          /* threads per workgroup of the colour-split sweeps
                                            (64..512, multiple of 64; default 256) */
+    MF3D_TUNE_MARCH_MAX_CELLS_FP64 = 6, /* ... and <= this many (default: no limit) */
+    MF3D_TUNE_MARCH_MAX_CELLS_FP32 = 7
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

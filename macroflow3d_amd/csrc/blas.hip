@@ -69,28 +69,60 @@ __global__ void __launch_bounds__(kBlock)
     if (i < n) y[i] = a * x[i];
 }
 
+// First-stage reductions: block blk owns cells [blk * kReduceCells, + kReduceCells),
+// thread t its cells blk * kReduceCells + u * kBlock + t (u = 0..7, summed in
+// that order), then the block tree; partials[reduce_blocks(n)].
+// All 8 slots are loaded before any is used (a slot past n reads the last
+// cell and is masked), so a thread keeps 8 loads per array in flight.
+constexpr int kPer = kReduceCells / kBlock;
+__device__ __forceinline__ long red_cell(long blk, int u) {
+    return blk * kReduceCells + (long)u * kBlock + threadIdx.x;
+}
+__device__ __forceinline__ long red_load(long blk, int u, long n) {
+    return min(red_cell(blk, u), n - 1);
+}
+
 __global__ void __launch_bounds__(kBlock)
     dot_partials_kernel(const double* __restrict__ x, const double* __restrict__ y, size_t n,
                         double* __restrict__ partials) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const double v = i < n ? x[i] * y[i] : 0.0;
+    double xv[kPer], yv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_load(blockIdx.x, u, (long)n);
+        xv[u] = x[i];
+        yv[u] = y[i];
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+        if (red_cell(blockIdx.x, u) < (long)n) v += xv[u] * yv[u];
     const double s = block_sum<kBlock>(v);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 __device__ void finalize_body(const double* __restrict__ partials, unsigned np, int op,
                               IterState* __restrict__ st, double* __restrict__ out) {
-    // four independent accumulators keep several loads in flight per thread
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    // The partials sit in L2, so the pass is latency-bound: 16 independent
+    // loads per thread and round (one round for 16K partials, four for 64K).
+    constexpr int U = 16;
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = 0.0;
     unsigned i = threadIdx.x;
-    for (; i + 3 * kFinalBlock < np; i += 4 * kFinalBlock) {
-        v0 += partials[i];
-        v1 += partials[i + kFinalBlock];
-        v2 += partials[i + 2 * kFinalBlock];
-        v3 += partials[i + 3 * kFinalBlock];
+    for (; i + (U - 1) * kFinalBlock < np; i += U * kFinalBlock) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] += partials[i + u * kFinalBlock];
     }
-    for (; i < np; i += kFinalBlock) v0 += partials[i];
-    const double S = block_sum<kFinalBlock>((v0 + v1) + (v2 + v3));
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u) {  // the remaining < U rounds
+        const unsigned t = i + u * kFinalBlock;
+        if (t < np) v[u] += partials[t];
+    }
+#pragma unroll
+    for (int w = U / 2; w > 0; w >>= 1)
+#pragma unroll
+        for (int u = 0; u < w; ++u) v[u] += v[u + w];
+    const double S = block_sum<kFinalBlock>(v[0]);
     if (threadIdx.x != 0) return;
     switch (op) {
     case FIN_STORE: *out = S; break;
@@ -111,6 +143,7 @@ __device__ void finalize_body(const double* __restrict__ partials, unsigned np, 
         break;
     case FIN_RR: // pcg.cuh:156-164
         if (st->stop) break;
+        st->xpend = 1;  // r is updated; x follows with the p update (or pcg_x_pending)
         st->rr = S;
         st->iter += 1;
         if (st->iter % st->check_every == 0 || st->iter >= st->max_iter) {
@@ -122,6 +155,7 @@ __device__ void finalize_body(const double* __restrict__ partials, unsigned np, 
         break;
     case FIN_RZ: // pcg.cuh:170-177 (flexible: Polak-Ribiere numerator)
         if (st->stop) break;
+        st->xpend = 0;  // the p update that follows also updates x
         st->beta = (st->flexible ? S - st->rzo : S) / st->rz;
         st->rz = S;
         break;
@@ -130,6 +164,7 @@ __device__ void finalize_body(const double* __restrict__ partials, unsigned np, 
         st->rzo = S;
         break;
     case FIN_RESUME:
+        st->xpend = 0;
         st->rr = S;
         st->stop = (sqrt(S) / st->den < st->rtol) ? 1 : 0;
         if (!st->stop && st->iter >= st->max_iter) st->stop = 3;
@@ -177,13 +212,14 @@ __global__ void __launch_bounds__(kFinalBlock)
 
 // ---- batched (one launch for many realisations, descriptors BatchVec) --------
 // second stage: one block per realisation; guard 0: always, 1: only running
-// realisations (stop == 0), 2: only converged ones (stop == 1)
+// realisations (stop == 0), 2: only converged ones (stop == 1); op_count:
+// the partials come from the operator (np_op of them)
 __global__ void __launch_bounds__(kFinalBlock)
-    finalize_batch_kernel(const BatchVec* __restrict__ Vs, int op, int guard) {
+    finalize_batch_kernel(const BatchVec* __restrict__ Vs, int op, int guard, int op_count) {
     const BatchVec& V = Vs[blockIdx.x];
     const int stop = V.st->stop;
     if ((guard == 1 && stop != 0) || (guard == 2 && stop != 1)) return;
-    finalize_body(V.partials, V.np, op, V.st, V.out);
+    finalize_body(V.partials, op_count ? V.np_op : V.np, op, V.st, V.out);
 }
 
 // MODE 0: (r, z), skipped once stopped; MODE 1: (rhs, rhs)
@@ -195,9 +231,19 @@ __global__ void __launch_bounds__(kBlock)
     if (MODE == 0 && V.st->stop) return;
     const long blk = (long)blockIdx.x - start[e];
     const long n = (long)V.nx * V.ny * V.nz;
-    const long i = blk * kBlock + threadIdx.x;
+    const double* a = MODE == 0 ? V.r : V.rhs;
+    const double* b = MODE == 0 ? V.z : V.rhs;
+    double av[kPer], bv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_load(blk, u, n);
+        av[u] = a[i];
+        bv[u] = b[i];
+    }
     double v = 0.0;
-    if (i < n) v = MODE == 0 ? V.r[i] * V.z[i] : V.rhs[i] * V.rhs[i];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+        if (red_cell(blk, u) < n) v += av[u] * bv[u];
     const double s = block_sum<kBlock>(v);
     if (threadIdx.x == 0) V.partials[blk] = s;
 }
@@ -210,29 +256,28 @@ __global__ void __launch_bounds__(kBlock)
     const double alpha = V.st->alpha;
     const long blk = (long)blockIdx.x - start[e];
     const long n = (long)V.nx * V.ny * V.nz;
-    const long i = blk * kBlock + threadIdx.x;
+    double pv[kPer], av[kPer], xv[kPer], rv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_load(blk, u, n);
+        pv[u] = V.p[i];
+        av[u] = V.Ap[i];
+        xv[u] = V.x[i];
+        rv[u] = V.r[i];
+    }
     double v = 0.0;
-    if (i < n) {
-        V.x[i] = alpha * V.p[i] + V.x[i];
-        const double ri = -alpha * V.Ap[i] + V.r[i];
-        V.r[i] = ri;
-        v = ri * ri;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_cell(blk, u);
+        if (i < n) {
+            V.x[i] = alpha * pv[u] + xv[u];
+            const double ri = -alpha * av[u] + rv[u];
+            V.r[i] = ri;
+            v += ri * ri;
+        }
     }
     const double s = block_sum<kBlock>(v);
     if (threadIdx.x == 0) V.partials[blk] = s;
-}
-
-// MODE 0: p = z + beta p; MODE 1: p = z (start of a cycle)
-template <int MODE>
-__global__ void __launch_bounds__(kBlock)
-    p_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m) {
-    const int e = batch_entry(start, m, blockIdx.x);
-    const BatchVec& V = Vs[e];
-    if (V.st->stop) return;
-    const double beta = V.st->beta;
-    const long n = (long)V.nx * V.ny * V.nz;
-    const long i = ((long)blockIdx.x - start[e]) * kBlock + threadIdx.x;
-    if (i < n) V.p[i] = MODE == 0 ? 1.0 * V.z[i] + beta * V.p[i] : V.z[i];
 }
 
 // PCG x/r update (pcg.cuh:149-153 as two axpy) fused with (r, r).
@@ -242,16 +287,66 @@ __global__ void __launch_bounds__(kBlock)
                          double* __restrict__ r, size_t n, double* __restrict__ partials) {
     if (st->stop) return;  // uniform across the grid
     const double alpha = st->alpha;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double pv[kPer], av[kPer], xv[kPer], rv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_load(blockIdx.x, u, (long)n);
+        pv[u] = p[i];
+        av[u] = Ap[i];
+        xv[u] = x[i];
+        rv[u] = r[i];
+    }
     double v = 0.0;
-    if (i < n) {
-        x[i] = alpha * p[i] + x[i];
-        const double ri = -alpha * Ap[i] + r[i];
-        r[i] = ri;
-        v = ri * ri;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_cell(blockIdx.x, u);
+        if (i < (long)n) {
+            x[i] = alpha * pv[u] + xv[u];
+            const double ri = -alpha * av[u] + rv[u];
+            r[i] = ri;
+            v += ri * ri;
+        }
     }
     const double s = block_sum<kBlock>(v);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// r -= alpha Ap with (r, r) partials: the r half of pcg_update_xr_kernel, for
+// the iteration that updates x together with p (x += alpha p; p = z + beta p).
+__global__ void __launch_bounds__(kBlock)
+    pcg_update_r_kernel(const IterState* __restrict__ st, const double* __restrict__ Ap,
+                        double* __restrict__ r, size_t n, double* __restrict__ partials) {
+    if (st->stop) return;  // uniform across the grid
+    const double alpha = st->alpha;
+    double av[kPer], rv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_load(blockIdx.x, u, (long)n);
+        av[u] = Ap[i];
+        rv[u] = r[i];
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_cell(blockIdx.x, u);
+        if (i < (long)n) {
+            const double ri = -alpha * av[u] + rv[u];
+            r[i] = ri;
+            v += ri * ri;
+        }
+    }
+    const double s = block_sum<kBlock>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// the x += alpha p still owed when the iteration stopped after its r update
+__global__ void __launch_bounds__(kBlock)
+    pcg_x_pending_kernel(const IterState* __restrict__ st, const double* __restrict__ p,
+                         double* __restrict__ x, size_t n) {
+    if (!st->xpend) return;
+    const double alpha = st->alpha;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = alpha * p[i] + x[i];
 }
 
 // PCG p = z + beta p (pcg.cuh:179, axpby(1, z, beta, p)).
@@ -272,13 +367,25 @@ __global__ void __launch_bounds__(kBlock)
                         double* __restrict__ r, size_t n, double* __restrict__ partials) {
     if (st->stop) return;
     const double alpha = st->alpha;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double pv[kPer], av[kPer], xv[kPer], rv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_load(blockIdx.x, u, (long)n);
+        pv[u] = p[i];
+        av[u] = Ap[i];
+        xv[u] = x[i];
+        rv[u] = r[i];
+    }
     double v = 0.0;
-    if (i < n) {
-        x[i] = x[i] + alpha * p[i];
-        const double ri = r[i] - alpha * (-Ap[i]);
-        r[i] = ri;
-        v = ri * ri;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_cell(blockIdx.x, u);
+        if (i < (long)n) {
+            x[i] = xv[u] + alpha * pv[u];
+            const double ri = rv[u] - alpha * (-av[u]);
+            r[i] = ri;
+            v += ri * ri;
+        }
     }
     const double s = block_sum<kBlock>(v);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -337,7 +444,7 @@ int scale_copy(hipStream_t s, double a, const double* x, double* y, size_t n) {
     return MF3D_OK;
 }
 int dot_partials(hipStream_t s, const double* x, const double* y, size_t n, double* partials) {
-    dot_partials_kernel<<<grid_for(n), kBlock, 0, s>>>(x, y, n, partials);
+    dot_partials_kernel<<<reduce_blocks(n), kBlock, 0, s>>>(x, y, n, partials);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }
@@ -349,7 +456,18 @@ int finalize(hipStream_t s, const double* partials, unsigned np, Finalize op, It
 }
 int pcg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
                   double* x, double* r, size_t n, double* partials) {
-    pcg_update_xr_kernel<<<grid_for(n), kBlock, 0, s>>>(st, p, Ap, x, r, n, partials);
+    pcg_update_xr_kernel<<<reduce_blocks(n), kBlock, 0, s>>>(st, p, Ap, x, r, n, partials);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int pcg_update_r(hipStream_t s, const IterState* st, const double* Ap, double* r, size_t n,
+                 double* partials) {
+    pcg_update_r_kernel<<<reduce_blocks(n), kBlock, 0, s>>>(st, Ap, r, n, partials);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int pcg_x_pending(hipStream_t s, const IterState* st, const double* p, double* x, size_t n) {
+    pcg_x_pending_kernel<<<grid_for(n), kBlock, 0, s>>>(st, p, x, n);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }
@@ -360,7 +478,7 @@ int pcg_update_p(hipStream_t s, const IterState* st, const double* z, double* p,
 }
 int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
                  double* x, double* r, size_t n, double* partials) {
-    cg_update_xr_kernel<<<grid_for(n), kBlock, 0, s>>>(st, p, Ap, x, r, n, partials);
+    cg_update_xr_kernel<<<reduce_blocks(n), kBlock, 0, s>>>(st, p, Ap, x, r, n, partials);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }
@@ -370,8 +488,9 @@ int cg_update_p(hipStream_t s, const IterState* st, const double* r, double* p, 
     return MF3D_OK;
 }
 
-int batch_finalize(hipStream_t s, const BatchVec* V, int m, Finalize op, int guard) {
-    finalize_batch_kernel<<<m, kFinalBlock, 0, s>>>(V, (int)op, guard);
+int batch_finalize(hipStream_t s, const BatchVec* V, int m, Finalize op, int guard,
+                   bool op_count) {
+    finalize_batch_kernel<<<m, kFinalBlock, 0, s>>>(V, (int)op, guard, op_count ? 1 : 0);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }
@@ -386,13 +505,5 @@ int batch_update_xr(hipStream_t s, const BatchVec* V, const int* start, int m, l
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }
-int batch_update_p(hipStream_t s, int mode, const BatchVec* V, const int* start, int m,
-                   long nblocks) {
-    if (mode == 0) p_batch_kernel<0><<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
-    else p_batch_kernel<1><<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
-    MF3D_LAUNCH_CHECK();
-    return MF3D_OK;
-}
-
 }  // namespace kern
 }  // namespace mf3d

@@ -384,15 +384,17 @@ __device__ __forceinline__ void lin_axis(int i, int n, int& i0, int& i1, float& 
     }
 }
 
-// h0 = h_lin + J trilinear(y) on the fine grid g (y: coarse, ych channels, channel 0)
+// h0 = h_lin + J trilinear(y) on the fine grid g (y: coarse, ych channels, channel 0).
+// Workgroup b covers segment b % segs (256 cells) of fine row b / segs, so j, k
+// and the y/z weights are uniform in a workgroup and each thread only derives i.
 __global__ void __launch_bounds__(256)
     output_kernel(Geo g, double hw, double he, const float* __restrict__ y, int ych,
                   double* __restrict__ h) {
-    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= g.n()) return;
-    const int i = (int)(v % g.nx);
-    const long q = v / g.nx;
-    const int j = (int)(q % g.ny), k = (int)(q / g.ny);
+    const int segs = (g.nx + 255) / 256;
+    const long row = (long)blockIdx.x / segs;
+    const int i = (int)((long)blockIdx.x - row * segs) * 256 + (int)threadIdx.x;
+    if (i >= g.nx) return;
+    const int j = (int)(row % g.ny), k = (int)(row / g.ny);
     const int NX = g.nx / kS, NY = g.ny / kS, NZ = g.nz / kS;
     int x0, x1, y0, y1, z0, z1;
     float wx, wy, wz;
@@ -409,7 +411,7 @@ __global__ void __launch_bounds__(256)
     const float c0 = c00 * (1.f - wy) + c10 * wy, c1 = c01 * (1.f - wy) + c11 * wy;
     const float yi = c0 * (1.f - wz) + c1 * wz;
     const double J = (hw - he) / g.nx;
-    h[v] = hw + (he - hw) * (i + 0.5) / g.nx + J * (double)yi;
+    h[row * g.nx + i] = hw + (he - hw) * (i + 0.5) / g.nx + J * (double)yi;
 }
 
 // per coarse cell V: target t = mean over its block of (h - h_lin) / J;
@@ -927,8 +929,9 @@ int mf3d_wsnet_predict(mf3d_wsnet* net, mf3d_stream stream, const mf3d_grid* gri
     (void)hipGetLastError();
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     MF3D_TRY(forward(net, s, *grid, K, k_is_log));
-    output_kernel<<<blocks((long)cells(*grid)), 256, 0, s>>>(Geo{grid->nx, grid->ny, grid->nz},
-                                                             hw, he, net->y, 16, h0);
+    const long rows = (long)grid->ny * grid->nz, segs = (grid->nx + 255) / 256;
+    output_kernel<<<(unsigned)(rows * segs), 256, 0, s>>>(Geo{grid->nx, grid->ny, grid->nz}, hw,
+                                                          he, net->y, 16, h0);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

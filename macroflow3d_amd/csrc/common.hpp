@@ -112,6 +112,7 @@ struct IterState {
     int max_iter;
     int stop_norm; // MF3D_STOP_*
     int flexible;  // beta = (r_new, z_new - z_old) / (r_old, z_old)
+    int xpend;     // x += alpha p still owed (x updated with the next p, see FIN_RR)
 };
 
 // Reduction finalisers (what thread 0 does with the reduced sum S).
@@ -133,6 +134,7 @@ enum Finalize : int {
 
 constexpr int kReduceBlock = 256;   // threads per first-stage block
 constexpr int kFinalBlock = 1024;   // threads of the second stage
+constexpr int kReduceCells = 2048;  // cells per first-stage partial (8 per thread, 256 threads)
 
 // ---------------------------------------------------------------------------
 // Batched solves: one launch covers many realisations. Entry e of a launch
@@ -178,7 +180,9 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
     float* b32;
     float* x32;
     double* partials;
-    unsigned np;
+    unsigned np;      // partials of the 256-cell reductions (reduce_blocks)
+    unsigned np_op;   // partials of the operator (op_partials)
+    int otiles, okz;  // operator geometry (op_geom)
     IterState* st;
     double* out;  // scalar result of a FIN_STORE / FIN_SQRT second stage
 };

@@ -23,6 +23,7 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "faces.hpp"
@@ -486,7 +487,10 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     if (MODE == 2) {
         // prolongation: each in-domain neighbour gains its coarse parent's value.
         // Own cells i = 8q + 2e + p have parents I = 4q + e; their y/z neighbours
-        // too (in the parent rows of j +- 1, k +- 1); x neighbours (i +- 1) >> 1.
+        // too (in the parent rows of j +- 1, k +- 1); x neighbours (i +- 1) >> 1,
+        // i.e. I - 1 .. I + 1 of the own parent row. A parent row's 4 values
+        // I = 4q .. 4q + 3 are one float4 (natural coarse) or two float2 (one
+        // per colour: places 2q, 2q + 1) of a colour-split coarse level.
         const int NXc = d.nx >> 1, NYc = d.ny >> 1, NZc = d.nz >> 1;
         const int J0 = j >> 1, K0 = k >> 1;
         const long hc = NXc >> 1, HC = hc * NYc * NZc;
@@ -494,23 +498,64 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
             if (CS) return xc[(long)((I + J + K) & 1) * HC + (I >> 1) + hc * ((long)J + (long)NYc * K)];
             return xc[(long)I + (long)NXc * ((long)J + (long)NYc * K)];
         };
+        auto row4 = [&](int J, int K, float (&o)[4]) {
+            if (CS) {
+                const long rb = 2L * q + hc * ((long)J + (long)NYc * K);
+                const long c0 = (long)((J + K) & 1) * HC, c1 = (long)((J + K + 1) & 1) * HC;
+                const float2 A = *reinterpret_cast<const float2*>(xc + c0 + rb);
+                const float2 B = *reinterpret_cast<const float2*>(xc + c1 + rb);
+                o[0] = A.x, o[1] = B.x, o[2] = A.y, o[3] = B.y;
+            } else {
+                const float4 A = *reinterpret_cast<const float4*>(
+                    xc + 4L * q + (long)NXc * ((long)J + (long)NYc * K));
+                o[0] = A.x, o[1] = A.y, o[2] = A.z, o[3] = A.w;
+            }
+        };
         const bool yp = j + 1 < d.ny || bc.t[3] == MF3D_BC_PERIODIC;
         const bool ym = j > 0 || bc.t[2] == MF3D_BC_PERIODIC;
         const bool zp = k + 1 < d.nz || bc.t[5] == MF3D_BC_PERIODIC;
         const bool zm = k > 0 || bc.t[4] == MF3D_BC_PERIODIC;
         const int Jp = j + 1 < d.ny ? (j + 1) >> 1 : 0, Jm = j > 0 ? (j - 1) >> 1 : NYc - 1;
         const int Kp = k + 1 < d.nz ? (k + 1) >> 1 : 0, Km = k > 0 ? (k - 1) >> 1 : NZc - 1;
+        float cr[4];
+        if (yp) {
+            row4(Jp, K0, cr);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int I = 4 * q + e, i = 2 * I + p;
-            if (yp) vyp[e] += cv(I, Jp, K0);
-            if (ym) vym[e] += cv(I, Jm, K0);
-            if (zp) vzp[e] += cv(I, J0, Kp);
-            if (zm) vzm[e] += cv(I, J0, Km);
-            if (i + 1 < d.nx) vxp[e] += cv((i + 1) >> 1, J0, K0);
-            else if (bc.t[1] == MF3D_BC_PERIODIC) vxp[e] += cv(0, J0, K0);
-            if (i > 0) vxm[e] += cv((i - 1) >> 1, J0, K0);
-            else if (bc.t[0] == MF3D_BC_PERIODIC) vxm[e] += cv(NXc - 1, J0, K0);
+            for (int e = 0; e < 4; ++e) vyp[e] += cr[e];
+        }
+        if (ym) {
+            row4(Jm, K0, cr);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vym[e] += cr[e];
+        }
+        if (zp) {
+            row4(J0, Kp, cr);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vzp[e] += cr[e];
+        }
+        if (zm) {
+            row4(J0, Km, cr);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vzm[e] += cr[e];
+        }
+        // own parent row, with I = 4q - 1 and I = 4q + 4 from the adjacent lanes
+        float R[4];
+        row4(J0, K0, R);
+        float rm = __shfl_up(R[3], 1, 64), rp = __shfl_down(R[0], 1, 64);
+        if (lane == 0 && !xlo) rm = cv(4 * q - 1, J0, K0);
+        if (lane == 63 && !xhi) rp = cv(4 * q + 4, J0, K0);
+        if (p == 0) {  // x+ neighbour 2I + 1: parent I; x- neighbour 2I - 1: parent I - 1
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vxp[e] += R[e];
+            vxm[1] += R[0], vxm[2] += R[1], vxm[3] += R[2];
+            if (!xlo) vxm[0] += rm;
+            else if (bc.t[0] == MF3D_BC_PERIODIC) vxm[0] += cv(NXc - 1, J0, K0);
+        } else {       // x- neighbour 2I: parent I; x+ neighbour 2I + 2: parent I + 1
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vxm[e] += R[e];
+            vxp[0] += R[1], vxp[1] += R[2], vxp[2] += R[3];
+            if (!xhi) vxp[3] += rp;
+            else if (bc.t[1] == MF3D_BC_PERIODIC) vxp[3] += cv(0, J0, K0);
         }
     }
     float res[4];
@@ -979,72 +1024,119 @@ __device__ inline double block_sum(double v) {
     return v;
 }
 
-// A x = -sum f (x_C - x_nb) / dx^2 (varcoeff_laplacian.cu:38-239); the pin's
-// diagonal counts in-domain neighbours only (varcoeff_laplacian.cu:110-234).
-__device__ __forceinline__ double apply_cell(const Dims& d, const BcTypes& bc, bool pin,
-                                             const double* x, const double* FX,
-                                             const double* FY, const double* FZ, long o, int i,
-                                             int j, int k, double inv_dx2) {
-    double f[6], v[6];
-    cell_faces(d, FX, FY, FZ, i, j, k, f);
-    neighbours(d, bc, x, o, i, j, k, v);
-    const double xC = x[o];
-    double Ax = f[1] * (xC - v[1]) + f[0] * (xC - v[0]) + f[3] * (xC - v[3]) +
-                f[2] * (xC - v[2]) + f[5] * (xC - v[5]) + f[4] * (xC - v[4]);
-    if (pin && o == 0) {
-        double aC = 0.0;
-        if (i < d.nx - 1) aC += f[0];
-        if (i > 0) aC += f[1];
-        if (j < d.ny - 1) aC += f[2];
-        if (j > 0) aC += f[3];
-        if (k < d.nz - 1) aC += f[4];
-        if (k > 0) aC += f[5];
-        Ax += aC * xC;
-    }
-    return -Ax * inv_dx2;
-}
+// A x = -sum f (x_C - x_nb) / dx^2 (varcoeff_laplacian.cu:38-239), summed
+// x-, x+, y-, y+, z-, z+; the pin's diagonal counts in-domain neighbours only
+// (varcoeff_laplacian.cu:110-234). Evaluated by opz_body below.
 
 // MODE 0: y = A x, partials of (x, y).  MODE 1: r = b - A x, partials of (r, r).
+//
+// A workgroup owns a plane tile (kBlock consecutive cells of an xy plane) and
+// marches kz planes up in z. Each thread keeps its column's x at k-1, k, k+1
+// and the z face below k in registers, so per cell it loads the next plane's
+// x and z face, the four in-plane neighbours and the four in-plane faces, and
+// its indices are advanced by adding plane strides (no per-cell division).
+// Workgroup blk covers tile blk % tiles of z-chunk blk / tiles and writes one
+// partial (its column sums in z order, then the block tree). Each cell's value
+// is that of apply_cell (same faces, same sum order).
 template <int MODE>
-__device__ __forceinline__ void op_body(const Dims& d, const BcTypes& bc, int pin, double inv_dx2,
-                                        const double* __restrict__ x,
-                                        const double* __restrict__ FX,
-                                        const double* __restrict__ FY,
-                                        const double* __restrict__ FZ,
-                                        const double* __restrict__ b, double* __restrict__ y,
-                                        double* __restrict__ partials, long blk) {
-    const long n = (long)d.nx * d.ny * d.nz;
-    const long o = blk * kBlock + threadIdx.x;
-    double prod = 0.0;
-    if (o < n) {
-        const int i = (int)(o % d.nx);
-        const long q = o / d.nx;
-        const int j = (int)(q % d.ny);
-        const int k = (int)(q / d.ny);
-        const double Ax = apply_cell(d, bc, pin != 0, x, FX, FY, FZ, o, i, j, k, inv_dx2);
-        if (MODE == 0) {
-            y[o] = Ax;
-            prod = x[o] * Ax;
-        } else {
-            const double r = -1.0 * Ax + b[o];
-            if (y) y[o] = r;
-            prod = r * r;
+__device__ __forceinline__ void opz_body(const Dims& d, const BcTypes& bc, int pin, double inv_dx2,
+                                         int tiles, int kz, const double* __restrict__ x,
+                                         const double* __restrict__ FX,
+                                         const double* __restrict__ FY,
+                                         const double* __restrict__ FZ,
+                                         const double* __restrict__ b, double* __restrict__ y,
+                                         double* __restrict__ partials, long blk) {
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int P = nx * ny;
+    const int chunk = (int)(blk / tiles);
+    const int p = (int)(blk - (long)chunk * tiles) * kBlock + (int)threadIdx.x;
+    double acc = 0.0;
+    if (p < P) {
+        const int j = p / nx;
+        const int i = p - j * nx;
+        const long sz = P;
+        const int k0 = chunk * kz;
+        const int k1 = min(nz, k0 + kz);
+        const bool xper_lo = bc.t[0] == MF3D_BC_PERIODIC, xper_hi = bc.t[1] == MF3D_BC_PERIODIC;
+        const bool yper_lo = bc.t[2] == MF3D_BC_PERIODIC, yper_hi = bc.t[3] == MF3D_BC_PERIODIC;
+        const bool zper_lo = bc.t[4] == MF3D_BC_PERIODIC, zper_hi = bc.t[5] == MF3D_BC_PERIODIC;
+        long o = (long)k0 * sz + p;
+        long ofx = (long)i + (long)(nx + 1) * ((long)j + (long)ny * k0);
+        long ofy = (long)i + (long)nx * ((long)j + (long)(ny + 1) * k0);
+        const long sfx = (long)(nx + 1) * ny, sfy = (long)nx * (ny + 1);
+        double xm = k0 > 0 ? x[o - sz] : (zper_lo ? x[o + (long)(nz - 1) * sz] : 0.0);
+        double xc = x[o];
+        double fzl = FZ[o];  // FZ(i, j, k) sits at the cell index of (i, j, k)
+        // the loads of plane k + 1 are issued before plane k is computed
+        struct Ld {
+            double xp, fzu, v0, v1, v2, v3, f0, f1, f2, f3, bo;
+        };
+        auto load = [&](int k, long o, long ofx, long ofy) {
+            Ld l;
+            l.xp = k < nz - 1 ? x[o + sz] : (zper_hi ? x[o - (long)(nz - 1) * sz] : 0.0);
+            l.fzu = FZ[o + sz];
+            l.v0 = i < nx - 1 ? x[o + 1] : (xper_hi ? x[o - (nx - 1)] : 0.0);
+            l.v1 = i > 0 ? x[o - 1] : (xper_lo ? x[o + (nx - 1)] : 0.0);
+            l.v2 = j < ny - 1 ? x[o + nx] : (yper_hi ? x[o - (long)(ny - 1) * nx] : 0.0);
+            l.v3 = j > 0 ? x[o - nx] : (yper_lo ? x[o + (long)(ny - 1) * nx] : 0.0);
+            l.f0 = FX[ofx + 1];
+            l.f1 = FX[ofx];
+            l.f2 = FY[ofy + nx];
+            l.f3 = FY[ofy];
+            l.bo = MODE == 1 ? b[o] : 0.0;
+            return l;
+        };
+        Ld cur = load(k0, o, ofx, ofy);
+        for (int k = k0; k < k1; ++k) {
+            Ld nxt = cur;
+            if (k + 1 < k1) nxt = load(k + 1, o + sz, ofx + sfx, ofy + sfy);
+            const double xp = cur.xp, fzu = cur.fzu;
+            const double v0 = cur.v0, v1 = cur.v1, v2 = cur.v2, v3 = cur.v3;
+            const double f0 = cur.f0, f1 = cur.f1, f2 = cur.f2, f3 = cur.f3;
+            double Ax = f1 * (xc - v1) + f0 * (xc - v0) + f3 * (xc - v3) + f2 * (xc - v2) +
+                        fzl * (xc - xm) + fzu * (xc - xp);
+            if (pin && o == 0) {
+                double aC = 0.0;
+                if (i < nx - 1) aC += f0;
+                if (i > 0) aC += f1;
+                if (j < ny - 1) aC += f2;
+                if (j > 0) aC += f3;
+                if (k < nz - 1) aC += fzu;
+                if (k > 0) aC += fzl;
+                Ax += aC * xc;
+            }
+            Ax = -Ax * inv_dx2;
+            if (MODE == 0) {
+                y[o] = Ax;
+                acc += xc * Ax;
+            } else {
+                const double r = -1.0 * Ax + cur.bo;
+                if (y) y[o] = r;
+                acc += r * r;
+            }
+            xm = xc;
+            xc = xp;
+            fzl = fzu;
+            o += sz;
+            ofx += sfx;
+            ofy += sfy;
+            cur = nxt;
         }
     }
-    const double s = block_sum<kBlock>(prod);
+    const double s = block_sum<kBlock>(acc);
     if (threadIdx.x == 0) partials[blk] = s;
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
-    op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, const double* __restrict__ x,
-              const double* __restrict__ FX, const double* __restrict__ FY,
-              const double* __restrict__ FZ, const double* __restrict__ b,
-              double* __restrict__ y, double* __restrict__ partials) {
-    const long n = (long)d.nx * d.ny * d.nz;
-    const long blk = xcd_block((n + kBlock - 1) / kBlock);
+    op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, int tiles, int kz, long nblocks,
+              const double* __restrict__ x, const double* __restrict__ FX,
+              const double* __restrict__ FY, const double* __restrict__ FZ,
+              const double* __restrict__ b, double* __restrict__ y,
+              double* __restrict__ partials) {
+    const long blk = xcd_block(nblocks);
     if (blk < 0) return;  // padding block: uniform exit before the block reduction
-    op_body<MODE>(d, bc, pin, inv_dx2, x, FX, FY, FZ, b, y, partials, blk);
+    opz_body<MODE>(d, bc, pin, inv_dx2, tiles, kz, x, FX, FY, FZ, b, y, partials, blk);
 }
 
 template <typename T>
@@ -1053,10 +1145,18 @@ __global__ void __launch_bounds__(kBlock) fill_t_kernel(T* __restrict__ x, long 
     if (i < n) x[i] = T(0);
 }
 
+// partials[reduce_blocks(n)] of ln K: 8 cells per thread, strided by kBlock
 __global__ void __launch_bounds__(kBlock)
     sum_log_kernel(const double* __restrict__ K, long n, double* __restrict__ partials) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const double v = i < n ? log(K[i]) : 0.0;
+    constexpr int P8 = kReduceCells / kBlock;
+    double kv[P8];
+#pragma unroll
+    for (int u = 0; u < P8; ++u)
+        kv[u] = K[min((long)blockIdx.x * kReduceCells + (long)u * kBlock + threadIdx.x, n - 1)];
+    double v = 0.0;
+#pragma unroll
+    for (int u = 0; u < P8; ++u)
+        if ((long)blockIdx.x * kReduceCells + (long)u * kBlock + threadIdx.x < n) v += log(kv[u]);
     const double s = block_sum<kBlock>(v);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -1093,22 +1193,135 @@ __device__ __forceinline__ void to_f64_split_body(const Dims& d, const float* __
                                                   const double* __restrict__ r,
                                                   double* __restrict__ partials, long blk) {
     const long n = (long)d.nx * d.ny * d.nz;
-    const long e = blk * kBlock + threadIdx.x;
     double prod = 0.0;
-    if (e < n) {
-        const int i = (int)(e % d.nx);
-        const long q = e / d.nx;
-        const int j = (int)(q % d.ny);
-        const int k = (int)(q / d.ny);
-        const double z = (double)a[split_of(d).at(d, i, j, k)];
-        o[e] = z;
-        if (DOT) prod = r[e] * z;
+#pragma unroll
+    for (int u = 0; u < kReduceCells / kBlock; ++u) {
+        const long e = blk * kReduceCells + (long)u * kBlock + threadIdx.x;
+        if (e < n) {
+            const int i = (int)(e % d.nx);
+            const long q = e / d.nx;
+            const int j = (int)(q % d.ny);
+            const int k = (int)(q / d.ny);
+            const double z = (double)a[split_of(d).at(d, i, j, k)];
+            o[e] = z;
+            if (DOT) prod += r[e] * z;
+        }
     }
     if (DOT) {
         const double s = block_sum<kBlock>(prod);
         if (threadIdx.x == 0) partials[blk] = s;
     }
 }
+// nx % 8 == 0: the FP32 colour-split x of the V-cycle against FP64 natural
+// vectors, 8 consecutive cells of a row per thread (one float4 of each colour,
+// places 4q .. 4q + 3). KIND:
+//   S8_Z     o = z                      (z = (double) x32)
+//   S8_ZDOT  o = z, (r, z) partials
+//   S8_DOT   (r, z) partials only
+//   S8_P     o = 1 z + beta o           (PCG p update, pcg.cuh:179; skipped once stopped)
+//   S8_P0    o = z                      (first p), skipped once stopped
+// The (r, z) partial of each block (2048 consecutive cells): its threads'
+// 8-cell sums, then the block tree; partials[reduce_blocks(n)].
+enum { S8_Z = 0, S8_ZDOT = 1, S8_DOT = 2, S8_P = 3, S8_P0 = 4 };
+template <int KIND>
+__device__ __forceinline__ void split8_body(const Dims& d, const float* __restrict__ a,
+                                            double* __restrict__ o, const double* __restrict__ r,
+                                            double* __restrict__ partials, double beta, long blk) {
+    constexpr bool DOT = KIND == S8_ZDOT || KIND == S8_DOT;
+    const long total = ((long)d.nx * d.ny * d.nz) >> 3;
+    const long t = blk * kBlock + threadIdx.x;
+    double prod = 0.0;
+    if (t < total) {
+        const int h8 = d.nx >> 3;
+        const long row = t / h8;
+        const int q = (int)(t - row * h8);
+        const int j = (int)(row % d.ny);
+        const int k = (int)(row / d.ny);
+        const int c0 = (j + k) & 1;  // colour of the even cells i = 8q + 2e
+        const long half = ((long)(d.nx >> 1) * d.ny) * d.nz;
+        const long pl = (long)(d.nx >> 1) * row + 4L * q;
+        const float4 A = *reinterpret_cast<const float4*>(a + (long)c0 * half + pl);
+        const float4 B = *reinterpret_cast<const float4*>(a + (long)(c0 ^ 1) * half + pl);
+        const double z[8] = {A.x, B.x, A.y, B.y, A.z, B.z, A.w, B.w};
+        const long e0 = row * d.nx + 8L * q;
+        double2* op = reinterpret_cast<double2*>(o + e0);
+        if (KIND == S8_Z || KIND == S8_ZDOT || KIND == S8_P0) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) op[e] = make_double2(z[2 * e], z[2 * e + 1]);
+        } else if (KIND == S8_P) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double2 pv = op[e];
+                op[e] = make_double2(1.0 * z[2 * e] + beta * pv.x, 1.0 * z[2 * e + 1] + beta * pv.y);
+            }
+        }
+        if (DOT) {
+            const double2* rp = reinterpret_cast<const double2*>(r + e0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double2 rv = rp[e];
+                prod += rv.x * z[2 * e];
+                prod += rv.y * z[2 * e + 1];
+            }
+        }
+    }
+    if (DOT) {  // the block's 2048 consecutive cells: partials[reduce_blocks(n)]
+        const double sb = block_sum<kBlock>(prod);
+        if (threadIdx.x == 0) partials[blk] = sb;
+    }
+}
+// x += alpha p; p = z + beta p (z = x32), 8 cells per thread as split8_body:
+// the x half of pcg_update_xr_kernel moved to the p update, so the iteration
+// reads and writes p once for both. Skipped once stopped.
+__device__ __forceinline__ void xp_split8_body(const Dims& d, const float* __restrict__ a,
+                                               double* __restrict__ x, double* __restrict__ p,
+                                               double alpha, double beta, long blk) {
+    const long total = ((long)d.nx * d.ny * d.nz) >> 3;
+    const long t = blk * kBlock + threadIdx.x;
+    if (t >= total) return;
+    const int h8 = d.nx >> 3;
+    const long row = t / h8;
+    const int q = (int)(t - row * h8);
+    const int j = (int)(row % d.ny);
+    const int k = (int)(row / d.ny);
+    const int c0 = (j + k) & 1;
+    const long half = ((long)(d.nx >> 1) * d.ny) * d.nz;
+    const long pl = (long)(d.nx >> 1) * row + 4L * q;
+    const float4 A = *reinterpret_cast<const float4*>(a + (long)c0 * half + pl);
+    const float4 B = *reinterpret_cast<const float4*>(a + (long)(c0 ^ 1) * half + pl);
+    const double z[8] = {A.x, B.x, A.y, B.y, A.z, B.z, A.w, B.w};
+    const long e0 = row * d.nx + 8L * q;
+    double2* pp = reinterpret_cast<double2*>(p + e0);
+    double2* xp = reinterpret_cast<double2*>(x + e0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const double2 pv = pp[e], xv = xp[e];
+        xp[e] = make_double2(alpha * pv.x + xv.x, alpha * pv.y + xv.y);
+        pp[e] = make_double2(1.0 * z[2 * e] + beta * pv.x, 1.0 * z[2 * e + 1] + beta * pv.y);
+    }
+}
+__global__ void __launch_bounds__(kBlock)
+    xp_split8_kernel(Dims d, const float* __restrict__ a, double* __restrict__ x,
+                     double* __restrict__ p, const IterState* __restrict__ st) {
+    if (st->stop) return;  // uniform across the grid
+    const long total = ((long)d.nx * d.ny * d.nz) >> 3;
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;
+    xp_split8_body(d, a, x, p, st->alpha, st->beta, blk);
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kBlock)
+    split8_kernel(Dims d, const float* __restrict__ a, double* __restrict__ o,
+                  const double* __restrict__ r, double* __restrict__ partials,
+                  const IterState* __restrict__ st) {
+    if ((KIND == S8_P || KIND == S8_P0) && st->stop) return;  // uniform across the grid
+    const long total = ((long)d.nx * d.ny * d.nz) >> 3;
+    const long blk = xcd_block((total + kBlock - 1) / kBlock);
+    if (blk < 0) return;  // whole padding block
+    split8_body<KIND>(d, a, o, r, partials, KIND == S8_P ? st->beta : 0.0, blk);
+}
+
 __global__ void __launch_bounds__(kBlock)
     to_f32_split_kernel(Dims d, const double* __restrict__ a, float* __restrict__ o) {
     const long blk = xcd_block(((long)d.nx * d.ny * d.nz + kBlock - 1) / kBlock);
@@ -1119,7 +1332,7 @@ template <bool DOT>
 __global__ void __launch_bounds__(kBlock)
     to_f64_split_kernel(Dims d, const float* __restrict__ a, double* __restrict__ o,
                         const double* __restrict__ r, double* __restrict__ partials) {
-    const long blk = xcd_block(((long)d.nx * d.ny * d.nz + kBlock - 1) / kBlock);
+    const long blk = xcd_block(((long)d.nx * d.ny * d.nz + kReduceCells - 1) / kReduceCells);
     if (blk < 0) return;  // whole padding block, before the block reduction
     to_f64_split_body<DOT>(d, a, o, r, partials, blk);
 }
@@ -1223,7 +1436,9 @@ __global__ void __launch_bounds__(kBlock)
     to_f32_split_body(Dims{V.nx, V.ny, V.nz}, V.r, V.b32, blk - start[e]);
 }
 
-// z = x32 with the (r, z) partials of each block
+// the (r, z) partials of each 256 cells, z = x32 not stored (p is updated
+// from x32 directly); start[] counts ceil(n / 2048) workgroups per
+// realisation (8 cells per thread)
 __global__ void __launch_bounds__(kBlock)
     tof64_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
                        long nblocks) {
@@ -1232,11 +1447,28 @@ __global__ void __launch_bounds__(kBlock)
     const int e = batch_entry(start, m, blk);
     const BatchVec& V = Vs[e];
     if (V.st->stop) return;
-    to_f64_split_body<true>(Dims{V.nx, V.ny, V.nz}, V.x32, V.z, V.r, V.partials, blk - start[e]);
+    split8_body<S8_DOT>(Dims{V.nx, V.ny, V.nz}, V.x32, nullptr, V.r, V.partials, 0.0,
+                        blk - start[e]);
+}
+
+// PCG p update from x32 (MODE 0: p = x32 + beta p; MODE 1: p = x32) for the
+// running realisations; start[] as tof64_batch_kernel's
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    p_split8_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
+                          long nblocks) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchVec& V = Vs[e];
+    if (V.st->stop) return;
+    split8_body<MODE == 0 ? S8_P : S8_P0>(Dims{V.nx, V.ny, V.nz}, V.x32, V.p, nullptr, nullptr,
+                                          MODE == 0 ? V.st->beta : 0.0, blk - start[e]);
 }
 
 // MODE 0: Ap = A p with (p, Ap) partials, skipped once stopped. MODE 1:
 // r = rhs - A x with (r, r) partials, always (the true-residual check).
+// start[] counts op_partials() workgroups per realisation.
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
     op_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
@@ -1248,11 +1480,11 @@ __global__ void __launch_bounds__(kBlock)
     if (MODE == 0 && V.st->stop) return;
     const Dims d{V.nx, V.ny, V.nz};
     if (MODE == 0)
-        op_body<0>(d, V.bc, V.pin, V.inv_dx2, V.p, V.fx, V.fy, V.fz, nullptr, V.Ap, V.partials,
-                   blk - start[e]);
+        opz_body<0>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.p, V.fx, V.fy, V.fz, nullptr,
+                    V.Ap, V.partials, blk - start[e]);
     else
-        op_body<1>(d, V.bc, V.pin, V.inv_dx2, V.x, V.fx, V.fy, V.fz, V.rhs, V.r, V.partials,
-                   blk - start[e]);
+        opz_body<1>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.x, V.fx, V.fy, V.fz, V.rhs,
+                    V.r, V.partials, blk - start[e]);
 }
 
 MarchGeom geometry(const mf3d_grid& g, int zchunk) {
@@ -1604,6 +1836,22 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
     }
 }
 
+OpGeom op_geom(const mf3d_grid& g) {
+    OpGeom o;
+    const long P = (long)g.nx * g.ny;
+    o.tiles = (int)((P + kBlock - 1) / kBlock);
+    // z-chunks of up to 16 planes, shortened while the launch has fewer than
+    // 2048 workgroups (8 per CU)
+    o.kz = (int)std::max(1L, std::min(16L, (long)o.tiles * g.nz / 2048));
+    o.chunks = (g.nz + o.kz - 1) / o.kz;
+    return o;
+}
+
+unsigned op_partials(const mf3d_grid& g) {
+    const OpGeom o = op_geom(g);
+    return (unsigned)o.tiles * (unsigned)o.chunks;
+}
+
 int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
              int mode, const double* x, const double* b, double* y, double* partials) {
     if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "operator needs FP64 faces");
@@ -1612,12 +1860,16 @@ int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, con
     const double* FX = cptr<double>(F, 0);
     const double* FY = cptr<double>(F, 1);
     const double* FZ = cptr<double>(F, 2);
+    const OpGeom o = op_geom(g);
+    const long nb = (long)o.tiles * o.chunks;
+    const unsigned grid = (unsigned)(((nb + 7) / 8) * 8);
+    (void)n;
     if (mode == 0)
-        op_kernel<0><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
-                                                    FZ, b, y, partials);
+        op_kernel<0><<<grid, kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, o.tiles, o.kz, nb,
+                                             x, FX, FY, FZ, b, y, partials);
     else
-        op_kernel<1><<<xcd_grid(n), kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, x, FX, FY,
-                                                    FZ, b, y, partials);
+        op_kernel<1><<<grid, kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, o.tiles, o.kz, nb,
+                                             x, FX, FY, FZ, b, y, partials);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -1630,7 +1882,7 @@ int fill_f32(hipStream_t s, float* x, size_t n, float v) {
 }
 
 int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials) {
-    sum_log_kernel<<<grid_for(n), kBlock, 0, s>>>(K, (long)n, partials);
+    sum_log_kernel<<<reduce_blocks(n), kBlock, 0, s>>>(K, (long)n, partials);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -1655,6 +1907,11 @@ int to_f32(hipStream_t s, const double* a, float* o, const mf3d_grid& g, int ord
     return MF3D_OK;
 }
 
+// grid of split8_kernel (8 cells per thread, padded to a multiple of 8 blocks)
+unsigned split8_grid(const mf3d_grid& g) {
+    return (unsigned)((((cells(g) / 8 + kBlock - 1) / kBlock + 7) / 8) * 8);
+}
+
 int to_f64(hipStream_t s, const float* a, double* o, const mf3d_grid& g, int order,
            const double* r, double* partials) {
     if (order != ORDER_SPLIT) {
@@ -1662,12 +1919,50 @@ int to_f64(hipStream_t s, const float* a, double* o, const mf3d_grid& g, int ord
         return r ? dot_partials(s, r, o, cells(g), partials) : MF3D_OK;
     }
     if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
+    if (!(g.nx & 7)) {
+        const unsigned gr = split8_grid(g);
+        if (r)
+            split8_kernel<S8_ZDOT><<<gr, kBlock, 0, s>>>(dims(g), a, o, r, partials, nullptr);
+        else
+            split8_kernel<S8_Z><<<gr, kBlock, 0, s>>>(dims(g), a, o, nullptr, nullptr, nullptr);
+        MF3D_HIP_TRY(hipGetLastError());
+        return MF3D_OK;
+    }
+    const unsigned gr = ((reduce_blocks(cells(g)) + 7u) / 8u) * 8u;
     if (r)
-        to_f64_split_kernel<true><<<xcd_grid(cells(g)), kBlock, 0, s>>>(dims(g), a, o, r,
-                                                                        partials);
+        to_f64_split_kernel<true><<<gr, kBlock, 0, s>>>(dims(g), a, o, r, partials);
     else
-        to_f64_split_kernel<false><<<xcd_grid(cells(g)), kBlock, 0, s>>>(dims(g), a, o, nullptr,
-                                                                         nullptr);
+        to_f64_split_kernel<false><<<gr, kBlock, 0, s>>>(dims(g), a, o, nullptr, nullptr);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int rz_split8(hipStream_t s, const float* x32, const mf3d_grid& g, const double* r,
+              double* partials) {
+    if (g.nx & 7) return fail(MF3D_ERR_UNSUPPORTED, "8-cell conversion needs nx % 8 == 0");
+    split8_kernel<S8_DOT><<<split8_grid(g), kBlock, 0, s>>>(dims(g), x32, nullptr, r, partials,
+                                                            nullptr);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int update_xp_split8(hipStream_t s, const IterState* st, const float* x32, const mf3d_grid& g,
+                     double* x, double* p) {
+    if (g.nx & 7) return fail(MF3D_ERR_UNSUPPORTED, "8-cell conversion needs nx % 8 == 0");
+    xp_split8_kernel<<<split8_grid(g), kBlock, 0, s>>>(dims(g), x32, x, p, st);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int update_p_split8(hipStream_t s, const IterState* st, const float* x32, const mf3d_grid& g,
+                    double* p, bool first) {
+    if (g.nx & 7) return fail(MF3D_ERR_UNSUPPORTED, "8-cell conversion needs nx % 8 == 0");
+    if (first)
+        split8_kernel<S8_P0><<<split8_grid(g), kBlock, 0, s>>>(dims(g), x32, p, nullptr, nullptr,
+                                                               st);
+    else
+        split8_kernel<S8_P><<<split8_grid(g), kBlock, 0, s>>>(dims(g), x32, p, nullptr, nullptr,
+                                                              st);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -1787,6 +2082,15 @@ int batch_to_f32(hipStream_t s, const BatchVec* V, const int* start, int m, long
 
 int batch_to_f64(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks) {
     tof64_batch_kernel<<<(unsigned)(((nblocks + 7) / 8) * 8), kBlock, 0, s>>>(V, start, m, nblocks);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_update_p_split8(hipStream_t s, int mode, const BatchVec* V, const int* start, int m,
+                          long nblocks) {
+    const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
+    if (mode == 0) p_split8_batch_kernel<0><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
+    else p_split8_batch_kernel<1><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

@@ -74,7 +74,13 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
                             float* bcoarse, int order = ORDER_NATURAL,
                             int corder = ORDER_NATURAL);
 // FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
-// mode 1: y = b - A x (y nullable), partials (y, y).
+// mode 1: y = b - A x (y nullable), partials (y, y). It writes op_partials(g)
+// partials: one per (plane tile of 256 cells, z-chunk of kz planes).
+struct OpGeom {
+    int tiles, kz, chunks;
+};
+OpGeom op_geom(const mf3d_grid& g);
+unsigned op_partials(const mf3d_grid& g);
 int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
              int mode, const double* x, const double* b, double* y, double* partials);
 int to_f32(hipStream_t s, const double* a, float* o, size_t n);
@@ -82,6 +88,16 @@ int to_f64(hipStream_t s, const float* a, double* o, size_t n);
 // conversions between a natural FP64 vector and an FP32 vector of `order`
 int to_f32(hipStream_t s, const double* a, float* o, const mf3d_grid& g, int order);
 // with r (nullable): also the (r, o) first-stage partials, as dot_partials
+// colour-split x32 with nx % 8 == 0: (r, x32) partials[reduce_blocks(n)] only,
+// and the PCG p update from x32 (first: p = x32; else p = x32 + beta p;
+// skipped once st->stop is set)
+int rz_split8(hipStream_t s, const float* x32, const mf3d_grid& g, const double* r,
+              double* partials);
+int update_p_split8(hipStream_t s, const IterState* st, const float* x32, const mf3d_grid& g,
+                    double* p, bool first);
+// x += alpha p; p = x32 + beta p (skipped once st->stop is set)
+int update_xp_split8(hipStream_t s, const IterState* st, const float* x32, const mf3d_grid& g,
+                     double* x, double* p);
 int to_f64(hipStream_t s, const float* a, double* o, const mf3d_grid& g, int order,
            const double* r = nullptr, double* partials = nullptr);
 int gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
@@ -116,6 +132,8 @@ int batch_gemv(hipStream_t s, const BatchLevel* L, const int* start, int m, long
 int batch_to_f32(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
 // x32 split -> z with the (r, z) partials (replaces a batch_dot mode 0)
 int batch_to_f64(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
+int batch_update_p_split8(hipStream_t s, int mode, const BatchVec* V, const int* start, int m,
+                          long nblocks);
 int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
 
 }  // namespace kern

@@ -13,11 +13,11 @@ int residual(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, con
              const double* b, const double* K, double* r);
 int residual_restrict(hipStream_t s, const mf3d_grid& fine, const BcTypes& bc, bool pin,
                       const double* x, const double* b, const double* K, double* b_coarse);
-unsigned reduce_blocks(size_t n);  // first-stage block count of the fused reductions
+// first-stage block count of the fused reductions: one partial per kReduceCells
+// consecutive cells (8 per thread of a 256-thread block)
+unsigned reduce_blocks(size_t n);
 int apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
-          const double* x, double* y, double* partials /* nullable */);
-int op_residual(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
-                const double* x, const double* b, double* r /* nullable */, double* partials);
+          const double* x, double* y);
 int restrict_avg(hipStream_t s, const mf3d_grid& coarse, const double* f, double* c);
 int prolong_add(hipStream_t s, const mf3d_grid& fine, const double* c, double* f);
 int coarsen(hipStream_t s, const mf3d_grid& coarse, const double* Kf, double* Kc);
@@ -54,6 +54,11 @@ int finalize(hipStream_t s, const double* partials, unsigned np, Finalize op, It
 int pcg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
                   double* x, double* r, size_t n, double* partials);
 // PCG: p = z + beta p
+// r -= alpha Ap with (r, r) partials (x is updated later, with p); the x +=
+// alpha p still owed after the last such update (IterState::xpend)
+int pcg_update_r(hipStream_t s, const IterState* st, const double* Ap, double* r, size_t n,
+                 double* partials);
+int pcg_x_pending(hipStream_t s, const IterState* st, const double* p, double* x, size_t n);
 int pcg_update_p(hipStream_t s, const IterState* st, const double* z, double* p, size_t n);
 // CG on -A: if !st->stop { x += alpha p; r -= alpha (-Ap) } with partials of (r, r)
 int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
@@ -67,14 +72,13 @@ int scale_copy(hipStream_t s, double a, const double* x, double* y, size_t n);
 // reduce_blocks(n) over the realisations, one partial per block) -------------
 // second stage per realisation (one block each); guard 0: all, 1: only
 // running realisations (stop == 0), 2: only converged ones (stop == 1)
-int batch_finalize(hipStream_t s, const BatchVec* V, int m, Finalize op, int guard);
+int batch_finalize(hipStream_t s, const BatchVec* V, int m, Finalize op, int guard,
+                   bool op_count = false);
 // mode 0: (r, z) partials (guarded), mode 1: (rhs, rhs) partials
 int batch_dot(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
 // x += alpha p, r -= alpha Ap with (r, r) partials (guarded)
 int batch_update_xr(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
 // mode 0: p = z + beta p, mode 1: p = z (guarded)
-int batch_update_p(hipStream_t s, int mode, const BatchVec* V, const int* start, int m,
-                   long nblocks);
 
 // ---- velocity (velocity.hip) -----------------------------------------------
 int velocity(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* h,

@@ -125,6 +125,7 @@ struct mf3d_workspace {
     bool ready = false;
     // tuning (mf3d_workspace_set_tuning / autotuner)
     long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
+    long march_max_cells[2] = {1L << 40, 1L << 40};
     int zchunk = 0;
     bool split = true;  // colour-split FP32 levels where the sizes allow
     int sweep_block = 256;  // threads per workgroup of the colour-split sweeps
@@ -245,7 +246,7 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     MF3D_TRY(ws->alloc(&ws->z, n));
     MF3D_TRY(ws->alloc(&ws->p, n));
     MF3D_TRY(ws->alloc(&ws->Ap, n));
-    ws->partials_cap = kern::reduce_blocks(n) + 64;
+    ws->partials_cap = std::max<size_t>(kern::reduce_blocks(n), kern::op_partials(g)) + 64;
     MF3D_TRY(ws->alloc(&ws->partials, ws->partials_cap));
     double* stp = nullptr;
     MF3D_TRY(ws->alloc(&stp, (sizeof(IterState) + 7) / 8));
@@ -435,10 +436,9 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
             }
             MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
         }
-        const long thr = ws->march_min_cells[mixed ? 1 : 0];
-        L.plan.kind = L.n <= 4096 ? SMOOTH_SMALL
-                      : (kern::march_ok(L.g, ws->bct) && (long)L.n >= thr) ? SMOOTH_MARCH
-                                                                           : SMOOTH_COLOUR;
+        const long lo = ws->march_min_cells[mixed ? 1 : 0], hi = ws->march_max_cells[mixed ? 1 : 0];
+        const bool march = kern::march_ok(L.g, ws->bct) && (long)L.n >= lo && (long)L.n <= hi;
+        L.plan.kind = L.n <= 4096 ? SMOOTH_SMALL : march ? SMOOTH_MARCH : SMOOTH_COLOUR;
         L.plan.zchunk = ws->zchunk;
         L.plan.block = ws->sweep_block;
         if (!mixed && L.plan.kind == SMOOTH_MARCH && l > 0 && !L.t) MF3D_TRY(ws->alloc(&L.t, L.n));
@@ -594,12 +594,24 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
                               false, (const T*)V::x(C), L.plan, fo, co);
 }
 
+// PCG may leave z = x32 unstored (the (r, z) partials and the p update read the
+// FP32 V-cycle result directly): FP32 V-cycle, colour-split finest level with
+// nx % 8 == 0, and not flexible CG (which keeps z_old).
+bool z_in_x32(const mf3d_workspace* ws, bool flexible) {
+    const Level& L0 = ws->lv[0];
+    return !flexible && ws->precision == MF3D_PRECOND_FP32 && ws->lv.size() > 1 &&
+           L0.order == ORDER_SPLIT && !(L0.g.nx & 7);
+}
+
 // MultigridPreconditioner::apply (mg_preconditioner.cu:16-45): z = V-cycle(b = r, x = 0),
 // in FP64 or with the FP32 V-cycle between two conversions.
 // rz_partials (nullable): also leave the first-stage partials of (r, z) there.
+// z == nullptr (z_in_x32 only): z stays in the finest level's x32.
 int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
                  double* rz_partials = nullptr) {
     Level& L0 = ws->lv[0];
+    if (!z && !z_in_x32(ws, false))
+        return fail(MF3D_ERR_UNSUPPORTED, "z can stay in x32 only on a colour-split FP32 level");
     if (ws->precision == MF3D_PRECOND_FP64) {
         MF3D_TRY(vcycle<double>(ws, s, 0, z, r, true));
         return rz_partials ? kern::dot_partials(s, r, z, L0.n, rz_partials) : MF3D_OK;
@@ -609,6 +621,7 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
                       ws->pre > 0 && L0.plan.kind == SMOOTH_COLOUR;
     if (!fold) MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.g, L0.order));
     MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true, fold ? r : nullptr));
+    if (!z) return rz_partials ? kern::rz_split8(s, L0.x32, L0.g, r, rz_partials) : MF3D_OK;
     return kern::to_f64(s, L0.x32, z, L0.g, L0.order, rz_partials ? r : nullptr, rz_partials);
 }
 
@@ -687,7 +700,11 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     (void)K;
     const size_t n = cells(g);
     const unsigned np = kern::reduce_blocks(n);
+    const unsigned npo = kern::op_partials(g);  // partials of the operator
     const bool flexible = cfg.flexible != 0;
+    const bool zx = z_in_x32(ws, flexible);  // z kept in the finest level's x32
+    double* z = zx ? nullptr : ws->z;
+    Level& L0 = ws->lv[0];
     // the reference's own rule (||r||/||r0||, FP64) keeps its exact semantics
     const bool judge_true =
         cfg.stop_norm == MF3D_STOP_RHS_NORM || ws->precision == MF3D_PRECOND_FP32;
@@ -696,12 +713,13 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     MF3D_TRY(kern::dot_partials(s, ws->rhs, ws->rhs, n, ws->partials));
     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_BNORM, ws->st, nullptr));
     MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials));
-    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_R0, ws->st, nullptr));
+    MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_R0, ws->st, nullptr));
     int restarts = 0;
     for (;;) {
-        MF3D_TRY(precondition(ws, s, ws->r, ws->z, ws->partials));
+        MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials));
         MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ0, ws->st, nullptr));
-        MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
+        if (zx) MF3D_TRY(kern::update_p_split8(s, ws->st, L0.x32, L0.g, ws->p, true));
+        else MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
         MF3D_TRY(read_state(ws, s));
         Chunker ch;
         if (cfg.stop_norm == MF3D_STOP_INITIAL_RESIDUAL && cfg.cg_check_every > 1)
@@ -711,23 +729,30 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             const int chunk = ch.next(*ws->host_st, target);
             for (int c = 0; c < chunk; ++c) {
                 MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials));
-                MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_PAP, ws->st, nullptr));
-                MF3D_TRY(kern::pcg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
+                MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_PAP, ws->st, nullptr));
+                if (zx)  // x is updated together with p below
+                    MF3D_TRY(kern::pcg_update_r(s, ws->st, ws->Ap, ws->r, n, ws->partials));
+                else
+                    MF3D_TRY(kern::pcg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n,
+                                                 ws->partials));
                 MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RR, ws->st, nullptr));
                 if (flexible) {  // (r_new, z_old) before z is overwritten
                     MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
                     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZO, ws->st, nullptr));
                 }
-                MF3D_TRY(precondition(ws, s, ws->r, ws->z, ws->partials));
+                MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials));
                 MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
-                MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
+                if (zx) MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
+                else MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
             }
             MF3D_TRY(read_state(ws, s));
         }
+        // the x += alpha p of the iteration that stopped after its r update
+        if (zx) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
         if (ws->host_st->stop != 1 || restarts >= 3 || !judge_true) break;
         // converged on the recursive residual: judge by the true one
         MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials));
-        MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RESUME, ws->st, nullptr));
+        MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_RESUME, ws->st, nullptr));
         MF3D_TRY(read_state(ws, s));
         if (ws->host_st->stop == 1) break;  // the true residual meets the tolerance
         ++restarts;
@@ -769,7 +794,8 @@ int run_cg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
         const int chunk = ch.next(*ws->host_st, cfg.cg_rtol * ws->host_st->den);
         for (int c = 0; c < chunk; ++c) {
             MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials));
-            MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_PAP, ws->st, nullptr));
+            MF3D_TRY(kern::finalize(s, ws->partials, kern::op_partials(g), FIN_CG_PAP, ws->st,
+                                    nullptr));
             MF3D_TRY(kern::cg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
             MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_RR, ws->st, nullptr));
             MF3D_TRY(kern::cg_update_p(s, ws->st, ws->r, ws->p, n));
@@ -839,6 +865,10 @@ struct BLaunch {
     int* start = nullptr;  // device, m + 1 block offsets
     long snblocks = 0;     // colour-split sweeps: blocks of sweep_block threads
     int* sstart = nullptr;
+    long onblocks = 0;  // FP64 operator: op_partials workgroups
+    int* ostart = nullptr;
+    long cnblocks = 0;  // FP32 -> FP64 conversion: 2048 cells per workgroup
+    int* cstart = nullptr;
 };
 
 struct BDepth {
@@ -902,7 +932,7 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
     for (int r = 0; r < count; ++r) maxl = std::max(maxl, ws[r]->lv.size());
     P.depth.resize(maxl);
     std::vector<BatchVec> vd;
-    std::vector<long> vb;
+    std::vector<long> vb, ob, cb;
     struct G {
         std::vector<BatchLevel> d;
         std::vector<long> b, sb;  // blocks of 256 threads, sweep blocks
@@ -927,10 +957,16 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
         v.b32 = L0.b32, v.x32 = L0.x32;
         v.partials = w->partials;
         v.np = kern::reduce_blocks(L0.n);
+        const kern::OpGeom og = kern::op_geom(L0.g);
+        v.np_op = kern::op_partials(L0.g);
+        v.otiles = og.tiles;
+        v.okz = og.kz;
         v.st = w->st;
         v.out = w->dscalar;
         vd.push_back(v);
         vb.push_back((long)kern::reduce_blocks(L0.n));
+        ob.push_back((long)v.np_op);
+        cb.push_back((long)((L0.n / 8 + 255) / 256));
         const size_t nl = w->lv.size();
         for (size_t l = 0; l < nl; ++l) {
             Level& L = w->lv[l];
@@ -976,6 +1012,8 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
         }
     }
     MF3D_TRY(upload_launch(P, P.vec, vd, vb));
+    MF3D_TRY(upload_starts(P, ob, &P.vec.ostart, &P.vec.onblocks));
+    MF3D_TRY(upload_starts(P, cb, &P.vec.cstart, &P.vec.cnblocks));
     for (size_t l = 0; l < maxl; ++l) {
         MF3D_TRY(upload_launch(P, P.depth[l].s4_cs, s4c[l].d, s4c[l].b, &s4c[l].sb));
         MF3D_TRY(upload_launch(P, P.depth[l].s4_cn, s4n[l].d, s4n[l].b, &s4n[l].sb));
@@ -989,6 +1027,8 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
 #define BL(L) static_cast<const BatchLevel*>((L).desc), (L).start, (L).m, (L).nblocks
 #define BS(L) static_cast<const BatchLevel*>((L).desc), (L).sstart, (L).m, (L).snblocks
 #define BV(L) static_cast<const BatchVec*>((L).desc), (L).start, (L).m, (L).nblocks
+#define BO(L) static_cast<const BatchVec*>((L).desc), (L).ostart, (L).m, (L).onblocks
+#define BC8(L) static_cast<const BatchVec*>((L).desc), (L).cstart, (L).m, (L).cnblocks
 
 // V-cycle from x = 0 at depth d for every realisation that has that level
 int vcycle_batch(const BPlan& P, hipStream_t s, size_t d) {
@@ -1030,20 +1070,20 @@ int vcycle_batch(const BPlan& P, hipStream_t s, size_t d) {
 // the finest levels' first half-sweep reads r (FP64) and fills b32
 int precondition_batch(const BPlan& P, hipStream_t s) {
     MF3D_TRY(vcycle_batch(P, s, 0));
-    return kern::batch_to_f64(s, BV(P.vec));
+    return kern::batch_to_f64(s, BC8(P.vec));
 }
 
 // one PCG iteration for every running realisation (pcg.cuh:139-180)
 int pcg_iteration_batch(const BPlan& P, hipStream_t s) {
     const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
     const int m = P.vec.m;
-    MF3D_TRY(kern::batch_op(s, 0, BV(P.vec)));
-    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_PAP, 1));
+    MF3D_TRY(kern::batch_op(s, 0, BO(P.vec)));
+    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_PAP, 1, true));
     MF3D_TRY(kern::batch_update_xr(s, BV(P.vec)));
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RR, 1));
     MF3D_TRY(precondition_batch(P, s));  // also the (r, z) partials
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RZ, 1));
-    return kern::batch_update_p(s, 0, BV(P.vec));
+    return kern::batch_update_p_split8(s, 0, BC8(P.vec));
 }
 
 int read_states(mf3d_workspace* const* ws, int count, hipStream_t s) {
@@ -1095,8 +1135,8 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
     MF3D_HIP_TRY(hipStreamSynchronize(s));
     MF3D_TRY(kern::batch_dot(s, 1, BV(P.vec)));
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_BNORM, 0));
-    MF3D_TRY(kern::batch_op(s, 1, BV(P.vec)));
-    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_R0, 0));
+    MF3D_TRY(kern::batch_op(s, 1, BO(P.vec)));
+    MF3D_TRY(kern::batch_finalize(s, V, m, FIN_R0, 0, true));
     GraphHolder G;
     if (use_graph) MF3D_TRY(capture_iteration(P, s, G));
     int restarts = 0;
@@ -1104,7 +1144,7 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
         // start of a cycle for the running realisations: z = M r, rz, p = z
         MF3D_TRY(precondition_batch(P, s));  // also the (r, z) partials
         MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RZ0, 1));
-        MF3D_TRY(kern::batch_update_p(s, 1, BV(P.vec)));
+        MF3D_TRY(kern::batch_update_p_split8(s, 1, BC8(P.vec)));
         MF3D_TRY(read_states(ws, count, s));
         for (;;) {
             bool running = false;
@@ -1117,8 +1157,8 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
             MF3D_TRY(read_states(ws, count, s));
         }
         // converged on the recursive residual: judge by the true one
-        MF3D_TRY(kern::batch_op(s, 1, BV(P.vec)));
-        MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RESUME, 2));
+        MF3D_TRY(kern::batch_op(s, 1, BO(P.vec)));
+        MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RESUME, 2, true));
         MF3D_TRY(read_states(ws, count, s));
         bool again = false;
         for (int r = 0; r < count; ++r) again = again || !ws[r]->host_st->stop;
@@ -1262,7 +1302,7 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     MF3D_HIP_TRY(hipEventRecord(ws->ev[2], s));
     if (cfg->true_residual) {
         MF3D_TRY(residual_op(ws, s, h, ws->rhs, nullptr, ws->partials));
-        MF3D_TRY(kern::finalize(s, ws->partials, kern::reduce_blocks(nc), FIN_SQRT, nullptr,
+        MF3D_TRY(kern::finalize(s, ws->partials, kern::op_partials(*grid), FIN_SQRT, nullptr,
                                 ws->dscalar));
         MF3D_HIP_TRY(hipMemcpyAsync(&result->true_residual, ws->dscalar, sizeof(double),
                                     hipMemcpyDeviceToHost, s));
@@ -1364,8 +1404,8 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
     MF3D_HIP_TRY(hipEventRecord(ev[2], s));
     if (cfg->true_residual) {
         const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
-        MF3D_TRY(kern::batch_op(s, 1, V, P.vec.start, P.vec.m, P.vec.nblocks));
-        MF3D_TRY(kern::batch_finalize(s, V, P.vec.m, FIN_SQRT, 0));
+        MF3D_TRY(kern::batch_op(s, 1, V, P.vec.ostart, P.vec.m, P.vec.onblocks));
+        MF3D_TRY(kern::batch_finalize(s, V, P.vec.m, FIN_SQRT, 0, true));
         for (int r = 0; r < count; ++r)
             MF3D_HIP_TRY(hipMemcpyAsync(&results[r].true_residual, ws[r]->dscalar, sizeof(double),
                                         hipMemcpyDeviceToHost, s));
@@ -1546,6 +1586,8 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     switch (key) {
     case MF3D_TUNE_MARCH_MIN_CELLS_FP64: ws->march_min_cells[0] = value; break;
     case MF3D_TUNE_MARCH_MIN_CELLS_FP32: ws->march_min_cells[1] = value; break;
+    case MF3D_TUNE_MARCH_MAX_CELLS_FP64: ws->march_max_cells[0] = value; break;
+    case MF3D_TUNE_MARCH_MAX_CELLS_FP32: ws->march_max_cells[1] = value; break;
     case MF3D_TUNE_ZCHUNK: ws->zchunk = (int)value; break;
     case MF3D_TUNE_SPLIT: ws->split = value != 0; break;
     case MF3D_TUNE_SWEEP_BLOCK:
@@ -1604,7 +1646,7 @@ int mf3d_varcoeff_apply(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc
     MF3D_CHECK_PTR(y);
     MF3D_CHECK_LEN(n, cells(*grid), "x/y/K");
     clear_stale_error();
-    return kern::apply(as_stream(stream), *grid, bc_types(*bc), pin != 0, K, x, y, nullptr);
+    return kern::apply(as_stream(stream), *grid, bc_types(*bc), pin != 0, K, x, y);
 }
 
 int mf3d_dot(mf3d_workspace* ws, mf3d_stream stream, const double* x, const double* y, size_t n,

@@ -239,52 +239,20 @@ __device__ inline double block_sum(double v) {
     return v;
 }
 
-// y = A x; when partials != nullptr also writes this block's share of (x, y).
-template <bool DOT>
+// y = A x
 __global__ void __launch_bounds__(kBlock)
     apply_kernel(int nx, int ny, int nz, double inv_dx2, BcTypes bc, int pin,
                  const double* __restrict__ K, const double* __restrict__ x,
-                 double* __restrict__ y, double* __restrict__ partials) {
+                 double* __restrict__ y) {
     const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)nx * ny * nz;
-    double prod = 0.0;
     if (o < n) {
         const int i = (int)(o % nx);
         const long q = o / nx;
         const int j = (int)(q % ny);
         const int k = (int)(q / ny);
-        const double v = apply_value(nx, ny, nz, bc, pin != 0, K, x, i, j, k, o, inv_dx2);
-        y[o] = v;
-        if (DOT) prod = x[o] * v;
+        y[o] = apply_value(nx, ny, nz, bc, pin != 0, K, x, i, j, k, o, inv_dx2);
     }
-    if (DOT) {
-        const double s = block_sum(prod);
-        if (threadIdx.x == 0) partials[blockIdx.x] = s;
-    }
-}
-
-// r = b - A x via the operator (PCG initial residual and true residual,
-// pcg.cuh:113-116), with this block's share of (r, r).
-__global__ void __launch_bounds__(kBlock)
-    op_residual_kernel(int nx, int ny, int nz, double inv_dx2, BcTypes bc, int pin,
-                       const double* __restrict__ K, const double* __restrict__ x,
-                       const double* __restrict__ b, double* __restrict__ r,
-                       double* __restrict__ partials) {
-    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n = (long)nx * ny * nz;
-    double prod = 0.0;
-    if (o < n) {
-        const int i = (int)(o % nx);
-        const long q = o / nx;
-        const int j = (int)(q % ny);
-        const int k = (int)(q / ny);
-        const double Ax = apply_value(nx, ny, nz, bc, pin != 0, K, x, i, j, k, o, inv_dx2);
-        const double v = -1.0 * Ax + b[o];
-        if (r) r[o] = v;
-        prod = v * v;
-    }
-    const double s = block_sum(prod);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 // restrict_kernel (src/multigrid/transfer/restrict_3d.cu:12-53)
@@ -583,27 +551,13 @@ int residual_restrict(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool
     return MF3D_OK;
 }
 
-unsigned reduce_blocks(size_t n) { return grid_for(n); }
+unsigned reduce_blocks(size_t n) { return (unsigned)((n + kReduceCells - 1) / kReduceCells); }
 
 int apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
-          const double* x, double* y, double* partials) {
+          const double* x, double* y) {
     const size_t n = cells(g);
-    const double inv_dx2 = 1.0 / (g.dx * g.dx);
-    if (partials)
-        apply_kernel<true><<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, inv_dx2, bc, pin, K,
-                                                          x, y, partials);
-    else
-        apply_kernel<false><<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, inv_dx2, bc, pin,
-                                                           K, x, y, nullptr);
-    MF3D_HIP_TRY(hipGetLastError());
-    return MF3D_OK;
-}
-
-int op_residual(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* K,
-                const double* x, const double* b, double* r, double* partials) {
-    const size_t n = cells(g);
-    op_residual_kernel<<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, 1.0 / (g.dx * g.dx), bc,
-                                                      pin, K, x, b, r, partials);
+    apply_kernel<<<grid_for(n), kBlock, 0, s>>>(g.nx, g.ny, g.nz, 1.0 / (g.dx * g.dx), bc, pin, K,
+                                                x, y);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

@@ -182,6 +182,8 @@ class FlowWorkspace:
     TUNE_ZCHUNK = 3
     TUNE_SPLIT = 4
     TUNE_SWEEP_BLOCK = 5
+    TUNE_MARCH_MAX_CELLS_FP64 = 6
+    TUNE_MARCH_MAX_CELLS_FP32 = 7
 
     def set_tuning(self, key: int, value: int) -> None:
         """Smoother selection knobs (levels with >= value cells use the one-pass
