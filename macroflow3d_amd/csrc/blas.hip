@@ -248,21 +248,21 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) V.partials[blk] = s;
 }
 
+// r -= alpha Ap with (r, r) partials for the running realisations (x is
+// updated with p, xp_split8_batch_kernel)
 __global__ void __launch_bounds__(kBlock)
-    xr_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m) {
+    r_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m) {
     const int e = batch_entry(start, m, blockIdx.x);
     const BatchVec& V = Vs[e];
     if (V.st->stop) return;
     const double alpha = V.st->alpha;
     const long blk = (long)blockIdx.x - start[e];
     const long n = (long)V.nx * V.ny * V.nz;
-    double pv[kPer], av[kPer], xv[kPer], rv[kPer];
+    double av[kPer], rv[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const long i = red_load(blk, u, n);
-        pv[u] = V.p[i];
         av[u] = V.Ap[i];
-        xv[u] = V.x[i];
         rv[u] = V.r[i];
     }
     double v = 0.0;
@@ -270,7 +270,6 @@ __global__ void __launch_bounds__(kBlock)
     for (int u = 0; u < kPer; ++u) {
         const long i = red_cell(blk, u);
         if (i < n) {
-            V.x[i] = alpha * pv[u] + xv[u];
             const double ri = -alpha * av[u] + rv[u];
             V.r[i] = ri;
             v += ri * ri;
@@ -278,6 +277,26 @@ __global__ void __launch_bounds__(kBlock)
     }
     const double s = block_sum<kBlock>(v);
     if (threadIdx.x == 0) V.partials[blk] = s;
+}
+
+// the x += alpha p still owed by realisations that stopped after their r update
+__global__ void __launch_bounds__(kBlock)
+    x_pending_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m) {
+    const int e = batch_entry(start, m, blockIdx.x);
+    const BatchVec& V = Vs[e];
+    if (!V.st->xpend) return;
+    const double alpha = V.st->alpha;
+    const long blk = (long)blockIdx.x - start[e];
+    const long n = (long)V.nx * V.ny * V.nz;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const long i = red_cell(blk, u);
+        if (i < n) V.x[i] = alpha * V.p[i] + V.x[i];
+    }
+}
+__global__ void xpend_clear_kernel(const BatchVec* __restrict__ Vs, int m) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < m) Vs[e].st->xpend = 0;
 }
 
 // PCG x/r update (pcg.cuh:149-153 as two axpy) fused with (r, r).
@@ -500,8 +519,15 @@ int batch_dot(hipStream_t s, int mode, const BatchVec* V, const int* start, int 
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }
-int batch_update_xr(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks) {
-    xr_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
+int batch_update_r(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks) {
+    r_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int batch_x_pending(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks) {
+    x_pending_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(V, start, m);
+    MF3D_LAUNCH_CHECK();
+    xpend_clear_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(V, m);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }

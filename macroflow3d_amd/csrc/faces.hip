@@ -943,12 +943,16 @@ __global__ void __launch_bounds__(kBlock)
     rr_split_body<TC, CS>(d, bc, pin, rs, x, b, F, bcoarse, blk);
 }
 
-// face coefficients of a colour-split level from K (values as faces_build_kernel)
-template <typename TC>
+// face coefficients of a colour-split level from K (values as faces_build_kernel);
+// with F64, also the unscaled FP64 faces in the natural MAC layout (FX, FY, FZ)
+// from the same harmonic means (the finest level of the mixed-precision solve)
+template <typename TC, bool F64>
 __global__ void __launch_bounds__(kBlock)
     faces_build_split_kernel(Dims d, BcTypes bc, double cs, const double* __restrict__ K,
                              TC* __restrict__ LX, TC* __restrict__ LY, TC* __restrict__ LZ,
-                             TC* __restrict__ UX, TC* __restrict__ UY, TC* __restrict__ UZ) {
+                             TC* __restrict__ UX, TC* __restrict__ UY, TC* __restrict__ UZ,
+                             double* __restrict__ FX, double* __restrict__ FY,
+                             double* __restrict__ FZ) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const long n = (long)nx * ny * nz;
     const long blk = xcd_block((n + kBlock - 1) / kBlock);
@@ -967,27 +971,33 @@ __global__ void __launch_bounds__(kBlock)
     if (i > 0) f = 2.0 / (iKc + 1.0 / K[o - 1]);
     else f = bc.t[0] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + nx - 1]) : bface(bc.t[0], Kc);
     cstore(LX, so, f * cs);
+    if (F64) FX[d.fx(i, j, k)] = f;
     if (j > 0) f = 2.0 / (iKc + 1.0 / K[o - sy]);
     else f = bc.t[2] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(ny - 1) * sy])
                                          : bface(bc.t[2], Kc);
     cstore(LY, so, f * cs);
+    if (F64) FY[d.fy(i, j, k)] = f;
     if (k > 0) f = 2.0 / (iKc + 1.0 / K[o - sz]);
     else f = bc.t[4] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(nz - 1) * sz])
                                          : bface(bc.t[4], Kc);
     cstore(LZ, so, f * cs);
+    if (F64) FZ[o] = f;
     if (i == nx - 1) {
         f = bc.t[1] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (nx - 1)]) : bface(bc.t[1], Kc);
         cstore(UX, j + (long)ny * k, f * cs);
+        if (F64) FX[d.fx(nx, j, k)] = f;
     }
     if (j == ny - 1) {
         f = bc.t[3] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(ny - 1) * sy])
                                         : bface(bc.t[3], Kc);
         cstore(UY, i + (long)nx * k, f * cs);
+        if (F64) FY[d.fy(i, ny, k)] = f;
     }
     if (k == nz - 1) {
         f = bc.t[5] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(nz - 1) * sz])
                                         : bface(bc.t[5], Kc);
         cstore(UZ, i + (long)nx * j, f * cs);
+        if (F64) FZ[o + sz] = f;
     }
 }
 
@@ -1466,6 +1476,20 @@ __global__ void __launch_bounds__(kBlock)
                                           MODE == 0 ? V.st->beta : 0.0, blk - start[e]);
 }
 
+// x += alpha p; p = x32 + beta p for the running realisations; start[] as
+// tof64_batch_kernel's
+__global__ void __launch_bounds__(kBlock)
+    xp_split8_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
+                           long nblocks) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;
+    const int e = batch_entry(start, m, blk);
+    const BatchVec& V = Vs[e];
+    if (V.st->stop) return;
+    xp_split8_body(Dims{V.nx, V.ny, V.nz}, V.x32, V.x, V.p, V.st->alpha, V.st->beta,
+                   blk - start[e]);
+}
+
 // MODE 0: Ap = A p with (p, Ap) partials, skipped once stopped. MODE 1:
 // r = rhs - A x with (r, r) partials, always (the true-residual check).
 // start[] counts op_partials() workgroups per realisation.
@@ -1524,10 +1548,15 @@ size_t coef_bytes(CoefType t) {
 namespace {
 template <typename TC>
 void launch_faces_split(hipStream_t s, const Dims& d, const BcTypes& bc, double cs,
-                        const double* K, FaceSet& f, unsigned gr) {
-    faces_build_split_kernel<TC><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (TC*)f.fx, (TC*)f.fy,
-                                                      (TC*)f.fz, (TC*)f.ux, (TC*)f.uy,
-                                                      (TC*)f.uz);
+                        const double* K, FaceSet& f, unsigned gr, FaceSet* f64 = nullptr) {
+    if (f64)
+        faces_build_split_kernel<TC, true><<<gr, kBlock, 0, s>>>(
+            d, bc, cs, K, (TC*)f.fx, (TC*)f.fy, (TC*)f.fz, (TC*)f.ux, (TC*)f.uy, (TC*)f.uz,
+            (double*)f64->fx, (double*)f64->fy, (double*)f64->fz);
+    else
+        faces_build_split_kernel<TC, false><<<gr, kBlock, 0, s>>>(
+            d, bc, cs, K, (TC*)f.fx, (TC*)f.fy, (TC*)f.fz, (TC*)f.ux, (TC*)f.uy, (TC*)f.uz,
+            nullptr, nullptr, nullptr);
 }
 }  // namespace
 
@@ -1563,6 +1592,22 @@ int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
         faces_build_kernel<__half><<<gr, kBlock, 0, s>>>(d, bc, cs, K, (__half*)f.fx,
                                                          (__half*)f.fy, (__half*)f.fz);
         break;
+    }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int faces_build_pair(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
+                     const double* K, FaceSet& flo, FaceSet& f64) {
+    if (!flo.split || f64.split || f64.type != COEF_F64 || !split_ok(g))
+        return fail(MF3D_ERR_UNSUPPORTED, "paired faces: split low-precision + natural FP64");
+    const Dims d = dims(g);
+    const unsigned gr = xcd_grid(cells(g));
+    switch (flo.type) {
+    case COEF_F32: launch_faces_split<float>(s, d, bc, cs, K, flo, gr, &f64); break;
+    case COEF_BF16: launch_faces_split<__hip_bfloat16>(s, d, bc, cs, K, flo, gr, &f64); break;
+    case COEF_F16: launch_faces_split<__half>(s, d, bc, cs, K, flo, gr, &f64); break;
+    default: return fail(MF3D_ERR_UNSUPPORTED, "split faces are 32- or 16-bit");
     }
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
@@ -2091,6 +2136,14 @@ int batch_update_p_split8(hipStream_t s, int mode, const BatchVec* V, const int*
     const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
     if (mode == 0) p_split8_batch_kernel<0><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
     else p_split8_batch_kernel<1><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int batch_update_xp_split8(hipStream_t s, const BatchVec* V, const int* start, int m,
+                           long nblocks) {
+    xp_split8_batch_kernel<<<(unsigned)(((nblocks + 7) / 8) * 8), kBlock, 0, s>>>(V, start, m,
+                                                                                 nblocks);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

@@ -44,6 +44,10 @@ size_t coef_bytes(CoefType t);
 // faces from K (cs: storage scale)
 int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs, const double* K,
                 FaceSet& f);
+// one pass for both face sets of the finest mixed-precision level: flo
+// colour-split (scaled by cs), f64 natural FP64 (unscaled)
+int faces_build_pair(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
+                     const double* K, FaceSet& flo, FaceSet& f64);
 bool march_ok(const mf3d_grid& g, const BcTypes& bc);
 int march_default_zchunk(const mf3d_grid& g);
 // iters GSRB iterations (red then black) with face coefficients; result in x.
@@ -134,6 +138,8 @@ int batch_to_f32(hipStream_t s, const BatchVec* V, const int* start, int m, long
 int batch_to_f64(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
 int batch_update_p_split8(hipStream_t s, int mode, const BatchVec* V, const int* start, int m,
                           long nblocks);
+int batch_update_xp_split8(hipStream_t s, const BatchVec* V, const int* start, int m,
+                           long nblocks);
 int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
 
 }  // namespace kern

@@ -25,9 +25,10 @@ int rhs(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* K, d
 int guess(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, int mode, double* h);
 int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
 int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv);
-// band_inverse for m factors in one launch (nblocks = sum of n, nmax = max n)
+// band_inverse for m factors in one launch (nblocks = sum of n, nmax = max n,
+// wmax = max bandwidth)
 int band_inverse_batch(hipStream_t s, const BandDesc* D, const int* start, int m, int nblocks,
-                       int nmax);
+                       int nmax, int wmax);
 
 // ---- fast fine-level kernels (stencil_fast.hip) ----------------------------
 // True when the z-marching kernels cover this level (size and BC limits).
@@ -77,7 +78,10 @@ int batch_finalize(hipStream_t s, const BatchVec* V, int m, Finalize op, int gua
 // mode 0: (r, z) partials (guarded), mode 1: (rhs, rhs) partials
 int batch_dot(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
 // x += alpha p, r -= alpha Ap with (r, r) partials (guarded)
-int batch_update_xr(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
+// r -= alpha Ap with (r, r) partials; the owed x += alpha p of stopped
+// realisations (IterState::xpend, cleared afterwards)
+int batch_update_r(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
+int batch_x_pending(hipStream_t s, const BatchVec* V, const int* start, int m, long nblocks);
 // mode 0: p = z + beta p, mode 1: p = z (guarded)
 
 // ---- velocity (velocity.hip) -----------------------------------------------

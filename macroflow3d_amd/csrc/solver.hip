@@ -104,6 +104,22 @@ struct mf3d_workspace {
     double* coarse_inv = nullptr;  // n*n inverse followed by the n*(w+1) band factor
     int coarse_n = 0;
     size_t coarse_cap = 0;         // elements allocated at coarse_inv
+    // the coarsest K travels to the host when the set-up is enqueued; the
+    // factorization waits for it only before the first coarsest solve
+    bool coarse_pending = false;
+    double* host_kc = nullptr;     // pinned coarsest K
+    double* host_band = nullptr;   // pinned band factor (source of the upload)
+    size_t host_kc_cap = 0, host_band_cap = 0;
+    hipEvent_t kc_ev = nullptr;
+    hipEvent_t inv_ev = nullptr;   // the inverse is complete (recorded on coarse_stream)
+    hipStream_t coarse_stream = nullptr;
+    std::thread coarse_worker;     // factorizes as soon as the coarsest K has arrived
+    std::vector<double> coarse_band_h;
+    int coarse_w = 0, coarse_status = 0;
+    std::string coarse_msg;
+    void join_coarse() {
+        if (coarse_worker.joinable()) coarse_worker.join();
+    }
     // Krylov vectors (PCGWorkspace, src/numerics/solvers/pcg.cuh:45-62) + rhs
     size_t nvec = 0;
     double *rhs = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
@@ -167,11 +183,13 @@ struct mf3d_workspace {
         return MF3D_OK;
     }
     void release() {
+        join_coarse();
         for (void* q : owned) (void)hipFree(q);
         owned.clear();
         bytes = 0;
         lv.clear();
         coarse_inv = nullptr;
+        coarse_pending = false;
         coarse_n = 0;
         coarse_cap = 0;
         nvec = 0;
@@ -185,9 +203,15 @@ struct mf3d_workspace {
         fine = mf3d_grid{};
     }
     ~mf3d_workspace() {
+        join_coarse();
         release();
         if (host_st) (void)hipHostFree(host_st);
         if (host_scalar) (void)hipHostFree(host_scalar);
+        if (host_kc) (void)hipHostFree(host_kc);
+        if (host_band) (void)hipHostFree(host_band);
+        if (kc_ev) (void)hipEventDestroy(kc_ev);
+        if (inv_ev) (void)hipEventDestroy(inv_ev);
+        if (coarse_stream) (void)hipStreamDestroy(coarse_stream);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -288,6 +312,11 @@ int host_scalar_sync(mf3d_workspace* ws, hipStream_t s, double* out) {
 // ---------------------------------------------------------------------------
 // Banded Cholesky factor of M = -A (coarsest level) from its host K: returns
 // the band width w and fills band (row-major, band[i*(w+1) + (i-j)] = L[i][j]).
+// The factorization runs right-looking on column storage (C[j*(w+1) + (i-j)] =
+// M[i][j], i >= j): column j is scaled, then its outer product is subtracted
+// from the next w columns with unit-stride updates. Every entry receives its
+// terms in the same order as the row-wise formula L[i][j] = (M[i][j] -
+// sum_p L[i][p] L[j][p]) / L[j][j], p ascending.
 int coarse_band(const mf3d_workspace* ws, const double* Kc, std::vector<double>& band, int* wout) {
     const Level& L = ws->lv.back();
     const int n = (int)L.n;
@@ -295,56 +324,62 @@ int coarse_band(const mf3d_workspace* ws, const double* Kc, std::vector<double>&
     const double dx2 = g.dx * g.dx;
     // band width from the neighbour offsets actually present
     int w = 0;
-    std::vector<std::vector<std::pair<int, double>>> rows(n);
-    std::vector<double> diag(n, 0.0);
     for (int k = 0; k < g.nz; ++k)
         for (int j = 0; j < g.ny; ++j)
             for (int i = 0; i < g.nx; ++i) {
                 const int c = i + g.nx * (j + g.ny * k);
-                double asum = 0.0;
                 for (int d = 0; d < 6; ++d) {
                     double coef;
                     const long nb = neighbour_coef(g.nx, g.ny, g.nz, ws->bct, Kc, i, j, k, d, Kc[c],
                                                    coef);
-                    asum += coef;
-                    if (nb >= 0) {
-                        rows[c].push_back({(int)nb, -coef / dx2});
-                        w = std::max(w, std::abs((int)nb - c));
-                    }
+                    if (nb >= 0) w = std::max(w, std::abs((int)nb - c));
                 }
-                if (ws->pin && c == 0) asum *= 2.0;
-                diag[c] = asum / dx2;
             }
     const int W1 = w + 1;
-    band.assign((size_t)n * W1, 0.0);  // band[i*W1 + (i-j)] = M[i][j], j <= i
-    for (int c = 0; c < n; ++c) {
-        band[(size_t)c * W1] += diag[c];
-        for (auto& e : rows[c]) {
-            const int j = e.first;
-            if (j < c) band[(size_t)c * W1 + (c - j)] += 0.5 * e.second;
-            if (j > c) band[(size_t)j * W1 + (j - c)] += 0.5 * e.second;
-            if (j == c) band[(size_t)c * W1] += e.second;  // self-wrap (dimension 1)
-        }
-    }
-    // banded Cholesky in place
+    std::vector<double> C((size_t)n * W1, 0.0);
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+                const int c = i + g.nx * (j + g.ny * k);
+                double coef[6], asum = 0.0;
+                long nb[6];
+                for (int d = 0; d < 6; ++d) {
+                    nb[d] = neighbour_coef(g.nx, g.ny, g.nz, ws->bct, Kc, i, j, k, d, Kc[c],
+                                           coef[d]);
+                    asum += coef[d];
+                }
+                if (ws->pin && c == 0) asum *= 2.0;
+                C[(size_t)c * W1] += asum / dx2;
+                for (int d = 0; d < 6; ++d) {  // symmetric part: half of each (c, nb) entry
+                    if (nb[d] < 0) continue;
+                    const int q = (int)nb[d];
+                    const double e = -coef[d] / dx2;
+                    if (q < c) C[(size_t)q * W1 + (c - q)] += 0.5 * e;
+                    if (q > c) C[(size_t)c * W1 + (q - c)] += 0.5 * e;
+                    if (q == c) C[(size_t)c * W1] += e;  // self-wrap (dimension 1)
+                }
+            }
+    double* cm = C.data();
     for (int j = 0; j < n; ++j) {
-        double d = band[(size_t)j * W1];
-        for (int p = std::max(0, j - w); p < j; ++p) {
-            const double l = band[(size_t)j * W1 + (j - p)];
-            d -= l * l;
-        }
-        if (!(d > 0.0))
+        double* cj = cm + (size_t)j * W1;
+        if (!(cj[0] > 0.0))
             return fail(MF3D_ERR_SINGULAR, "coarsest operator is not definite (pure Neumann/"
                                            "periodic problem without pin?)");
-        const double Ljj = std::sqrt(d);
-        band[(size_t)j * W1] = Ljj;
-        for (int i = j + 1; i <= std::min(n - 1, j + w); ++i) {
-            double sacc = band[(size_t)i * W1 + (i - j)];
-            for (int p = std::max(0, i - w); p < j; ++p)
-                sacc -= band[(size_t)i * W1 + (i - p)] * band[(size_t)j * W1 + (j - p)];
-            band[(size_t)i * W1 + (i - j)] = sacc / Ljj;
+        const double Ljj = std::sqrt(cj[0]);
+        cj[0] = Ljj;
+        const int m = std::min(w, n - 1 - j);
+        for (int d = 1; d <= m; ++d) cj[d] /= Ljj;
+        for (int dk = 1; dk <= m; ++dk) {  // column j + dk -= L[.][j] L[j+dk][j]
+            double* ck = cm + (size_t)(j + dk) * W1;
+            const double lkj = cj[dk];
+            const double* src = cj + dk;
+            const int len = m - dk + 1;
+            for (int t = 0; t < len; ++t) ck[t] -= src[t] * lkj;
         }
     }
+    band.assign((size_t)n * W1, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d <= std::min(w, i); ++d) band[(size_t)i * W1 + d] = C[(size_t)(i - d) * W1 + d];
     *wout = w;
     return MF3D_OK;
 }
@@ -365,25 +400,93 @@ int reserve_coarse(mf3d_workspace* ws, int n, int w) {
 }
 
 // Dense coarsest inverse: coarsest K to the host, banded Cholesky there, the
-// columns of the inverse on the GPU (band_inverse_kernel).
-int build_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
-    Level& L = ws->lv.back();
-    const int n = (int)L.n;
-    if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
-    std::vector<double> Kc(n);
-    MF3D_HIP_TRY(hipMemcpyAsync(Kc.data(), coarsest_K(ws), n * sizeof(double),
-                                hipMemcpyDeviceToHost, s));
-    MF3D_HIP_TRY(hipStreamSynchronize(s));
-    std::vector<double> band;
+// columns of the inverse on the GPU (band_inverse_kernel). start_coarse_inverse
+// only enqueues the download (after the coarsening kernels); finish_coarse_inverse
+// runs before the first coarsest solve, so the host factorization overlaps the
+// rest of the set-up on the GPU.
+int grow_pinned(double** p, size_t* cap, size_t need) {
+    if (*p && *cap >= need) return MF3D_OK;
+    if (*p) MF3D_HIP_TRY(hipHostFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    MF3D_HIP_TRY(hipHostMalloc((void**)p, need * sizeof(double)));
+    *cap = need;
+    return MF3D_OK;
+}
+
+int coarse_bandwidth(const mf3d_workspace* ws) {
+    const mf3d_grid& g = ws->lv.back().g;
+    std::vector<double> ones((size_t)g.nx * g.ny * g.nz, 1.0);
     int w = 0;
-    MF3D_TRY(coarse_band(ws, Kc.data(), band, &w));
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+                const int c = i + g.nx * (j + g.ny * k);
+                for (int d = 0; d < 6; ++d) {
+                    double coef;
+                    const long nb = neighbour_coef(g.nx, g.ny, g.nz, ws->bct, ones.data(), i, j, k,
+                                                   d, 1.0, coef);
+                    if (nb >= 0) w = std::max(w, std::abs((int)nb - c));
+                }
+            }
+    return w;
+}
+
+// The worker thread waits for the download, factorizes, uploads the factor
+// and launches the inverse on the workspace's second stream; the solve's
+// stream waits for that (finish_coarse_inverse) only before its first
+// coarsest solve, so the factorization and the inverse overlap the set-up
+// and the first descent of the V-cycle. Allocation stays on this thread.
+int start_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
+    const int n = (int)ws->lv.back().n;
+    if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
+    ws->join_coarse();  // a set-up not followed by a solve
+    const int w = coarse_bandwidth(ws);
     MF3D_TRY(reserve_coarse(ws, n, w));
-    double* dband = ws->coarse_inv + (size_t)n * n;
-    MF3D_HIP_TRY(hipMemcpyAsync(dband, band.data(), band.size() * sizeof(double),
-                                hipMemcpyHostToDevice, s));
-    MF3D_TRY(kern::band_inverse(s, n, w, dband, ws->coarse_inv));
-    // the host band must outlive the copy
-    MF3D_HIP_TRY(hipStreamSynchronize(s));
+    // the previous upload from host_band was waited for by the previous solve
+    MF3D_TRY(grow_pinned(&ws->host_kc, &ws->host_kc_cap, (size_t)n));
+    MF3D_TRY(grow_pinned(&ws->host_band, &ws->host_band_cap, (size_t)n * (w + 1)));
+    if (!ws->kc_ev) MF3D_HIP_TRY(hipEventCreateWithFlags(&ws->kc_ev, hipEventDisableTiming));
+    if (!ws->inv_ev) MF3D_HIP_TRY(hipEventCreateWithFlags(&ws->inv_ev, hipEventDisableTiming));
+    if (!ws->coarse_stream)
+        MF3D_HIP_TRY(hipStreamCreateWithFlags(&ws->coarse_stream, hipStreamNonBlocking));
+    MF3D_HIP_TRY(hipMemcpyAsync(ws->host_kc, coarsest_K(ws), n * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+    MF3D_HIP_TRY(hipEventRecord(ws->kc_ev, s));
+    // the inverse overwrites coarse_inv: the previous solve's stream work is done
+    ws->coarse_pending = true;
+    ws->coarse_worker = std::thread([ws, n, w]() {
+        ws->coarse_status = MF3D_OK;
+        auto failed = [ws](int st, const char* msg) {
+            ws->coarse_status = st;
+            ws->coarse_msg = msg;
+        };
+        if (hipEventSynchronize(ws->kc_ev) != hipSuccess)
+            return failed(MF3D_ERR_HIP, "coarsest K download failed");
+        int wb = 0;
+        const int st = coarse_band(ws, ws->host_kc, ws->coarse_band_h, &wb);
+        if (st != MF3D_OK) return failed(st, last_error_cstr());
+        if (wb != w) return failed(MF3D_ERR_HIP, "coarsest band width changed");
+        std::memcpy(ws->host_band, ws->coarse_band_h.data(),
+                    ws->coarse_band_h.size() * sizeof(double));
+        double* dband = ws->coarse_inv + (size_t)n * n;
+        if (hipMemcpyAsync(dband, ws->host_band, ws->coarse_band_h.size() * sizeof(double),
+                           hipMemcpyHostToDevice, ws->coarse_stream) != hipSuccess)
+            return failed(MF3D_ERR_HIP, "band factor upload failed");
+        const int bs = kern::band_inverse(ws->coarse_stream, n, w, dband, ws->coarse_inv);
+        if (bs != MF3D_OK) return failed(bs, last_error_cstr());
+        if (hipEventRecord(ws->inv_ev, ws->coarse_stream) != hipSuccess)
+            return failed(MF3D_ERR_HIP, "event record failed");
+    });
+    return MF3D_OK;
+}
+
+int finish_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
+    if (!ws->coarse_pending) return MF3D_OK;
+    ws->coarse_pending = false;
+    ws->join_coarse();
+    if (ws->coarse_status != MF3D_OK) return fail(ws->coarse_status, ws->coarse_msg);
+    MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->inv_ev, 0));
     return MF3D_OK;
 }
 
@@ -425,16 +528,23 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
         const double mean_log2 = slog / (double)n0 / std::log(2.0);
         cs = std::ldexp(1.0, -(int)std::lround(mean_log2));
     }
+    for (size_t l = 1; l < ws->lv.size(); ++l)
+        MF3D_TRY(kern::coarsen(s, ws->lv[l].g, level_K(ws, l - 1), ws->lv[l].K));
+    if (cfg.coarse_solver == MF3D_COARSE_DIRECT && !defer_coarse)
+        MF3D_TRY(start_coarse_inverse(ws, s));
+    else
+        ws->coarse_pending = false;
     for (size_t l = 0; l < ws->lv.size(); ++l) {
         Level& L = ws->lv[l];
-        if (l > 0) MF3D_TRY(kern::coarsen(s, L.g, level_K(ws, l - 1), L.K));
         const bool want64 = (l == 0) || !mixed;  // the FP64 operator, the FP64 V-cycle
         if (want64) {
             if (!L.has_f64) {
                 MF3D_TRY(ws->alloc_faces(L.f64, L.g, COEF_F64));
                 L.has_f64 = true;
             }
-            MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
+            // the finest mixed level builds them below, with its FP32/16-bit faces
+            if (!(mixed && l == 0))
+                MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
         }
         const long lo = ws->march_min_cells[mixed ? 1 : 0], hi = ws->march_max_cells[mixed ? 1 : 0];
         const bool march = kern::march_ok(L.g, ws->bct) && (long)L.n >= lo && (long)L.n <= hi;
@@ -459,13 +569,16 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
             L.has_flo = true;
         }
         L.cs = cs;
-        MF3D_TRY(kern::faces_build(s, L.g, ws->bct, cs, level_K(ws, l), L.flo));
+        if (l == 0 && split) {
+            MF3D_TRY(kern::faces_build_pair(s, L.g, ws->bct, cs, level_K(ws, l), L.flo, L.f64));
+        } else {
+            if (l == 0) MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
+            MF3D_TRY(kern::faces_build(s, L.g, ws->bct, cs, level_K(ws, l), L.flo));
+        }
         if (!L.x32) MF3D_TRY(ws->alloc(&L.x32, L.n));
         if (!L.b32) MF3D_TRY(ws->alloc(&L.b32, L.n));
         if (L.plan.kind == SMOOTH_MARCH && !L.t32) MF3D_TRY(ws->alloc(&L.t32, L.n));
     }
-    if (cfg.coarse_solver == MF3D_COARSE_DIRECT && !defer_coarse)
-        MF3D_TRY(build_coarse_inverse(ws, s));
     return MF3D_OK;
 }
 
@@ -521,10 +634,11 @@ int build_coarse_inverses(mf3d_workspace* const* ws, int count, hipStream_t s,
     BandDesc* dh = reinterpret_cast<BandDesc*>(bh + tot);
     int* sh = reinterpret_cast<int*>(dh + count);
     size_t bo = 0;
-    int nmax = 0;
+    int nmax = 0, wmax = 0;
     sh[0] = 0;
     for (int r = 0; r < count; ++r) {
         const int n = (int)ws[r]->lv.back().n;
+        wmax = std::max(wmax, wv[r]);
         MF3D_TRY(reserve_coarse(ws[r], n, wv[r]));
         double* dband = ws[r]->coarse_inv + (size_t)n * n;
         std::memcpy(bh + bo, bands[r].data(), bands[r].size() * sizeof(double));
@@ -542,7 +656,7 @@ int build_coarse_inverses(mf3d_workspace* const* ws, int count, hipStream_t s,
                                 hipMemcpyHostToDevice, s));
     const BandDesc* D = static_cast<const BandDesc*>(dd);
     const int* st = reinterpret_cast<const int*>(D + count);
-    return kern::band_inverse_batch(s, D, st, count, sh[count], nmax);
+    return kern::band_inverse_batch(s, D, st, count, sh[count], nmax, wmax);
 }
 
 // per-type views of a level's V-cycle buffers
@@ -578,8 +692,10 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
     const double cs = V::cs(L);
     const FaceSet& F = V::F(L);
     if (l == nl - 1) {
-        if (ws->coarse_solver == MF3D_COARSE_DIRECT)
+        if (ws->coarse_solver == MF3D_COARSE_DIRECT) {
+            MF3D_TRY(finish_coarse_inverse(ws, s));
             return kern::gemv(s, (int)L.n, ws->coarse_inv, b, x);
+        }
         return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F,
                                   ws->coarse_iters, zero_init, (const T*)nullptr, L.plan);
     }
@@ -1079,11 +1195,11 @@ int pcg_iteration_batch(const BPlan& P, hipStream_t s) {
     const int m = P.vec.m;
     MF3D_TRY(kern::batch_op(s, 0, BO(P.vec)));
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_PAP, 1, true));
-    MF3D_TRY(kern::batch_update_xr(s, BV(P.vec)));
+    MF3D_TRY(kern::batch_update_r(s, BV(P.vec)));  // x follows with p
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RR, 1));
     MF3D_TRY(precondition_batch(P, s));  // also the (r, z) partials
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RZ, 1));
-    return kern::batch_update_p_split8(s, 0, BC8(P.vec));
+    return kern::batch_update_xp_split8(s, BC8(P.vec));
 }
 
 int read_states(mf3d_workspace* const* ws, int count, hipStream_t s) {
@@ -1156,7 +1272,8 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
             }
             MF3D_TRY(read_states(ws, count, s));
         }
-        // converged on the recursive residual: judge by the true one
+        // the x += alpha p owed by the last iteration, then judge by the true residual
+        MF3D_TRY(kern::batch_x_pending(s, BV(P.vec)));
         MF3D_TRY(kern::batch_op(s, 1, BO(P.vec)));
         MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RESUME, 2, true));
         MF3D_TRY(read_states(ws, count, s));

@@ -433,12 +433,150 @@ __global__ void __launch_bounds__(256)
 
 // Columns of A^{-1} from a banded Cholesky factor of M = -A = L L^T
 // (row-major band storage, Lb[i*(w+1) + (i-j)] = L[i][j] for i-w <= j <= i).
-// One wave per column c: solves L y = e_c then L^T z = y with the band dot
-// products spread over the 64 lanes, y kept in LDS; writes row c of -M^{-1},
-// which is column c of the symmetric A^{-1}.
-__device__ void band_inverse_body(int n, int w, const double* __restrict__ Lb,
-                                  double* __restrict__ Ainv, int c) {
-    extern __shared__ double y[];
+// One wave per column c: solves L y = e_c then L^T z = y and writes row c of
+// -M^{-1}, which is column c of the symmetric A^{-1}; y lives in LDS.
+//
+// band_inverse_slab (w <= 64): the substitutions go in blocks of 32 rows, one
+// row per lane. The band rows of a block are first copied to LDS with
+// coalesced loads (transposed: S[d * 32 + r] = L[R0 + r][R0 + r - d]); a lane
+// then gathers its row's terms from the blocks already solved, and the block
+// is solved row by row, each new value broadcast from its lane with
+// v_readlane and folded into the later rows' sums, so a sequential step is a
+// multiply, a broadcast and a fused multiply-add. The backward pass keeps the
+// previous two blocks' rows too: they hold the terms from the rows already
+// solved.
+// LDS: n + 3 * kSlab doubles.
+// band_inverse_row (any w): one row per step, the band dot product spread
+// over the lanes and reduced across the wave. LDS: n doubles.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+constexpr int kSlabRows = 32;
+constexpr int kSlab = kSlabRows * 65;  // doubles per staged block of band rows (w <= 64)
+
+// S[d * 32 + r] = Lb[(R0 + r) * W1 + d] (0 outside the matrix); all loads of a
+// block are in flight together
+__device__ __forceinline__ void stage_band(const double* __restrict__ Lb, int W1, int n, int R0,
+                                           double* __restrict__ S) {
+    const int lane = threadIdx.x;
+    double v[kSlabRows];
+#pragma unroll
+    for (int r = 0; r < kSlabRows; ++r) {
+        const int row = R0 + r;
+        const bool ok = row >= 0 && row < n;
+        v[r] = ok && lane < W1 ? Lb[(long)row * W1 + lane] : 0.0;
+    }
+    double v64 = 0.0;  // d = 64 (w = 64): lane r < 32 takes row R0 + r
+    if (W1 > 64 && lane < kSlabRows) {
+        const int row = R0 + lane;
+        v64 = row >= 0 && row < n ? Lb[(long)row * W1 + 64] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kSlabRows; ++r)
+        if (lane < W1) S[lane * kSlabRows + r] = v[r];
+    if (W1 > 64 && lane < kSlabRows) S[64 * kSlabRows + lane] = v64;
+}
+
+__device__ void band_inverse_slab(int n, int w, const double* __restrict__ Lb,
+                                  double* __restrict__ Ainv, int c, double* __restrict__ y,
+                                  double* __restrict__ S0, double* __restrict__ S1,
+                                  double* __restrict__ S2) {
+    constexpr int B = kSlabRows;
+    const int lane = threadIdx.x;
+    const bool part = lane < B;  // lanes B..63 only help with the copies
+    const int W1 = w + 1;
+    // forward: L y = e_c; rows before c are zero
+    for (int i0 = c; i0 < n; i0 += B) {
+        const int i = i0 + lane;
+        const bool act = part && i < n;
+        __syncthreads();  // S0 of the previous block has been read
+        stage_band(Lb, W1, n, i0, S0);
+        __syncthreads();
+        // terms j = i - d (d = lane + 1 .. min(w, i - c)) from the blocks already solved
+        const int dmax = act ? min(w, i - c) : 0;
+        double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int d0 = 1; d0 <= w; d0 += 8) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int d = d0 + k;
+                const bool ok = d > lane && d <= dmax;
+                const double L = S0[min(d, w) * B + (lane & (B - 1))];
+                const double yv = y[ok ? i - d : 0];
+                p[k] += ok ? L * yv : 0.0;
+            }
+        }
+        double acc = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+        const double rd = act ? 1.0 / S0[lane] : 0.0;
+        const double e = i == c ? 1.0 : 0.0;
+        double yi = 0.0;
+        const int m = min(B, n - i0);
+#pragma unroll 4
+        for (int s = 0; s < m; ++s) {
+            const int d = lane - s;
+            const double L = S0[(d > 0 && d <= w ? d : 0) * B + (lane & (B - 1))];
+            const double ys = readlane_f64((e - acc) * rd, s);
+            if (lane == s) yi = ys;
+            if (act && d > 0 && d <= w) acc += L * ys;
+        }
+        if (act) y[i] = yi;
+    }
+    // backward: L^T z = y (y = 0 before c); lane l of a block holds row i1 - 1 - l.
+    // cur: rows i1 - B .. i1 - 1 (row i1 - 1 - l at r = B - 1 - l); p1, p2: the
+    // two blocks staged before (rows i1 .. i1 + B - 1 and i1 + B .. i1 + 2B - 1)
+    double* cur = S0;
+    double* p1 = S1;
+    double* p2 = S2;
+    for (int i1 = n; i1 > 0; i1 -= B) {
+        const int i = i1 - 1 - lane;
+        const bool act = part && i >= 0;
+        __syncthreads();
+        stage_band(Lb, W1, n, i1 - B, cur);
+        __syncthreads();
+        const double rhs = act && i >= c ? y[i] : 0.0;
+        // terms j = i1 + t (d = j - i = t + 1 + lane <= w) from the rows already solved
+        const int tmax = act ? min(w - 1 - lane, n - 1 - i1) : -1;
+        double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int t0 = 0; t0 < w; t0 += 8) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int t = t0 + k;
+                const bool ok = t <= tmax;
+                const int d = ok ? t + 1 + lane : 0;
+                const double L = t < B ? p1[d * B + (t & (B - 1))] : p2[d * B + (t & (B - 1))];
+                const double yv = y[ok ? i1 + t : 0];
+                p[k] += ok ? L * yv : 0.0;
+            }
+        }
+        double acc = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+        const double rd = act ? 1.0 / cur[B - 1 - (lane & (B - 1))] : 0.0;
+        double zi = 0.0;
+        const int m = min(B, i1);
+#pragma unroll 4
+        for (int s = 0; s < m; ++s) {
+            const int d = lane - s;  // L[i1 - 1 - s][i] = cur[d * B + B - 1 - s]
+            const double L = cur[(d > 0 && d <= w ? d : 0) * B + B - 1 - s];
+            const double zs = readlane_f64((rhs - acc) * rd, s);
+            if (lane == s) zi = zs;
+            if (act && d > 0 && d <= w) acc += L * zs;
+        }
+        __syncthreads();  // every lane has read its y[i] and the solved rows above
+        if (act) y[i] = zi;
+        double* t = p2;
+        p2 = p1;
+        p1 = cur;
+        cur = t;
+    }
+    __syncthreads();
+    double* out = Ainv + (long)c * n;
+    for (int i = lane; i < n; i += 64) out[i] = -y[i];
+}
+
+__device__ void band_inverse_row(int n, int w, const double* __restrict__ Lb,
+                                 double* __restrict__ Ainv, int c, double* __restrict__ y) {
     const int lane = threadIdx.x;
     const int W1 = w + 1;
     for (int i = lane; i < n; i += 64) y[i] = 0.0;
@@ -455,7 +593,7 @@ __device__ void band_inverse_body(int n, int w, const double* __restrict__ Lb,
         if (lane == 0) y[i] = yi;
         __syncthreads();
     }
-    for (int i = n - 1; i >= c; --i) {
+    for (int i = n - 1; i >= 0; --i) {
         double s = 0.0;
         for (int t = lane; t < w; t += 64) {
             const int j = i + 1 + t;
@@ -463,21 +601,8 @@ __device__ void band_inverse_body(int n, int w, const double* __restrict__ Lb,
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        const double zi = (y[i] - s) / Lb[(long)i * W1];
+        const double zi = ((i >= c ? y[i] : 0.0) - s) / Lb[(long)i * W1];
         __syncthreads();
-        if (lane == 0) y[i] = zi;
-        __syncthreads();
-    }
-    // rows i < c of the back substitution: y[i] is 0 on entry there
-    for (int i = c - 1; i >= 0; --i) {
-        double s = 0.0;
-        for (int t = lane; t < w; t += 64) {
-            const int j = i + 1 + t;
-            if (j < n) s += Lb[(long)j * W1 + (j - i)] * y[j];
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        const double zi = (0.0 - s) / Lb[(long)i * W1];
         if (lane == 0) y[i] = zi;
         __syncthreads();
     }
@@ -485,18 +610,32 @@ __device__ void band_inverse_body(int n, int w, const double* __restrict__ Lb,
     for (int i = lane; i < n; i += 64) out[i] = -y[i];
 }
 
+// LDS doubles per column for (n, w): the slab variant when it fits in 64 KB
+__host__ __device__ inline bool band_slab_ok(int n, int w) {
+    return w <= 64 && (long)(n + 3 * kSlab) * 8 <= 65536;
+}
+
+__device__ __forceinline__ void band_inverse_body(int n, int w, const double* __restrict__ Lb,
+                                  double* __restrict__ Ainv, int c, int nmax, bool slab) {
+    extern __shared__ double y[];
+    if (slab) band_inverse_slab(n, w, Lb, Ainv, c, y, y + nmax, y + nmax + kSlab,
+                                y + nmax + 2 * kSlab);
+    else band_inverse_row(n, w, Lb, Ainv, c, y);
+}
+
 __global__ void __launch_bounds__(64)
-    band_inverse_kernel(int n, int w, const double* __restrict__ Lb, double* __restrict__ Ainv) {
-    band_inverse_body(n, w, Lb, Ainv, blockIdx.x);
+    band_inverse_kernel(int n, int w, const double* __restrict__ Lb, double* __restrict__ Ainv,
+                        int slab) {
+    band_inverse_body(n, w, Lb, Ainv, blockIdx.x, n, slab != 0);
 }
 
 // many factors at once: entry e owns the blocks [start[e], start[e+1]), one per column
 __global__ void __launch_bounds__(64)
     band_inverse_batch_kernel(const BandDesc* __restrict__ D, const int* __restrict__ start,
-                              int m) {
+                              int m, int nmax, int slab) {
     const int e = batch_entry(start, m, blockIdx.x);
     const BandDesc& B = D[e];
-    band_inverse_body(B.n, B.w, B.Lb, B.Ainv, (int)blockIdx.x - start[e]);
+    band_inverse_body(B.n, B.w, B.Lb, B.Ainv, (int)blockIdx.x - start[e], nmax, slab != 0);
 }
 
 }  // namespace
@@ -609,14 +748,19 @@ int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double
 }
 
 int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv) {
-    band_inverse_kernel<<<n, 64, (size_t)n * sizeof(double), s>>>(n, w, Lb, Ainv);
+    const bool slab = band_slab_ok(n, w);
+    const size_t lds = ((size_t)n + (slab ? 3 * kSlab : 0)) * sizeof(double);
+    band_inverse_kernel<<<n, 64, lds, s>>>(n, w, Lb, Ainv, slab ? 1 : 0);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
 
+// wmax: the largest bandwidth of the batch (the slab variant needs every w <= 64)
 int band_inverse_batch(hipStream_t s, const BandDesc* D, const int* start, int m, int nblocks,
-                       int nmax) {
-    band_inverse_batch_kernel<<<nblocks, 64, (size_t)nmax * sizeof(double), s>>>(D, start, m);
+                       int nmax, int wmax) {
+    const bool slab = band_slab_ok(nmax, wmax);
+    const size_t lds = ((size_t)nmax + (slab ? 3 * kSlab : 0)) * sizeof(double);
+    band_inverse_batch_kernel<<<nblocks, 64, lds, s>>>(D, start, m, nmax, slab ? 1 : 0);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
