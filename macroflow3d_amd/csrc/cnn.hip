@@ -384,6 +384,12 @@ __device__ __forceinline__ void lin_axis(int i, int n, int& i0, int& i1, float& 
     }
 }
 
+__global__ void __launch_bounds__(256)
+    channel0_kernel(const float* __restrict__ y, int ych, long n, float* __restrict__ yd) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) yd[v] = y[v * ych];
+}
+
 // h0 = h_lin + J trilinear(y) on the fine grid g (y: coarse, ych channels, channel 0).
 // Workgroup b covers segment b % segs (256 cells) of fine row b / segs, so j, k
 // and the y/z weights are uniform in a workgroup and each thread only derives i.
@@ -557,6 +563,7 @@ struct mf3d_wsnet {
                    *e2 = nullptr, *e2b = nullptr, *u1 = nullptr, *d1 = nullptr, *u0 = nullptr,
                    *d0 = nullptr;
     float* y = nullptr;  // [N0][16] FP32 (channel 0 used)
+    float* yd = nullptr; // [N0] channel 0 of y, dense (read by the output interpolation)
     // backward scratch
     __hip_bfloat16 *g0a = nullptr, *g0b = nullptr, *g0c = nullptr, *g1a = nullptr,
                    *g1b = nullptr, *g2a = nullptr, *g2b = nullptr;
@@ -631,6 +638,9 @@ int ensure_acts(mf3d_wsnet* net, const mf3d_grid& g) {
     MF3D_TRY(net_alloc(&q, n0 * 16 * sizeof(float)));
     net->act.push_back(q);
     net->y = static_cast<float*>(q);
+    MF3D_TRY(net_alloc(&q, n0 * sizeof(float)));
+    net->act.push_back(q);
+    net->yd = static_cast<float*>(q);
     long wp = 0, gmax = 0;
     for (int l = 0; l < kNumLayers; ++l) {
         wp = std::max(wp, (long)kSlices * net->cout_pad[l] * 27L * net->cin_pad[l]);
@@ -929,9 +939,13 @@ int mf3d_wsnet_predict(mf3d_wsnet* net, mf3d_stream stream, const mf3d_grid* gri
     (void)hipGetLastError();
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     MF3D_TRY(forward(net, s, *grid, K, k_is_log));
+    // channel 0 densely first: the interpolation's 8 gathers per cell then
+    // share cache lines between neighbouring cells
+    const long n0 = cells(*grid) / ((long)kS * kS * kS);
+    channel0_kernel<<<blocks(n0), 256, 0, s>>>(net->y, 16, n0, net->yd);
     const long rows = (long)grid->ny * grid->nz, segs = (grid->nx + 255) / 256;
     output_kernel<<<(unsigned)(rows * segs), 256, 0, s>>>(Geo{grid->nx, grid->ny, grid->nz}, hw,
-                                                          he, net->y, 16, h0);
+                                                          he, net->yd, 1, h0);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
