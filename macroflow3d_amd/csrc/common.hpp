@@ -133,7 +133,7 @@ enum Finalize : int {
 };
 
 constexpr int kReduceBlock = 256;   // threads per first-stage block
-constexpr int kFinalBlock = 1024;   // threads of the second stage
+constexpr int kFinalBlock = 256;    // threads of the second stage (= first-stage workgroups)
 constexpr int kReduceCells = 2048;  // cells per first-stage partial (8 per thread, 256 threads)
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "faces.hpp"
+#include "reduce.hpp"
 
 namespace mf3d {
 namespace {
@@ -1018,21 +1019,6 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---- FP64 operator on the finest level ----------------------------------------
-template <int BS>
-__device__ inline double block_sum(double v) {
-    __shared__ double part[BS / 64];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) part[w] = v;
-    __syncthreads();
-    if (w == 0) {
-        v = lane < BS / 64 ? part[lane] : 0.0;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    }
-    return v;
-}
 
 // A x = -sum f (x_C - x_nb) / dx^2 (varcoeff_laplacian.cu:38-239), summed
 // x-, x+, y-, y+, z-, z+; the pin's diagonal counts in-domain neighbours only
