@@ -264,6 +264,20 @@ def main():
 
     head_err = rel(h, h64)
     vel_err = max(rel(vel.U, v64.U), rel(vel.V, v64.V), rel(vel.W, v64.W))
+    # mass-conservation check of the headline head's padded velocity (outside
+    # the timed step): the divergence is the solve's residual b - A h
+    pv = ph.PaddedVelocityField(g, ctx)
+    ph.compute_velocity_from_head(pv, h, K, g, bc, ctx)
+    diag = ph.VelocityDiagnostics(g, ctx)
+    ph.compute_velocity_diagnostics(pv, diag, g, ctx)
+    div_l2 = float(torch.linalg.vector_norm(diag.divergence))
+    diagnostics = {"div_min": float(diag.stats[0, 0]), "div_max": float(diag.stats[0, 1]),
+                   "div_mean": float(diag.stats[0, 2]), "div_l2_over_bnorm": div_l2 / res.bnorm,
+                   "vorticity_mag_mean": float(diag.stats[1, 2]),
+                   "helicity_mean": float(diag.stats[2, 2])}
+    log(args, f"velocity diagnostics: ||div|| / ||b|| = {div_l2 / res.bnorm:.3e}, "
+              f"mean |curl| {diagnostics['vorticity_mag_mean']:.4e}")
+    del pv, diag
 
     # dominant kernel: finest-level colour sweep of the headline V-cycle,
     # timed in place on the workspace's own level-0 buffers
@@ -317,6 +331,7 @@ def main():
         "speedup_vs_fp64": round(ms64 / ms, 3) if ms > 0 else None,
         "head_rel_l2_vs_fp64": head_err,
         "velocity_rel_l2_vs_fp64": vel_err,
+        "velocity_diagnostics": diagnostics,
         "roofline": {
             "kernel": f"colour_split4_kernel<{args.coef}> (finest level, one colour)",
             "bound": "hbm",

@@ -10,6 +10,9 @@
 //   coarsen_K                                src/physics/flow/coarsen_K.cuh:46-47
 //   compute_velocity_from_head (x2), compute_norm2/sum/mean, check_no_nans
 //                                            src/physics/flow/velocity_from_head.cuh:54-107
+//   VelocityDiagnostics, compute_velocity_diagnostics, print_velocity_diagnostics
+//                                            src/physics/flow/velocity_diagnostics.cuh:31-92
+//   compute_field_stats                      src/physics/stochastic/stochastic.cu:369-414
 //   solve_head_batched, WarmStartNet         extensions without a counterpart
 //
 // HeadSolveConfig keeps the reference fields and defaults; the extension
@@ -353,6 +356,78 @@ inline void print_velocity_checksums(const VelocityField& vel, CudaContext& ctx)
         std::printf("[velocity] %s: sum=%.17g norm2=%.17g nan=%s\n", names[c],
                     compute_sum(spans[c], ctx), nrm, std::isnan(nrm) ? "yes" : "no");
     }
+}
+
+// ---- velocity diagnostics ------------------------------------------------------
+// min, max and mean of a device field (synchronous); zeros when empty
+inline void compute_field_stats(DeviceSpan<const real> data, real& min_val, real& max_val,
+                                real& mean_val, const CudaContext& ctx) {
+    if (data.empty()) {
+        min_val = max_val = mean_val = 0.0;
+        return;
+    }
+    blas::ReductionWorkspace red;
+    DeviceBuffer<real> out(3);
+    mf3d_check(mf3d_field_stats(red.handle(), ctx.c_stream(), data.data(), data.size(),
+                                out.data()),
+               "compute_field_stats");
+    real h[3];
+    hip_check(hipMemcpyAsync(h, out.data(), sizeof(h), hipMemcpyDeviceToHost, ctx.stream()),
+              "compute_field_stats");
+    ctx.synchronize();
+    min_val = h[0], max_val = h[1], mean_val = h[2];
+}
+
+// Cell-centred divergence, |curl| and helicity (nx*ny*nz values each).
+struct VelocityDiagnostics {
+    ScalarField divergence;
+    ScalarField vorticity_mag;
+    ScalarField helicity;
+
+    VelocityDiagnostics() = default;
+    explicit VelocityDiagnostics(const Grid3D& grid)
+        : divergence(grid), vorticity_mag(grid), helicity(grid) {}
+    void resize(const Grid3D& grid) {
+        divergence.resize(grid);
+        vorticity_mag.resize(grid);
+        helicity.resize(grid);
+    }
+    bool empty() const { return divergence.empty(); }
+};
+
+// One launch over all cells: divergence everywhere, |curl| and helicity at
+// interior cells (0 at the border, and everywhere when a dimension is below
+// 3, with a warning). diag must be allocated to the grid size.
+inline void compute_velocity_diagnostics(const PaddedVelocityField& vel, VelocityDiagnostics& diag,
+                                         const Grid3D& grid, const CudaContext& ctx) {
+    const size_t total = grid.num_cells();
+    if (total == 0) return;
+    if (grid.nx < 3 || grid.ny < 3 || grid.nz < 3)
+        std::printf("       [diag] WARNING: grid too small for curl/helicity "
+                    "(%dx%dx%d, need >=3 in each dim). Only divergence computed.\n",
+                    grid.nx, grid.ny, grid.nz);
+    const mf3d_grid g = to_c(grid);
+    mf3d_check(mf3d_velocity_diagnostics(nullptr, ctx.c_stream(), &g, vel.U.data(), vel.V.data(),
+                                         vel.W.data(), vel.U.size(),
+                                         diag.divergence.device_ptr(),
+                                         diag.vorticity_mag.device_ptr(),
+                                         diag.helicity.device_ptr(), diag.divergence.data.size(),
+                                         nullptr),
+               "compute_velocity_diagnostics");
+}
+
+inline void print_velocity_diagnostics(const VelocityDiagnostics& diag, int realization_id,
+                                       const CudaContext& ctx) {
+    real lo, hi, mean;
+    compute_field_stats(diag.divergence.span(), lo, hi, mean, ctx);
+    std::printf("       [diag] r=%d  div:  min=%.4e  max=%.4e  mean=%.4e\n", realization_id, lo,
+                hi, mean);
+    compute_field_stats(diag.vorticity_mag.span(), lo, hi, mean, ctx);
+    std::printf("       [diag] r=%d  |w|:  min=%.4e  max=%.4e  mean=%.4e  (incl. border=0)\n",
+                realization_id, lo, hi, mean);
+    compute_field_stats(diag.helicity.span(), lo, hi, mean, ctx);
+    std::printf("       [diag] r=%d  h:    min=%.4e  max=%.4e  mean=%.4e  (incl. border=0)\n",
+                realization_id, lo, hi, mean);
 }
 
 // ---- extensions without a counterpart in MacroFlow3D ---------------------------

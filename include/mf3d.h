@@ -177,6 +177,26 @@ int mf3d_velocity_padded(mf3d_stream stream, const mf3d_grid* grid, const mf3d_b
                          const double* h, const double* K, size_t n, double* U, double* V,
                          double* W, size_t n_padded);
 
+/* physics::compute_velocity_diagnostics (src/physics/flow/velocity_diagnostics.cuh:61-67,
+ * velocity_diagnostics.cu:24-177) on a padded field (layout of mf3d_velocity_padded):
+ * div, vort_mag and helicity (n >= nx*ny*nz cell values each). The divergence
+ * covers every cell; |curl| and helicity are computed at interior cells
+ * (1 <= i <= nx-2, same in j and k) and are 0 elsewhere, and everywhere when a
+ * dimension is below 3 (the reference also prints a warning then; callers of
+ * this entry point do that themselves). d_stats (nullable, device, 9 values):
+ * min, max, mean of div, vort_mag and helicity, the numbers
+ * print_velocity_diagnostics prints (velocity_diagnostics.cu:179-199); it
+ * needs ws for the reduction scratch (ws may be null when d_stats is null). */
+int mf3d_velocity_diagnostics(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
+                              const double* U, const double* V, const double* W,
+                              size_t n_padded, double* div, double* vort_mag, double* helicity,
+                              size_t n, double* d_stats);
+
+/* compute_field_stats (src/physics/stochastic/stochastic.cu:369-414): d_out
+ * (device, 3 values) = min, max, mean of x[0..n); zeros when n == 0. */
+int mf3d_field_stats(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
+                     double* d_out);
+
 /* ---- multigrid ------------------------------------------------------- */
 /* multigrid::gsrb_smooth_3d (src/multigrid/smoothers/gsrb_3d.cuh:29-31, gsrb_3d.cu:368-576). */
 int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, const double* b,

@@ -77,6 +77,9 @@ SIGNATURES = {
     "mf3d_coarsen_K": (_i, [_vp, _g, _g, _vp, _sz, _vp, _sz]),
     "mf3d_velocity_compact": (_i, [_vp, _g, _b, _vp, _vp, _sz, _vp, _sz, _vp, _sz, _vp, _sz]),
     "mf3d_velocity_padded": (_i, [_vp, _g, _b, _vp, _vp, _sz, _vp, _vp, _vp, _sz]),
+    "mf3d_velocity_diagnostics": (_i, [_vp, _vp, _g, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz,
+                                       _vp]),
+    "mf3d_field_stats": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "mf3d_gsrb_smooth_3d": (_i, [_vp, _g, _vp, _vp, _vp, _sz, _i, _b, _i]),
     "mf3d_gsrb_smooth_3d_scratch": (_i, [_vp, _g, _vp, _vp, _vp, _sz, _i, _b, _i, _vp, _i]),
     "mf3d_compute_residual_3d": (_i, [_vp, _g, _vp, _vp, _vp, _vp, _sz, _b, _i]),

@@ -33,6 +33,19 @@ int fail(int status, const std::string& msg);
 
 inline size_t cells(const mf3d_grid& g) { return (size_t)g.nx * (size_t)g.ny * (size_t)g.nz; }
 
+// Workgroups are dealt round-robin to the 8 XCDs (blockIdx % 8), each with its
+// own L2. xcd_block maps the launch so that the blocks of one XCD cover a
+// contiguous range of the logical grid (a slab of rows and planes), keeping
+// the y- and z-neighbour rows of a stencil in that XCD's L2. The grid is
+// rounded up to a multiple of 8 blocks (xcd_round); the padding blocks get -1.
+inline unsigned xcd_round(size_t blocks) { return (unsigned)((blocks + 7u) / 8u * 8u); }
+__device__ __forceinline__ long xcd_block(long nblocks) {
+    const long per = (nblocks + 7) >> 3;
+    const long b = blockIdx.x;
+    const long t = (b & 7) * per + (b >> 3);
+    return t < nblocks ? t : -1;
+}
+
 // Boundary types only (values enter through the RHS and velocity kernels).
 struct BcTypes {
     int t[6];

@@ -35,19 +35,7 @@ namespace {
 constexpr int kBlock = 256;
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 // grid of a 1-D launch remapped by xcd_block: padded to a multiple of 8 blocks
-inline unsigned xcd_grid(size_t n) { return ((grid_for(n) + 7u) / 8u) * 8u; }
-
-// Workgroups are dealt round-robin to the 8 XCDs (blockIdx % 8), each with its
-// own L2. This maps the launch so that the blocks of one XCD cover a
-// contiguous range of the logical grid (a slab of rows and planes), keeping
-// the y- and z-neighbour rows of a stencil in that XCD's L2. Returns -1 for the
-// padding blocks of a grid rounded up to a multiple of 8.
-__device__ __forceinline__ long xcd_block(long nblocks) {
-    const long per = (nblocks + 7) >> 3;
-    const long b = blockIdx.x;
-    const long t = (b & 7) * per + (b >> 3);
-    return t < nblocks ? t : -1;
-}
+inline unsigned xcd_grid(size_t n) { return xcd_round(grid_for(n)); }
 
 // ---- coefficient storage conversions ---------------------------------------
 __device__ __forceinline__ double cload(const double* p, long i) { return p[i]; }

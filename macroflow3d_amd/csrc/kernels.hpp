@@ -87,6 +87,17 @@ int batch_x_pending(hipStream_t s, const BatchVec* V, const int* start, int m, l
 // ---- velocity (velocity.hip) -----------------------------------------------
 int velocity(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* h,
              const double* K, double* U, double* V, double* W, bool padded);
+// divergence, |curl| and helicity at cell centres from padded U, V, W; with
+// stats (device, 9 values: min, max, mean of each field) the partials buffer
+// holds diag_partials(g) values
+size_t diag_partials(const mf3d_grid& g);
+int velocity_diagnostics(hipStream_t s, const mf3d_grid& g, const double* U, const double* V,
+                         const double* W, double* div, double* omag, double* hel,
+                         double* partials, double* stats);
+// out (device, 3 values) = min, max, mean of x[0..n), n > 0; partials holds
+// stats_partials(n) values
+size_t stats_partials(size_t n);
+int field_stats(hipStream_t s, const double* x, size_t n, double* partials, double* out);
 
 }  // namespace kern
 }  // namespace mf3d

@@ -286,8 +286,8 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     return MF3D_OK;
 }
 
-int scratch_partials(mf3d_workspace* ws, size_t n) {
-    const size_t need = kern::reduce_blocks(n) + 64;
+// reduction scratch of at least need doubles (grown, never shrunk)
+int scratch_count(mf3d_workspace* ws, size_t need) {
     if (ws->partials && ws->partials_cap >= need) return MF3D_OK;
     double* p = nullptr;
     MF3D_TRY(ws->alloc(&p, need));
@@ -295,6 +295,9 @@ int scratch_partials(mf3d_workspace* ws, size_t n) {
     ws->partials_cap = need;
     if (!ws->dscalar) MF3D_TRY(ws->alloc(&ws->dscalar, 8));
     return MF3D_OK;
+}
+int scratch_partials(mf3d_workspace* ws, size_t n) {
+    return scratch_count(ws, kern::reduce_blocks(n) + 64);
 }
 
 int host_scalar_sync(mf3d_workspace* ws, hipStream_t s, double* out) {
@@ -1614,6 +1617,41 @@ int mf3d_velocity_padded(mf3d_stream stream, const mf3d_grid* grid, const mf3d_b
     MF3D_CHECK_LEN(n_padded, (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1), "padded velocity");
     clear_stale_error();
     return kern::velocity(as_stream(stream), g, *bc, h, K, U, V, W, true);
+}
+
+int mf3d_velocity_diagnostics(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
+                              const double* U, const double* V, const double* W,
+                              size_t n_padded, double* div, double* vort_mag, double* helicity,
+                              size_t n, double* d_stats) {
+    MF3D_TRY(validate_grid(grid));
+    MF3D_CHECK_PTR(U);
+    MF3D_CHECK_PTR(V);
+    MF3D_CHECK_PTR(W);
+    MF3D_CHECK_PTR(div);
+    MF3D_CHECK_PTR(vort_mag);
+    MF3D_CHECK_PTR(helicity);
+    const mf3d_grid& g = *grid;
+    MF3D_CHECK_LEN(n_padded, (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1), "padded velocity");
+    MF3D_CHECK_LEN(n, cells(g), "diagnostic field");
+    if (d_stats) {
+        MF3D_CHECK_PTR(ws);
+        MF3D_TRY(scratch_count(ws, kern::diag_partials(g)));
+    }
+    clear_stale_error();
+    return kern::velocity_diagnostics(as_stream(stream), g, U, V, W, div, vort_mag, helicity,
+                                      d_stats ? ws->partials : nullptr, d_stats);
+}
+
+int mf3d_field_stats(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
+                     double* d_out) {
+    MF3D_CHECK_PTR(ws);
+    MF3D_CHECK_PTR(d_out);
+    clear_stale_error();
+    hipStream_t s = as_stream(stream);
+    if (n == 0) return kern::fill(s, d_out, 3, 0.0);  // stochastic.cu:372-375
+    MF3D_CHECK_PTR(x);
+    MF3D_TRY(scratch_count(ws, kern::stats_partials(n)));
+    return kern::field_stats(s, x, n, ws->partials, d_out);
 }
 
 int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, const double* b,

@@ -6,8 +6,10 @@ Mirrors MacroFlow3D's ``src/physics/flow`` and ``src/physics/common``:
 ``build_rhs_head`` (rhs_head.cuh:50-51), ``coarsen_K`` (coarsen_K.cuh:46-47),
 ``compute_velocity_from_head`` (velocity_from_head.cuh:54-73) with
 ``VelocityField``/``PaddedVelocityField`` (fields.cuh:108-319), the checksum
-utilities (velocity_from_head.cuh:79-107) and ``FlowWorkspace``
-(workspaces.cuh:111-167). Every computation runs in libmf3d_hip.so.
+utilities (velocity_from_head.cuh:79-107), the velocity diagnostics
+(velocity_diagnostics.cuh:31-92), ``compute_field_stats`` (stochastic.cu:369-414)
+and ``FlowWorkspace`` (workspaces.cuh:111-167). Every computation runs in
+libmf3d_hip.so.
 """
 from __future__ import annotations
 
@@ -392,6 +394,78 @@ def compute_mean(data: torch.Tensor, ctx: HipContext) -> float:
 def check_no_nans(data: torch.Tensor, ctx: HipContext) -> bool:
     """True when no value is NaN (a NaN makes the sum of squares NaN)."""
     return not math.isnan(compute_norm2(data, ctx))
+
+
+def compute_field_stats(data: torch.Tensor, ctx: HipContext) -> tuple:
+    """(min, max, mean) of a device field (stochastic.cu:369-414), synchronous;
+    zeros for an empty field."""
+    n = data.numel()
+    if n == 0:
+        return 0.0, 0.0, 0.0
+    out = ctx.empty(3)
+    _lib.call("mf3d_field_stats", _ws().handle, ctx.handle, dptr(data, n, "data", ctx), n,
+              dptr(out, 3, "out", ctx))
+    ctx.synchronize()
+    lo, hi, mean = out.tolist()
+    return lo, hi, mean
+
+
+class VelocityDiagnostics:
+    """Cell-centred divergence, |curl| and helicity of a velocity field
+    (velocity_diagnostics.cuh:31-50), each nx*ny*nz values at i + nx*(j + ny*k)."""
+
+    def __init__(self, grid: Grid3D, ctx: HipContext):
+        self.resize(grid, ctx)
+
+    def resize(self, grid: Grid3D, ctx: HipContext) -> None:
+        n = grid.num_cells()
+        self.divergence = ctx.zeros(n)
+        self.vorticity_mag = ctx.zeros(n)
+        self.helicity = ctx.zeros(n)
+        self.stats = None  # (3, 3): min, max, mean per field, after compute
+
+    def empty(self) -> bool:
+        return self.divergence.numel() == 0
+
+
+def compute_velocity_diagnostics(vel: "PaddedVelocityField", diag: VelocityDiagnostics,
+                                 grid: Grid3D, ctx: HipContext) -> None:
+    """physics::compute_velocity_diagnostics (velocity_diagnostics.cu:147-177).
+
+    Divergence at every cell; |curl| and helicity at interior cells, 0 at the
+    border and everywhere when a dimension is below 3 (a warning is printed
+    then, as the reference does). Also fills ``diag.stats`` (min, max, mean of
+    each field, computed on the device in the same pass). Synchronous.
+    """
+    if not isinstance(vel, PaddedVelocityField):
+        raise TypeError("compute_velocity_diagnostics takes a PaddedVelocityField")
+    n = grid.num_cells()
+    if n == 0:
+        return
+    if min(grid.nx, grid.ny, grid.nz) < 3:
+        print(f"       [diag] WARNING: grid too small for curl/helicity "
+              f"({grid.nx}x{grid.ny}x{grid.nz}, need >=3 in each dim). "
+              f"Only divergence computed.")
+    s = vel.field_size()
+    stats = ctx.empty(9)
+    _lib.call("mf3d_velocity_diagnostics", _ws().handle, ctx.handle, C.byref(grid.c()),
+              dptr(vel.U, s, "U", ctx), dptr(vel.V, s, "V", ctx), dptr(vel.W, s, "W", ctx), s,
+              dptr(diag.divergence, n, "divergence", ctx),
+              dptr(diag.vorticity_mag, n, "vorticity_mag", ctx),
+              dptr(diag.helicity, n, "helicity", ctx), n, dptr(stats, 9, "stats", ctx))
+    ctx.synchronize()
+    diag.stats = stats.view(3, 3).cpu()
+
+
+def print_velocity_diagnostics(diag: VelocityDiagnostics, realization_id: int,
+                               ctx: HipContext) -> None:
+    """One line per field with min, max, mean (velocity_diagnostics.cu:179-199)."""
+    rows = [("div: ", ""), ("|w|: ", "  (incl. border=0)"), ("h:   ", "  (incl. border=0)")]
+    fields = (diag.divergence, diag.vorticity_mag, diag.helicity)
+    for (label, tail), f in zip(rows, fields):
+        lo, hi, mean = compute_field_stats(f, ctx)
+        print(f"       [diag] r={realization_id}  {label} min={lo:.4e}  max={hi:.4e}  "
+              f"mean={mean:.4e}{tail}")
 
 
 def velocity_checksums(vel, ctx: HipContext) -> dict:

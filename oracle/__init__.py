@@ -15,6 +15,10 @@ Parity status of each oracle function (how it is pinned, see tests/):
   * PCG, MG, CG solves: dense brute-force solve on small grids and the
     homogeneous / layered closed forms;
   * velocity: homogeneous and layered closed forms, flux balance;
+  * velocity diagnostics: closed forms (linear fields with exact divergence,
+    curl and helicity), telescoping flux sums, and the divergence of the
+    velocity of a converged head solve (= the solve's residual);
+  * field stats: numpy min / max / mean;
   * dot / nrm2: exact-rational checks; the reference's cuBLAS reduction order
     is closed source and parity unpinned (not reproducible, not needed);
   * lognormal generator: covariance / variance recovery (statistical) -- it is
@@ -90,6 +94,8 @@ def lib():
             "orc_solve_head": (C.c_int, [g, b, cfg, d, d, P(Result)]),
             "orc_velocity_compact": (None, [g, b, d, d, d, d, d]),
             "orc_velocity_padded": (None, [g, b, d, d, d, d, d]),
+            "orc_velocity_diagnostics": (None, [g, d, d, d, d, d, d]),
+            "orc_field_stats": (None, [C.c_size_t, d, d]),
             "orc_lognormal_lnK": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double,
                                            C.c_double, C.c_uint64, d, d]),
             "orc_gauss_pair": (None, [C.c_uint64, C.c_uint64, d]),
@@ -304,6 +310,30 @@ def velocity_padded(g, b_, h, K):
     U, V, W = np.zeros(n), np.zeros(n), np.zeros(n)
     lib().orc_velocity_padded(C.byref(g), C.byref(b_), _p(h), _p(K), _p(U), _p(V), _p(W))
     return U, V, W
+
+
+def velocity_diagnostics(g, U, V, W):
+    """compute_velocity_diagnostics on a padded field (velocity_diagnostics.cu:46-177):
+    (divergence, |curl|, helicity) at the cell centres."""
+    n = (g.nx + 1) * (g.ny + 1) * (g.nz + 1)
+    arrs = []
+    for a in (U, V, W):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != n:
+            raise ValueError(f"padded component has {a.size} values, expected {n}")
+        arrs.append(a)
+    nc = g.nx * g.ny * g.nz
+    div, om, hel = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+    lib().orc_velocity_diagnostics(C.byref(g), *(_p(a) for a in arrs), _p(div), _p(om), _p(hel))
+    return div, om, hel
+
+
+def field_stats(x):
+    """compute_field_stats (stochastic.cu:369-414): (min, max, mean)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros(3)
+    lib().orc_field_stats(x.size, _p(x), _p(out))
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def lognormal_lnK(nx, ny, nz, sigma2, corr_len, seed):

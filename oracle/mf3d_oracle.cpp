@@ -958,6 +958,72 @@ void orc_velocity_padded(const orc_grid* g, const orc_bc* bc, const double* H, c
     velocity_impl(*g, *bc, H, K, U, V, W, true);
 }
 
+// Cell-centred diagnostics of a padded velocity field, compute_velocity_diagnostics
+// (src/physics/flow/velocity_diagnostics.cu:46-141 kernel, 147-177 host):
+//   div = (U(i+1)-U(i))/dx + (V(j+1)-V(j))/dy + (W(k+1)-W(k))/dz at every cell;
+//   at interior cells (1 <= i <= nx-2, same in j and k) the face averages
+//   uc = (U(i)+U(i+1))/2 (and vc, wc) are differenced centrally,
+//   w = (dwc/dy - dvc/dz, duc/dz - dwc/dx, dvc/dx - duc/dy), and
+//   |w| and the helicity uc*wx + vc*wy + wc*wz are stored; elsewhere, and
+//   everywhere when a dimension is below 3 (:157), both are 0.
+void orc_velocity_diagnostics(const orc_grid* gp, const double* U, const double* V,
+                              const double* W, double* div, double* omag, double* hel) {
+    const orc_grid& g = *gp;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool curl = nx >= 3 && ny >= 3 && nz >= 3;
+    const double ix = 1.0 / g.dx, iy = 1.0 / g.dy, iz = 1.0 / g.dz;
+    const double hx = 0.5 * ix, hy = 0.5 * iy, hz = 0.5 * iz;
+    auto P = [&](int i, int j, int k) {
+        return ((size_t)k * (size_t)(ny + 1) + (size_t)j) * (size_t)(nx + 1) + (size_t)i;
+    };
+    // face averages at the centre of cell (i, j, k)
+    auto uc = [&](int i, int j, int k) { return 0.5 * (U[P(i, j, k)] + U[P(i + 1, j, k)]); };
+    auto vc = [&](int i, int j, int k) { return 0.5 * (V[P(i, j, k)] + V[P(i, j + 1, k)]); };
+    auto wc = [&](int i, int j, int k) { return 0.5 * (W[P(i, j, k)] + W[P(i, j, k + 1)]); };
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+                div[c] = (U[P(i + 1, j, k)] - U[P(i, j, k)]) * ix +
+                         (V[P(i, j + 1, k)] - V[P(i, j, k)]) * iy +
+                         (W[P(i, j, k + 1)] - W[P(i, j, k)]) * iz;
+                const bool inner = curl && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 &&
+                                   k >= 1 && k <= nz - 2;
+                if (!inner) {
+                    omag[c] = 0.0;
+                    hel[c] = 0.0;
+                    continue;
+                }
+                const double wx = (wc(i, j + 1, k) - wc(i, j - 1, k)) * hy -
+                                  (vc(i, j, k + 1) - vc(i, j, k - 1)) * hz;
+                const double wy = (uc(i, j, k + 1) - uc(i, j, k - 1)) * hz -
+                                  (wc(i + 1, j, k) - wc(i - 1, j, k)) * hx;
+                const double wz = (vc(i + 1, j, k) - vc(i - 1, j, k)) * hx -
+                                  (uc(i, j + 1, k) - uc(i, j - 1, k)) * hy;
+                omag[c] = std::sqrt(wx * wx + wy * wy + wz * wz);
+                hel[c] = uc(i, j, k) * wx + vc(i, j, k) * wy + wc(i, j, k) * wz;
+            }
+}
+
+// compute_field_stats (src/physics/stochastic/stochastic.cu:369-414):
+// out = {min, max, mean}; all zero for n == 0. The sum runs in index order.
+void orc_field_stats(size_t n, const double* x, double* out) {
+    if (n == 0) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    double lo = x[0], hi = x[0], sum = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        lo = std::min(lo, x[i]);
+        hi = std::max(hi, x[i]);
+        sum += x[i];
+    }
+    out[0] = lo;
+    out[1] = hi;
+    out[2] = sum / (double)n;
+}
+
 // ---------------------------------------------------------------------------
 // Log-normal conductivity by FFT circulant embedding (Dietrich & Newsam 1997).
 // Not a reference function: the reference draws K with a randomized spectral

@@ -88,6 +88,15 @@ int main(int argc, char** argv) {
         write_bin(out + ".Up.bin", to_host(pv.U_span(), ctx));
         const double usum = physics::compute_sum(vel.U_span(), ctx);
         const bool finite = physics::check_no_nans(vel.U_span(), ctx);
+        // mass-conservation check of the padded hand-off (velocity_diagnostics.cu:147-199)
+        physics::VelocityDiagnostics diag(grid);
+        physics::compute_velocity_diagnostics(pv, diag, grid, ctx);
+        physics::print_velocity_diagnostics(diag, 0, ctx);
+        write_bin(out + ".div.bin", to_host(diag.divergence.span(), ctx));
+        write_bin(out + ".omag.bin", to_host(diag.vorticity_mag.span(), ctx));
+        write_bin(out + ".hel.bin", to_host(diag.helicity.span(), ctx));
+        real dmin = 0, dmax = 0, dmean = 0;
+        physics::compute_field_stats(diag.divergence.span(), dmin, dmax, dmean, ctx);
 
         // generic PCG: operator + MG preconditioner of the reference interfaces
         multigrid::MGHierarchy hier;
@@ -151,11 +160,13 @@ int main(int argc, char** argv) {
                     "\"ref_rfinal\": %.17g, \"ns_iters\": %d, \"ns_converged\": %d, "
                     "\"ns_true_residual\": %.17g, \"ns_bnorm\": %.17g, \"pcg_iters\": %d, "
                     "\"pcg_converged\": %d, \"pcg_r0\": %.17g, \"U_sum\": %.17g, "
-                    "\"U_finite\": %d, \"levels\": %d}\n",
+                    "\"U_finite\": %d, \"levels\": %d, \"div_min\": %.17g, "
+                    "\"div_max\": %.17g, \"div_mean\": %.17g}\n",
                     n, r_ref.num_iterations, (int)r_ref.converged, r_ref.initial_residual,
                     r_ref.final_residual, r_ns.num_iterations, (int)r_ns.converged,
                     r_ns.true_residual, r_ns.bnorm, r_pcg.iterations, (int)r_pcg.converged,
-                    r_pcg.initial_residual, usum, (int)finite, hier.num_levels());
+                    r_pcg.initial_residual, usum, (int)finite, hier.num_levels(), dmin, dmax,
+                    dmean);
 
         // errors surface as exceptions (BCSpec validation, cuda_check.cuh:12-18)
         BCSpec bad = bc;

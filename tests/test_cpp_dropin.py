@@ -4,7 +4,8 @@ pipeline calls the head solve, checked against the CPU oracle.
 CPU: every header compiles on its own. GPU: tests/cpp/dropin_solve.cpp (built
 by macroflow3d_amd.build) runs solve_head with the reference and extension
 semantics, both velocity layouts, and the generic pcg_solve over
-VarCoeffLaplacian + MultigridPreconditioner, and the extensions (mixed
+VarCoeffLaplacian + MultigridPreconditioner, the velocity diagnostics, and the
+extensions (mixed
 precision, solve_head_batched, WarmStartNet); results are compared with the
 oracle on the same K."""
 from __future__ import annotations
@@ -74,6 +75,17 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     Up = rd("Up").reshape(n + 1, n + 1, n + 1)
     assert np.array_equal(Up[:n, :n, :], rd("U").reshape(n, n, n + 1))
     assert np.abs(info["U_sum"] - U.sum()) <= 1e-10 * abs(U.sum()) and info["U_finite"]
+    # velocity diagnostics of the padded field vs the oracle's on its own padded
+    # velocity of the same head; the divergence is the solve's residual (tiny)
+    Uo, Vo, Wo = O.velocity_padded(g, bc, h_ns, K)
+    dref, oref, href = O.velocity_diagnostics(g, Uo, Vo, Wo)
+    scale = np.abs(U).max()
+    assert np.abs(rd("div") - dref).max() <= 1e-10 * scale
+    assert np.abs(rd("omag") - oref).max() <= 1e-10 * scale
+    assert np.abs(rd("hel") - href).max() <= 1e-10 * scale * scale
+    d = rd("div")
+    assert info["div_min"] == d.min() and info["div_max"] == d.max()
+    assert abs(info["div_mean"] - d.mean()) <= 1e-12 * scale
     assert info["levels"] == 3
     # extensions: mixed precision reaches the FP64 residual bar, the batched
     # solve reproduces the single solves bit for bit, the C++ network loader
