@@ -238,7 +238,12 @@ enum {
     MF3D_TUNE_SWEEP_BLOCK = 5,          /* threads per workgroup of the colour-split sweeps
                                            (64..512, multiple of 64; default 256) */
     MF3D_TUNE_MARCH_MAX_CELLS_FP64 = 6, /* ... and <= this many (default: no limit) */
-    MF3D_TUNE_MARCH_MAX_CELLS_FP32 = 7
+    MF3D_TUNE_MARCH_MAX_CELLS_FP32 = 7,
+    MF3D_TUNE_X_OVERLAP = 8,            /* L > 0: PCG x update on a second stream from V-cycle
+                                           level L on (overlapping the coarse levels);
+                                           0 (default): fused with the p update */
+    MF3D_TUNE_X_STEP_BLOCKS = 9         /* workgroups of that x update (0, default: one per
+                                           2048 cells) */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

@@ -126,6 +126,7 @@ struct IterState {
     int stop_norm; // MF3D_STOP_*
     int flexible;  // beta = (r_new, z_new - z_old) / (r_old, z_old)
     int xpend;     // x += alpha p still owed (x updated with the next p, see FIN_RR)
+    double xa;     // this iteration's x step: alpha once (r, r) was formed, else 0 (FIN_RR)
 };
 
 // Reduction finalisers (what thread 0 does with the reduced sum S).

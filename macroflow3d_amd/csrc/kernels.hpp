@@ -60,6 +60,10 @@ int pcg_update_xr(hipStream_t s, const IterState* st, const double* p, const dou
 int pcg_update_r(hipStream_t s, const IterState* st, const double* Ap, double* r, size_t n,
                  double* partials);
 int pcg_x_pending(hipStream_t s, const IterState* st, const double* p, double* x, size_t n);
+// x += IterState::xa p (the x step of the iteration whose (r, r) was just formed);
+// max_blocks > 0 caps the grid (workgroups then loop over the vector)
+int pcg_x_step(hipStream_t s, const IterState* st, const double* p, double* x, size_t n,
+               unsigned max_blocks = 0);
 int pcg_update_p(hipStream_t s, const IterState* st, const double* z, double* p, size_t n);
 // CG on -A: if !st->stop { x += alpha p; r -= alpha (-Ap) } with partials of (r, r)
 int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,

@@ -71,8 +71,12 @@ __device__ inline void finalize_apply(int op, double S, IterState* __restrict__ 
         if (!isfinite(st->alpha)) st->stop = 2;
         break;
     case FIN_RR: // pcg.cuh:156-164
-        if (st->stop) break;
+        if (st->stop) {
+            st->xa = 0.0;
+            break;
+        }
         st->xpend = 1;  // r is updated; x follows with the p update (or pcg_x_pending)
+        st->xa = st->alpha;  // ... or with pcg_x_step on a second stream
         st->rr = S;
         st->iter += 1;
         if (st->iter % st->check_every == 0 || st->iter >= st->max_iter) {

@@ -145,6 +145,20 @@ struct mf3d_workspace {
     int zchunk = 0;
     bool split = true;  // colour-split FP32 levels where the sizes allow
     int sweep_block = 256;  // threads per workgroup of the colour-split sweeps
+    // PCG with z kept in x32, optionally: the x step of each iteration runs on
+    // x_stream while the main stream works through the coarse levels of the
+    // V-cycle (forked when the V-cycle reaches level x_overlap, joined before
+    // the p update, which reads the p the x step also reads). Off by default:
+    // at 256^3 it was slower than the fused x/p update at every fork level
+    // and grid size tried (the x step takes CU slots and bandwidth from the
+    // level kernels it runs beside, and the cross-stream waits cost more
+    // than the idle bandwidth it fills).
+    int x_overlap = 0;  // 0: off; else the level at which the x step forks
+    unsigned x_step_blocks = 0;  // grid cap of the x step (0: one block per 2048 cells)
+    hipStream_t x_stream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool fork_pending = false;
+    double* fork_x = nullptr;
     size_t bytes = 0;
     std::vector<void*> owned;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -212,6 +226,9 @@ struct mf3d_workspace {
         if (kc_ev) (void)hipEventDestroy(kc_ev);
         if (inv_ev) (void)hipEventDestroy(inv_ev);
         if (coarse_stream) (void)hipStreamDestroy(coarse_stream);
+        if (x_stream) (void)hipStreamDestroy(x_stream);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -685,10 +702,22 @@ template <> struct LevelView<float> {
 // zero_init: x is zero on entry (the fills of v_cycle.cu:62 and
 // mg_preconditioner.cu:37 are folded into the first smoothing sweep); the
 // prolongation (v_cycle.cu:69) is folded into the first post-smoothing sweep.
+// enqueues the pending x step on x_stream after the work already on s
+int fork_x_step(mf3d_workspace* ws, hipStream_t s) {
+    ws->fork_pending = false;
+    MF3D_HIP_TRY(hipEventRecord(ws->fork_ev, s));
+    MF3D_HIP_TRY(hipStreamWaitEvent(ws->x_stream, ws->fork_ev, 0));
+    MF3D_TRY(kern::pcg_x_step(ws->x_stream, ws->st, ws->p, ws->fork_x, ws->nvec,
+                              ws->x_step_blocks));
+    MF3D_HIP_TRY(hipEventRecord(ws->join_ev, ws->x_stream));
+    return MF3D_OK;
+}
+
 template <typename T>
 int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool zero_init,
            const double* r64 = nullptr) {
     using V = LevelView<T>;
+    if ((int)l == ws->x_overlap && ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
     const size_t nl = ws->lv.size();
     Level& L = ws->lv[l];
     const double dx2 = L.g.dx * L.g.dx;
@@ -823,6 +852,15 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     const bool flexible = cfg.flexible != 0;
     const bool zx = z_in_x32(ws, flexible);  // z kept in the finest level's x32
     double* z = zx ? nullptr : ws->z;
+    const bool ovl = zx && ws->x_overlap > 0;  // x step on x_stream, see mf3d_workspace
+    ws->fork_pending = false;
+    if (ovl) {
+        if (!ws->x_stream)
+            MF3D_HIP_TRY(hipStreamCreateWithFlags(&ws->x_stream, hipStreamNonBlocking));
+        if (!ws->fork_ev) MF3D_HIP_TRY(hipEventCreateWithFlags(&ws->fork_ev, hipEventDisableTiming));
+        if (!ws->join_ev) MF3D_HIP_TRY(hipEventCreateWithFlags(&ws->join_ev, hipEventDisableTiming));
+        ws->fork_x = x;
+    }
     Level& L0 = ws->lv[0];
     // the reference's own rule (||r||/||r0||, FP64) keeps its exact semantics
     const bool judge_true =
@@ -859,15 +897,24 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                     MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
                     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZO, ws->st, nullptr));
                 }
+                ws->fork_pending = ovl;
                 MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials));
                 MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
-                if (zx) MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
-                else MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
+                if (ovl) {
+                    if (ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
+                    MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->join_ev, 0));
+                    MF3D_TRY(kern::update_p_split8(s, ws->st, L0.x32, L0.g, ws->p, false));
+                } else if (zx) {
+                    MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
+                } else {
+                    MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
+                }
             }
             MF3D_TRY(read_state(ws, s));
         }
         // the x += alpha p of the iteration that stopped after its r update
-        if (zx) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
+        // (with the overlapped x step every iteration has done its own)
+        if (zx && !ovl) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
         if (ws->host_st->stop != 1 || restarts >= 3 || !judge_true) break;
         // converged on the recursive residual: judge by the true one
         MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials));
@@ -1745,6 +1792,11 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     case MF3D_TUNE_MARCH_MAX_CELLS_FP32: ws->march_max_cells[1] = value; break;
     case MF3D_TUNE_ZCHUNK: ws->zchunk = (int)value; break;
     case MF3D_TUNE_SPLIT: ws->split = value != 0; break;
+    case MF3D_TUNE_X_OVERLAP:
+        if (value < 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "x overlap level must be >= 0");
+        ws->x_overlap = (int)value;
+        break;
+    case MF3D_TUNE_X_STEP_BLOCKS: ws->x_step_blocks = (unsigned)std::max(0L, value); break;
     case MF3D_TUNE_SWEEP_BLOCK:
         if (value < 64 || value > 512 || (value & 63))
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");

@@ -186,6 +186,8 @@ class FlowWorkspace:
     TUNE_SWEEP_BLOCK = 5
     TUNE_MARCH_MAX_CELLS_FP64 = 6
     TUNE_MARCH_MAX_CELLS_FP32 = 7
+    TUNE_X_OVERLAP = 8
+    TUNE_X_STEP_BLOCKS = 9
 
     def set_tuning(self, key: int, value: int) -> None:
         """Smoother selection knobs (levels with >= value cells use the one-pass
