@@ -125,6 +125,10 @@ struct PaddedVelocityField {
                 b->resize(s);
                 hip_check(hipMemset(b->data(), 0, s * sizeof(real)), "PaddedVelocityField");
             }
+            // hipMemset runs on the null stream, which does not order against
+            // the contexts' non-blocking streams: finish it before any kernel
+            // writes the faces
+            hip_check(hipDeviceSynchronize(), "PaddedVelocityField");
         }
     }
     void resize(const Grid3D& g) {

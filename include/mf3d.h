@@ -138,8 +138,9 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
  * covers all realisations; one PCG iteration is captured once into a hipGraph
  * and replayed between host checks (flags & MF3D_BATCH_NO_GRAPH: plain
  * launches). Requires PCG_MG with the FP32 V-cycle, the direct coarsest solve,
- * pre/post smoothing >= 1, flexible = 0, and nx % 8 == 0 on every
- * colour-swept level. Each realisation stops on its own tolerance; results[r]
+ * pre/post smoothing >= 1, flexible = 0, nx % 8 == 0 on every
+ * colour-swept level and 16-byte aligned heads (other cases: status
+ * MF3D_ERR_UNSUPPORTED). Each realisation stops on its own tolerance; results[r]
  * holds its iterations and residuals (setup_ms/solve_ms: the whole batch).
  * Synchronises. */
 enum { MF3D_BATCH_NO_GRAPH = 1 };
@@ -257,7 +258,8 @@ int mf3d_mg_smooth(mf3d_workspace* ws, mf3d_stream stream, int level, int num_it
 int mf3d_mg_num_levels(const mf3d_workspace* ws);
 
 /* solvers::MultigridPreconditioner::apply (src/numerics/solvers/mg_preconditioner.cuh:44-71,
- * mg_preconditioner.cu:16-45): z = one V-cycle (v_cycle.cu:36-74) from x = 0 with b = r. */
+ * mg_preconditioner.cu:16-45): z = one V-cycle (v_cycle.cu:36-74) from x = 0 with b = r.
+ * r and z must be 16-byte aligned (MF3D_ERR_INVALID_ARGUMENT otherwise). */
 int mf3d_mg_precondition(mf3d_workspace* ws, mf3d_stream stream, const double* r, double* z,
                          size_t n);
 

@@ -182,9 +182,7 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
     BcTypes bc;
     int pin;
     double inv_dx2;
-    const double* fx;  // FP64 faces of the finest level
-    const double* fy;
-    const double* fz;
+    const double* hk;  // 1 / (2 K) of the finest level (FP64 operator)
     double* x;
     double* r;
     double* z;
@@ -197,6 +195,7 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
     unsigned np;      // partials of the 256-cell reductions (reduce_blocks)
     unsigned np_op;   // partials of the operator (op_partials)
     int otiles, okz;  // operator geometry (op_geom)
+    int opairs;       // operator on cell pairs (op_geom)
     IterState* st;
     double* out;  // scalar result of a FIN_STORE / FIN_SQRT second stage
 };

@@ -933,15 +933,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // face coefficients of a colour-split level from K (values as faces_build_kernel);
-// with F64, also the unscaled FP64 faces in the natural MAC layout (FX, FY, FZ)
-// from the same harmonic means (the finest level of the mixed-precision solve)
-template <typename TC, bool F64>
+// with HK, also H = 1 / (2 K) of the FP64 operator (the finest level of the
+// mixed-precision solve)
+template <typename TC, bool HK>
 __global__ void __launch_bounds__(kBlock)
     faces_build_split_kernel(Dims d, BcTypes bc, double cs, const double* __restrict__ K,
                              TC* __restrict__ LX, TC* __restrict__ LY, TC* __restrict__ LZ,
                              TC* __restrict__ UX, TC* __restrict__ UY, TC* __restrict__ UZ,
-                             double* __restrict__ FX, double* __restrict__ FY,
-                             double* __restrict__ FZ) {
+                             double* __restrict__ H) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const long n = (long)nx * ny * nz;
     const long blk = xcd_block((n + kBlock - 1) / kBlock);
@@ -956,38 +955,40 @@ __global__ void __launch_bounds__(kBlock)
     const long so = split_of(d).at(d, i, j, k);
     const double Kc = K[o];
     const double iKc = 1.0 / Kc;
+    if (HK) H[o] = 0.5 * iKc;
     double f;
     if (i > 0) f = 2.0 / (iKc + 1.0 / K[o - 1]);
     else f = bc.t[0] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + nx - 1]) : bface(bc.t[0], Kc);
     cstore(LX, so, f * cs);
-    if (F64) FX[d.fx(i, j, k)] = f;
     if (j > 0) f = 2.0 / (iKc + 1.0 / K[o - sy]);
     else f = bc.t[2] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(ny - 1) * sy])
                                          : bface(bc.t[2], Kc);
     cstore(LY, so, f * cs);
-    if (F64) FY[d.fy(i, j, k)] = f;
     if (k > 0) f = 2.0 / (iKc + 1.0 / K[o - sz]);
     else f = bc.t[4] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(nz - 1) * sz])
                                          : bface(bc.t[4], Kc);
     cstore(LZ, so, f * cs);
-    if (F64) FZ[o] = f;
     if (i == nx - 1) {
         f = bc.t[1] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (nx - 1)]) : bface(bc.t[1], Kc);
         cstore(UX, j + (long)ny * k, f * cs);
-        if (F64) FX[d.fx(nx, j, k)] = f;
     }
     if (j == ny - 1) {
         f = bc.t[3] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(ny - 1) * sy])
                                         : bface(bc.t[3], Kc);
         cstore(UY, i + (long)nx * k, f * cs);
-        if (F64) FY[d.fy(i, ny, k)] = f;
     }
     if (k == nz - 1) {
         f = bc.t[5] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(nz - 1) * sz])
                                         : bface(bc.t[5], Kc);
         cstore(UZ, i + (long)nx * j, f * cs);
-        if (F64) FZ[o + sz] = f;
     }
+}
+
+// H = 1 / (2 K) (0.5 times the rounded 1 / K, exact scaling)
+__global__ void __launch_bounds__(kBlock)
+    half_recip_kernel(const double* __restrict__ K, double* __restrict__ H, long n) {
+    const long o = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (o < n) H[o] = 0.5 * (1.0 / K[o]);
 }
 
 // ---- prolongation for levels without the march kernel -----------------------
@@ -1010,24 +1011,57 @@ __global__ void __launch_bounds__(kBlock)
 
 // A x = -sum f (x_C - x_nb) / dx^2 (varcoeff_laplacian.cu:38-239), summed
 // x-, x+, y-, y+, z-, z+; the pin's diagonal counts in-domain neighbours only
-// (varcoeff_laplacian.cu:110-234). Evaluated by opz_body below.
-
-// MODE 0: y = A x, partials of (x, y).  MODE 1: r = b - A x, partials of (r, r).
+// (varcoeff_laplacian.cu:110-234).
 //
-// A workgroup owns a plane tile (kBlock consecutive cells of an xy plane) and
-// marches kz planes up in z. Each thread keeps its column's x at k-1, k, k+1
-// and the z face below k in registers, so per cell it loads the next plane's
-// x and z face, the four in-plane neighbours and the four in-plane faces, and
-// its indices are advanced by adding plane strides (no per-cell division).
-// Workgroup blk covers tile blk % tiles of z-chunk blk / tiles and writes one
-// partial (its column sums in z order, then the block tree). Each cell's value
-// is that of apply_cell (same faces, same sum order).
+// The face coefficients are not stored: with h = 1 / (2 K) per cell (8 bytes
+// per cell instead of 24 for three FP64 face arrays) the harmonic face is
+// f = 2 Ka Kb / (Ka + Kb) = 1 / (ha + hb), a Dirichlet face (2 K_C, the
+// ghost value lives in b) is 1 / h_C, i.e. the same form with h_nb = 0, and
+// a Neumann face is 0. Periodic faces pair with the wrapped cell.
+//
+// 1/s by the hardware reciprocal estimate refined with two Newton-Raphson
+// steps r <- r + r (1 - s r) (each step squares the relative error); for the
+// positive, finite sums here the result is within a few ulp of 1/s.
+__device__ __forceinline__ double rcp_nr(double s) {
+    double r = __builtin_amdgcn_rcp(s);
+    double e = fma(-s, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-s, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Boundary of one side of a cell: the neighbour's offset (0 without one), and
+// whether the neighbour exists (in the domain or periodic) and the face does
+// (neighbour or Dirichlet).
+struct Side {
+    long off;
+    bool nb, face;
+};
+__device__ __forceinline__ Side side(bool inside, long in_off, int t, long wrap_off) {
+    Side s;
+    s.nb = inside || t == MF3D_BC_PERIODIC;
+    s.face = s.nb || t == MF3D_BC_DIRICHLET;
+    s.off = inside ? in_off : (t == MF3D_BC_PERIODIC ? wrap_off : 0);
+    return s;
+}
+// face coefficient between a cell (hc) and a side (hn, loaded at the side's
+// offset; ignored without a neighbour)
+__device__ __forceinline__ double hface(const Side& s, double hc, double hn) {
+    return s.face ? rcp_nr(hc + (s.nb ? hn : 0.0)) : 0.0;
+}
+
+// MODE 0: y = A x, partials of (x, y).  MODE 1: r = b - A x (y may be null),
+// partials of (r, r). One partial per workgroup: the column sums in z order,
+// then the block tree.
+//
+// Single-cell variant (odd nx): a workgroup owns kBlock consecutive cells of
+// an xy plane and marches kz planes up in z, each thread keeping its column's
+// x and h at k-1, k, k+1 and the z face below k in registers; the next plane's
+// loads are issued before the current plane is computed.
 template <int MODE>
 __device__ __forceinline__ void opz_body(const Dims& d, const BcTypes& bc, int pin, double inv_dx2,
                                          int tiles, int kz, const double* __restrict__ x,
-                                         const double* __restrict__ FX,
-                                         const double* __restrict__ FY,
-                                         const double* __restrict__ FZ,
+                                         const double* __restrict__ H,
                                          const double* __restrict__ b, double* __restrict__ y,
                                          double* __restrict__ partials, long blk) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
@@ -1041,52 +1075,53 @@ __device__ __forceinline__ void opz_body(const Dims& d, const BcTypes& bc, int p
         const long sz = P;
         const int k0 = chunk * kz;
         const int k1 = min(nz, k0 + kz);
-        const bool xper_lo = bc.t[0] == MF3D_BC_PERIODIC, xper_hi = bc.t[1] == MF3D_BC_PERIODIC;
-        const bool yper_lo = bc.t[2] == MF3D_BC_PERIODIC, yper_hi = bc.t[3] == MF3D_BC_PERIODIC;
-        const bool zper_lo = bc.t[4] == MF3D_BC_PERIODIC, zper_hi = bc.t[5] == MF3D_BC_PERIODIC;
+        const Side E = side(i < nx - 1, 1, bc.t[1], -(long)(nx - 1));
+        const Side Wt = side(i > 0, -1, bc.t[0], (long)(nx - 1));
+        const Side N = side(j < ny - 1, nx, bc.t[3], -(long)(ny - 1) * nx);
+        const Side S = side(j > 0, -(long)nx, bc.t[2], (long)(ny - 1) * nx);
         long o = (long)k0 * sz + p;
-        long ofx = (long)i + (long)(nx + 1) * ((long)j + (long)ny * k0);
-        long ofy = (long)i + (long)nx * ((long)j + (long)(ny + 1) * k0);
-        const long sfx = (long)(nx + 1) * ny, sfy = (long)nx * (ny + 1);
-        double xm = k0 > 0 ? x[o - sz] : (zper_lo ? x[o + (long)(nz - 1) * sz] : 0.0);
-        double xc = x[o];
-        double fzl = FZ[o];  // FZ(i, j, k) sits at the cell index of (i, j, k)
-        // the loads of plane k + 1 are issued before plane k is computed
+        const Side D0 = side(k0 > 0, -sz, bc.t[4], (long)(nz - 1) * sz);
+        double xc = x[o], hc = H[o];
+        const double xd = x[o + D0.off], hd = H[o + D0.off];
+        double xm = D0.nb ? xd : 0.0;
+        double fzl = hface(D0, hc, hd);
         struct Ld {
-            double xp, fzu, v0, v1, v2, v3, f0, f1, f2, f3, bo;
+            double xp, hp, v0, v1, v2, v3, h0, h1, h2, h3, bo;
         };
-        auto load = [&](int k, long o, long ofx, long ofy) {
+        auto load = [&](int k, long o) {
+            const Side U = side(k < nz - 1, sz, bc.t[5], -(long)(nz - 1) * sz);
             Ld l;
-            l.xp = k < nz - 1 ? x[o + sz] : (zper_hi ? x[o - (long)(nz - 1) * sz] : 0.0);
-            l.fzu = FZ[o + sz];
-            l.v0 = i < nx - 1 ? x[o + 1] : (xper_hi ? x[o - (nx - 1)] : 0.0);
-            l.v1 = i > 0 ? x[o - 1] : (xper_lo ? x[o + (nx - 1)] : 0.0);
-            l.v2 = j < ny - 1 ? x[o + nx] : (yper_hi ? x[o - (long)(ny - 1) * nx] : 0.0);
-            l.v3 = j > 0 ? x[o - nx] : (yper_lo ? x[o + (long)(ny - 1) * nx] : 0.0);
-            l.f0 = FX[ofx + 1];
-            l.f1 = FX[ofx];
-            l.f2 = FY[ofy + nx];
-            l.f3 = FY[ofy];
+            l.xp = x[o + U.off];
+            l.hp = H[o + U.off];
+            l.v0 = x[o + E.off];
+            l.v1 = x[o + Wt.off];
+            l.v2 = x[o + N.off];
+            l.v3 = x[o + S.off];
+            l.h0 = H[o + E.off];
+            l.h1 = H[o + Wt.off];
+            l.h2 = H[o + N.off];
+            l.h3 = H[o + S.off];
             l.bo = MODE == 1 ? b[o] : 0.0;
             return l;
         };
-        Ld cur = load(k0, o, ofx, ofy);
+        Ld cur = load(k0, o);
         for (int k = k0; k < k1; ++k) {
             Ld nxt = cur;
-            if (k + 1 < k1) nxt = load(k + 1, o + sz, ofx + sfx, ofy + sfy);
-            const double xp = cur.xp, fzu = cur.fzu;
-            const double v0 = cur.v0, v1 = cur.v1, v2 = cur.v2, v3 = cur.v3;
-            const double f0 = cur.f0, f1 = cur.f1, f2 = cur.f2, f3 = cur.f3;
+            if (k + 1 < k1) nxt = load(k + 1, o + sz);
+            const Side U = side(k < nz - 1, sz, bc.t[5], -(long)(nz - 1) * sz);
+            const double f0 = hface(E, hc, cur.h0), f1 = hface(Wt, hc, cur.h1);
+            const double f2 = hface(N, hc, cur.h2), f3 = hface(S, hc, cur.h3);
+            const double fzu = hface(U, hc, cur.hp);
+            const double v0 = E.nb ? cur.v0 : 0.0, v1 = Wt.nb ? cur.v1 : 0.0;
+            const double v2 = N.nb ? cur.v2 : 0.0, v3 = S.nb ? cur.v3 : 0.0;
+            const double xp = U.nb ? cur.xp : 0.0;
             double Ax = f1 * (xc - v1) + f0 * (xc - v0) + f3 * (xc - v3) + f2 * (xc - v2) +
                         fzl * (xc - xm) + fzu * (xc - xp);
-            if (pin && o == 0) {
+            if (pin && o == 0) {  // in-domain faces of cell 0: x+, y+, z+
                 double aC = 0.0;
-                if (i < nx - 1) aC += f0;
-                if (i > 0) aC += f1;
-                if (j < ny - 1) aC += f2;
-                if (j > 0) aC += f3;
-                if (k < nz - 1) aC += fzu;
-                if (k > 0) aC += fzl;
+                if (nx > 1) aC += f0;
+                if (ny > 1) aC += f2;
+                if (nz > 1) aC += fzu;
                 Ax += aC * xc;
             }
             Ax = -Ax * inv_dx2;
@@ -1100,10 +1135,9 @@ __device__ __forceinline__ void opz_body(const Dims& d, const BcTypes& bc, int p
             }
             xm = xc;
             xc = xp;
+            hc = U.nb ? cur.hp : 0.0;
             fzl = fzu;
             o += sz;
-            ofx += sfx;
-            ofy += sfy;
             cur = nxt;
         }
     }
@@ -1111,16 +1145,141 @@ __device__ __forceinline__ void opz_body(const Dims& d, const BcTypes& bc, int p
     if (threadIdx.x == 0) partials[blk] = s;
 }
 
+// Paired variant (even nx): each thread owns two x-adjacent cells and moves
+// them with 16-byte loads and stores, which halves the vector-memory
+// instructions per cell (the single-cell kernel is bound by address
+// processing, not by HBM). A workgroup owns 2 kBlock consecutive cells of an
+// xy plane and marches kz planes up in z. The x neighbours across a pair
+// boundary come from the adjacent lanes; the first and last lane of a wave,
+// and periodic wraps, load them. Per cell: the same faces and the same sum
+// order as opz_body.
+typedef double __attribute__((ext_vector_type(2))) dbl2;
+__device__ __forceinline__ dbl2 ld2(const double* __restrict__ q, long off) {
+    return *reinterpret_cast<const dbl2*>(q + off);
+}
+
 template <int MODE>
+__device__ __forceinline__ void opz2_body(const Dims& d, const BcTypes& bc, int pin,
+                                          double inv_dx2, int tiles, int kz,
+                                          const double* __restrict__ x,
+                                          const double* __restrict__ H,
+                                          const double* __restrict__ b, double* __restrict__ y,
+                                          double* __restrict__ partials, long blk) {
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int P = nx * ny;
+    const int chunk = (int)(blk / tiles);
+    const int p = (int)(blk - (long)chunk * tiles) * (2 * kBlock) + 2 * (int)threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const bool act = p < P;
+    const int pc = act ? p : 0;  // idle lanes read a valid pair and store nothing
+    const int j = pc / nx;
+    const int i = pc - j * nx;  // even
+    const long sz = P;
+    const int k0 = chunk * kz;
+    const int k1 = min(nz, k0 + kz);
+    // x sides of the pair: west of cell i, east of cell i + 1
+    const Side Wt = side(i > 0, -1, bc.t[0], (long)(nx - 1));
+    const Side E = side(i + 2 < nx, 2, bc.t[1], 2 - (long)nx);
+    // lanes that load the x neighbour instead of taking it from the next lane
+    const bool wload = Wt.nb && (lane == 0 || i == 0);
+    const bool eload = E.nb && (lane == 63 || i + 2 == nx);
+    const Side N = side(j < ny - 1, nx, bc.t[3], -(long)(ny - 1) * nx);
+    const Side S = side(j > 0, -(long)nx, bc.t[2], (long)(ny - 1) * nx);
+    long o = (long)k0 * sz + pc;
+    const Side D0 = side(k0 > 0, -sz, bc.t[4], (long)(nz - 1) * sz);
+    dbl2 xc = ld2(x, o), hc = ld2(H, o);
+    dbl2 xm = ld2(x, o + D0.off);
+    const dbl2 hd = ld2(H, o + D0.off);
+    if (!D0.nb) xm = dbl2{0.0, 0.0};
+    dbl2 fzl = dbl2{hface(D0, hc.x, hd.x), hface(D0, hc.y, hd.y)};
+    struct Ld {
+        dbl2 xp, hp, xn, xs, hn, hs, bo;
+        double xw, hw, xe, he;
+    };
+    auto load = [&](int k, long o) {
+        const Side U = side(k < nz - 1, sz, bc.t[5], -(long)(nz - 1) * sz);
+        Ld l;
+        l.xp = ld2(x, o + U.off);
+        l.hp = ld2(H, o + U.off);
+        l.xn = ld2(x, o + N.off);
+        l.hn = ld2(H, o + N.off);
+        l.xs = ld2(x, o + S.off);
+        l.hs = ld2(H, o + S.off);
+        l.bo = MODE == 1 ? ld2(b, o) : dbl2{0.0, 0.0};
+        l.xw = wload ? x[o + Wt.off] : 0.0;
+        l.hw = wload ? H[o + Wt.off] : 0.0;
+        l.xe = eload ? x[o + E.off] : 0.0;
+        l.he = eload ? H[o + E.off] : 0.0;
+        return l;
+    };
+    double acc = 0.0;
+    Ld cur = load(k0, o);
+    for (int k = k0; k < k1; ++k) {
+        Ld nxt = cur;
+        if (k + 1 < k1) nxt = load(k + 1, o + sz);
+        const Side U = side(k < nz - 1, sz, bc.t[5], -(long)(nz - 1) * sz);
+        double xw = __shfl_up(xc.y, 1, 64), hw = __shfl_up(hc.y, 1, 64);
+        double xe = __shfl_down(xc.x, 1, 64), he = __shfl_down(hc.x, 1, 64);
+        if (wload) xw = cur.xw, hw = cur.hw;
+        if (eload) xe = cur.xe, he = cur.he;
+        if (!Wt.nb) xw = 0.0;
+        if (!E.nb) xe = 0.0;
+        const double fw = hface(Wt, hc.x, hw);
+        const double fm = rcp_nr(hc.x + hc.y);  // between the two cells
+        const double fe = hface(E, hc.y, he);
+        const double fn0 = hface(N, hc.x, cur.hn.x), fn1 = hface(N, hc.y, cur.hn.y);
+        const double fs0 = hface(S, hc.x, cur.hs.x), fs1 = hface(S, hc.y, cur.hs.y);
+        const double fu0 = hface(U, hc.x, cur.hp.x), fu1 = hface(U, hc.y, cur.hp.y);
+        const double n0 = N.nb ? cur.xn.x : 0.0, n1 = N.nb ? cur.xn.y : 0.0;
+        const double s0 = S.nb ? cur.xs.x : 0.0, s1 = S.nb ? cur.xs.y : 0.0;
+        const double u0 = U.nb ? cur.xp.x : 0.0, u1 = U.nb ? cur.xp.y : 0.0;
+        double A0 = fw * (xc.x - xw) + fm * (xc.x - xc.y) + fs0 * (xc.x - s0) + fn0 * (xc.x - n0) +
+                    fzl.x * (xc.x - xm.x) + fu0 * (xc.x - u0);
+        double A1 = fm * (xc.y - xc.x) + fe * (xc.y - xe) + fs1 * (xc.y - s1) + fn1 * (xc.y - n1) +
+                    fzl.y * (xc.y - xm.y) + fu1 * (xc.y - u1);
+        if (pin && o == 0) {  // in-domain faces of cell 0: x+ (fm), y+, z+
+            double aC = fm;
+            if (ny > 1) aC += fn0;
+            if (nz > 1) aC += fu0;
+            A0 += aC * xc.x;
+        }
+        A0 = -A0 * inv_dx2;
+        A1 = -A1 * inv_dx2;
+        if (MODE == 0) {
+            if (act) {
+                *reinterpret_cast<dbl2*>(y + o) = dbl2{A0, A1};
+                acc += xc.x * A0;
+                acc += xc.y * A1;
+            }
+        } else {
+            const double r0 = -1.0 * A0 + cur.bo.x, r1 = -1.0 * A1 + cur.bo.y;
+            if (act) {
+                if (y) *reinterpret_cast<dbl2*>(y + o) = dbl2{r0, r1};
+                acc += r0 * r0;
+                acc += r1 * r1;
+            }
+        }
+        xm = xc;
+        xc = dbl2{u0, u1};
+        hc = U.nb ? cur.hp : dbl2{0.0, 0.0};
+        fzl = dbl2{fu0, fu1};
+        o += sz;
+        cur = nxt;
+    }
+    const double s = block_sum<kBlock>(acc);
+    if (threadIdx.x == 0) partials[blk] = s;
+}
+
+template <int MODE, bool PAIRS>
 __global__ void __launch_bounds__(kBlock)
     op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, int tiles, int kz, long nblocks,
-              const double* __restrict__ x, const double* __restrict__ FX,
-              const double* __restrict__ FY, const double* __restrict__ FZ,
+              const double* __restrict__ x, const double* __restrict__ H,
               const double* __restrict__ b, double* __restrict__ y,
               double* __restrict__ partials) {
     const long blk = xcd_block(nblocks);
     if (blk < 0) return;  // padding block: uniform exit before the block reduction
-    opz_body<MODE>(d, bc, pin, inv_dx2, tiles, kz, x, FX, FY, FZ, b, y, partials, blk);
+    if (PAIRS) opz2_body<MODE>(d, bc, pin, inv_dx2, tiles, kz, x, H, b, y, partials, blk);
+    else opz_body<MODE>(d, bc, pin, inv_dx2, tiles, kz, x, H, b, y, partials, blk);
 }
 
 template <typename T>
@@ -1477,12 +1636,22 @@ __global__ void __launch_bounds__(kBlock)
     const BatchVec& V = Vs[e];
     if (MODE == 0 && V.st->stop) return;
     const Dims d{V.nx, V.ny, V.nz};
-    if (MODE == 0)
-        opz_body<0>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.p, V.fx, V.fy, V.fz, nullptr,
-                    V.Ap, V.partials, blk - start[e]);
-    else
-        opz_body<1>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.x, V.fx, V.fy, V.fz, V.rhs,
-                    V.r, V.partials, blk - start[e]);
+    const long b = blk - start[e];
+    if (MODE == 0) {
+        if (V.opairs)
+            opz2_body<0>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.p, V.hk, nullptr, V.Ap,
+                         V.partials, b);
+        else
+            opz_body<0>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.p, V.hk, nullptr, V.Ap,
+                        V.partials, b);
+    } else {
+        if (V.opairs)
+            opz2_body<1>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.x, V.hk, V.rhs, V.r,
+                         V.partials, b);
+        else
+            opz_body<1>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.x, V.hk, V.rhs, V.r,
+                        V.partials, b);
+    }
 }
 
 MarchGeom geometry(const mf3d_grid& g, int zchunk) {
@@ -1522,15 +1691,14 @@ size_t coef_bytes(CoefType t) {
 namespace {
 template <typename TC>
 void launch_faces_split(hipStream_t s, const Dims& d, const BcTypes& bc, double cs,
-                        const double* K, FaceSet& f, unsigned gr, FaceSet* f64 = nullptr) {
-    if (f64)
+                        const double* K, FaceSet& f, unsigned gr, double* H = nullptr) {
+    if (H)
         faces_build_split_kernel<TC, true><<<gr, kBlock, 0, s>>>(
-            d, bc, cs, K, (TC*)f.fx, (TC*)f.fy, (TC*)f.fz, (TC*)f.ux, (TC*)f.uy, (TC*)f.uz,
-            (double*)f64->fx, (double*)f64->fy, (double*)f64->fz);
+            d, bc, cs, K, (TC*)f.fx, (TC*)f.fy, (TC*)f.fz, (TC*)f.ux, (TC*)f.uy, (TC*)f.uz, H);
     else
         faces_build_split_kernel<TC, false><<<gr, kBlock, 0, s>>>(
             d, bc, cs, K, (TC*)f.fx, (TC*)f.fy, (TC*)f.fz, (TC*)f.ux, (TC*)f.uy, (TC*)f.uz,
-            nullptr, nullptr, nullptr);
+            nullptr);
 }
 }  // namespace
 
@@ -1571,18 +1739,24 @@ int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
     return MF3D_OK;
 }
 
-int faces_build_pair(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
-                     const double* K, FaceSet& flo, FaceSet& f64) {
-    if (!flo.split || f64.split || f64.type != COEF_F64 || !split_ok(g))
-        return fail(MF3D_ERR_UNSUPPORTED, "paired faces: split low-precision + natural FP64");
+int faces_build_split_h(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
+                        const double* K, FaceSet& flo, double* H) {
+    if (!flo.split || !split_ok(g) || !H)
+        return fail(MF3D_ERR_UNSUPPORTED, "faces with H: colour-split low-precision faces");
     const Dims d = dims(g);
     const unsigned gr = xcd_grid(cells(g));
     switch (flo.type) {
-    case COEF_F32: launch_faces_split<float>(s, d, bc, cs, K, flo, gr, &f64); break;
-    case COEF_BF16: launch_faces_split<__hip_bfloat16>(s, d, bc, cs, K, flo, gr, &f64); break;
-    case COEF_F16: launch_faces_split<__half>(s, d, bc, cs, K, flo, gr, &f64); break;
+    case COEF_F32: launch_faces_split<float>(s, d, bc, cs, K, flo, gr, H); break;
+    case COEF_BF16: launch_faces_split<__hip_bfloat16>(s, d, bc, cs, K, flo, gr, H); break;
+    case COEF_F16: launch_faces_split<__half>(s, d, bc, cs, K, flo, gr, H); break;
     default: return fail(MF3D_ERR_UNSUPPORTED, "split faces are 32- or 16-bit");
     }
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int half_recip(hipStream_t s, const double* K, double* H, size_t n) {
+    half_recip_kernel<<<grid_for(n), kBlock, 0, s>>>(K, H, (long)n);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -1855,10 +2029,12 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
     }
 }
 
-OpGeom op_geom(const mf3d_grid& g) {
+OpGeom op_geom(const mf3d_grid& g, bool pairs) {
     OpGeom o;
+    o.pairs = pairs && !(g.nx & 1);
     const long P = (long)g.nx * g.ny;
-    o.tiles = (int)((P + kBlock - 1) / kBlock);
+    const long per = o.pairs ? 2 * kBlock : kBlock;
+    o.tiles = (int)((P + per - 1) / per);
     // z-chunks of up to 16 planes, shortened while the launch has fewer than
     // 2048 workgroups (8 per CU)
     o.kz = (int)std::max(1L, std::min(16L, (long)o.tiles * g.nz / 2048));
@@ -1867,29 +2043,42 @@ OpGeom op_geom(const mf3d_grid& g) {
 }
 
 unsigned op_partials(const mf3d_grid& g) {
-    const OpGeom o = op_geom(g);
-    return (unsigned)o.tiles * (unsigned)o.chunks;
+    const OpGeom a = op_geom(g, false), b = op_geom(g, true);
+    return (unsigned)std::max((long)a.tiles * a.chunks, (long)b.tiles * b.chunks);
 }
 
-int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
-             int mode, const double* x, const double* b, double* y, double* partials) {
-    if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "operator needs FP64 faces");
-    const size_t n = cells(g);
+bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y,
+                 const void* H) {
+    auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+    return !(g.nx & 1) && al(x) && al(b) && al(y) && al(H);
+}
+
+int op_apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* H,
+             int mode, const double* x, const double* b, double* y, double* partials,
+             unsigned* np, bool allow_pairs) {
     const double inv_dx2 = 1.0 / (g.dx * g.dx);
-    const double* FX = cptr<double>(F, 0);
-    const double* FY = cptr<double>(F, 1);
-    const double* FZ = cptr<double>(F, 2);
-    const OpGeom o = op_geom(g);
+    const OpGeom o = op_geom(g, allow_pairs && op_pairs_ok(g, x, b, y, H));
     const long nb = (long)o.tiles * o.chunks;
-    const unsigned grid = (unsigned)(((nb + 7) / 8) * 8);
-    (void)n;
-    if (mode == 0)
-        op_kernel<0><<<grid, kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, o.tiles, o.kz, nb,
-                                             x, FX, FY, FZ, b, y, partials);
-    else
-        op_kernel<1><<<grid, kBlock, 0, s>>>(dims(g), bc, pin ? 1 : 0, inv_dx2, o.tiles, o.kz, nb,
-                                             x, FX, FY, FZ, b, y, partials);
+    const unsigned grid = xcd_round((size_t)nb);
+    const Dims d = dims(g);
+    const int pn = pin ? 1 : 0;
+    if (mode == 0) {
+        if (o.pairs)
+            op_kernel<0, true><<<grid, kBlock, 0, s>>>(d, bc, pn, inv_dx2, o.tiles, o.kz, nb, x, H,
+                                                       b, y, partials);
+        else
+            op_kernel<0, false><<<grid, kBlock, 0, s>>>(d, bc, pn, inv_dx2, o.tiles, o.kz, nb, x,
+                                                        H, b, y, partials);
+    } else {
+        if (o.pairs)
+            op_kernel<1, true><<<grid, kBlock, 0, s>>>(d, bc, pn, inv_dx2, o.tiles, o.kz, nb, x, H,
+                                                       b, y, partials);
+        else
+            op_kernel<1, false><<<grid, kBlock, 0, s>>>(d, bc, pn, inv_dx2, o.tiles, o.kz, nb, x,
+                                                        H, b, y, partials);
+    }
     MF3D_HIP_TRY(hipGetLastError());
+    if (np) *np = (unsigned)nb;
     return MF3D_OK;
 }
 

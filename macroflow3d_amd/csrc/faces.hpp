@@ -44,10 +44,12 @@ size_t coef_bytes(CoefType t);
 // faces from K (cs: storage scale)
 int faces_build(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs, const double* K,
                 FaceSet& f);
-// one pass for both face sets of the finest mixed-precision level: flo
-// colour-split (scaled by cs), f64 natural FP64 (unscaled)
-int faces_build_pair(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
-                     const double* K, FaceSet& flo, FaceSet& f64);
+// one pass for the finest mixed-precision level: flo colour-split (scaled by
+// cs) and H = 1 / (2 K) of the FP64 operator
+int faces_build_split_h(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, double cs,
+                        const double* K, FaceSet& flo, double* H);
+// H = 1 / (2 K)
+int half_recip(hipStream_t s, const double* K, double* H, size_t n);
 bool march_ok(const mf3d_grid& g, const BcTypes& bc);
 int march_default_zchunk(const mf3d_grid& g);
 // iters GSRB iterations (red then black) with face coefficients; result in x.
@@ -80,13 +82,23 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
 // FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
 // mode 1: y = b - A x (y nullable), partials (y, y). It writes op_partials(g)
 // partials: one per (plane tile of 256 cells, z-chunk of kz planes).
+// FP64 operator of the finest level from h = 1 / (2 K): workgroups of
+// tiles x chunks (one partial each); pairs: two x-adjacent cells per thread
+// with 16-byte accesses (nx even and every vector 16-byte aligned)
 struct OpGeom {
     int tiles, kz, chunks;
+    bool pairs;
 };
-OpGeom op_geom(const mf3d_grid& g);
-unsigned op_partials(const mf3d_grid& g);
-int op_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const FaceSet& F,
-             int mode, const double* x, const double* b, double* y, double* partials);
+OpGeom op_geom(const mf3d_grid& g, bool pairs);
+unsigned op_partials(const mf3d_grid& g);  // the larger count of the two variants
+bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y,
+                 const void* H);
+// mode 0: y = A x with (x, y) partials; mode 1: y = b - A x (y nullable) with
+// (y, y) partials; np (nullable): the number of partials written; pairs only
+// when allowed and op_pairs_ok
+int op_apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* H,
+             int mode, const double* x, const double* b, double* y, double* partials,
+             unsigned* np, bool allow_pairs = true);
 int to_f32(hipStream_t s, const double* a, float* o, size_t n);
 int to_f64(hipStream_t s, const float* a, double* o, size_t n);
 // conversions between a natural FP64 vector and an FP32 vector of `order`

@@ -96,12 +96,14 @@ __device__ inline void finalize_apply(int op, double S, IterState* __restrict__ 
         if (st->stop) break;
         st->rzo = S;
         break;
-    case FIN_RESUME:
+    case FIN_RESUME: {  // true residual after a converged pass; vanishing as in FIN_R0
         st->xpend = 0;
         st->rr = S;
-        st->stop = (sqrt(S) / st->den < st->rtol) ? 1 : 0;
+        const double rn = sqrt(S);
+        st->stop = (rn < 1e-15 || rn / st->den < st->rtol) ? 1 : 0;
         if (!st->stop && st->iter >= st->max_iter) st->stop = 3;
         break;
+    }
     case FIN_CG_RR0: // cg.cuh:47-77: rr = (r,r), tol = r0 * rtol
         st->rz = S;
         st->rr = S;

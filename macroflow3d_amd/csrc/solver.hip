@@ -138,6 +138,13 @@ struct mf3d_workspace {
     int precision = MF3D_PRECOND_FP64;
     CoefType lo_type = COEF_BF16;
     const double* K0 = nullptr;
+    double* h0 = nullptr;  // 1 / (2 K) of the finest level: the FP64 operator's faces
+    // the operator may run on cell pairs (decided per solve from the vectors'
+    // alignment, the same way for single and batched solves)
+    bool op_pairs = true;
+    // aligned stand-in for a head vector that is not 16-byte aligned
+    double* hcopy = nullptr;
+    size_t hcopy_n = 0;
     bool ready = false;
     // tuning (mf3d_workspace_set_tuning / autotuner)
     long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
@@ -212,6 +219,9 @@ struct mf3d_workspace {
         partials_cap = 0;
         st = nullptr;
         dscalar = nullptr;
+        hcopy = nullptr;
+        hcopy_n = 0;
+        h0 = nullptr;
         ready = false;
         req_levels = 0;
         fine = mf3d_grid{};
@@ -554,17 +564,19 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
         MF3D_TRY(start_coarse_inverse(ws, s));
     else
         ws->coarse_pending = false;
+    if (!ws->h0) MF3D_TRY(ws->alloc(&ws->h0, cells(g)));
+    // the FP64 operator's h = 1 / (2 K); the finest mixed colour-split level
+    // builds it below, with its FP32/16-bit faces
+    if (!mixed || !(ws->split && kern::split_ok(g)))
+        MF3D_TRY(kern::half_recip(s, K, ws->h0, cells(g)));
     for (size_t l = 0; l < ws->lv.size(); ++l) {
         Level& L = ws->lv[l];
-        const bool want64 = (l == 0) || !mixed;  // the FP64 operator, the FP64 V-cycle
-        if (want64) {
+        if (!mixed) {  // FP64 faces of the FP64 V-cycle
             if (!L.has_f64) {
                 MF3D_TRY(ws->alloc_faces(L.f64, L.g, COEF_F64));
                 L.has_f64 = true;
             }
-            // the finest mixed level builds them below, with its FP32/16-bit faces
-            if (!(mixed && l == 0))
-                MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
+            MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
         }
         const long lo = ws->march_min_cells[mixed ? 1 : 0], hi = ws->march_max_cells[mixed ? 1 : 0];
         const bool march = kern::march_ok(L.g, ws->bct) && (long)L.n >= lo && (long)L.n <= hi;
@@ -590,9 +602,11 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
         }
         L.cs = cs;
         if (l == 0 && split) {
-            MF3D_TRY(kern::faces_build_pair(s, L.g, ws->bct, cs, level_K(ws, l), L.flo, L.f64));
+            if (!(ws->split && kern::split_ok(g)))
+                return fail(MF3D_ERR_UNSUPPORTED, "finest level split without split_ok");
+            MF3D_TRY(kern::faces_build_split_h(s, L.g, ws->bct, cs, level_K(ws, l), L.flo,
+                                               ws->h0));
         } else {
-            if (l == 0) MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
             MF3D_TRY(kern::faces_build(s, L.g, ws->bct, cs, level_K(ws, l), L.flo));
         }
         if (!L.x32) MF3D_TRY(ws->alloc(&L.x32, L.n));
@@ -773,15 +787,25 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
     return kern::to_f64(s, L0.x32, z, L0.g, L0.order, rz_partials ? r : nullptr, rz_partials);
 }
 
-// operator on the finest level with its FP64 faces
-int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, double* partials) {
+// whether the finest-level operator runs on cell pairs for a solve into x
+bool op_pairs_for(const mf3d_workspace* ws, const double* x) {
+    const mf3d_grid& g = ws->lv[0].g;
+    return kern::op_pairs_ok(g, x, ws->rhs, ws->r, ws->h0) &&
+           kern::op_pairs_ok(g, ws->p, nullptr, ws->Ap, ws->h0);
+}
+
+// FP64 operator on the finest level (faces from h0); np: partials written
+int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, double* partials,
+             unsigned* np = nullptr) {
     Level& L0 = ws->lv[0];
-    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 0, x, nullptr, y, partials);
+    return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 0, x, nullptr, y, partials, np,
+                          ws->op_pairs);
 }
 int residual_op(mf3d_workspace* ws, hipStream_t s, const double* x, const double* b, double* r,
-                double* partials) {
+                double* partials, unsigned* np = nullptr) {
     Level& L0 = ws->lv[0];
-    return kern::op_faces(s, L0.g, ws->bct, ws->pin, L0.f64, 1, x, b, r, partials);
+    return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 1, x, b, r, partials, np,
+                          ws->op_pairs);
 }
 
 IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
@@ -848,7 +872,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     (void)K;
     const size_t n = cells(g);
     const unsigned np = kern::reduce_blocks(n);
-    const unsigned npo = kern::op_partials(g);  // partials of the operator
+    unsigned npo = 0;  // partials of the last operator launch
     const bool flexible = cfg.flexible != 0;
     const bool zx = z_in_x32(ws, flexible);  // z kept in the finest level's x32
     double* z = zx ? nullptr : ws->z;
@@ -869,7 +893,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     MF3D_TRY(state_upload(ws, s, st0));
     MF3D_TRY(kern::dot_partials(s, ws->rhs, ws->rhs, n, ws->partials));
     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_BNORM, ws->st, nullptr));
-    MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials));
+    MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials, &npo));
     MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_R0, ws->st, nullptr));
     int restarts = 0;
     for (;;) {
@@ -885,7 +909,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             const double target = cfg.cg_rtol * ws->host_st->den;
             const int chunk = ch.next(*ws->host_st, target);
             for (int c = 0; c < chunk; ++c) {
-                MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials));
+                MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials, &npo));
                 MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_PAP, ws->st, nullptr));
                 if (zx)  // x is updated together with p below
                     MF3D_TRY(kern::pcg_update_r(s, ws->st, ws->Ap, ws->r, n, ws->partials));
@@ -917,7 +941,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
         if (zx && !ovl) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
         if (ws->host_st->stop != 1 || restarts >= 3 || !judge_true) break;
         // converged on the recursive residual: judge by the true one
-        MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials));
+        MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials, &npo));
         MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_RESUME, ws->st, nullptr));
         MF3D_TRY(read_state(ws, s));
         if (ws->host_st->stop == 1) break;  // the true residual meets the tolerance
@@ -959,9 +983,9 @@ int run_cg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     while (!ws->host_st->stop) {
         const int chunk = ch.next(*ws->host_st, cfg.cg_rtol * ws->host_st->den);
         for (int c = 0; c < chunk; ++c) {
-            MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials));
-            MF3D_TRY(kern::finalize(s, ws->partials, kern::op_partials(g), FIN_CG_PAP, ws->st,
-                                    nullptr));
+            unsigned npa = 0;
+            MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials, &npa));
+            MF3D_TRY(kern::finalize(s, ws->partials, npa, FIN_CG_PAP, ws->st, nullptr));
             MF3D_TRY(kern::cg_update_xr(s, ws->st, ws->p, ws->Ap, x, ws->r, n, ws->partials));
             MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_CG_RR, ws->st, nullptr));
             MF3D_TRY(kern::cg_update_p(s, ws->st, ws->r, ws->p, n));
@@ -1110,23 +1134,25 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
         if (L0.order != ORDER_SPLIT || (L0.g.nx & 7))
             return fail(MF3D_ERR_UNSUPPORTED,
                         "batched solve: the finest level must be colour-split with nx % 8 == 0");
+        if (reinterpret_cast<uintptr_t>(h[r]) & 15)
+            return fail(MF3D_ERR_UNSUPPORTED, "batched solve: heads must be 16-byte aligned");
         BatchVec v{};
         v.nx = L0.g.nx, v.ny = L0.g.ny, v.nz = L0.g.nz;
         v.bc = w->bct;
         v.pin = w->pin ? 1 : 0;
         v.inv_dx2 = 1.0 / (L0.g.dx * L0.g.dx);
-        v.fx = (const double*)L0.f64.fx;
-        v.fy = (const double*)L0.f64.fy;
-        v.fz = (const double*)L0.f64.fz;
+        v.hk = w->h0;
         v.x = h[r];
         v.r = w->r, v.z = w->z, v.p = w->p, v.Ap = w->Ap, v.rhs = w->rhs;
         v.b32 = L0.b32, v.x32 = L0.x32;
         v.partials = w->partials;
         v.np = kern::reduce_blocks(L0.n);
-        const kern::OpGeom og = kern::op_geom(L0.g);
-        v.np_op = kern::op_partials(L0.g);
+        // the geometry single solves use for the same vectors (op_apply)
+        const kern::OpGeom og = kern::op_geom(L0.g, op_pairs_for(w, v.x));
+        v.np_op = (unsigned)og.tiles * (unsigned)og.chunks;
         v.otiles = og.tiles;
         v.okz = og.kz;
+        v.opairs = og.pairs ? 1 : 0;
         v.st = w->st;
         v.out = w->dscalar;
         vd.push_back(v);
@@ -1453,11 +1479,23 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     MF3D_HIP_TRY(hipEventRecord(ws->ev[0], s));
     MF3D_TRY(mg_setup(ws, s, *grid, *bc, *cfg, K));
     MF3D_TRY(kern::rhs(s, *grid, *bc, K, ws->rhs));
+    // the kernels move FP64 vectors with 16-byte accesses: a head that is not
+    // 16-byte aligned (a view at an odd offset) is solved in a workspace copy
+    double* const hu = h;
+    if (reinterpret_cast<uintptr_t>(h) & 15) {
+        if (!ws->hcopy || ws->hcopy_n < nc) {
+            MF3D_TRY(ws->alloc(&ws->hcopy, nc));
+            ws->hcopy_n = nc;
+        }
+        if (cfg->init_guess == MF3D_GUESS_PROVIDED) MF3D_TRY(kern::copy(s, hu, ws->hcopy, nc));
+        h = ws->hcopy;
+    }
     // solve_head.cu:123 overwrites h with the reference guess; the extension
     // keeps a caller-provided guess.
     if (cfg->init_guess != MF3D_GUESS_PROVIDED)
         MF3D_TRY(kern::guess(s, *grid, *bc, cfg->init_guess, h));
     MF3D_HIP_TRY(hipEventRecord(ws->ev[1], s));
+    ws->op_pairs = op_pairs_for(ws, h);
     int st = MF3D_OK;
     if (cfg->solver_type == MF3D_SOLVER_PCG_MG)
         st = run_pcg(ws, s, *grid, *cfg, K, h, result);
@@ -1468,12 +1506,13 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     if (st != MF3D_OK) return st;
     MF3D_HIP_TRY(hipEventRecord(ws->ev[2], s));
     if (cfg->true_residual) {
-        MF3D_TRY(residual_op(ws, s, h, ws->rhs, nullptr, ws->partials));
-        MF3D_TRY(kern::finalize(s, ws->partials, kern::op_partials(*grid), FIN_SQRT, nullptr,
-                                ws->dscalar));
+        unsigned npt = 0;
+        MF3D_TRY(residual_op(ws, s, h, ws->rhs, nullptr, ws->partials, &npt));
+        MF3D_TRY(kern::finalize(s, ws->partials, npt, FIN_SQRT, nullptr, ws->dscalar));
         MF3D_HIP_TRY(hipMemcpyAsync(&result->true_residual, ws->dscalar, sizeof(double),
                                     hipMemcpyDeviceToHost, s));
     }
+    if (h != hu) MF3D_TRY(kern::copy(s, h, hu, nc));
     MF3D_HIP_TRY(hipStreamSynchronize(s));
     result->setup_ms = elapsed(ws->ev[0], ws->ev[1]);
     result->solve_ms = elapsed(ws->ev[1], ws->ev[2]);
@@ -1840,6 +1879,9 @@ int mf3d_mg_precondition(mf3d_workspace* ws, mf3d_stream stream, const double* r
     if (!ws->ready || !ws->K0)
         return fail(MF3D_ERR_INVALID_ARGUMENT, "mf3d_mg_setup has not been called");
     MF3D_CHECK_LEN(n, cells(ws->fine), "r/z");
+    // the FP32 V-cycle reads r and writes z with 16-byte accesses
+    if ((reinterpret_cast<uintptr_t>(r) | reinterpret_cast<uintptr_t>(z)) & 15)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "r and z must be 16-byte aligned");
     clear_stale_error();
     return precondition(ws, as_stream(stream), r, z);
 }

@@ -279,6 +279,47 @@ def test_periodic_and_dirichlet_solves(ctx, workspace, mix):
     assert rel_l2(h, ho) < 1e-9
 
 
+@pytest.mark.parametrize("mix", MIXES)
+@pytest.mark.parametrize("shape", [(15, 8, 6), (2, 6, 4), (40, 6, 4), (130, 4, 4)])
+def test_solve_operator_variants_match_oracle(ctx, workspace, mix, shape):
+    """The FP64 operator of the solve in both forms: single cells (odd nx, a
+    one-level direct-preconditioned solve) and cell pairs (even nx; rows of 20
+    and 65 pairs cross wave boundaries, 2 x 1 pairs is the smallest), every
+    boundary mix including the pinned singular ones. FP64 PCG to 1e-11 against the oracle's solve (faces 2/(1/a + 1/b)
+    there, 1/(h_a + h_b) here, h = 1/(2K))."""
+    K = random_K(int(np.prod(shape)), 21, 0.8)
+    print(f"operator variant {shape} {mix}", flush=True)
+    lv = 1 if shape[0] % 2 else 2  # odd sizes cannot be restricted: one (direct) level
+    cfg = ph.HeadSolveConfig(mg_levels=lv, cg_rtol=1e-11, coarse_solver=ph.CoarseSolver.DIRECT,
+                             stop_norm=ph.StopNorm.RHS_NORM, cg_check_every=1)
+    p = O.SolveParams(mg_levels=lv, cg_rtol=1e-11, coarse_solver=1, stop_norm=1, cg_check_every=1)
+    h, res, ho, ro = _solve_pair(ctx, workspace, shape, mix, cfg, p, K)
+    assert res.converged and ro.converged
+    assert abs(res.num_iterations - ro.num_iterations) <= 1
+    assert rel_l2(h, ho) < 1e-9
+    assert abs(res.initial_residual - ro.initial_residual) <= 1e-12 * ro.initial_residual
+
+
+def test_misaligned_head_uses_single_cell_operator(ctx, workspace):
+    """A head vector that is not 16-byte aligned (a view at an odd offset)
+    runs the single-cell operator; the solve matches the aligned one."""
+    shape = (32, 16, 16)
+    n = int(np.prod(shape))
+    K = random_K(n, 22, 0.8)
+    g, _ = grid_pair(*shape)
+    bc, _ = bc_pair("darcy_x")
+    cfg = ph.HeadSolveConfig.north_star(1e-11, mg_levels=3)
+    h = ctx.zeros(n)
+    r1 = ph.solve_head(h, dev(K, ctx), g, bc, cfg, ctx, workspace)
+    buf = ctx.zeros(n + 1)
+    hv = buf[1:]
+    assert hv.data_ptr() % 16 == 8
+    r2 = ph.solve_head(hv, dev(K, ctx), g, bc, cfg, ctx, workspace)
+    assert r1.converged and r2.converged
+    assert abs(r1.num_iterations - r2.num_iterations) <= 1
+    assert rel_l2(host(hv), host(h)) < 1e-10
+
+
 def test_provided_initial_guess_keeps_caller_head(ctx, workspace):
     """InitGuess.PROVIDED: the solve starts from the caller's h (warm start)."""
     K = random_K(16 ** 3, 16, 0.8)
