@@ -892,8 +892,11 @@ __device__ __forceinline__ float residual_cell_split(const Dims& d, const BcType
 }
 
 // residual + restriction of a colour-split fine level into a coarse level of
-// either order
-template <typename TC, bool CS>
+// either order. RED: after a red-black GS sweep that ended with the black
+// half-sweep, each black cell satisfies its own equation given its (final)
+// red neighbours, so its residual is zero up to FP32 rounding; only the four
+// red children of a coarse cell are evaluated.
+template <typename TC, bool CS, bool RED>
 __device__ __forceinline__ void rr_split_body(const Dims& d, const BcTypes& bc, int pin, float rs,
                                               const float* __restrict__ x,
                                               const float* __restrict__ b, const SFaces<TC>& F,
@@ -910,18 +913,25 @@ __device__ __forceinline__ void rr_split_body(const Dims& d, const BcTypes& bc, 
     const bool p = pin != 0;
     const Split sp = split_of(d);
     float s = 0.0f;
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j, k + 1, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j, k + 1, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k + 1, rs);
-    s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k + 1, rs);
+    if (RED) {  // children of colour 0: (0,0,0), (1,1,0), (1,0,1), (0,1,1)
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i, j, k, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j, k + 1, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k + 1, rs);
+    } else {
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i, j, k, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j, k, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i, j, k + 1, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j, k + 1, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k + 1, rs);
+        s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k + 1, rs);
+    }
     bcoarse[CS ? split_of(dc).at(dc, I, J, Kc) : t] = s / 8.0f;
 }
 
-template <typename TC, bool CS>
+template <typename TC, bool CS, bool RED>
 __global__ void __launch_bounds__(kBlock)
     residual_restrict_split_kernel(Dims d, BcTypes bc, int pin, float rs,
                                    const float* __restrict__ x, const float* __restrict__ b,
@@ -929,7 +939,7 @@ __global__ void __launch_bounds__(kBlock)
     const long total = (long)(d.nx >> 1) * (d.ny >> 1) * (d.nz >> 1);
     const long blk = xcd_block((total + kBlock - 1) / kBlock);
     if (blk < 0) return;
-    rr_split_body<TC, CS>(d, bc, pin, rs, x, b, F, bcoarse, blk);
+    rr_split_body<TC, CS, RED>(d, bc, pin, rs, x, b, F, bcoarse, blk);
 }
 
 // face coefficients of a colour-split level from K (values as faces_build_kernel);
@@ -1530,8 +1540,9 @@ __global__ void __launch_bounds__(kBlock)
     const int e = batch_entry(start, m, blk);
     const BatchLevel& L = Ls[e];
     if (L.st->stop) return;
-    rr_split_body<TC, CS>(Dims{L.nx, L.ny, L.nz}, L.bc, L.pin, L.rs, L.x, L.b,
-                          batch_sfaces<TC>(L), L.bcoarse, blk - start[e]);
+    // batched solves smooth at least once before each restriction
+    rr_split_body<TC, CS, true>(Dims{L.nx, L.ny, L.nz}, L.bc, L.pin, L.rs, L.x, L.b,
+                                batch_sfaces<TC>(L), L.bcoarse, blk - start[e]);
 }
 
 // small (single-workgroup) levels: one workgroup per entry
@@ -1931,17 +1942,26 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
     return MF3D_OK;
 }
 
-template <typename TC>
-int rr_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double rs,
-             const float* x, const float* b, const FaceSet& F, float* bcoarse, bool csplit) {
-    const size_t nc = (size_t)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
-    const Dims d = dims(g);
+template <typename TC, bool RED>
+void launch_rr_split(hipStream_t s, const Dims& d, const BcTypes& bc, bool pin, double rs,
+                     const float* x, const float* b, const FaceSet& F, float* bcoarse,
+                     bool csplit, size_t nc) {
     if (csplit)
-        residual_restrict_split_kernel<TC, true><<<xcd_grid(nc), kBlock, 0, s>>>(
+        residual_restrict_split_kernel<TC, true, RED><<<xcd_grid(nc), kBlock, 0, s>>>(
             d, bc, pin ? 1 : 0, (float)rs, x, b, sfaces<TC>(F), bcoarse);
     else
-        residual_restrict_split_kernel<TC, false><<<xcd_grid(nc), kBlock, 0, s>>>(
+        residual_restrict_split_kernel<TC, false, RED><<<xcd_grid(nc), kBlock, 0, s>>>(
             d, bc, pin ? 1 : 0, (float)rs, x, b, sfaces<TC>(F), bcoarse);
+}
+
+template <typename TC>
+int rr_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double rs,
+             const float* x, const float* b, const FaceSet& F, float* bcoarse, bool csplit,
+             bool red) {
+    const size_t nc = (size_t)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
+    const Dims d = dims(g);
+    if (red) launch_rr_split<TC, true>(s, d, bc, pin, rs, x, b, F, bcoarse, csplit, nc);
+    else launch_rr_split<TC, false>(s, d, bc, pin, rs, x, b, F, bcoarse, csplit, nc);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -1998,7 +2018,8 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
 
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const double* x, const double* b, const FaceSet& F,
-                            double* bcoarse, int order, int corder) {
+                            double* bcoarse, int order, int corder, bool red_only) {
+    (void)red_only;  // natural-order FP64 levels evaluate every child
     if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
         return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels are stored in natural order");
     return rr_t<double, double>(s, g, bc, pin, rs, x, b, F, bcoarse);
@@ -2006,16 +2027,17 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
 
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const float* x, const float* b, const FaceSet& F,
-                            float* bcoarse, int order, int corder) {
+                            float* bcoarse, int order, int corder, bool red_only) {
     if ((order == ORDER_SPLIT) != F.split)
         return fail(MF3D_ERR_UNSUPPORTED, "vector order and face layout differ");
     if (order == ORDER_SPLIT) {
         if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
         const bool cs = corder == ORDER_SPLIT;
         switch (F.type) {
-        case COEF_F32: return rr_split<float>(s, g, bc, pin, rs, x, b, F, bcoarse, cs);
-        case COEF_BF16: return rr_split<__hip_bfloat16>(s, g, bc, pin, rs, x, b, F, bcoarse, cs);
-        case COEF_F16: return rr_split<__half>(s, g, bc, pin, rs, x, b, F, bcoarse, cs);
+        case COEF_F32: return rr_split<float>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only);
+        case COEF_BF16:
+            return rr_split<__hip_bfloat16>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only);
+        case COEF_F16: return rr_split<__half>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only);
         default: return fail(MF3D_ERR_UNSUPPORTED, "FP32 residual needs 32- or 16-bit faces");
         }
     }

@@ -74,11 +74,13 @@ bool split_ok(const mf3d_grid& g);
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const double* x, const double* b, const FaceSet& F,
                             double* bcoarse, int order = ORDER_NATURAL,
-                            int corder = ORDER_NATURAL);
+                            int corder = ORDER_NATURAL, bool red_only = false);
+// red_only (colour-split levels after at least one red-black sweep): the
+// black children's residuals are taken as zero (see rr_split_body)
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const float* x, const float* b, const FaceSet& F,
                             float* bcoarse, int order = ORDER_NATURAL,
-                            int corder = ORDER_NATURAL);
+                            int corder = ORDER_NATURAL, bool red_only = false);
 // FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
 // mode 1: y = b - A x (y nullable), partials (y, y). It writes op_partials(g)
 // partials: one per (plane tile of 256 cells, z-chunk of kz planes).

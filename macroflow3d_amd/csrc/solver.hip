@@ -749,8 +749,10 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
     const int fo = V::order(L), co = V::order(C);
     MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->pre,
                                 zero_init, (const T*)nullptr, L.plan, fo, co, r64));
+    // after pre-smoothing that ends with a black half-sweep, the split
+    // levels evaluate only the red children's residuals
     MF3D_TRY(kern::residual_restrict_faces(s, L.g, ws->bct, ws->pin, 1.0 / (cs * dx2), x, b, F,
-                                           V::b(C), fo, co));
+                                           V::b(C), fo, co, ws->pre > 0));
     MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true));
     return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->post,
                               false, (const T*)V::x(C), L.plan, fo, co);
