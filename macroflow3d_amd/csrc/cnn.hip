@@ -336,32 +336,53 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- input, output, loss -------------------------------------------------------
-// x0 on the coarse grid from the fine K (or ln K) of the kS^3 block of each cell
+// x0 on the coarse grid from the fine K (or ln K) of the kS^3 block of each cell:
+// 0.5 mean ln K, 0.5 ln mean K, -0.5 ln mean 1/K, the x coordinate in [-1, 1].
+// The features end in BF16, so they are accumulated in FP32. A workgroup holds
+// 64 coarse cells x kS z-slices (one thread each); the slices' sums are added
+// in LDS in a fixed order.
 __global__ void __launch_bounds__(256)
     input_kernel(Geo g, const double* __restrict__ K, int is_log, __hip_bfloat16* __restrict__ x0) {
+    static_assert(kS == 4, "64 cells x 4 slices per workgroup");
     const int NX = g.nx / kS, NY = g.ny / kS, NZ = g.nz / kS;
-    const long V = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (V >= (long)NX * NY * NZ) return;
-    const int I = (int)(V % NX);
-    const long q = V / NX;
-    const int J = (int)(q % NY), Kc = (int)(q / NY);
-    double sl = 0.0, sk = 0.0, si = 0.0;
-    for (int dz = 0; dz < kS; ++dz)
+    const long nv = (long)NX * NY * NZ;
+    const int lane = (int)(threadIdx.x & 63), dz = (int)(threadIdx.x >> 6);
+    const long V = (long)blockIdx.x * 64 + lane;
+    float sl = 0.0f, sk = 0.0f, si = 0.0f;
+    int I = 0, J = 0, Kc = 0;
+    if (V < nv) {
+        I = (int)(V % NX);
+        const long q = V / NX;
+        J = (int)(q % NY);
+        Kc = (int)(q / NY);
+#pragma unroll
         for (int dy = 0; dy < kS; ++dy)
+#pragma unroll
             for (int dx = 0; dx < kS; ++dx) {
                 const long o = (long)(kS * I + dx) +
                                (long)g.nx * ((kS * J + dy) + (long)g.ny * (kS * Kc + dz));
-                const double l = is_log ? K[o] : log(K[o]);
-                const double k = is_log ? exp(K[o]) : K[o];
+                const float v = (float)K[o];
+                const float l = is_log ? v : __logf(v);
+                const float kv = is_log ? __expf(v) : v;
                 sl += l;
-                sk += k;
-                si += 1.0 / k;
+                sk += kv;
+                si += 1.0f / kv;
             }
-    const double w = 1.0 / (kS * kS * kS);
+    }
+    __shared__ float part[3][kS][64];
+    part[0][dz][lane] = sl;
+    part[1][dz][lane] = sk;
+    part[2][dz][lane] = si;
+    __syncthreads();
+    if (dz != 0 || V >= nv) return;
+    sl = ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
+    sk = ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) + part[1][3][lane];
+    si = ((part[2][0][lane] + part[2][1][lane]) + part[2][2][lane]) + part[2][3][lane];
+    const float w = 1.0f / (kS * kS * kS);
     __hip_bfloat16* o = x0 + V * 16;
-    o[0] = f2bf((float)(0.5 * sl * w));
-    o[1] = f2bf((float)(0.5 * log(sk * w)));
-    o[2] = f2bf((float)(-0.5 * log(si * w)));
+    o[0] = f2bf(0.5f * sl * w);
+    o[1] = f2bf(0.5f * __logf(sk * w));
+    o[2] = f2bf(-0.5f * __logf(si * w));
     o[3] = f2bf((float)(2.0 * (I + 0.5) / NX - 1.0));
 #pragma unroll
     for (int c = 4; c < 16; ++c) o[c] = f2bf(0.0f);
@@ -392,32 +413,46 @@ __global__ void __launch_bounds__(256)
 
 // h0 = h_lin + J trilinear(y) on the fine grid g (y: coarse, ych channels, channel 0).
 // Workgroup b covers segment b % segs (256 cells) of fine row b / segs, so j, k
-// and the y/z weights are uniform in a workgroup and each thread only derives i.
+// and the y/z weights are uniform in a workgroup; the segment's coarse values
+// (at most 66 along x for each of the 4 y/z corners) are staged in LDS first,
+// so each cell makes one coalesced store and no gathers from global memory.
 __global__ void __launch_bounds__(256)
-    output_kernel(Geo g, double hw, double he, const float* __restrict__ y, int ych,
-                  double* __restrict__ h) {
+    output_kernel(Geo g, double hw, double he, double inv_nx, const float* __restrict__ y,
+                  int ych, double* __restrict__ h) {
+    constexpr int kSpan = 256 / kS + 2;
+    __shared__ float cy[4][kSpan];
     const int segs = (g.nx + 255) / 256;
     const long row = (long)blockIdx.x / segs;
-    const int i = (int)((long)blockIdx.x - row * segs) * 256 + (int)threadIdx.x;
-    if (i >= g.nx) return;
+    const int i0 = (int)((long)blockIdx.x - row * segs) * 256;
+    const int i = i0 + (int)threadIdx.x;
     const int j = (int)(row % g.ny), k = (int)(row / g.ny);
     const int NX = g.nx / kS, NY = g.ny / kS, NZ = g.nz / kS;
-    int x0, x1, y0, y1, z0, z1;
-    float wx, wy, wz;
-    lin_axis(i, NX, x0, x1, wx);
+    int y0, y1, z0, z1;
+    float wy, wz;
     lin_axis(j, NY, y0, y1, wy);
     lin_axis(k, NZ, z0, z1, wz);
-    auto Y = [&](int a, int b, int c) {
-        return y[((long)a + (long)NX * (b + (long)NY * c)) * ych];
-    };
-    const float c00 = Y(x0, y0, z0) * (1.f - wx) + Y(x1, y0, z0) * wx;
-    const float c10 = Y(x0, y1, z0) * (1.f - wx) + Y(x1, y1, z0) * wx;
-    const float c01 = Y(x0, y0, z1) * (1.f - wx) + Y(x1, y0, z1) * wx;
-    const float c11 = Y(x0, y1, z1) * (1.f - wx) + Y(x1, y1, z1) * wx;
+    const int Ib = max(i0 / kS - 1, 0);  // first staged coarse x
+    const int Ie = min((i0 + 255) / kS + 1, NX - 1);
+    const int span = Ie - Ib + 1;  // <= kSpan
+    for (int t = threadIdx.x; t < 4 * span; t += 256) {
+        const int c = t / span, a = Ib + (t - c * span);
+        const int b = (c & 1) ? y1 : y0, z = (c & 2) ? z1 : z0;
+        cy[c][t - c * span] = y[((long)a + (long)NX * (b + (long)NY * z)) * ych];
+    }
+    __syncthreads();
+    if (i >= g.nx) return;
+    int x0, x1;
+    float wx;
+    lin_axis(i, NX, x0, x1, wx);
+    const int a0 = x0 - Ib, a1 = x1 - Ib;
+    const float c00 = cy[0][a0] * (1.f - wx) + cy[0][a1] * wx;
+    const float c10 = cy[1][a0] * (1.f - wx) + cy[1][a1] * wx;
+    const float c01 = cy[2][a0] * (1.f - wx) + cy[2][a1] * wx;
+    const float c11 = cy[3][a0] * (1.f - wx) + cy[3][a1] * wx;
     const float c0 = c00 * (1.f - wy) + c10 * wy, c1 = c01 * (1.f - wy) + c11 * wy;
     const float yi = c0 * (1.f - wz) + c1 * wz;
-    const double J = (hw - he) / g.nx;
-    h[row * g.nx + i] = hw + (he - hw) * (i + 0.5) / g.nx + J * (double)yi;
+    const double J = (hw - he) * inv_nx;
+    h[row * g.nx + i] = hw + (he - hw) * ((i + 0.5) * inv_nx) + J * (double)yi;
 }
 
 // per coarse cell V: target t = mean over its block of (h - h_lin) / J;
@@ -740,7 +775,8 @@ int forward(mf3d_wsnet* net, hipStream_t s, const mf3d_grid& gf, const double* K
     const Geo G0{g.nx, g.ny, g.nz}, G1{g1.nx, g1.ny, g1.nz};
     const long n0 = (long)cells(g), n1 = (long)cells(g1), n2 = (long)cells(g2);
     float* P = net->params;
-    input_kernel<<<blocks(n0), 256, 0, s>>>(Geo{gf.nx, gf.ny, gf.nz}, K, is_log, net->x0);
+    input_kernel<<<(unsigned)((n0 + 63) / 64), 256, 0, s>>>(Geo{gf.nx, gf.ny, gf.nz}, K, is_log,
+                                                          net->x0);
     MF3D_TRY(conv(s, g, 16, 16, net->kpad[0], 16, net->x0, net->W[0], P + net->boff[0], nullptr,
                   nullptr, true, false, net->e0, false));
     pool_kernel<<<blocks(n1 * 16), 256, 0, s>>>(G0, 16, net->e0, net->p1);
@@ -945,7 +981,7 @@ int mf3d_wsnet_predict(mf3d_wsnet* net, mf3d_stream stream, const mf3d_grid* gri
     channel0_kernel<<<blocks(n0), 256, 0, s>>>(net->y, 16, n0, net->yd);
     const long rows = (long)grid->ny * grid->nz, segs = (grid->nx + 255) / 256;
     output_kernel<<<(unsigned)(rows * segs), 256, 0, s>>>(Geo{grid->nx, grid->ny, grid->nz}, hw,
-                                                          he, net->yd, 1, h0);
+                                                          he, 1.0 / grid->nx, net->yd, 1, h0);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
