@@ -243,8 +243,10 @@ enum {
     MF3D_TUNE_X_OVERLAP = 8,            /* L > 0: PCG x update on a second stream from V-cycle
                                            level L on (overlapping the coarse levels);
                                            0 (default): fused with the p update */
-    MF3D_TUNE_X_STEP_BLOCKS = 9         /* workgroups of that x update (0, default: one per
+    MF3D_TUNE_X_STEP_BLOCKS = 9,        /* workgroups of that x update (0, default: one per
                                            2048 cells) */
+    MF3D_TUNE_FOLD_FIRST = 10           /* 1 (default): a colour-split coarse level's first red
+                                           half-sweep is computed by the restriction kernel */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

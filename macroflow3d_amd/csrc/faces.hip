@@ -896,11 +896,22 @@ __device__ __forceinline__ float residual_cell_split(const Dims& d, const BcType
 // half-sweep, each black cell satisfies its own equation given its (final)
 // red neighbours, so its residual is zero up to FP32 rounding; only the four
 // red children of a coarse cell are evaluated.
-template <typename TC, bool CS, bool RED>
+// Coarse first half-sweep folded in (FUSE, colour-split coarse level whose x
+// starts at zero): a red coarse cell's first GS value depends only on its own
+// b and faces, x = (0 - b cs dx^2) / sum f, so it is written here (values as
+// the zero-start half-sweep) and the coarse smoothing starts with black.
+struct CoarseFirst {
+    float* x;     // coarse x (split order); null: not folded
+    float bs;     // coarse cs dx^2
+};
+
+template <typename TC, bool CS, bool RED, bool FUSE = false>
 __device__ __forceinline__ void rr_split_body(const Dims& d, const BcTypes& bc, int pin, float rs,
                                               const float* __restrict__ x,
                                               const float* __restrict__ b, const SFaces<TC>& F,
-                                              float* __restrict__ bcoarse, long blk) {
+                                              float* __restrict__ bcoarse, long blk,
+                                              const SFaces<TC>* Fc = nullptr,
+                                              CoarseFirst cf = {}) {
     const Dims dc{d.nx >> 1, d.ny >> 1, d.nz >> 1};
     const long total = (long)dc.nx * dc.ny * dc.nz;
     const long t = blk * kBlock + threadIdx.x;
@@ -928,18 +939,28 @@ __device__ __forceinline__ void rr_split_body(const Dims& d, const BcTypes& bc, 
         s += residual_cell_split(d, bc, p, sp, x, b, F, i, j + 1, k + 1, rs);
         s += residual_cell_split(d, bc, p, sp, x, b, F, i + 1, j + 1, k + 1, rs);
     }
-    bcoarse[CS ? split_of(dc).at(dc, I, J, Kc) : t] = s / 8.0f;
+    const long oc = CS ? split_of(dc).at(dc, I, J, Kc) : t;
+    const float bcv = s / 8.0f;
+    bcoarse[oc] = bcv;
+    if (FUSE && CS && ((I + J + Kc) & 1) == 0) {
+        float f[6], v[6];
+        cell_faces_split(dc, split_of(dc), *Fc, I, J, Kc, f);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] = 0.0f;
+        cf.x[oc] = gs_value(f, v, bcv, cf.bs, pin && (I | J | Kc) == 0);
+    }
 }
 
-template <typename TC, bool CS, bool RED>
+template <typename TC, bool CS, bool RED, bool FUSE>
 __global__ void __launch_bounds__(kBlock)
     residual_restrict_split_kernel(Dims d, BcTypes bc, int pin, float rs,
                                    const float* __restrict__ x, const float* __restrict__ b,
-                                   SFaces<TC> F, float* __restrict__ bcoarse) {
+                                   SFaces<TC> F, float* __restrict__ bcoarse, SFaces<TC> Fc,
+                                   CoarseFirst cf) {
     const long total = (long)(d.nx >> 1) * (d.ny >> 1) * (d.nz >> 1);
     const long blk = xcd_block((total + kBlock - 1) / kBlock);
     if (blk < 0) return;
-    rr_split_body<TC, CS, RED>(d, bc, pin, rs, x, b, F, bcoarse, blk);
+    rr_split_body<TC, CS, RED, FUSE>(d, bc, pin, rs, x, b, F, bcoarse, blk, &Fc, cf);
 }
 
 // face coefficients of a colour-split level from K (values as faces_build_kernel);
@@ -1870,7 +1891,8 @@ int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan, int order,
-                 int corder, const double* r64) {
+                 int corder, const double* r64, bool red_done) {
+    if (red_done) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels do not fold the first sweep");
     if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 smoothing needs FP64 faces");
     if (r64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels read b directly");
     if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
@@ -1905,7 +1927,7 @@ void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin,
 template <typename TC>
 int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, const float* b, const FaceSet& F, int iters, bool zero_init,
-                 const float* xc, bool csplit, int tb, const double* r64) {
+                 const float* xc, bool csplit, int tb, const double* r64, bool red_done) {
     const Dims d = dims(g);
     const SFaces<TC> SF = sfaces<TC>(F);
     const size_t n = cells(g);
@@ -1927,6 +1949,7 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
     }
     for (int it = 0; it < iters; ++it)
         for (int colour = 0; colour < 2; ++colour) {
+            if (it == 0 && colour == 0 && zero_init && red_done) continue;  // folded upstream
             if (it == 0 && colour == 0 && zero_init && r64)
                 launch_split<TC, 3, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb, r64);
             else if (it == 0 && colour == 0 && zero_init)
@@ -1945,23 +1968,28 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
 template <typename TC, bool RED>
 void launch_rr_split(hipStream_t s, const Dims& d, const BcTypes& bc, bool pin, double rs,
                      const float* x, const float* b, const FaceSet& F, float* bcoarse,
-                     bool csplit, size_t nc) {
-    if (csplit)
-        residual_restrict_split_kernel<TC, true, RED><<<xcd_grid(nc), kBlock, 0, s>>>(
-            d, bc, pin ? 1 : 0, (float)rs, x, b, sfaces<TC>(F), bcoarse);
+                     bool csplit, size_t nc, const FaceSet* Fc, CoarseFirst cf) {
+    const SFaces<TC> sf = sfaces<TC>(F);
+    const SFaces<TC> sfc = Fc ? sfaces<TC>(*Fc) : sf;
+    if (csplit && cf.x)
+        residual_restrict_split_kernel<TC, true, RED, true><<<xcd_grid(nc), kBlock, 0, s>>>(
+            d, bc, pin ? 1 : 0, (float)rs, x, b, sf, bcoarse, sfc, cf);
+    else if (csplit)
+        residual_restrict_split_kernel<TC, true, RED, false><<<xcd_grid(nc), kBlock, 0, s>>>(
+            d, bc, pin ? 1 : 0, (float)rs, x, b, sf, bcoarse, sfc, cf);
     else
-        residual_restrict_split_kernel<TC, false, RED><<<xcd_grid(nc), kBlock, 0, s>>>(
-            d, bc, pin ? 1 : 0, (float)rs, x, b, sfaces<TC>(F), bcoarse);
+        residual_restrict_split_kernel<TC, false, RED, false><<<xcd_grid(nc), kBlock, 0, s>>>(
+            d, bc, pin ? 1 : 0, (float)rs, x, b, sf, bcoarse, sfc, cf);
 }
 
 template <typename TC>
 int rr_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double rs,
              const float* x, const float* b, const FaceSet& F, float* bcoarse, bool csplit,
-             bool red) {
+             bool red, const FaceSet* Fc, CoarseFirst cf) {
     const size_t nc = (size_t)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
     const Dims d = dims(g);
-    if (red) launch_rr_split<TC, true>(s, d, bc, pin, rs, x, b, F, bcoarse, csplit, nc);
-    else launch_rr_split<TC, false>(s, d, bc, pin, rs, x, b, F, bcoarse, csplit, nc);
+    if (red) launch_rr_split<TC, true>(s, d, bc, pin, rs, x, b, F, bcoarse, csplit, nc, Fc, cf);
+    else launch_rr_split<TC, false>(s, d, bc, pin, rs, x, b, F, bcoarse, csplit, nc, Fc, cf);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -1975,9 +2003,11 @@ bool split_ok(const mf3d_grid& g) {
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder,
-                 const double* r64) {
+                 const double* r64, bool red_done) {
     if (r64 && !(order == ORDER_SPLIT && (g.nx & 7) == 0 && zero_init && iters > 0 && !xc))
         return fail(MF3D_ERR_UNSUPPORTED, "b from FP64 needs a zero-start split4 sweep");
+    if (red_done && !(order == ORDER_SPLIT && zero_init && iters > 0 && !xc && !r64))
+        return fail(MF3D_ERR_UNSUPPORTED, "a folded first half-sweep needs a zero-start split sweep");
     if ((order == ORDER_SPLIT) != F.split)
         return fail(MF3D_ERR_UNSUPPORTED, "vector order and face layout differ");
     if (order == ORDER_SPLIT) {
@@ -1989,13 +2019,13 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
         switch (F.type) {
         case COEF_F32:
             return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
-                                      r64);
+                                      r64, red_done);
         case COEF_BF16:
             return smooth_split<__hip_bfloat16>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc,
-                                                cs, tb, r64);
+                                                cs, tb, r64, red_done);
         case COEF_F16:
             return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
-                                       r64);
+                                       r64, red_done);
         default:
             return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");
         }
@@ -2018,8 +2048,12 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
 
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const double* x, const double* b, const FaceSet& F,
-                            double* bcoarse, int order, int corder, bool red_only) {
+                            double* bcoarse, int order, int corder, bool red_only,
+                            double* xcoarse, const FaceSet* Fc, double bsc) {
     (void)red_only;  // natural-order FP64 levels evaluate every child
+    (void)Fc;
+    (void)bsc;
+    if (xcoarse) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels do not fold the first sweep");
     if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
         return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels are stored in natural order");
     return rr_t<double, double>(s, g, bc, pin, rs, x, b, F, bcoarse);
@@ -2027,17 +2061,24 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
 
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const float* x, const float* b, const FaceSet& F,
-                            float* bcoarse, int order, int corder, bool red_only) {
+                            float* bcoarse, int order, int corder, bool red_only,
+                            float* xcoarse, const FaceSet* Fc, double bsc) {
     if ((order == ORDER_SPLIT) != F.split)
         return fail(MF3D_ERR_UNSUPPORTED, "vector order and face layout differ");
     if (order == ORDER_SPLIT) {
         if (!split_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "colour-split order needs even sizes");
         const bool cs = corder == ORDER_SPLIT;
+        if (xcoarse && (!cs || !Fc || !Fc->split || Fc->type != F.type))
+            return fail(MF3D_ERR_UNSUPPORTED, "folded coarse sweep needs a split coarse level");
+        const CoarseFirst cf{xcoarse, (float)bsc};
         switch (F.type) {
-        case COEF_F32: return rr_split<float>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only);
+        case COEF_F32:
+            return rr_split<float>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only, Fc, cf);
         case COEF_BF16:
-            return rr_split<__hip_bfloat16>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only);
-        case COEF_F16: return rr_split<__half>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only);
+            return rr_split<__hip_bfloat16>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only, Fc,
+                                            cf);
+        case COEF_F16:
+            return rr_split<__half>(s, g, bc, pin, rs, x, b, F, bcoarse, cs, red_only, Fc, cf);
         default: return fail(MF3D_ERR_UNSUPPORTED, "FP32 residual needs 32- or 16-bit faces");
         }
     }

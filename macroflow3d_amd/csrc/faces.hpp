@@ -59,28 +59,37 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan,
                  int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
-                 const double* r64 = nullptr);
+                 const double* r64 = nullptr, bool red_done = false);
 // order: storage order of x/b, corder: of xc (FP64 levels are natural only);
 // a split level's coarse level may be natural, not the reverse.
 // r64 (nullable): b is the FP32 copy of this FP64 natural-order vector, made
 // by the first half-sweep (colour-split level, nx % 8 == 0, zero_init, no xc)
+// red_done: the first (red) half-sweep from x = 0 was folded into the
+// restriction that produced b (residual_restrict_faces with xcoarse)
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan,
                  int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
-                 const double* r64 = nullptr);
+                 const double* r64 = nullptr, bool red_done = false);
 bool split_ok(const mf3d_grid& g);
 // coarse b = restriction of (b - A x), rs = 1 / (cs * dx^2)
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const double* x, const double* b, const FaceSet& F,
                             double* bcoarse, int order = ORDER_NATURAL,
-                            int corder = ORDER_NATURAL, bool red_only = false);
+                            int corder = ORDER_NATURAL, bool red_only = false,
+                            double* xcoarse = nullptr, const FaceSet* Fc = nullptr,
+                            double bsc = 0.0);
 // red_only (colour-split levels after at least one red-black sweep): the
 // black children's residuals are taken as zero (see rr_split_body)
+// xcoarse (split coarse level only): also the coarse level's first red
+// half-sweep from x = 0 (faces Fc, bsc = its cs dx^2); the coarse smoothing
+// then runs with red_done
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                             double rs, const float* x, const float* b, const FaceSet& F,
                             float* bcoarse, int order = ORDER_NATURAL,
-                            int corder = ORDER_NATURAL, bool red_only = false);
+                            int corder = ORDER_NATURAL, bool red_only = false,
+                            float* xcoarse = nullptr, const FaceSet* Fc = nullptr,
+                            double bsc = 0.0);
 // FP64 operator with FP64 faces (cs = 1). mode 0: y = A x, partials (x, y);
 // mode 1: y = b - A x (y nullable), partials (y, y). It writes op_partials(g)
 // partials: one per (plane tile of 256 cells, z-chunk of kz planes).

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "faces.hpp"
@@ -161,6 +162,7 @@ struct mf3d_workspace {
     // level kernels it runs beside, and the cross-stream waits cost more
     // than the idle bandwidth it fills).
     int x_overlap = 0;  // 0: off; else the level at which the x step forks
+    bool fold_first = true;  // coarse first half-sweeps folded into the restrictions
     unsigned x_step_blocks = 0;  // grid cap of the x step (0: one block per 2048 cells)
     hipStream_t x_stream = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
@@ -727,9 +729,11 @@ int fork_x_step(mf3d_workspace* ws, hipStream_t s) {
     return MF3D_OK;
 }
 
+// red_done: level l's first red half-sweep from x = 0 was folded into the
+// restriction from level l - 1
 template <typename T>
 int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool zero_init,
-           const double* r64 = nullptr) {
+           const double* r64 = nullptr, bool red_done = false) {
     using V = LevelView<T>;
     if ((int)l == ws->x_overlap && ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
     const size_t nl = ws->lv.size();
@@ -748,12 +752,19 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
     Level& C = ws->lv[l + 1];
     const int fo = V::order(L), co = V::order(C);
     MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->pre,
-                                zero_init, (const T*)nullptr, L.plan, fo, co, r64));
+                                zero_init, (const T*)nullptr, L.plan, fo, co, r64, red_done));
     // after pre-smoothing that ends with a black half-sweep, the split
-    // levels evaluate only the red children's residuals
+    // levels evaluate only the red children's residuals; a colour-split
+    // coarse level (not the coarsest) gets its first red half-sweep here too
+    const bool fold = std::is_same<T, float>::value && ws->fold_first && ws->pre > 0 &&
+                      fo == ORDER_SPLIT && co == ORDER_SPLIT && l + 2 < nl &&
+                      C.plan.kind == SMOOTH_COLOUR;
+    const double bsc = V::cs(C) * C.g.dx * C.g.dx;
     MF3D_TRY(kern::residual_restrict_faces(s, L.g, ws->bct, ws->pin, 1.0 / (cs * dx2), x, b, F,
-                                           V::b(C), fo, co, ws->pre > 0));
-    MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true));
+                                           V::b(C), fo, co, ws->pre > 0,
+                                           fold ? V::x(C) : (T*)nullptr, fold ? &V::F(C) : nullptr,
+                                           bsc));
+    MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true, nullptr, fold));
     return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->post,
                               false, (const T*)V::x(C), L.plan, fo, co);
 }
@@ -1838,6 +1849,7 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
         ws->x_overlap = (int)value;
         break;
     case MF3D_TUNE_X_STEP_BLOCKS: ws->x_step_blocks = (unsigned)std::max(0L, value); break;
+    case MF3D_TUNE_FOLD_FIRST: ws->fold_first = value != 0; break;
     case MF3D_TUNE_SWEEP_BLOCK:
         if (value < 64 || value > 512 || (value & 63))
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");

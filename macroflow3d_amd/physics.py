@@ -188,6 +188,7 @@ class FlowWorkspace:
     TUNE_MARCH_MAX_CELLS_FP32 = 7
     TUNE_X_OVERLAP = 8
     TUNE_X_STEP_BLOCKS = 9
+    TUNE_FOLD_FIRST = 10
 
     def set_tuning(self, key: int, value: int) -> None:
         """Smoother selection knobs (levels with >= value cells use the one-pass
