@@ -245,8 +245,10 @@ enum {
                                            0 (default): fused with the p update */
     MF3D_TUNE_X_STEP_BLOCKS = 9,        /* workgroups of that x update (0, default: one per
                                            2048 cells) */
-    MF3D_TUNE_FOLD_FIRST = 10           /* 1 (default): a colour-split coarse level's first red
+    MF3D_TUNE_FOLD_FIRST = 10,          /* 1 (default): a colour-split coarse level's first red
                                            half-sweep is computed by the restriction kernel */
+    MF3D_TUNE_FOLD_R = 11               /* 1 (default): the FP32-preconditioned PCG's residual
+                                           update runs in the first finest-level half-sweep */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

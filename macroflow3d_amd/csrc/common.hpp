@@ -129,6 +129,16 @@ struct IterState {
     double xa;     // this iteration's x step: alpha once (r, r) was formed, else 0 (FIN_RR)
 };
 
+// The PCG residual update folded into the first half-sweep of the FP32
+// V-cycle on the finest level: r -= alpha Ap (skipped once stopped) before r
+// becomes b, with (r, r) partials, one per sweep workgroup.
+struct RUpd {
+    const IterState* st;
+    const double* Ap;
+    double* r;
+    double* partials;
+};
+
 // Reduction finalisers (what thread 0 does with the reduced sum S).
 enum Finalize : int {
     FIN_STORE = 0,   // *out = S

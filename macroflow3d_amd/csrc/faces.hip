@@ -355,12 +355,13 @@ __device__ __forceinline__ void load4(const TC* p, float (&o)[4]) {
 // MODE 3: as MODE 1, with b taken from the FP64 natural-order r64 (both
 // colours of b written as FP32 on the way: the conversion at the entry of the
 // FP32 V-cycle folded into its first half-sweep).
-template <typename TC, int MODE, bool CS>
+template <typename TC, int MODE, bool CS, bool RU = false>
 __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, int pin, float bs,
                                             float* __restrict__ x, const float* __restrict__ b,
                                             const SFaces<TC>& F, int colour,
                                             const float* __restrict__ xc,
-                                            const double* __restrict__ r64, long blk) {
+                                            const double* r64, long blk,
+                                            const RUpd* ru = nullptr) {
     const Split sp = split_of(d);
     const int h4 = sp.h >> 2;
     const long total = sp.half >> 2;
@@ -458,6 +459,33 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
             const double2 v2 = rp[e];
             rv[2 * e] = v2.x;
             rv[2 * e + 1] = v2.y;
+        }
+        if (RU) {  // r -= alpha Ap as pcg_update_r_kernel, then (r, r) of the 8 cells
+            const double2* ap = reinterpret_cast<const double2*>(ru->Ap + row * d.nx + 8L * q);
+            double av[8];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double2 v2 = ap[e];
+                av[2 * e] = v2.x;
+                av[2 * e + 1] = v2.y;
+            }
+            if (!ru->st->stop) {  // uniform across the grid
+                const double alpha = ru->st->alpha;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) rv[u] = -alpha * av[u] + rv[u];
+                if (valid) {
+                    double2* rw = reinterpret_cast<double2*>(ru->r + row * d.nx + 8L * q);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) rw[e] = make_double2(rv[2 * e], rv[2 * e + 1]);
+                }
+            }
+            double acc = 0.0;
+            if (valid) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += rv[u] * rv[u];
+            }
+            const double sb = block_sum_dyn(acc);
+            if (threadIdx.x == 0) ru->partials[blk] = sb;
         }
         float bo[4];
 #pragma unroll
@@ -569,6 +597,18 @@ __global__ void __launch_bounds__(kMaxSweepBlock)
     const long blk = xcd_block((total + blockDim.x - 1) / blockDim.x);
     if (blk < 0) return;  // whole workgroup
     split4_body<TC, MODE, CS>(d, bc, pin, bs, x, b, F, colour, xc, r64, blk);
+}
+
+// the first half-sweep from x = 0 with b from r, after r -= alpha Ap (RUpd)
+template <typename TC>
+__global__ void __launch_bounds__(kMaxSweepBlock)
+    colour_split4_r_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
+                           const float* __restrict__ b, SFaces<TC> F, int colour,
+                           const double* r64, RUpd ru) {
+    const long total = ((long)(d.nx >> 1) * d.ny * d.nz) >> 2;
+    const long blk = xcd_block((total + blockDim.x - 1) / blockDim.x);
+    if (blk < 0) return;  // whole workgroup
+    split4_body<TC, 3, false, true>(d, bc, pin, bs, x, b, F, colour, nullptr, r64, blk, &ru);
 }
 
 // prolong-add between any two orders (used when a level has no post-smoothing)
@@ -1891,8 +1931,9 @@ int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan, int order,
-                 int corder, const double* r64, bool red_done) {
-    if (red_done) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels do not fold the first sweep");
+                 int corder, const double* r64, bool red_done, const RUpd* ru) {
+    if (red_done || ru)
+        return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels do not fold the first sweep");
     if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 smoothing needs FP64 faces");
     if (r64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels read b directly");
     if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
@@ -1911,9 +1952,15 @@ SFaces<TC> sfaces(const FaceSet& F) {
 template <typename TC, int MODE, bool CS>
 void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin, float bs,
                   float* x, const float* b, const SFaces<TC>& F, int colour, const float* xc,
-                  int tb, const double* r64 = nullptr) {
+                  int tb, const double* r64 = nullptr, const RUpd* ru = nullptr) {
     const Dims d = dims(g);
     const size_t n = cells(g);
+    if (MODE == 3 && ru) {  // nx % 8 == 0 (MODE 3 needs it)
+        const size_t nb = (n / 8 + tb - 1) / tb;
+        colour_split4_r_kernel<TC><<<(unsigned)((nb + 7) / 8 * 8), tb, 0, s>>>(
+            d, bc, pin, bs, x, b, F, colour, r64, *ru);
+        return;
+    }
     if ((g.nx & 7) == 0) {
         const size_t nb = (n / 8 + tb - 1) / tb;
         colour_split4_kernel<TC, MODE, CS><<<(unsigned)((nb + 7) / 8 * 8), tb, 0, s>>>(
@@ -1927,7 +1974,8 @@ void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin,
 template <typename TC>
 int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, const float* b, const FaceSet& F, int iters, bool zero_init,
-                 const float* xc, bool csplit, int tb, const double* r64, bool red_done) {
+                 const float* xc, bool csplit, int tb, const double* r64, bool red_done,
+                 const RUpd* ru) {
     const Dims d = dims(g);
     const SFaces<TC> SF = sfaces<TC>(F);
     const size_t n = cells(g);
@@ -1951,7 +1999,8 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
         for (int colour = 0; colour < 2; ++colour) {
             if (it == 0 && colour == 0 && zero_init && red_done) continue;  // folded upstream
             if (it == 0 && colour == 0 && zero_init && r64)
-                launch_split<TC, 3, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb, r64);
+                launch_split<TC, 3, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb, r64,
+                                           ru);
             else if (it == 0 && colour == 0 && zero_init)
                 launch_split<TC, 1, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb);
             else if (it == 0 && colour == 0 && xc && csplit)
@@ -2003,7 +2052,8 @@ bool split_ok(const mf3d_grid& g) {
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder,
-                 const double* r64, bool red_done) {
+                 const double* r64, bool red_done, const RUpd* ru) {
+    if (ru && !r64) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on the b-from-r sweep");
     if (r64 && !(order == ORDER_SPLIT && (g.nx & 7) == 0 && zero_init && iters > 0 && !xc))
         return fail(MF3D_ERR_UNSUPPORTED, "b from FP64 needs a zero-start split4 sweep");
     if (red_done && !(order == ORDER_SPLIT && zero_init && iters > 0 && !xc && !r64))
@@ -2019,13 +2069,13 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
         switch (F.type) {
         case COEF_F32:
             return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
-                                      r64, red_done);
+                                      r64, red_done, ru);
         case COEF_BF16:
             return smooth_split<__hip_bfloat16>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc,
-                                                cs, tb, r64, red_done);
+                                                cs, tb, r64, red_done, ru);
         case COEF_F16:
             return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
-                                       r64, red_done);
+                                       r64, red_done, ru);
         default:
             return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");
         }

@@ -25,6 +25,23 @@ __device__ inline double block_sum(double v) {
     return v;
 }
 
+// sum over a workgroup of any size that is a multiple of 64 (at most 1024);
+// the result is valid in thread 0
+__device__ inline double block_sum_dyn(double v) {
+    __shared__ double part[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = lane < (int)(blockDim.x >> 6) ? part[lane] : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    }
+    return v;
+}
+
 // sum of partials[0 .. np) by one workgroup of kFinalBlock threads in a fixed
 // order (16 independent loads per thread and round; the partials sit in L2);
 // valid in thread 0

@@ -163,6 +163,10 @@ struct mf3d_workspace {
     // than the idle bandwidth it fills).
     int x_overlap = 0;  // 0: off; else the level at which the x step forks
     bool fold_first = true;  // coarse first half-sweeps folded into the restrictions
+    // PCG with z kept in x32: the r update (and its (r, r) partials, in
+    // rr_partials) done by the V-cycle's first finest-level half-sweep
+    bool fold_r = true;
+    double* rr_partials = nullptr;
     unsigned x_step_blocks = 0;  // grid cap of the x step (0: one block per 2048 cells)
     hipStream_t x_stream = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
@@ -219,6 +223,7 @@ struct mf3d_workspace {
         rhs = r = z = p = Ap = nullptr;
         partials = nullptr;
         partials_cap = 0;
+        rr_partials = nullptr;
         st = nullptr;
         dscalar = nullptr;
         hcopy = nullptr;
@@ -301,6 +306,8 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     MF3D_TRY(ws->alloc(&ws->Ap, n));
     ws->partials_cap = std::max<size_t>(kern::reduce_blocks(n), kern::op_partials(g)) + 64;
     MF3D_TRY(ws->alloc(&ws->partials, ws->partials_cap));
+    // one per workgroup of the finest sweep (8 cells per thread, >= 64 threads)
+    MF3D_TRY(ws->alloc(&ws->rr_partials, n / 512 + 64));
     double* stp = nullptr;
     MF3D_TRY(ws->alloc(&stp, (sizeof(IterState) + 7) / 8));
     ws->st = reinterpret_cast<IterState*>(stp);
@@ -733,7 +740,7 @@ int fork_x_step(mf3d_workspace* ws, hipStream_t s) {
 // restriction from level l - 1
 template <typename T>
 int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool zero_init,
-           const double* r64 = nullptr, bool red_done = false) {
+           const double* r64 = nullptr, bool red_done = false, const RUpd* ru = nullptr) {
     using V = LevelView<T>;
     if ((int)l == ws->x_overlap && ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
     const size_t nl = ws->lv.size();
@@ -752,7 +759,8 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
     Level& C = ws->lv[l + 1];
     const int fo = V::order(L), co = V::order(C);
     MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->pre,
-                                zero_init, (const T*)nullptr, L.plan, fo, co, r64, red_done));
+                                zero_init, (const T*)nullptr, L.plan, fo, co, r64, red_done,
+                                ru));
     // after pre-smoothing that ends with a black half-sweep, the split
     // levels evaluate only the red children's residuals; a colour-split
     // coarse level (not the coarsest) gets its first red half-sweep here too
@@ -778,24 +786,34 @@ bool z_in_x32(const mf3d_workspace* ws, bool flexible) {
            L0.order == ORDER_SPLIT && !(L0.g.nx & 7);
 }
 
+// the FP32 V-cycle's colour-split finest level takes r directly in its first
+// half-sweep
+bool r_in_first_sweep(const mf3d_workspace* ws) {
+    const Level& L0 = ws->lv[0];
+    return ws->precision == MF3D_PRECOND_FP32 && ws->lv.size() > 1 &&
+           L0.order == ORDER_SPLIT && !(L0.g.nx & 7) && ws->pre > 0 &&
+           L0.plan.kind == SMOOTH_COLOUR;
+}
+
 // MultigridPreconditioner::apply (mg_preconditioner.cu:16-45): z = V-cycle(b = r, x = 0),
 // in FP64 or with the FP32 V-cycle between two conversions.
 // rz_partials (nullable): also leave the first-stage partials of (r, z) there.
 // z == nullptr (z_in_x32 only): z stays in the finest level's x32.
+// ru: r -= alpha Ap is done by that first half-sweep (r_in_first_sweep only)
 int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
-                 double* rz_partials = nullptr) {
+                 double* rz_partials = nullptr, const RUpd* ru = nullptr) {
     Level& L0 = ws->lv[0];
     if (!z && !z_in_x32(ws, false))
         return fail(MF3D_ERR_UNSUPPORTED, "z can stay in x32 only on a colour-split FP32 level");
     if (ws->precision == MF3D_PRECOND_FP64) {
+        if (ru) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on the FP32 V-cycle");
         MF3D_TRY(vcycle<double>(ws, s, 0, z, r, true));
         return rz_partials ? kern::dot_partials(s, r, z, L0.n, rz_partials) : MF3D_OK;
     }
-    // a colour-split finest level takes r directly in its first half-sweep
-    const bool fold = ws->lv.size() > 1 && L0.order == ORDER_SPLIT && !(L0.g.nx & 7) &&
-                      ws->pre > 0 && L0.plan.kind == SMOOTH_COLOUR;
+    const bool fold = r_in_first_sweep(ws);
+    if (ru && !fold) return fail(MF3D_ERR_UNSUPPORTED, "the r update needs the folded r sweep");
     if (!fold) MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.g, L0.order));
-    MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true, fold ? r : nullptr));
+    MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true, fold ? r : nullptr, false, ru));
     if (!z) return rz_partials ? kern::rz_split8(s, L0.x32, L0.g, r, rz_partials) : MF3D_OK;
     return kern::to_f64(s, L0.x32, z, L0.g, L0.order, rz_partials ? r : nullptr, rz_partials);
 }
@@ -890,6 +908,10 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     const bool zx = z_in_x32(ws, flexible);  // z kept in the finest level's x32
     double* z = zx ? nullptr : ws->z;
     const bool ovl = zx && ws->x_overlap > 0;  // x step on x_stream, see mf3d_workspace
+    // r update in the V-cycle's first sweep (its FIN_RR then follows the V-cycle)
+    const bool fuse_r = zx && !ovl && ws->fold_r && r_in_first_sweep(ws);
+    const unsigned np_sweep =
+        (unsigned)((n / 8 + ws->lv[0].plan.block - 1) / ws->lv[0].plan.block);
     ws->fork_pending = false;
     if (ovl) {
         if (!ws->x_stream)
@@ -924,6 +946,15 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             for (int c = 0; c < chunk; ++c) {
                 MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials, &npo));
                 MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_PAP, ws->st, nullptr));
+                if (fuse_r) {
+                    const RUpd ru{ws->st, ws->Ap, ws->r, ws->rr_partials};
+                    MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, &ru));
+                    MF3D_TRY(kern::finalize(s, ws->rr_partials, np_sweep, FIN_RR, ws->st,
+                                            nullptr));
+                    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
+                    MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
+                    continue;
+                }
                 if (zx)  // x is updated together with p below
                     MF3D_TRY(kern::pcg_update_r(s, ws->st, ws->Ap, ws->r, n, ws->partials));
                 else
@@ -1850,6 +1881,7 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
         break;
     case MF3D_TUNE_X_STEP_BLOCKS: ws->x_step_blocks = (unsigned)std::max(0L, value); break;
     case MF3D_TUNE_FOLD_FIRST: ws->fold_first = value != 0; break;
+    case MF3D_TUNE_FOLD_R: ws->fold_r = value != 0; break;
     case MF3D_TUNE_SWEEP_BLOCK:
         if (value < 64 || value > 512 || (value & 63))
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");
