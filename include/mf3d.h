@@ -247,8 +247,9 @@ enum {
                                            2048 cells) */
     MF3D_TUNE_FOLD_FIRST = 10,          /* 1 (default): a colour-split coarse level's first red
                                            half-sweep is computed by the restriction kernel */
-    MF3D_TUNE_FOLD_R = 11               /* 1 (default): the FP32-preconditioned PCG's residual
+    MF3D_TUNE_FOLD_R = 11,              /* 1 (default): the FP32-preconditioned PCG's residual
                                            update runs in the first finest-level half-sweep */
+    MF3D_TUNE_FOLD_RZ = 12              /* 1 (default): ... and its (r, z) in the last one */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

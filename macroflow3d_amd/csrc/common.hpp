@@ -139,6 +139,14 @@ struct RUpd {
     double* partials;
 };
 
+// The (r, z) partials of PCG folded into the last half-sweep of the FP32
+// V-cycle on the finest level (z = x32 of both colours once it ends): one
+// partial per sweep workgroup.
+struct RzDot {
+    const double* r;
+    double* partials;
+};
+
 // Reduction finalisers (what thread 0 does with the reduced sum S).
 enum Finalize : int {
     FIN_STORE = 0,   // *out = S

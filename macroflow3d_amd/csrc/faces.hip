@@ -355,13 +355,17 @@ __device__ __forceinline__ void load4(const TC* p, float (&o)[4]) {
 // MODE 3: as MODE 1, with b taken from the FP64 natural-order r64 (both
 // colours of b written as FP32 on the way: the conversion at the entry of the
 // FP32 V-cycle folded into its first half-sweep).
-template <typename TC, int MODE, bool CS, bool RU = false>
+// RZ (MODE 0, the last black half-sweep of a V-cycle): with the red values it
+// loads as neighbours and the black values it writes, the thread holds z of
+// its 8 natural cells and adds (r, z) to the workgroup's partial.
+template <typename TC, int MODE, bool CS, bool RU = false, bool RZ = false>
 __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, int pin, float bs,
                                             float* __restrict__ x, const float* __restrict__ b,
                                             const SFaces<TC>& F, int colour,
                                             const float* __restrict__ xc,
                                             const double* r64, long blk,
-                                            const RUpd* ru = nullptr) {
+                                            const RUpd* ru = nullptr,
+                                            const RzDot* rz = nullptr) {
     const Split sp = split_of(d);
     const int h4 = sp.h >> 2;
     const long total = sp.half >> 2;
@@ -415,6 +419,7 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     }
 
     float vxp[4], vxm[4], vyp[4], vym[4], vzp[4], vzm[4];
+    float oc[4];  // the other colour at the thread's 4 places (MODE 0 and 2)
     if (MODE == 1 || MODE == 3) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) vxp[e] = vxm[e] = vyp[e] = vym[e] = vzp[e] = vzm[e] = 0.0f;
@@ -422,6 +427,8 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
         const float* O = x + (long)(colour ^ 1) * sp.half;
         float a[4];
         load4(O + base, a);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) oc[e] = a[e];
         float prv = __shfl_up(a[3], 1, 64);
         float nxt = __shfl_down(a[0], 1, 64);
         if (lane == 0 && !xlo) prv = O[base - 1];
@@ -583,6 +590,19 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
         res[e] = gs_value(f, v, bb[e], bs, pin && colour == 0 && row == 0 && q == 0 && e == 0);
     }
     if (valid) *reinterpret_cast<float4*>(x + own) = make_float4(res[0], res[1], res[2], res[3]);
+    if (RZ) {  // own cells at natural 2e + p, the other colour at 2e + 1 - p
+        const double2* rp = reinterpret_cast<const double2*>(rz->r + row * d.nx + 8L * q);
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double2 v2 = rp[e];
+            const double ro = p ? v2.y : v2.x, rt = p ? v2.x : v2.y;
+            acc += ro * (double)res[e];
+            acc += rt * (double)oc[e];
+        }
+        const double sb = block_sum_dyn(valid ? acc : 0.0);
+        if (threadIdx.x == 0) rz->partials[blk] = sb;
+    }
 }
 
 // any block size up to kMaxSweepBlock (a tuning choice: results do not depend on it)
@@ -609,6 +629,18 @@ __global__ void __launch_bounds__(kMaxSweepBlock)
     const long blk = xcd_block((total + blockDim.x - 1) / blockDim.x);
     if (blk < 0) return;  // whole workgroup
     split4_body<TC, 3, false, true>(d, bc, pin, bs, x, b, F, colour, nullptr, r64, blk, &ru);
+}
+
+// the last (black) half-sweep of the V-cycle with the (r, z) partials (RzDot)
+template <typename TC>
+__global__ void __launch_bounds__(kMaxSweepBlock)
+    colour_split4_rz_kernel(Dims d, BcTypes bc, int pin, float bs, float* __restrict__ x,
+                            const float* __restrict__ b, SFaces<TC> F, int colour, RzDot rz) {
+    const long total = ((long)(d.nx >> 1) * d.ny * d.nz) >> 2;
+    const long blk = xcd_block((total + blockDim.x - 1) / blockDim.x);
+    if (blk < 0) return;  // whole workgroup
+    split4_body<TC, 0, false, false, true>(d, bc, pin, bs, x, b, F, colour, nullptr, nullptr,
+                                           blk, nullptr, &rz);
 }
 
 // prolong-add between any two orders (used when a level has no post-smoothing)
@@ -1931,8 +1963,9 @@ int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  double* x, double* tmp, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan, int order,
-                 int corder, const double* r64, bool red_done, const RUpd* ru) {
-    if (red_done || ru)
+                 int corder, const double* r64, bool red_done, const RUpd* ru,
+                 const RzDot* rz) {
+    if (red_done || ru || rz)
         return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels do not fold the first sweep");
     if (F.type != COEF_F64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 smoothing needs FP64 faces");
     if (r64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels read b directly");
@@ -1952,13 +1985,20 @@ SFaces<TC> sfaces(const FaceSet& F) {
 template <typename TC, int MODE, bool CS>
 void launch_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, int pin, float bs,
                   float* x, const float* b, const SFaces<TC>& F, int colour, const float* xc,
-                  int tb, const double* r64 = nullptr, const RUpd* ru = nullptr) {
+                  int tb, const double* r64 = nullptr, const RUpd* ru = nullptr,
+                  const RzDot* rz = nullptr) {
     const Dims d = dims(g);
     const size_t n = cells(g);
     if (MODE == 3 && ru) {  // nx % 8 == 0 (MODE 3 needs it)
         const size_t nb = (n / 8 + tb - 1) / tb;
         colour_split4_r_kernel<TC><<<(unsigned)((nb + 7) / 8 * 8), tb, 0, s>>>(
             d, bc, pin, bs, x, b, F, colour, r64, *ru);
+        return;
+    }
+    if (MODE == 0 && rz) {  // nx % 8 == 0 (checked by smooth_faces)
+        const size_t nb = (n / 8 + tb - 1) / tb;
+        colour_split4_rz_kernel<TC><<<(unsigned)((nb + 7) / 8 * 8), tb, 0, s>>>(
+            d, bc, pin, bs, x, b, F, colour, *rz);
         return;
     }
     if ((g.nx & 7) == 0) {
@@ -1975,7 +2015,7 @@ template <typename TC>
 int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, const float* b, const FaceSet& F, int iters, bool zero_init,
                  const float* xc, bool csplit, int tb, const double* r64, bool red_done,
-                 const RUpd* ru) {
+                 const RUpd* ru, const RzDot* rz) {
     const Dims d = dims(g);
     const SFaces<TC> SF = sfaces<TC>(F);
     const size_t n = cells(g);
@@ -2008,7 +2048,9 @@ int smooth_split(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
             else if (it == 0 && colour == 0 && xc)
                 launch_split<TC, 2, false>(s, g, bc, p, fbs, x, b, SF, colour, xc, tb);
             else
-                launch_split<TC, 0, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb);
+                launch_split<TC, 0, false>(s, g, bc, p, fbs, x, b, SF, colour, nullptr, tb,
+                                           nullptr, nullptr,
+                                           it == iters - 1 && colour == 1 ? rz : nullptr);
         }
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
@@ -2052,8 +2094,10 @@ bool split_ok(const mf3d_grid& g) {
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder,
-                 const double* r64, bool red_done, const RUpd* ru) {
+                 const double* r64, bool red_done, const RUpd* ru, const RzDot* rz) {
     if (ru && !r64) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on the b-from-r sweep");
+    if (rz && !(order == ORDER_SPLIT && (g.nx & 7) == 0 && iters > 0))
+        return fail(MF3D_ERR_UNSUPPORTED, "(r, z) partials need a split4 sweep to end on");
     if (r64 && !(order == ORDER_SPLIT && (g.nx & 7) == 0 && zero_init && iters > 0 && !xc))
         return fail(MF3D_ERR_UNSUPPORTED, "b from FP64 needs a zero-start split4 sweep");
     if (red_done && !(order == ORDER_SPLIT && zero_init && iters > 0 && !xc && !r64))
@@ -2069,13 +2113,13 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
         switch (F.type) {
         case COEF_F32:
             return smooth_split<float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
-                                      r64, red_done, ru);
+                                      r64, red_done, ru, rz);
         case COEF_BF16:
             return smooth_split<__hip_bfloat16>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc,
-                                                cs, tb, r64, red_done, ru);
+                                                cs, tb, r64, red_done, ru, rz);
         case COEF_F16:
             return smooth_split<__half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, cs, tb,
-                                       r64, red_done, ru);
+                                       r64, red_done, ru, rz);
         default:
             return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");
         }

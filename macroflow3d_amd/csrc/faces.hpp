@@ -60,7 +60,7 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
                  bool zero_init, const double* xc, const SmoothPlan& plan,
                  int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
                  const double* r64 = nullptr, bool red_done = false,
-                 const RUpd* ru = nullptr);
+                 const RUpd* ru = nullptr, const RzDot* rz = nullptr);
 // order: storage order of x/b, corder: of xc (FP64 levels are natural only);
 // a split level's coarse level may be natural, not the reverse.
 // r64 (nullable): b is the FP32 copy of this FP64 natural-order vector, made
@@ -68,12 +68,14 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
 // red_done: the first (red) half-sweep from x = 0 was folded into the
 // restriction that produced b (residual_restrict_faces with xcoarse);
 // ru (with r64): the PCG r update rides on the first half-sweep
+// rz (colour-split level, nx % 8 == 0): the last half-sweep also writes the
+// (r, x) partials of its natural-order FP64 r, one per sweep workgroup
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                  float* x, float* tmp, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan,
                  int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
                  const double* r64 = nullptr, bool red_done = false,
-                 const RUpd* ru = nullptr);
+                 const RUpd* ru = nullptr, const RzDot* rz = nullptr);
 bool split_ok(const mf3d_grid& g);
 // coarse b = restriction of (b - A x), rs = 1 / (cs * dx^2)
 int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,

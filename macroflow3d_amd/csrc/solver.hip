@@ -167,6 +167,8 @@ struct mf3d_workspace {
     // rr_partials) done by the V-cycle's first finest-level half-sweep
     bool fold_r = true;
     double* rr_partials = nullptr;
+    // ... and the (r, z) partials by its last finest-level half-sweep
+    bool fold_rz = true;
     unsigned x_step_blocks = 0;  // grid cap of the x step (0: one block per 2048 cells)
     hipStream_t x_stream = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
@@ -304,7 +306,9 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     MF3D_TRY(ws->alloc(&ws->z, n));
     MF3D_TRY(ws->alloc(&ws->p, n));
     MF3D_TRY(ws->alloc(&ws->Ap, n));
-    ws->partials_cap = std::max<size_t>(kern::reduce_blocks(n), kern::op_partials(g)) + 64;
+    // (the finest split4 sweeps write one per workgroup of >= 64 threads)
+    ws->partials_cap =
+        std::max<size_t>({kern::reduce_blocks(n), kern::op_partials(g), n / 512}) + 64;
     MF3D_TRY(ws->alloc(&ws->partials, ws->partials_cap));
     // one per workgroup of the finest sweep (8 cells per thread, >= 64 threads)
     MF3D_TRY(ws->alloc(&ws->rr_partials, n / 512 + 64));
@@ -740,7 +744,8 @@ int fork_x_step(mf3d_workspace* ws, hipStream_t s) {
 // restriction from level l - 1
 template <typename T>
 int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool zero_init,
-           const double* r64 = nullptr, bool red_done = false, const RUpd* ru = nullptr) {
+           const double* r64 = nullptr, bool red_done = false, const RUpd* ru = nullptr,
+           const RzDot* rz = nullptr) {
     using V = LevelView<T>;
     if ((int)l == ws->x_overlap && ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
     const size_t nl = ws->lv.size();
@@ -774,7 +779,8 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
                                            bsc));
     MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true, nullptr, fold));
     return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->post,
-                              false, (const T*)V::x(C), L.plan, fo, co);
+                              false, (const T*)V::x(C), L.plan, fo, co, nullptr, false, nullptr,
+                              rz);
 }
 
 // PCG may leave z = x32 unstored (the (r, z) partials and the p update read the
@@ -784,6 +790,12 @@ bool z_in_x32(const mf3d_workspace* ws, bool flexible) {
     const Level& L0 = ws->lv[0];
     return !flexible && ws->precision == MF3D_PRECOND_FP32 && ws->lv.size() > 1 &&
            L0.order == ORDER_SPLIT && !(L0.g.nx & 7);
+}
+
+// partials of a finest-level split4 sweep: one per workgroup
+unsigned sweep_partials(const mf3d_workspace* ws) {
+    const Level& L0 = ws->lv[0];
+    return (unsigned)((L0.n / 8 + L0.plan.block - 1) / L0.plan.block);
 }
 
 // the FP32 V-cycle's colour-split finest level takes r directly in its first
@@ -800,11 +812,16 @@ bool r_in_first_sweep(const mf3d_workspace* ws) {
 // rz_partials (nullable): also leave the first-stage partials of (r, z) there.
 // z == nullptr (z_in_x32 only): z stays in the finest level's x32.
 // ru: r -= alpha Ap is done by that first half-sweep (r_in_first_sweep only)
+// np_rz: the number of (r, z) partials written (z == nullptr: the last
+// finest-level half-sweep writes one per workgroup when it can, else
+// reduce_blocks(n))
 int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
-                 double* rz_partials = nullptr, const RUpd* ru = nullptr) {
+                 double* rz_partials = nullptr, const RUpd* ru = nullptr,
+                 unsigned* np_rz = nullptr) {
     Level& L0 = ws->lv[0];
     if (!z && !z_in_x32(ws, false))
         return fail(MF3D_ERR_UNSUPPORTED, "z can stay in x32 only on a colour-split FP32 level");
+    if (np_rz) *np_rz = kern::reduce_blocks(L0.n);
     if (ws->precision == MF3D_PRECOND_FP64) {
         if (ru) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on the FP32 V-cycle");
         MF3D_TRY(vcycle<double>(ws, s, 0, z, r, true));
@@ -813,7 +830,15 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
     const bool fold = r_in_first_sweep(ws);
     if (ru && !fold) return fail(MF3D_ERR_UNSUPPORTED, "the r update needs the folded r sweep");
     if (!fold) MF3D_TRY(kern::to_f32(s, r, L0.b32, L0.g, L0.order));
-    MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true, fold ? r : nullptr, false, ru));
+    // the last finest half-sweep forms (r, z): z in x32, post-smoothing on level 0
+    const bool rz_fold = !z && rz_partials && ws->fold_rz && ws->post > 0 && fold;
+    const RzDot rzd{r, rz_partials};
+    MF3D_TRY(vcycle<float>(ws, s, 0, L0.x32, L0.b32, true, fold ? r : nullptr, false, ru,
+                           rz_fold ? &rzd : nullptr));
+    if (rz_fold) {
+        if (np_rz) *np_rz = sweep_partials(ws);
+        return MF3D_OK;
+    }
     if (!z) return rz_partials ? kern::rz_split8(s, L0.x32, L0.g, r, rz_partials) : MF3D_OK;
     return kern::to_f64(s, L0.x32, z, L0.g, L0.order, rz_partials ? r : nullptr, rz_partials);
 }
@@ -910,8 +935,8 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     const bool ovl = zx && ws->x_overlap > 0;  // x step on x_stream, see mf3d_workspace
     // r update in the V-cycle's first sweep (its FIN_RR then follows the V-cycle)
     const bool fuse_r = zx && !ovl && ws->fold_r && r_in_first_sweep(ws);
-    const unsigned np_sweep =
-        (unsigned)((n / 8 + ws->lv[0].plan.block - 1) / ws->lv[0].plan.block);
+    const unsigned np_sweep = sweep_partials(ws);
+    unsigned nprz = 0;  // (r, z) partials of the last preconditioner application
     ws->fork_pending = false;
     if (ovl) {
         if (!ws->x_stream)
@@ -932,8 +957,8 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_R0, ws->st, nullptr));
     int restarts = 0;
     for (;;) {
-        MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials));
-        MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ0, ws->st, nullptr));
+        MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, nullptr, &nprz));
+        MF3D_TRY(kern::finalize(s, ws->partials, nprz, FIN_RZ0, ws->st, nullptr));
         if (zx) MF3D_TRY(kern::update_p_split8(s, ws->st, L0.x32, L0.g, ws->p, true));
         else MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
         MF3D_TRY(read_state(ws, s));
@@ -948,10 +973,10 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                 MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_PAP, ws->st, nullptr));
                 if (fuse_r) {
                     const RUpd ru{ws->st, ws->Ap, ws->r, ws->rr_partials};
-                    MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, &ru));
+                    MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, &ru, &nprz));
                     MF3D_TRY(kern::finalize(s, ws->rr_partials, np_sweep, FIN_RR, ws->st,
                                             nullptr));
-                    MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
+                    MF3D_TRY(kern::finalize(s, ws->partials, nprz, FIN_RZ, ws->st, nullptr));
                     MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
                     continue;
                 }
@@ -966,8 +991,8 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZO, ws->st, nullptr));
                 }
                 ws->fork_pending = ovl;
-                MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials));
-                MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZ, ws->st, nullptr));
+                MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, nullptr, &nprz));
+                MF3D_TRY(kern::finalize(s, ws->partials, nprz, FIN_RZ, ws->st, nullptr));
                 if (ovl) {
                     if (ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
                     MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->join_ev, 0));
@@ -1882,6 +1907,7 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     case MF3D_TUNE_X_STEP_BLOCKS: ws->x_step_blocks = (unsigned)std::max(0L, value); break;
     case MF3D_TUNE_FOLD_FIRST: ws->fold_first = value != 0; break;
     case MF3D_TUNE_FOLD_R: ws->fold_r = value != 0; break;
+    case MF3D_TUNE_FOLD_RZ: ws->fold_rz = value != 0; break;
     case MF3D_TUNE_SWEEP_BLOCK:
         if (value < 64 || value > 512 || (value & 63))
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");

@@ -190,6 +190,7 @@ class FlowWorkspace:
     TUNE_X_STEP_BLOCKS = 9
     TUNE_FOLD_FIRST = 10
     TUNE_FOLD_R = 11
+    TUNE_FOLD_RZ = 12
 
     def set_tuning(self, key: int, value: int) -> None:
         """Smoother selection knobs (levels with >= value cells use the one-pass

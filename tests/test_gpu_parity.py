@@ -581,24 +581,28 @@ def test_folded_first_coarse_sweep_matches_separate(ctx, mix, shape, levels, pre
 def test_fused_residual_update_matches_separate(ctx, mix, shape):
     """The PCG residual update r -= alpha Ap done by the first finest-level
     half-sweep of the FP32 V-cycle (with its (r, r) partials, one per sweep
-    workgroup) gives the solve of the separate update kernel: same stopping
-    iteration (within one: the (r, r) sums are added in another order) and
-    the same head."""
+    workgroup), and the (r, z) partials formed by the last one, give the
+    solve of the separate kernels, in every combination: same stopping
+    iteration (within one: the sums are added in another order) and the same
+    head."""
     g, _ = grid_pair(*shape)
     bc, _ = bc_pair(mix)
     n = g.num_cells()
     K = dev(random_K(n, 73, 1.5), ctx)
     cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
-    print(f"fused r update {shape} {mix}", flush=True)
-    out = []
-    for fold in (1, 0):
+    print(f"fused r update / (r, z) {shape} {mix}", flush=True)
+    out = {}
+    for fr, frz in ((0, 0), (1, 0), (0, 1), (1, 1)):
         ws = ph.FlowWorkspace()
-        ws.set_tuning(ph.FlowWorkspace.TUNE_FOLD_R, fold)
+        ws.set_tuning(ph.FlowWorkspace.TUNE_FOLD_R, fr)
+        ws.set_tuning(ph.FlowWorkspace.TUNE_FOLD_RZ, frz)
         h = ctx.zeros(n)
         res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
-        out.append((host(h), res))
+        out[(fr, frz)] = (host(h), res)
         ws.close()
-    (h1, r1), (h0, r0) = out
-    assert r1.converged and r0.converged
-    assert abs(r1.num_iterations - r0.num_iterations) <= 1
-    assert rel_l2(h1, h0) < 1e-9
+    h0, r0 = out[(0, 0)]
+    assert r0.converged
+    for key, (h1, r1) in out.items():
+        assert r1.converged, key
+        assert abs(r1.num_iterations - r0.num_iterations) <= 1, key
+        assert rel_l2(h1, h0) < 1e-9, key
