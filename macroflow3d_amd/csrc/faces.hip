@@ -355,6 +355,29 @@ __device__ __forceinline__ void load4(const TC* p, float (&o)[4]) {
 // MODE 3: as MODE 1, with b taken from the FP64 natural-order r64 (both
 // colours of b written as FP32 on the way: the conversion at the entry of the
 // FP32 V-cycle folded into its first half-sweep).
+// sweep workgroups: any block size up to kMaxSweepBlock (a tuning choice:
+// results do not depend on it)
+constexpr int kMaxSweepBlock = 512;
+
+// (i, j, k) of a natural cell index, advanced along the natural order without
+// divisions: the unit-stride passes below visit e, e + 128, ... and divide once
+struct CellWalk {
+    int i, j, k;
+    __device__ CellWalk(const Dims& d, long e) {
+        const long row = e / d.nx;
+        i = (int)(e - row * d.nx);
+        j = (int)(row % d.ny);
+        k = (int)(row / d.ny);
+    }
+    __device__ void advance(const Dims& d, int by) {
+        i += by;
+        while (i >= d.nx) {
+            i -= d.nx;
+            if (++j == d.ny) j = 0, ++k;
+        }
+    }
+};
+
 // RZ (MODE 0, the last black half-sweep of a V-cycle): with the red values it
 // loads as neighbours and the black values it writes, the thread holds z of
 // its 8 natural cells and adds (r, z) to the workgroup's partial.
@@ -457,7 +480,50 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
         else vzm[0] = vzm[1] = vzm[2] = vzm[3] = 0.0f;
     }
     float bb[4];
-    if (MODE == 3) {
+    if (MODE == 3 && RU) {
+        // The wave's threads own the 512 natural cells [w0, w0 + 512) (8 per
+        // thread, rows are contiguous). First pass over them in unit stride:
+        // the cell pairs (2m, 2m + 1) of a row, one of each colour at place m,
+        // pair P = 64 u + lane: r -= alpha Ap as pcg_update_r_kernel, (r, r),
+        // both colours of b (FP32) out, and the own colour's b into LDS at the
+        // pair's index, where the GS pass's thread finds its 4 places
+        // (pairs 4 lane .. 4 lane + 3) as one float4.
+        __shared__ float bown[kMaxSweepBlock * 4];
+        const long n = (long)d.nx * d.ny * d.nz;
+        const long w0 = 8L * (blk * (long)blockDim.x + (long)(threadIdx.x & ~63));
+        const bool go = !ru->st->stop;  // uniform across the grid
+        const double alpha = ru->st->alpha;
+        float* bw = const_cast<float*>(b);
+        double acc = 0.0;
+        CellWalk cw(d, min(w0 + 2L * lane, n - 2));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long e = w0 + 128L * u + 2L * lane;
+            if (u > 0) cw.advance(d, 128);
+            if (e < n) {
+                const int c0 = (cw.j + cw.k) & 1;  // colour of cell 2m
+                const long pl = e >> 1;
+                double2 rv = *reinterpret_cast<const double2*>(r64 + e);
+                const double2 av = *reinterpret_cast<const double2*>(ru->Ap + e);
+                if (go) {
+                    rv.x = -alpha * av.x + rv.x;
+                    rv.y = -alpha * av.y + rv.y;
+                    *reinterpret_cast<double2*>(ru->r + e) = rv;
+                }
+                acc += rv.x * rv.x;
+                acc += rv.y * rv.y;
+                const float b0 = (float)rv.x, b1 = (float)rv.y;
+                bw[(long)c0 * sp.half + pl] = b0;
+                bw[(long)(c0 ^ 1) * sp.half + pl] = b1;
+                bown[(threadIdx.x & ~63) * 4 + 64 * u + lane] = c0 == colour ? b0 : b1;
+            }
+        }
+        const double sb = block_sum_dyn(acc);  // (synchronizes the workgroup)
+        if (threadIdx.x == 0) ru->partials[blk] = sb;
+        __syncthreads();
+        const float4 o4 = *reinterpret_cast<const float4*>(bown + 4 * threadIdx.x);
+        bb[0] = o4.x, bb[1] = o4.y, bb[2] = o4.z, bb[3] = o4.w;
+    } else if (MODE == 3) {
         // the thread's 8 natural cells of the row: own colour at 2e + p, the other at 2e + 1 - p
         const double2* rp = reinterpret_cast<const double2*>(r64 + row * d.nx + 8L * q);
         double rv[8];
@@ -466,33 +532,6 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
             const double2 v2 = rp[e];
             rv[2 * e] = v2.x;
             rv[2 * e + 1] = v2.y;
-        }
-        if (RU) {  // r -= alpha Ap as pcg_update_r_kernel, then (r, r) of the 8 cells
-            const double2* ap = reinterpret_cast<const double2*>(ru->Ap + row * d.nx + 8L * q);
-            double av[8];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const double2 v2 = ap[e];
-                av[2 * e] = v2.x;
-                av[2 * e + 1] = v2.y;
-            }
-            if (!ru->st->stop) {  // uniform across the grid
-                const double alpha = ru->st->alpha;
-#pragma unroll
-                for (int u = 0; u < 8; ++u) rv[u] = -alpha * av[u] + rv[u];
-                if (valid) {
-                    double2* rw = reinterpret_cast<double2*>(ru->r + row * d.nx + 8L * q);
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) rw[e] = make_double2(rv[2 * e], rv[2 * e + 1]);
-                }
-            }
-            double acc = 0.0;
-            if (valid) {
-#pragma unroll
-                for (int u = 0; u < 8; ++u) acc += rv[u] * rv[u];
-            }
-            const double sb = block_sum_dyn(acc);
-            if (threadIdx.x == 0) ru->partials[blk] = sb;
         }
         float bo[4];
 #pragma unroll
@@ -604,9 +643,6 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
         if (threadIdx.x == 0) rz->partials[blk] = sb;
     }
 }
-
-// any block size up to kMaxSweepBlock (a tuning choice: results do not depend on it)
-constexpr int kMaxSweepBlock = 512;
 
 template <typename TC, int MODE, bool CS>
 __global__ void __launch_bounds__(kMaxSweepBlock)
@@ -1516,34 +1552,34 @@ __device__ __forceinline__ void split8_body(const Dims& d, const float* __restri
         if (threadIdx.x == 0) partials[blk] = sb;
     }
 }
-// x += alpha p; p = z + beta p (z = x32), 8 cells per thread as split8_body:
-// the x half of pcg_update_xr_kernel moved to the p update, so the iteration
-// reads and writes p once for both. Skipped once stopped.
+// x += alpha p; p = z + beta p (z = x32): the x half of pcg_update_xr_kernel
+// moved to the p update, so the iteration reads and writes p once for both.
+// Skipped once stopped. A workgroup covers the 2048 natural cells
+// [2048 blk, 2048 blk + 2048) (nx % 8 == 0) in pairs (2m, 2m + 1) of a row,
+// one cell of each colour at colour-split place m; a wave takes 4 runs of 64
+// consecutive pairs, so every load and store of a wave is unit-stride.
 __device__ __forceinline__ void xp_split8_body(const Dims& d, const float* __restrict__ a,
                                                double* __restrict__ x, double* __restrict__ p,
                                                double alpha, double beta, long blk) {
-    const long total = ((long)d.nx * d.ny * d.nz) >> 3;
-    const long t = blk * kBlock + threadIdx.x;
-    if (t >= total) return;
-    const int h8 = d.nx >> 3;
-    const long row = t / h8;
-    const int q = (int)(t - row * h8);
-    const int j = (int)(row % d.ny);
-    const int k = (int)(row / d.ny);
-    const int c0 = (j + k) & 1;
-    const long half = ((long)(d.nx >> 1) * d.ny) * d.nz;
-    const long pl = (long)(d.nx >> 1) * row + 4L * q;
-    const float4 A = *reinterpret_cast<const float4*>(a + (long)c0 * half + pl);
-    const float4 B = *reinterpret_cast<const float4*>(a + (long)(c0 ^ 1) * half + pl);
-    const double z[8] = {A.x, B.x, A.y, B.y, A.z, B.z, A.w, B.w};
-    const long e0 = row * d.nx + 8L * q;
-    double2* pp = reinterpret_cast<double2*>(p + e0);
-    double2* xp = reinterpret_cast<double2*>(x + e0);
+    const long n = (long)d.nx * d.ny * d.nz;
+    const long half = n >> 1;
+    const long e0 = blk * (8L * kBlock) + (long)(threadIdx.x >> 6) * 512 +
+                    2L * (threadIdx.x & 63);
+    if (e0 >= n) return;
+    CellWalk cw(d, e0);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const double2 pv = pp[e], xv = xp[e];
-        xp[e] = make_double2(alpha * pv.x + xv.x, alpha * pv.y + xv.y);
-        pp[e] = make_double2(1.0 * z[2 * e] + beta * pv.x, 1.0 * z[2 * e + 1] + beta * pv.y);
+    for (int u = 0; u < 4; ++u) {
+        const long e = e0 + 128L * u;  // even: i = 2m
+        if (e >= n) break;
+        if (u > 0) cw.advance(d, 128);
+        const long pl = e >> 1;  // (nx / 2) row + m
+        const int c0 = (cw.j + cw.k) & 1;
+        const double z0 = a[(long)c0 * half + pl], z1 = a[(long)(c0 ^ 1) * half + pl];
+        double2* pp = reinterpret_cast<double2*>(p + e);
+        double2* xp = reinterpret_cast<double2*>(x + e);
+        const double2 pv = *pp, xv = *xp;
+        *xp = make_double2(alpha * pv.x + xv.x, alpha * pv.y + xv.y);
+        *pp = make_double2(1.0 * z0 + beta * pv.x, 1.0 * z1 + beta * pv.y);
     }
 }
 __global__ void __launch_bounds__(kBlock)
