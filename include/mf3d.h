@@ -249,7 +249,9 @@ enum {
                                            half-sweep is computed by the restriction kernel */
     MF3D_TUNE_FOLD_R = 11,              /* 1 (default): the FP32-preconditioned PCG's residual
                                            update runs in the first finest-level half-sweep */
-    MF3D_TUNE_FOLD_RZ = 12              /* 1 (default): ... and its (r, z) in the last one */
+    MF3D_TUNE_FOLD_RZ = 12,             /* 1 (default): ... and its (r, z) in the last one */
+    MF3D_TUNE_OP_TILE = 13              /* 1 (default): the FP64 operator stages plane tiles in
+                                           LDS when nx <= 512 (even, 16-byte aligned vectors) */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

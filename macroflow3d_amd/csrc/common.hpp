@@ -213,7 +213,7 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
     unsigned np;      // partials of the 256-cell reductions (reduce_blocks)
     unsigned np_op;   // partials of the operator (op_partials)
     int otiles, okz;  // operator geometry (op_geom)
-    int opairs;       // operator on cell pairs (op_geom)
+    int opairs;       // operator variant: 0 single cells, 1 pairs, 2 plane tiles (op_plan)
     IterState* st;
     double* out;  // scalar result of a FIN_STORE / FIN_SQRT second stage
 };

@@ -1409,6 +1409,202 @@ __device__ __forceinline__ void opz2_body(const Dims& d, const BcTypes& bc, int 
     if (threadIdx.x == 0) partials[blk] = s;
 }
 
+// Plane-tile variant (even nx <= kOpLdsMaxNx): a workgroup owns the kOpTile
+// consecutive cells [t0, t0 + kOpTile) of an xy plane and marches kz planes up
+// in z. Each plane's tile and the nx cells on either side of it (the y
+// neighbours) are staged in LDS, slot L holding plane cell (t0 - nx + L) mod
+// nx*ny, so that every x and y neighbour of a tile cell, periodic wraps
+// included, is the slot at a fixed distance: one unit-stride pass per array
+// and plane replaces the three row loads of the paired kernel. Thread tid
+// owns the tile's cell pairs tid and tid + kBlock and loads them itself (their
+// plane-k+1 values are the up neighbours); the halo pairs are dealt to the
+// threads in order. Per cell: the faces and the sum order of opz2_body.
+constexpr int kOpTile = 4 * kBlock;  // cells: two pairs per thread
+constexpr int kOpLdsMaxNx = 512;
+constexpr int kOpHalo = kOpLdsMaxNx / kBlock;  // halo pairs per thread, at most
+
+template <int MODE>
+__device__ __forceinline__ void opz_lds_body(const Dims& d, const BcTypes& bc, int pin,
+                                             double inv_dx2, int tiles, int kz,
+                                             const double* __restrict__ x,
+                                             const double* __restrict__ H,
+                                             const double* __restrict__ b,
+                                             double* __restrict__ y,
+                                             double* __restrict__ partials, long blk) {
+    extern __shared__ __align__(16) double op_lds[];  // x then h, kOpTile + 2 nx slots each
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long P = (long)nx * ny;
+    const int slots = kOpTile + 2 * nx;
+    double* lx = op_lds;
+    double* lh = op_lds + slots;
+    const int chunk = (int)(blk / tiles);
+    const long t0 = (blk - (long)chunk * tiles) * kOpTile;
+    const int tid = (int)threadIdx.x;
+    const int k0 = chunk * kz;
+    const int k1 = min(nz, k0 + kz);
+    const int nh = nx;  // halo pairs: nx / 2 below the tile, nx / 2 above
+    auto wrapP = [&](long c) { c %= P; return c < 0 ? c + P : c; };
+    // plane cell of slot L (even L: the pair's left cell, same row)
+    auto cell_of = [&](int L) { return wrapP(t0 - nx + L); };
+    auto plane_off = [&](int k) { return (long)((k % nz + nz) % nz) * P; };
+    // own pairs: tile pairs tid and tid + kBlock (slots nx + 2 tid, + 2 kBlock)
+    int oi[2], oj[2];
+    bool act[2];
+    long oc[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const long c = t0 + 2L * (tid + m * kBlock);
+        act[m] = c < P;
+        // pairs past the plane's end stage their slot's wrapped cells (the
+        // periodic y neighbours of the last row) and store nothing
+        oc[m] = wrapP(c);
+        oj[m] = (int)(oc[m] / nx);
+        oi[m] = (int)(oc[m] - (long)oj[m] * nx);
+    }
+    long hc_[kOpHalo];  // halo pairs of this thread: plane cells and slots
+    int hs_[kOpHalo];
+#pragma unroll
+    for (int m = 0; m < kOpHalo; ++m) {
+        const int q = tid + m * kBlock;  // halo pair index, < nh when used
+        const int L = q < nh / 2 ? 2 * q : kOpTile + nx + 2 * (q - nh / 2);
+        hs_[m] = q < nh ? L : -1;
+        hc_[m] = q < nh ? cell_of(L) : 0;
+    }
+    struct Plane {
+        dbl2 x[2], h[2], hx[kOpHalo], hh[kOpHalo], bo[2];
+    };
+    auto load = [&](int k, bool halo) {
+        const long po = plane_off(k);
+        Plane l;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            l.x[m] = ld2(x, po + oc[m]);
+            l.h[m] = ld2(H, po + oc[m]);
+            l.bo[m] = MODE == 1 ? ld2(b, po + oc[m]) : dbl2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int m = 0; m < kOpHalo; ++m) {
+            if (halo && hs_[m] >= 0) {
+                l.hx[m] = ld2(x, po + hc_[m]);
+                l.hh[m] = ld2(H, po + hc_[m]);
+            }
+        }
+        return l;
+    };
+    auto stage = [&](const Plane& l) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int L = nx + 2 * (tid + m * kBlock);
+            *reinterpret_cast<dbl2*>(lx + L) = l.x[m];
+            *reinterpret_cast<dbl2*>(lh + L) = l.h[m];
+        }
+#pragma unroll
+        for (int m = 0; m < kOpHalo; ++m) {
+            if (hs_[m] >= 0) {
+                *reinterpret_cast<dbl2*>(lx + hs_[m]) = l.hx[m];
+                *reinterpret_cast<dbl2*>(lh + hs_[m]) = l.hh[m];
+            }
+        }
+    };
+    // the plane below k0 (own pairs) and the first plane
+    const Side D0 = side(k0 > 0, 0, bc.t[4], 0);
+    dbl2 xm[2], fzl[2];
+    Plane cur = load(k0, true);
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const long po = plane_off(k0 - 1);
+        const dbl2 xd = ld2(x, po + oc[m]), hd = ld2(H, po + oc[m]);
+        xm[m] = D0.nb ? xd : dbl2{0.0, 0.0};
+        fzl[m] = dbl2{hface(D0, cur.h[m].x, hd.x), hface(D0, cur.h[m].y, hd.y)};
+    }
+    stage(cur);
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        // the next plane (wraps to plane 0 past the top, used if periodic); its
+        // halo only if it is staged
+        Plane nxt = load(k + 1, k + 1 < k1);
+        __syncthreads();          // plane k staged
+        const Side U = side(k < nz - 1, 0, bc.t[5], 0);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int i = oi[m], j = oj[m];
+            const int L = nx + 2 * (tid + m * kBlock);
+            const Side Wt = side(i > 0, 0, bc.t[0], 0);
+            const Side E = side(i + 2 < nx, 0, bc.t[1], 0);
+            const Side N = side(j < ny - 1, 0, bc.t[3], 0);
+            const Side S = side(j > 0, 0, bc.t[2], 0);
+            const dbl2 xc = cur.x[m], hc = cur.h[m];
+            // x neighbours: the slot before the pair (west; at i = 0 the row's
+            // last cell, nx - 1 slots on) and after it (east; wrap: nx - 2 back)
+            const int Lw = i > 0 ? L - 1 : L + nx - 1;
+            const int Le = i + 2 < nx ? L + 2 : L + 2 - nx;
+            const double xw = Wt.nb ? lx[Lw] : 0.0, hw = lh[Lw];
+            const double xe = E.nb ? lx[Le] : 0.0, he = lh[Le];
+            const dbl2 xn = *reinterpret_cast<const dbl2*>(lx + L + nx);
+            const dbl2 hn = *reinterpret_cast<const dbl2*>(lh + L + nx);
+            const dbl2 xs = *reinterpret_cast<const dbl2*>(lx + L - nx);
+            const dbl2 hs = *reinterpret_cast<const dbl2*>(lh + L - nx);
+            const dbl2 xp = nxt.x[m], hp = nxt.h[m];
+            const double fw = hface(Wt, hc.x, hw);
+            const double fm = rcp_nr(hc.x + hc.y);  // between the two cells
+            const double fe = hface(E, hc.y, he);
+            const double fn0 = hface(N, hc.x, hn.x), fn1 = hface(N, hc.y, hn.y);
+            const double fs0 = hface(S, hc.x, hs.x), fs1 = hface(S, hc.y, hs.y);
+            const double fu0 = hface(U, hc.x, hp.x), fu1 = hface(U, hc.y, hp.y);
+            const double n0 = N.nb ? xn.x : 0.0, n1 = N.nb ? xn.y : 0.0;
+            const double s0 = S.nb ? xs.x : 0.0, s1 = S.nb ? xs.y : 0.0;
+            const double u0 = U.nb ? xp.x : 0.0, u1 = U.nb ? xp.y : 0.0;
+            double A0 = fw * (xc.x - xw) + fm * (xc.x - xc.y) + fs0 * (xc.x - s0) +
+                        fn0 * (xc.x - n0) + fzl[m].x * (xc.x - xm[m].x) + fu0 * (xc.x - u0);
+            double A1 = fm * (xc.y - xc.x) + fe * (xc.y - xe) + fs1 * (xc.y - s1) +
+                        fn1 * (xc.y - n1) + fzl[m].y * (xc.y - xm[m].y) + fu1 * (xc.y - u1);
+            if (pin && k == 0 && oc[m] == 0 && act[m]) {  // cell 0: x+ (fm), y+, z+
+                double aC = fm;
+                if (ny > 1) aC += fn0;
+                if (nz > 1) aC += fu0;
+                A0 += aC * xc.x;
+            }
+            A0 = -A0 * inv_dx2;
+            A1 = -A1 * inv_dx2;
+            const long o = (long)k * P + oc[m];
+            if (MODE == 0) {
+                if (act[m]) {
+                    *reinterpret_cast<dbl2*>(y + o) = dbl2{A0, A1};
+                    acc += xc.x * A0;
+                    acc += xc.y * A1;
+                }
+            } else {
+                const double r0 = -1.0 * A0 + cur.bo[m].x, r1 = -1.0 * A1 + cur.bo[m].y;
+                if (act[m]) {
+                    if (y) *reinterpret_cast<dbl2*>(y + o) = dbl2{r0, r1};
+                    acc += r0 * r0;
+                    acc += r1 * r1;
+                }
+            }
+            xm[m] = xc;
+            fzl[m] = dbl2{fu0, fu1};
+        }
+        __syncthreads();  // every read of plane k done
+        if (k + 1 < k1) stage(nxt);
+        // the next plane's own values: a non-periodic top has no neighbour there
+        // (u = 0, h unused: the march ends at nz - 1)
+        cur = nxt;
+    }
+    const double s = block_sum<kBlock>(acc);
+    if (threadIdx.x == 0) partials[blk] = s;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    op_lds_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, int tiles, int kz, long nblocks,
+                  const double* __restrict__ x, const double* __restrict__ H,
+                  const double* __restrict__ b, double* __restrict__ y,
+                  double* __restrict__ partials) {
+    const long blk = xcd_block(nblocks);
+    if (blk < 0) return;  // padding block: uniform exit before the block reduction
+    opz_lds_body<MODE>(d, bc, pin, inv_dx2, tiles, kz, x, H, b, y, partials, blk);
+}
+
 template <int MODE, bool PAIRS>
 __global__ void __launch_bounds__(kBlock)
     op_kernel(Dims d, BcTypes bc, int pin, double inv_dx2, int tiles, int kz, long nblocks,
@@ -1778,14 +1974,20 @@ __global__ void __launch_bounds__(kBlock)
     const Dims d{V.nx, V.ny, V.nz};
     const long b = blk - start[e];
     if (MODE == 0) {
-        if (V.opairs)
+        if (V.opairs == 2)
+            opz_lds_body<0>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.p, V.hk, nullptr, V.Ap,
+                            V.partials, b);
+        else if (V.opairs)
             opz2_body<0>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.p, V.hk, nullptr, V.Ap,
                          V.partials, b);
         else
             opz_body<0>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.p, V.hk, nullptr, V.Ap,
                         V.partials, b);
     } else {
-        if (V.opairs)
+        if (V.opairs == 2)
+            opz_lds_body<1>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.x, V.hk, V.rhs, V.r,
+                            V.partials, b);
+        else if (V.opairs)
             opz2_body<1>(d, V.bc, V.pin, V.inv_dx2, V.otiles, V.okz, V.x, V.hk, V.rhs, V.r,
                          V.partials, b);
         else
@@ -2235,9 +2437,34 @@ OpGeom op_geom(const mf3d_grid& g, bool pairs) {
     return o;
 }
 
+// the plane-tile variant: kOpTile cells per workgroup
+static OpGeom op_tile_geom(const mf3d_grid& g) {
+    OpGeom o;
+    o.pairs = true;
+    const long P = (long)g.nx * g.ny;
+    o.tiles = (int)((P + kOpTile - 1) / kOpTile);
+    o.kz = (int)std::max(1L, std::min(16L, (long)o.tiles * g.nz / 2048));
+    o.chunks = (g.nz + o.kz - 1) / o.kz;
+    return o;
+}
+
+OpGeom op_plan(const mf3d_grid& g, bool pairs, bool allow_tile) {
+    if (pairs && allow_tile && !(g.nx & 1) && g.nx <= kOpLdsMaxNx) {
+        OpGeom o = op_tile_geom(g);
+        o.tile = true;
+        return o;
+    }
+    return op_geom(g, pairs);
+}
+
+size_t op_tile_lds(const mf3d_grid& g) {
+    return 2 * (size_t)(kOpTile + 2 * g.nx) * sizeof(double);
+}
+
 unsigned op_partials(const mf3d_grid& g) {
-    const OpGeom a = op_geom(g, false), b = op_geom(g, true);
-    return (unsigned)std::max((long)a.tiles * a.chunks, (long)b.tiles * b.chunks);
+    const OpGeom a = op_geom(g, false), b = op_geom(g, true), c = op_tile_geom(g);
+    return (unsigned)std::max({(long)a.tiles * a.chunks, (long)b.tiles * b.chunks,
+                               (long)c.tiles * c.chunks});
 }
 
 bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y,
@@ -2248,13 +2475,26 @@ bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y
 
 int op_apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* H,
              int mode, const double* x, const double* b, double* y, double* partials,
-             unsigned* np, bool allow_pairs) {
+             unsigned* np, bool allow_pairs, bool allow_tile) {
     const double inv_dx2 = 1.0 / (g.dx * g.dx);
-    const OpGeom o = op_geom(g, allow_pairs && op_pairs_ok(g, x, b, y, H));
-    const long nb = (long)o.tiles * o.chunks;
-    const unsigned grid = xcd_round((size_t)nb);
+    const bool pairs = allow_pairs && op_pairs_ok(g, x, b, y, H);
     const Dims d = dims(g);
     const int pn = pin ? 1 : 0;
+    const OpGeom o = op_plan(g, pairs, allow_tile);
+    const long nb = (long)o.tiles * o.chunks;
+    const unsigned grid = xcd_round((size_t)nb);
+    if (o.tile) {
+        const size_t lds = op_tile_lds(g);
+        if (mode == 0)
+            op_lds_kernel<0><<<grid, kBlock, lds, s>>>(d, bc, pn, inv_dx2, o.tiles, o.kz, nb, x,
+                                                       H, b, y, partials);
+        else
+            op_lds_kernel<1><<<grid, kBlock, lds, s>>>(d, bc, pn, inv_dx2, o.tiles, o.kz, nb, x,
+                                                       H, b, y, partials);
+        MF3D_HIP_TRY(hipGetLastError());
+        if (np) *np = (unsigned)nb;
+        return MF3D_OK;
+    }
     if (mode == 0) {
         if (o.pairs)
             op_kernel<0, true><<<grid, kBlock, 0, s>>>(d, bc, pn, inv_dx2, o.tiles, o.kz, nb, x, H,
@@ -2504,10 +2744,11 @@ int batch_update_xp_split8(hipStream_t s, const BatchVec* V, const int* start, i
     return MF3D_OK;
 }
 
-int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks) {
+int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks,
+             size_t lds) {
     const unsigned g = (unsigned)(((nblocks + 7) / 8) * 8);
-    if (mode == 0) op_batch_kernel<0><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
-    else op_batch_kernel<1><<<g, kBlock, 0, s>>>(V, start, m, nblocks);
+    if (mode == 0) op_batch_kernel<0><<<g, kBlock, lds, s>>>(V, start, m, nblocks);
+    else op_batch_kernel<1><<<g, kBlock, lds, s>>>(V, start, m, nblocks);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

@@ -104,17 +104,23 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
 struct OpGeom {
     int tiles, kz, chunks;
     bool pairs;
+    bool tile = false;  // plane tiles staged in LDS (pairs too)
 };
 OpGeom op_geom(const mf3d_grid& g, bool pairs);
+// the variant op_apply runs: plane tiles with pairs when allowed and nx <= 512
+OpGeom op_plan(const mf3d_grid& g, bool pairs, bool allow_tile);
+// dynamic LDS bytes of the plane-tile variant
+size_t op_tile_lds(const mf3d_grid& g);
 unsigned op_partials(const mf3d_grid& g);  // the larger count of the two variants
 bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y,
                  const void* H);
 // mode 0: y = A x with (x, y) partials; mode 1: y = b - A x (y nullable) with
 // (y, y) partials; np (nullable): the number of partials written; pairs only
-// when allowed and op_pairs_ok
+// when allowed and op_pairs_ok; with pairs and nx <= 512 (and allow_tile) the
+// plane-tile variant with the y neighbours staged in LDS
 int op_apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* H,
              int mode, const double* x, const double* b, double* y, double* partials,
-             unsigned* np, bool allow_pairs = true);
+             unsigned* np, bool allow_pairs = true, bool allow_tile = true);
 int to_f32(hipStream_t s, const double* a, float* o, size_t n);
 int to_f64(hipStream_t s, const float* a, double* o, size_t n);
 // conversions between a natural FP64 vector and an FP32 vector of `order`
@@ -168,7 +174,10 @@ int batch_update_p_split8(hipStream_t s, int mode, const BatchVec* V, const int*
                           long nblocks);
 int batch_update_xp_split8(hipStream_t s, const BatchVec* V, const int* start, int m,
                            long nblocks);
-int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks);
+// lds: dynamic LDS bytes, the largest op_tile_lds of the entries that use
+// plane tiles (BatchVec::opairs == 2), else 0
+int batch_op(hipStream_t s, int mode, const BatchVec* V, const int* start, int m, long nblocks,
+             size_t lds);
 
 }  // namespace kern
 }  // namespace mf3d

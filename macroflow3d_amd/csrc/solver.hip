@@ -143,6 +143,7 @@ struct mf3d_workspace {
     // the operator may run on cell pairs (decided per solve from the vectors'
     // alignment, the same way for single and batched solves)
     bool op_pairs = true;
+    bool op_tile = true;  // plane-tile operator where it applies (MF3D_TUNE_OP_TILE)
     // aligned stand-in for a head vector that is not 16-byte aligned
     double* hcopy = nullptr;
     size_t hcopy_n = 0;
@@ -855,13 +856,13 @@ int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, doub
              unsigned* np = nullptr) {
     Level& L0 = ws->lv[0];
     return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 0, x, nullptr, y, partials, np,
-                          ws->op_pairs);
+                          ws->op_pairs, ws->op_tile);
 }
 int residual_op(mf3d_workspace* ws, hipStream_t s, const double* x, const double* b, double* r,
                 double* partials, unsigned* np = nullptr) {
     Level& L0 = ws->lv[0];
     return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 1, x, b, r, partials, np,
-                          ws->op_pairs);
+                          ws->op_pairs, ws->op_tile);
 }
 
 IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
@@ -1143,6 +1144,7 @@ struct BPlan {
     CoefType type = COEF_BF16;
     int pre = 2, post = 2;
     int tb = 256;  // sweep threads per workgroup
+    size_t op_lds = 0;  // dynamic LDS of the batched operator (plane-tile entries)
     ~BPlan() {
         for (void* q : mem) (void)hipFree(q);
     }
@@ -1216,12 +1218,13 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
         v.b32 = L0.b32, v.x32 = L0.x32;
         v.partials = w->partials;
         v.np = kern::reduce_blocks(L0.n);
-        // the geometry single solves use for the same vectors (op_apply)
-        const kern::OpGeom og = kern::op_geom(L0.g, op_pairs_for(w, v.x));
+        // the variant and geometry single solves use for the same vectors (op_apply)
+        const kern::OpGeom og = kern::op_plan(L0.g, op_pairs_for(w, v.x), w->op_tile);
         v.np_op = (unsigned)og.tiles * (unsigned)og.chunks;
         v.otiles = og.tiles;
         v.okz = og.kz;
-        v.opairs = og.pairs ? 1 : 0;
+        v.opairs = og.tile ? 2 : og.pairs ? 1 : 0;
+        if (og.tile) P.op_lds = std::max(P.op_lds, kern::op_tile_lds(L0.g));
         v.st = w->st;
         v.out = w->dscalar;
         vd.push_back(v);
@@ -1338,7 +1341,7 @@ int precondition_batch(const BPlan& P, hipStream_t s) {
 int pcg_iteration_batch(const BPlan& P, hipStream_t s) {
     const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
     const int m = P.vec.m;
-    MF3D_TRY(kern::batch_op(s, 0, BO(P.vec)));
+    MF3D_TRY(kern::batch_op(s, 0, BO(P.vec), P.op_lds));
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_PAP, 1, true));
     MF3D_TRY(kern::batch_update_r(s, BV(P.vec)));  // x follows with p
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RR, 1));
@@ -1396,7 +1399,7 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
     MF3D_HIP_TRY(hipStreamSynchronize(s));
     MF3D_TRY(kern::batch_dot(s, 1, BV(P.vec)));
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_BNORM, 0));
-    MF3D_TRY(kern::batch_op(s, 1, BO(P.vec)));
+    MF3D_TRY(kern::batch_op(s, 1, BO(P.vec), P.op_lds));
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_R0, 0, true));
     GraphHolder G;
     if (use_graph) MF3D_TRY(capture_iteration(P, s, G));
@@ -1419,7 +1422,7 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
         }
         // the x += alpha p owed by the last iteration, then judge by the true residual
         MF3D_TRY(kern::batch_x_pending(s, BV(P.vec)));
-        MF3D_TRY(kern::batch_op(s, 1, BO(P.vec)));
+        MF3D_TRY(kern::batch_op(s, 1, BO(P.vec), P.op_lds));
         MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RESUME, 2, true));
         MF3D_TRY(read_states(ws, count, s));
         bool again = false;
@@ -1679,7 +1682,7 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
     MF3D_HIP_TRY(hipEventRecord(ev[2], s));
     if (cfg->true_residual) {
         const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
-        MF3D_TRY(kern::batch_op(s, 1, V, P.vec.ostart, P.vec.m, P.vec.onblocks));
+        MF3D_TRY(kern::batch_op(s, 1, V, P.vec.ostart, P.vec.m, P.vec.onblocks, P.op_lds));
         MF3D_TRY(kern::batch_finalize(s, V, P.vec.m, FIN_SQRT, 0, true));
         for (int r = 0; r < count; ++r)
             MF3D_HIP_TRY(hipMemcpyAsync(&results[r].true_residual, ws[r]->dscalar, sizeof(double),
@@ -1908,6 +1911,7 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     case MF3D_TUNE_FOLD_FIRST: ws->fold_first = value != 0; break;
     case MF3D_TUNE_FOLD_R: ws->fold_r = value != 0; break;
     case MF3D_TUNE_FOLD_RZ: ws->fold_rz = value != 0; break;
+    case MF3D_TUNE_OP_TILE: ws->op_tile = value != 0; break;
     case MF3D_TUNE_SWEEP_BLOCK:
         if (value < 64 || value > 512 || (value & 63))
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");

@@ -191,6 +191,7 @@ class FlowWorkspace:
     TUNE_FOLD_FIRST = 10
     TUNE_FOLD_R = 11
     TUNE_FOLD_RZ = 12
+    TUNE_OP_TILE = 13
 
     def set_tuning(self, key: int, value: int) -> None:
         """Smoother selection knobs (levels with >= value cells use the one-pass

@@ -606,3 +606,37 @@ def test_fused_residual_update_matches_separate(ctx, mix, shape):
         assert r1.converged, key
         assert abs(r1.num_iterations - r0.num_iterations) <= 1, key
         assert rel_l2(h1, h0) < 1e-9, key
+
+
+@pytest.mark.parametrize("mix", MIXES)
+@pytest.mark.parametrize("shape", [(64, 40, 6), (512, 4, 4), (34, 34, 4), (2, 6, 4)])
+def test_plane_tile_operator_matches_paired(ctx, mix, shape):
+    """The FP64 operator with plane tiles and their y neighbours staged in LDS
+    (1024 cells per workgroup: several tiles per plane with a partial last
+    one, nx at its 512 limit, rows straddling tiles, a plane smaller than a
+    tile) solves as the paired kernel does: the same faces and per-cell sums,
+    so the same iterations and the head to rounding of the reductions."""
+    g, og = grid_pair(*shape)
+    bc, obc = bc_pair(mix)
+    n = g.num_cells()
+    Kh = random_K(n, 74, 1.0)
+    K = dev(Kh, ctx)
+    print(f"plane-tile operator {shape} {mix}", flush=True)
+    cfg = ph.HeadSolveConfig(mg_levels=2, cg_rtol=1e-11, coarse_solver=ph.CoarseSolver.DIRECT,
+                             stop_norm=ph.StopNorm.RHS_NORM, cg_check_every=1)
+    out = []
+    for tile in (1, 0):
+        ws = ph.FlowWorkspace()
+        ws.set_tuning(ph.FlowWorkspace.TUNE_OP_TILE, tile)
+        h = ctx.zeros(n)
+        res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+        out.append((host(h), res))
+        ws.close()
+    (h1, r1), (h0, r0) = out
+    assert r1.converged and r0.converged
+    assert abs(r1.num_iterations - r0.num_iterations) <= 1  # reduction order only
+    assert abs(r1.initial_residual - r0.initial_residual) <= 1e-13 * r0.initial_residual
+    assert rel_l2(h1, h0) < 1e-10
+    p = O.SolveParams(mg_levels=2, cg_rtol=1e-11, coarse_solver=1, stop_norm=1, cg_check_every=1)
+    ho, ro = O.solve_head(og, obc, p, Kh)
+    assert rel_l2(h1, ho) < 1e-9
