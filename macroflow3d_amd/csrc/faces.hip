@@ -2443,7 +2443,7 @@ static OpGeom op_tile_geom(const mf3d_grid& g) {
     o.pairs = true;
     const long P = (long)g.nx * g.ny;
     o.tiles = (int)((P + kOpTile - 1) / kOpTile);
-    o.kz = (int)std::max(1L, std::min(16L, (long)o.tiles * g.nz / 2048));
+    o.kz = (int)std::max(1L, std::min(16L, (long)o.tiles * g.nz / 1024));
     o.chunks = (g.nz + o.kz - 1) / o.kz;
     return o;
 }
