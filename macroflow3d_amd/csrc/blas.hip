@@ -92,6 +92,20 @@ __global__ void __launch_bounds__(kFinalBlock)
     finalize_body(partials, np, op, st, out);
 }
 
+// two second stages in one launch: op1 on the sum of p1, then op2 on that of p2
+__global__ void __launch_bounds__(kFinalBlock)
+    finalize2_kernel(const double* __restrict__ p1, unsigned n1, int op1,
+                     const double* __restrict__ p2, unsigned n2, int op2,
+                     IterState* __restrict__ st) {
+    const double S1 = finalize_sum(p1, n1);
+    __syncthreads();  // the block tree's LDS is reused
+    const double S2 = finalize_sum(p2, n2);
+    if (threadIdx.x == 0) {
+        finalize_apply(op1, S1, st, nullptr);
+        finalize_apply(op2, S2, st, nullptr);
+    }
+}
+
 // ---- batched (one launch for many realisations, descriptors BatchVec) --------
 // second stage: one block per realisation; guard 0: always, 1: only running
 // realisations (stop == 0), 2: only converged ones (stop == 1); op_count:
@@ -377,6 +391,12 @@ int dot_partials(hipStream_t s, const double* x, const double* y, size_t n, doub
 int finalize(hipStream_t s, const double* partials, unsigned np, Finalize op, IterState* st,
              double* out) {
     finalize_kernel<<<1, kFinalBlock, 0, s>>>(partials, np, (int)op, st, out);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int finalize2(hipStream_t s, const double* p1, unsigned n1, Finalize op1, const double* p2,
+              unsigned n2, Finalize op2, IterState* st) {
+    finalize2_kernel<<<1, kFinalBlock, 0, s>>>(p1, n1, (int)op1, p2, n2, (int)op2, st);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }

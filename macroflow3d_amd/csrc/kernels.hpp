@@ -51,6 +51,9 @@ int dot_partials(hipStream_t s, const double* x, const double* y, size_t n, doub
 // second stage: sum partials[0..np) in a fixed order and finalise into st / out
 int finalize(hipStream_t s, const double* partials, unsigned np, Finalize op, IterState* st,
              double* out);
+// finalize(p1, n1, op1) then finalize(p2, n2, op2) in one launch (state ops)
+int finalize2(hipStream_t s, const double* p1, unsigned n1, Finalize op1, const double* p2,
+              unsigned n2, Finalize op2, IterState* st);
 // PCG: if !st->stop { x += alpha p; r -= alpha Ap } with partials of (r, r)
 int pcg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
                   double* x, double* r, size_t n, double* partials);

@@ -975,9 +975,8 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                 if (fuse_r) {
                     const RUpd ru{ws->st, ws->Ap, ws->r, ws->rr_partials};
                     MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, &ru, &nprz));
-                    MF3D_TRY(kern::finalize(s, ws->rr_partials, np_sweep, FIN_RR, ws->st,
-                                            nullptr));
-                    MF3D_TRY(kern::finalize(s, ws->partials, nprz, FIN_RZ, ws->st, nullptr));
+                    MF3D_TRY(kern::finalize2(s, ws->rr_partials, np_sweep, FIN_RR, ws->partials,
+                                             nprz, FIN_RZ, ws->st));
                     MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
                     continue;
                 }
