@@ -589,7 +589,7 @@ def test_fused_residual_update_matches_separate(ctx, mix, shape):
     bc, _ = bc_pair(mix)
     n = g.num_cells()
     K = dev(random_K(n, 73, 1.5), ctx)
-    cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
+    cfg = ph.HeadSolveConfig.mixed(1e-10, mg_levels=3)
     print(f"fused r update / (r, z) {shape} {mix}", flush=True)
     out = {}
     for fr, frz in ((0, 0), (1, 0), (0, 1), (1, 1)):
@@ -640,3 +640,36 @@ def test_plane_tile_operator_matches_paired(ctx, mix, shape):
     p = O.SolveParams(mg_levels=2, cg_rtol=1e-11, coarse_solver=1, stop_norm=1, cg_check_every=1)
     ho, ro = O.solve_head(og, obc, p, Kh)
     assert rel_l2(h1, ho) < 1e-9
+
+
+@pytest.mark.parametrize("tb,pre,post", [(64, 2, 2), (512, 2, 2), (128, 3, 1), (256, 2, 1),
+                                         (256, 2, 3)])
+def test_fused_pcg_steps_across_sweep_blocks_and_smoothing(ctx, tb, pre, post):
+    """The r update in the first finest half-sweep and the (r, z) partials in
+    the last one write one partial per sweep workgroup: every workgroup size
+    (64..512 threads) and smoothing count (the first half-sweep red with x = 0,
+    the last black after the prolongation's half-sweep) solves as the separate
+    kernels do. (With a single pre-smoothing iteration this red-black V-cycle
+    is too far from symmetric for CG on this field: V(1, k) stagnates near
+    1e-4 in FP64 too, fused or not, so the cases keep pre >= 2.)"""
+    g, _ = grid_pair(64, 32, 48)
+    bc, _ = bc_pair("periodic_yz")
+    n = g.num_cells()
+    K = dev(random_K(n, 75, 1.5), ctx)
+    cfg = ph.HeadSolveConfig.mixed(1e-10, mg_levels=4)
+    cfg.mg_pre_smooth, cfg.mg_post_smooth = pre, post
+    print(f"fused PCG steps tb={tb} V({pre},{post})", flush=True)
+    out = []
+    for fold in (1, 0):
+        ws = ph.FlowWorkspace()
+        ws.set_tuning(ph.FlowWorkspace.TUNE_SWEEP_BLOCK, tb)
+        ws.set_tuning(ph.FlowWorkspace.TUNE_FOLD_R, fold)
+        ws.set_tuning(ph.FlowWorkspace.TUNE_FOLD_RZ, fold)
+        h = ctx.zeros(n)
+        res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+        out.append((host(h), res))
+        ws.close()
+    (h1, r1), (h0, r0) = out
+    assert r1.converged and r0.converged
+    assert abs(r1.num_iterations - r0.num_iterations) <= 1
+    assert rel_l2(h1, h0) < 1e-9
