@@ -580,30 +580,25 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
         const bool zm = k > 0 || bc.t[4] == MF3D_BC_PERIODIC;
         const int Jp = j + 1 < d.ny ? (j + 1) >> 1 : 0, Jm = j > 0 ? (j - 1) >> 1 : NYc - 1;
         const int Kp = k + 1 < d.nz ? (k + 1) >> 1 : 0, Km = k > 0 ? (k - 1) >> 1 : NZc - 1;
-        float cr[4];
-        if (yp) {
-            row4(Jp, K0, cr);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) vyp[e] += cr[e];
-        }
-        if (ym) {
-            row4(Jm, K0, cr);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) vym[e] += cr[e];
-        }
-        if (zp) {
-            row4(J0, Kp, cr);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) vzp[e] += cr[e];
-        }
-        if (zm) {
-            row4(J0, Km, cr);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) vzm[e] += cr[e];
-        }
         // own parent row, with I = 4q - 1 and I = 4q + 4 from the adjacent lanes
         float R[4];
         row4(J0, K0, R);
+        // sizes are even: an even j shares its parent row with j + 1 (always in
+        // the domain), an odd j with j - 1, so one of the two y neighbours'
+        // rows is R and only the other is loaded; the same in z
+        const bool jodd = j & 1, kodd = k & 1;
+        float cy[4], cz[4];
+        row4(jodd ? Jp : Jm, K0, cy);
+        row4(J0, kodd ? Kp : Km, cz);
+        const bool ylo = jodd || ym, yhi = !jodd || yp;  // in-domain or periodic
+        const bool zlo = kodd || zm, zhi = !kodd || zp;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (yhi) vyp[e] += jodd ? cy[e] : R[e];
+            if (ylo) vym[e] += jodd ? R[e] : cy[e];
+            if (zhi) vzp[e] += kodd ? cz[e] : R[e];
+            if (zlo) vzm[e] += kodd ? R[e] : cz[e];
+        }
         float rm = __shfl_up(R[3], 1, 64), rp = __shfl_down(R[0], 1, 64);
         if (lane == 0 && !xlo) rm = cv(4 * q - 1, J0, K0);
         if (lane == 63 && !xhi) rp = cv(4 * q + 4, J0, K0);
