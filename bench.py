@@ -90,9 +90,10 @@ def summarize_pmc(d: str, counter: str) -> int:
     return 0
 
 
-def summarize_trace(d: str) -> int:
+def summarize_trace(d: str, tail: int = 0) -> int:
     """Per-kernel, per-grid launch count and average duration of a rocprofv3
-    --kernel-trace --output-format csv run of this script."""
+    --kernel-trace --output-format csv run of this script; with tail > 0 also
+    the average of the finest half-sweep's last `tail` launches."""
     import csv
     import glob
     files = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
@@ -108,13 +109,25 @@ def summarize_trace(d: str) -> int:
             grid = 1
             for ax in ("X", "Y", "Z"):
                 grid *= int(row.get(f"Grid_Size_{ax}", 1) or 1)
-            dur = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
-            c = acc.setdefault((name, grid), [0, 0])
+            start = int(row["Start_Timestamp"])
+            dur = int(row["End_Timestamp"]) - start
+            c = acc.setdefault((name, grid), [0, 0, []])
             c[0] += 1
             c[1] += dur
-    for (name, grid), (cnt, tot) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+            c[2].append((start, dur))
+    for (name, grid), (cnt, tot, _) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
         print(f"{tot / cnt / 1e3:10.2f} us avg\tcalls={cnt}\tgrid={grid}\ttotal={tot / 1e6:.3f} ms"
               f"\t{name[:110]}")
+    # the finest-level half-sweep's last launches are the timed smoother
+    # repetitions (2 per mg_smooth iteration), the launches bench.py's HIP
+    # events cover; the averages above also hold the in-solve launches
+    sweeps = [(k, v) for k, v in acc.items()
+              if "colour_split4_kernel<" in k[0] and ", 0, false>" in k[0]]
+    if sweeps and tail > 0:
+        (name, grid), (_, _, launches) = max(sweeps, key=lambda kv: kv[0][1])
+        last = [d for _, d in sorted(launches)[-tail:]]
+        print(f"{sum(last) / len(last) / 1e3:10.2f} us avg\tcalls={len(last)}\tgrid={grid}"
+              f"\tlast {len(last)} launches (timed smoother repetitions)\t{name[:80]}")
     return 0
 
 
@@ -162,7 +175,7 @@ def main():
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc)
     if args.summarize_trace:
-        return summarize_trace(args.summarize_trace)
+        return summarize_trace(args.summarize_trace, 2 * args.smoother_reps)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: one realisation's head "
                                    "solve is not sharded across GPUs",
