@@ -214,7 +214,7 @@ def main():
             "fp32": ph.CoefFormat.FP32}[args.coef]
     # plain FP64 MG-PCG starts, like the pipeline, from the linear head
     # (init_head_guess before solve_head, EnsembleRunner.cu:286-290)
-    cfg64 = ph.HeadSolveConfig.north_star(args.rtol, args.levels)
+    cfg64 = ph.HeadSolveConfig.fp64_direct(args.rtol, args.levels)
     cfg64.init_guess = ph.InitGuess.REFERENCE
     cfglin = ph.HeadSolveConfig.mixed(args.rtol, args.levels, coef)
     cfglin.init_guess = ph.InitGuess.REFERENCE

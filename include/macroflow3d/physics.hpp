@@ -191,7 +191,7 @@ struct HeadSolveConfig {
     bool flexible = false;
 
     // direct coarsest solve, ||b - A h|| <= rtol ||b|| tested every iteration, zero guess
-    static HeadSolveConfig north_star(real rtol = 1e-10, int levels = 6) {
+    static HeadSolveConfig fp64_direct(real rtol = 1e-10, int levels = 6) {
         HeadSolveConfig c;
         c.mg_levels = levels;
         c.cg_rtol = rtol;
@@ -201,10 +201,10 @@ struct HeadSolveConfig {
         c.stop_norm = StopNorm::RhsNorm;
         return c;
     }
-    // north_star with the FP32 V-cycle and 16-bit face coefficients
+    // fp64_direct with the FP32 V-cycle and 16-bit face coefficients
     static HeadSolveConfig mixed(real rtol = 1e-10, int levels = 6,
                                  CoefFormat coef = CoefFormat::BF16) {
-        HeadSolveConfig c = north_star(rtol, levels);
+        HeadSolveConfig c = fp64_direct(rtol, levels);
         c.precond_precision = PrecondPrecision::FP32;
         c.coef_format = coef;
         return c;

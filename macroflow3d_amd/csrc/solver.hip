@@ -948,7 +948,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     }
     Level& L0 = ws->lv[0];
     // the reference's own rule (||r||/||r0||, FP64) keeps its exact semantics
-    const bool judge_true =
+    const bool true_check =
         cfg.stop_norm == MF3D_STOP_RHS_NORM || ws->precision == MF3D_PRECOND_FP32;
     IterState st0 = make_state(cfg, cfg.cg_rtol);
     MF3D_TRY(state_upload(ws, s, st0));
@@ -1008,8 +1008,8 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
         // the x += alpha p of the iteration that stopped after its r update
         // (with the overlapped x step every iteration has done its own)
         if (zx && !ovl) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
-        if (ws->host_st->stop != 1 || restarts >= 3 || !judge_true) break;
-        // converged on the recursive residual: judge by the true one
+        if (ws->host_st->stop != 1 || restarts >= 3 || !true_check) break;
+        // converged on the recursive residual: decide by the true one
         MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials, &npo));
         MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_RESUME, ws->st, nullptr));
         MF3D_TRY(read_state(ws, s));
@@ -1383,7 +1383,7 @@ int capture_iteration(const BPlan& P, hipStream_t s, GraphHolder& G) {
 }
 
 // PCG of run_pcg for all realisations (FP32 preconditioner: convergence is
-// judged on the true residual, with up to 3 restarts)
+// decided on the true residual, with up to 3 restarts)
 int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
                   const mf3d_head_solve_config& cfg, const BPlan& P, bool use_graph,
                   int* restarts_out) {
@@ -1419,7 +1419,7 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
             }
             MF3D_TRY(read_states(ws, count, s));
         }
-        // the x += alpha p owed by the last iteration, then judge by the true residual
+        // the x += alpha p owed by the last iteration, then decide by the true residual
         MF3D_TRY(kern::batch_x_pending(s, BV(P.vec)));
         MF3D_TRY(kern::batch_op(s, 1, BO(P.vec), P.op_lds));
         MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RESUME, 2, true));

@@ -101,7 +101,7 @@ class HeadSolveConfig:
         return cfg
 
     @staticmethod
-    def north_star(rtol: float = 1e-10, mg_levels: int = 6) -> "HeadSolveConfig":
+    def fp64_direct(rtol: float = 1e-10, mg_levels: int = 6) -> "HeadSolveConfig":
         """Direct coarsest solve, ||r||/||b|| <= rtol checked every iteration, zero guess."""
         return HeadSolveConfig(mg_levels=mg_levels, cg_rtol=rtol, cg_check_every=1,
                                init_guess=InitGuess.ZERO, coarse_solver=CoarseSolver.DIRECT,
@@ -110,9 +110,9 @@ class HeadSolveConfig:
     @staticmethod
     def mixed(rtol: float = 1e-10, mg_levels: int = 6, coef: "CoefFormat" = None,
               flexible: bool = False) -> "HeadSolveConfig":
-        """north_star() with the FP32 V-cycle and 16-bit (default BF16) face coefficients;
+        """fp64_direct() with the FP32 V-cycle and 16-bit (default BF16) face coefficients;
         the PCG, its dot products, the operator and the coarsest solve stay FP64."""
-        c = HeadSolveConfig.north_star(rtol, mg_levels)
+        c = HeadSolveConfig.fp64_direct(rtol, mg_levels)
         c.precond_precision = PrecondPrecision.FP32
         c.coef_format = CoefFormat.BF16 if coef is None else coef
         c.flexible = flexible

@@ -13,7 +13,7 @@ training data with the plain FP64 MG-PCG solve and drives the training.
 
 The solve keeps its own guarantees: the prediction is only the initial
 iterate (``InitGuess.PROVIDED``), PCG still runs to ``||b - A h|| <= rtol ||b||``
-judged on the true residual, and a zero guess remains the fallback.
+decided on the true residual, and a zero guess remains the fallback.
 
 Command line (the GPU runs; writes FP32 parameters as .npy):
     python -m macroflow3d_amd.warmstart train --out weights.npy [--train 2000 --held 200]
@@ -144,7 +144,7 @@ def make_samples(ctx: HipContext, n: int, params, levels: int = 5, rtol: float =
     """lnK and the plain FP64 MG-PCG head for each (sigma2, corr, seed); device tensors."""
     g = Grid3D(n, n, n)
     bc = BCSpec.darcy_x()
-    cfg = ph.HeadSolveConfig.north_star(rtol, levels)
+    cfg = ph.HeadSolveConfig.fp64_direct(rtol, levels)
     ws = ph.FlowWorkspace()
     out = []
     t0 = time.perf_counter()
@@ -228,7 +228,7 @@ def compare_guesses(ctx: HipContext, net: WarmStartNet, samples, n: int, levels:
         K = torch.exp(lnK)
         row = {}
         for name in ("zero", "linear", "predicted"):
-            cfg = ph.HeadSolveConfig.north_star(rtol, levels)
+            cfg = ph.HeadSolveConfig.fp64_direct(rtol, levels)
             cfg.precond_precision = precond
             h = ctx.zeros(g.num_cells())
             torch.cuda.synchronize()

@@ -73,7 +73,7 @@ int main(int argc, char** argv) {
         write_bin(out + ".h_ref.bin", to_host(h.span(), ctx));
 
         // extension semantics in the same workspace
-        const physics::HeadSolveConfig ns = physics::HeadSolveConfig::north_star(1e-10, 3);
+        const physics::HeadSolveConfig ns = physics::HeadSolveConfig::fp64_direct(1e-10, 3);
         const physics::HeadSolveResult r_ns =
             physics::solve_head(h.span(), K.span(), grid, bc, ns, ctx, ws);
         write_bin(out + ".h_ns.bin", to_host(h.span(), ctx));

@@ -27,7 +27,7 @@ def _realisations(ctx, shapes, seed0=500):
 
 
 def test_batched_rejects_unsupported_config():
-    cfg = ph.HeadSolveConfig.north_star(1e-10, 4).c()  # FP64 preconditioner
+    cfg = ph.HeadSolveConfig.fp64_direct(1e-10, 4).c()  # FP64 preconditioner
     bc = BCSpec.darcy_x().c()
     g = (_lib.mf3d_grid * 1)(Grid3D(32, 32, 32).c())
     null = (C.c_void_p * 1)(None)

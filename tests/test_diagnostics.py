@@ -200,7 +200,7 @@ def test_divergence_of_solved_velocity_is_residual_256(ctx, workspace):
     K, _ = kfield.lognormal_K(n, n, n, 4.0, 8.0, 7, device=ctx.device)
     K = K.to(torch.float64).contiguous()
     h = ctx.zeros(g.num_cells())
-    res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.north_star(1e-10, 6), ctx, workspace)
+    res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.fp64_direct(1e-10, 6), ctx, workspace)
     assert res.converged
     vel = ph.PaddedVelocityField(g, ctx)
     ph.compute_velocity_from_head(vel, h, K, g, bc, ctx)

@@ -95,7 +95,7 @@ def test_gpu_solve_reproduces_fixture(ctx, golden, mode):
     if mode == "mode_r":
         cfg = ph.HeadSolveConfig(mg_levels=meta["mg_levels"], cg_rtol=meta["rtol"])
     else:
-        cfg = ph.HeadSolveConfig.north_star(meta["rtol"], mg_levels=meta["mg_levels"])
+        cfg = ph.HeadSolveConfig.fp64_direct(meta["rtol"], mg_levels=meta["mg_levels"])
     K = torch.as_tensor(arrays["K"]).to(ctx.device)
     h = ctx.zeros(g.num_cells())
     ws = ph.FlowWorkspace()

@@ -244,7 +244,7 @@ def test_solve_head_extension_semantics_32cubed(ctx, workspace):
     zero guess; head within 1e-8 relative L2 of an oracle solve taken to 1e-13."""
     from macroflow3d_amd import kfield
     K = np.exp(kfield.lognormal_lnK(32, 32, 32, 1.0, 4.0, 1234)[0].numpy())
-    cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
+    cfg = ph.HeadSolveConfig.fp64_direct(1e-10, mg_levels=3)
     p = O.mode_n(mg_levels=3)
     h, res, ho, ro = _solve_pair(ctx, workspace, (32, 32, 32), "darcy_x", cfg, p, K)
     assert res.converged and res.num_iterations == ro.num_iterations
@@ -308,7 +308,7 @@ def test_misaligned_head_uses_single_cell_operator(ctx, workspace):
     K = random_K(n, 22, 0.8)
     g, _ = grid_pair(*shape)
     bc, _ = bc_pair("darcy_x")
-    cfg = ph.HeadSolveConfig.north_star(1e-11, mg_levels=3)
+    cfg = ph.HeadSolveConfig.fp64_direct(1e-11, mg_levels=3)
     h = ctx.zeros(n)
     r1 = ph.solve_head(h, dev(K, ctx), g, bc, cfg, ctx, workspace)
     buf = ctx.zeros(n + 1)
@@ -327,11 +327,11 @@ def test_provided_initial_guess_keeps_caller_head(ctx, workspace):
     bc, obc = bc_pair("darcy_x")
     ho, _ = O.solve_head(og, obc, O.mode_n(mg_levels=3, cg_rtol=1e-12), K)
     h0 = ho + 1e-6 * np.random.default_rng(17).standard_normal(ho.size)
-    cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
+    cfg = ph.HeadSolveConfig.fp64_direct(1e-10, mg_levels=3)
     cfg.init_guess = ph.InitGuess.PROVIDED
     h = dev(h0, ctx)
     res = ph.solve_head(h, dev(K, ctx), g, bc, cfg, ctx, workspace)
-    cfg0 = ph.HeadSolveConfig.north_star(1e-10, mg_levels=3)
+    cfg0 = ph.HeadSolveConfig.fp64_direct(1e-10, mg_levels=3)
     h2 = ctx.zeros(K.size)
     res0 = ph.solve_head(h2, dev(K, ctx), g, bc, cfg0, ctx, workspace)
     assert res.converged and res0.converged
@@ -343,7 +343,7 @@ def test_odd_coarsest_level_and_two_cell_levels(ctx, workspace):
     """48^3 coarsens to 3^3 (odd coarsest) and 8x4x2 has a 2-cell dimension."""
     for shape, levels in (((48, 48, 48), 6), ((8, 4, 2), 4)):
         K = random_K(int(np.prod(shape)), 18, 0.5)
-        cfg = ph.HeadSolveConfig.north_star(1e-10, mg_levels=levels)
+        cfg = ph.HeadSolveConfig.fp64_direct(1e-10, mg_levels=levels)
         p = O.mode_n(mg_levels=levels)
         h, res, ho, ro = _solve_pair(ctx, workspace, shape, "darcy_x", cfg, p, K)
         assert res.converged and res.num_iterations == ro.num_iterations, shape
@@ -391,7 +391,7 @@ def test_large_grid_properties_256cubed(ctx):
     bc = BCSpec.darcy_x()
     ws = ph.FlowWorkspace()
     h = ctx.zeros(g.num_cells())
-    res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.north_star(1e-10, 6), ctx, ws)
+    res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.fp64_direct(1e-10, 6), ctx, ws)
     assert res.converged
     assert res.true_residual / res.bnorm <= 2e-10
     v = ph.VelocityField(g, ctx)
@@ -459,7 +459,7 @@ def test_fp64_march_and_colour_plans_agree(ctx):
         ws = ph.FlowWorkspace()
         ws.set_tuning(ph.FlowWorkspace.TUNE_MARCH_MIN_CELLS_FP64, thr)
         h = ctx.zeros(g.num_cells())
-        res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.north_star(1e-10, 4), ctx, ws)
+        res = ph.solve_head(h, K, g, bc, ph.HeadSolveConfig.fp64_direct(1e-10, 4), ctx, ws)
         out.append((host(h), res.num_iterations))
     assert out[0][1] == out[1][1]
     assert rel_l2(out[0][0], out[1][0]) < 1e-12
