@@ -177,7 +177,7 @@ def main():
     if args.summarize_trace:
         return summarize_trace(args.summarize_trace, 2 * args.smoother_reps)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only: one realisation's head "
+        print(json.dumps({"error": "bench.py runs on one GPU only: one realisation's head "
                                    "solve is not sharded across GPUs",
                           "n_gpus": args.gpus}))
         return 2

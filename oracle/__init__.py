@@ -2,8 +2,8 @@
 
 ctypes binding of ``libmf3d_oracle.so`` (built from ``mf3d_oracle.cpp`` by the
 Makefile next to this file). Only ``tests/``, ``__graft_entry__.smoke()`` and
-the CPU-baseline leg of ``bench.py`` may import this module, and only as the
-checker; the product package ``macroflow3d_amd`` never imports it.
+the CPU-baseline leg of ``bench.py`` may import this module, and only to check
+GPU results against; the product package ``macroflow3d_amd`` never imports it.
 
 Each wrapper names the reference function it restates (MacroFlow3D source
 paths, file:line).

@@ -2,8 +2,8 @@
 
 numpy restatement of the encoder-decoder that libmf3d_hip runs on BF16 MFMA
 kernels (macroflow3d_amd/csrc/cnn.hip); it has no counterpart in MacroFlow3D.
-Only tests/ may import it, as the checker of the GPU forward and backward
-passes. With ``bf16=True`` every tensor the GPU stores in BF16 (inputs,
+Only tests/ may import it, to check the GPU forward and backward passes
+against. With ``bf16=True`` every tensor the GPU stores in BF16 (inputs,
 weights, activations, activation gradients) is rounded to BF16 at the same
 place, so the remaining difference is FP32-vs-FP64 accumulation; with
 ``bf16=False`` it is the exact FP64 function, whose gradient the tests pin

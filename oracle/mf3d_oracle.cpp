@@ -2,8 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY. This file is a host restatement of the reference
 // algorithm, used by the tests, by __graft_entry__.smoke() and by bench.py's
-// CPU-baseline leg as the checker. The product (macroflow3d_amd) never links,
-// loads or calls it.
+// CPU-baseline leg, to check the GPU results against. The product
+// (macroflow3d_amd) never links, loads or calls it.
 //
 // Every function names the reference file:line it restates (paths relative to
 // the MacroFlow3D source root). The reference path is CUDA C++ and cannot be
