@@ -15,8 +15,13 @@ Also measured here:
     the headline V-cycle): algorithmic bytes per launch / average launch time
     from HIP events on the solve stream; HBM traffic per launch from the
     rocprofv3 FETCH_SIZE/WRITE_SIZE summaries committed under profiles/;
-  * CPU baseline: the CPU FP64 oracle (OpenMP) on the same K field, a bounded
-    sample of PCG iterations extrapolated to the FP64 solve's iteration count.
+  * CPU baseline: the CPU FP64 oracle (OpenMP) solving the same 256^3 K to
+    1e-10 (set-up + solve, median of 3), plus one oracle solve to 1e-13 that
+    the GPU heads and velocity checksums are compared with;
+  * 512^3 (sigma2_lnK = 4, correlation length 16, 7 levels): plain FP64 and
+    mixed-precision solve times, head/velocity error of mixed vs FP64;
+  * the uneven Monte-Carlo batch (32 x 48^3, 32 x 64^3, 32 x 96^3): solves/s
+    of the batched solve and of one-after-another solves.
 
 Prints one JSON line. `--summarize-pmc DIR COUNTER` instead prints the
 per-dispatch counter totals of a rocprofv3 --pmc run of this script.
@@ -55,7 +60,10 @@ def parse():
     ap.add_argument("--coef", choices=["bf16", "fp16", "fp32"], default="bf16")
     ap.add_argument("--smoother-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=3, help="timed oracle solves (median)")
+    ap.add_argument("--no-512", action="store_true", help="skip the 512^3 solves")
+    ap.add_argument("--no-batch", action="store_true", help="skip the Monte-Carlo batch")
+    ap.add_argument("--batch-per-size", type=int, default=32)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"))
     ap.add_argument("--summarize-trace", metavar="DIR")
@@ -131,29 +139,66 @@ def summarize_trace(d: str, tail: int = 0) -> int:
     return 0
 
 
-def cpu_baseline(args, K_host, gpu_iters):
-    """Oracle at the same size: setup + 1 PCG iteration, then setup + 1 + cpu_iters
-    iterations; per-iteration time from the difference, extrapolated."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def checksums(a):
+    """(sum, sum of squares, max |.|) in FP64: the velocity checksums the
+    reference prints (velocity_from_head.cu:942-962)."""
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64)
+    return np.array([a.sum(), float(np.dot(a, a)), float(np.abs(a).max())])
+
+
+def cpu_baseline(args, K_host, gpu):
+    """The oracle's plain FP64 MG-PCG (direct coarsest, ||b - A h|| <= rtol ||b||,
+    linear initial head like the GPU's FP64 solve) on the same K: set-up +
+    solve timed, median of --cpu-reps runs; then one oracle solve to 1e-13 as
+    the accuracy reference of the GPU heads (gpu: name -> (head, (U, V, W)) as
+    host arrays)."""
+    import numpy as np
+
     import oracle as O
     threads = O.num_threads()
     g = O.grid(args.n, args.n, args.n)
     b = O.darcy_bc()
-    times = []
-    for its in (1, 1 + args.cpu_iters):
-        p = O.mode_n(mg_levels=args.levels, cg_rtol=args.rtol, cg_max_iter=its, init_guess=0)
+    p = O.mode_n(mg_levels=args.levels, cg_rtol=args.rtol, init_guess=0)
+    times, its = [], 0
+    for _ in range(max(1, args.cpu_reps)):
         t0 = time.perf_counter()
-        O.solve_head(g, b, p, K_host)
+        _, r = O.solve_head(g, b, p, K_host)
         times.append(time.perf_counter() - t0)
+        its = r.num_iterations
         log(args, f"cpu oracle: {its} PCG iterations in {times[-1]:.2f} s")
-    per_iter = max(times[1] - times[0], 1e-9) / args.cpu_iters
-    est = times[0] + (gpu_iters - 1) * per_iter
-    return {"value": round(est * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
-            "sample": (f"CPU FP64 oracle (OpenMP, {threads} threads) on the same {args.n}^3 K: "
-                       f"setup+1 and setup+{1 + args.cpu_iters} PCG iterations timed "
-                       f"({times[0]:.2f} s, {times[1]:.2f} s), extrapolated to the plain FP64 "
-                       f"GPU solve's {gpu_iters} iterations"),
-            "per_iteration_ms": round(per_iter * 1e3, 1),
-            "setup_plus_first_iteration_ms": round(times[0] * 1e3, 1)}
+    t0 = time.perf_counter()
+    ho, _ = O.solve_head(g, b, O.mode_n(mg_levels=args.levels, cg_rtol=1e-13), K_host)
+    vo = O.velocity_compact(g, b, ho, K_host)
+    log(args, f"cpu oracle: 1e-13 reference solve + velocities in {time.perf_counter() - t0:.2f} s")
+    parity = {}
+    for name, (h, vel) in gpu.items():
+        worst = 0.0
+        for comp, ref in zip(vel, vo):
+            cg, co = checksums(comp), checksums(ref)
+            scale = np.array([np.abs(ref).sum(), co[1], co[2]])
+            worst = max(worst, float(np.max(np.abs(cg - co) / scale)))
+        parity[name] = {"head_rel_l2_vs_oracle": float(np.linalg.norm(h - ho) / np.linalg.norm(ho)),
+                        "velocity_checksums_max_rel_dev_vs_oracle": worst}
+    ref_sums = {c: checksums(a).tolist() for c, a in zip("UVW", vo)}
+    med = float(np.median(times))
+    return ({"value": round(med * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+             "sample": (f"CPU FP64 oracle (OpenMP, {threads} threads, {cpu_model()}) solving the "
+                        f"same {args.n}^3 K to ||b - A h|| <= {args.rtol:g} ||b|| from the linear "
+                        f"head (set-up + solve, {its} PCG iterations), median of {len(times)}"),
+             "runs_ms": [round(t * 1e3, 1) for t in times], "iterations": its},
+            parity, ref_sums)
 
 
 def committed_traffic(n: int, coef: str):
@@ -357,11 +402,130 @@ def main():
             "avg_launch_us": round(t_launch_ms * 1e3, 2),
         },
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, K.cpu().numpy(), r64.num_iterations)
-    print(json.dumps(out), flush=True)
     ws.close()
+    if not args.no_cpu:
+        torch.cuda.synchronize()
+        gpu_heads = {"fp64": (h64.cpu().numpy(), [v64.U.cpu().numpy(), v64.V.cpu().numpy(),
+                                                  v64.W.cpu().numpy()]),
+                     "headline": (h.cpu().numpy(), [vel.U.cpu().numpy(), vel.V.cpu().numpy(),
+                                                    vel.W.cpu().numpy()])}
+        out["cpu_baseline"], parity, sums = cpu_baseline(args, K.cpu().numpy(), gpu_heads)
+        out["parity_vs_cpu_oracle"] = parity
+        out["oracle_velocity_checksums"] = sums
+        log(args, "parity vs oracle: " + json.dumps(parity))
+        del gpu_heads
+    del K, h, h64, vel, v64
+    torch.cuda.empty_cache()
+    if not args.no_512:
+        out.update(bench_512(args, ctx, coef))
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        out.update(bench_batch(args, ctx))
+    print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_512(args, ctx, coef):
+    """512^3, sigma2_lnK = 4, correlation length 16, 7 levels: plain FP64 and
+    mixed-precision solve + velocities from the linear head (K resident)."""
+    import torch
+
+    from macroflow3d_amd import physics as ph
+    from macroflow3d_amd.core import BCSpec, Grid3D
+    from macroflow3d_amd.kfield import lognormal_lnK
+    n, levels = 512, 7
+    g, bc = Grid3D(n, n, n), BCSpec.darcy_x()
+    t0 = time.perf_counter()
+    Y, _ = lognormal_lnK(n, n, n, 4.0, 16.0, 3031, device=ctx.device)
+    K = torch.exp(Y)
+    del Y
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log(args, f"K field 512^3 generated in {time.perf_counter() - t0:.2f} s")
+    out = {}
+    heads = {}
+    for name, cfg in (("fp64", ph.HeadSolveConfig.fp64_direct(args.rtol, levels)),
+                      ("mixed", ph.HeadSolveConfig.mixed(args.rtol, levels, coef))):
+        cfg.init_guess = ph.InitGuess.REFERENCE
+        ws = ph.FlowWorkspace()
+        h = ctx.zeros(g.num_cells())
+        v = ph.VelocityField(g, ctx)
+        res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)  # warm-up
+        ph.compute_velocity_from_head(v, h, K, g, bc, ctx)
+        reps = 2 if name == "fp64" else 3
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+            ph.compute_velocity_from_head(v, h, K, g, bc, ctx)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / reps
+        ws.close()
+        out[f"{name}_ms_512"] = round(ms, 3)
+        out[f"{name}_iterations_512"] = res.num_iterations
+        out[f"{name}_rel_true_residual_512"] = res.true_residual / res.bnorm
+        heads[name] = (h, v)
+        log(args, f"512^3 {name}: {res.num_iterations} iterations, {ms:.2f} ms per solve")
+        torch.cuda.empty_cache()
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    (h64, v64), (hmx, vmx) = heads["fp64"], heads["mixed"]
+    out["head_rel_l2_vs_fp64_512"] = rel(hmx, h64)
+    out["velocity_rel_l2_vs_fp64_512"] = max(rel(vmx.U, v64.U), rel(vmx.V, v64.V),
+                                             rel(vmx.W, v64.W))
+    out["speedup_vs_fp64_512"] = round(out["fp64_ms_512"] / out["mixed_ms_512"], 3)
+    return out
+
+
+def bench_batch(args, ctx):
+    """The uneven Monte-Carlo batch (32 x 48^3, 32 x 64^3, 32 x 96^3; sigma2_lnK
+    ~ U[0.5, 4], correlation length ~ U[2, 8]; mixed precision, 6 levels, K
+    resident): one batched solve of all realisations vs one solve after another."""
+    import numpy as np
+    import torch
+
+    from macroflow3d_amd import ensemble
+    from macroflow3d_amd import physics as ph
+    from macroflow3d_amd.core import BCSpec
+    reals = ensemble.monte_carlo_set(4000, args.batch_per_size)
+    Ks = ensemble.generate_fields(ctx, reals)
+    grids = [re.grid for re in reals]
+    bc = BCSpec.darcy_x()
+    cfg = ph.HeadSolveConfig.mixed(args.rtol, 6)
+    ws = [ph.FlowWorkspace() for _ in reals]
+    hs = [ctx.zeros(gr.num_cells()) for gr in grids]
+    res = ph.solve_head_batched(hs, Ks, grids, bc, cfg, ctx, ws)  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        res = ph.solve_head_batched(hs, Ks, grids, bc, cfg, ctx, ws)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t) / reps
+    for w in ws:
+        w.close()
+    ensemble.solve_sequential(ctx, reals[:4], Ks[:4], bc, cfg)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, rseq = ensemble.solve_sequential(ctx, reals, Ks, bc, cfg)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t
+    its = [r.num_iterations for r in res]
+    log(args, f"batch of {len(reals)}: {t_batch * 1e3:.1f} ms batched, {t_seq * 1e3:.1f} ms "
+              f"sequential")
+    return {"batch_workload": f"{len(reals)} realisations: " + ", ".join(
+                f"{args.batch_per_size} x {n}^3" for n in (48, 64, 96)),
+            "batched_solves_per_s": round(len(reals) / t_batch, 1),
+            "sequential_solves_per_s": round(len(reals) / t_seq, 1),
+            "batched_ms": round(t_batch * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
+            "batch_iterations_min": int(min(its)), "batch_iterations_mean": float(np.mean(its)),
+            "batch_iterations_max": int(max(its)),
+            "batch_same_iterations_as_sequential": all(
+                a.num_iterations == b.num_iterations for a, b in zip(res, rseq)),
+            "batch_all_converged": all(r.converged and r.true_residual <= args.rtol * r.bnorm
+                                       for r in res)}
 
 
 if __name__ == "__main__":

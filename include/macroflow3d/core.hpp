@@ -258,8 +258,29 @@ inline bool pin_enabled(PinMode mode, const BCSpec& bc) {
 }
 
 namespace io {
+// PinConfig (src/io/config/Config.hpp:52-54)
 struct PinConfig {
     PinMode mode = PinMode::Auto;
+};
+
+// FlowYamlConfig (src/io/config/Config.hpp:86-114): the `flow:` block of the
+// pipeline's YAML as the parser leaves it; HeadSolveConfig::from_yaml turns it
+// into solver settings. Same members and defaults (no YAML parser here: the
+// caller fills the struct).
+struct FlowYamlConfig {
+    std::string solver = "mg";  // "mg", "cg", "mg_cg" / "pcg_mg"
+    int mg_levels = 4;
+    int mg_pre_smooth = 2;
+    int mg_post_smooth = 2;
+    int mg_coarse_iters = 50;
+    int mg_max_cycles = 20;
+    int cg_max_iter = 1000;
+    real cg_rtol = 1e-8;
+    int cg_check_every = 10;
+    real rtol = 1e-6;
+    BCSpec bc;
+    PinConfig pin;
+    bool verify_velocity = false;
 };
 }  // namespace io
 

@@ -33,6 +33,9 @@ class MGHierarchy {
     MGHierarchy(const MGHierarchy&) = delete;
     MGHierarchy& operator=(const MGHierarchy&) = delete;
     MGHierarchy(MGHierarchy&& o) noexcept : ws_(o.ws_) { o.ws_ = nullptr; }
+    // MGHierarchy(grid, num_levels) (mg_types.hpp:75-98): levels and vectors
+    // allocated now (halving rule, stopping before a dimension below 2)
+    MGHierarchy(const Grid3D& grid, int num_levels) : MGHierarchy() { reserve(grid, num_levels); }
     MGHierarchy& operator=(MGHierarchy&& o) noexcept {
         std::swap(ws_, o.ws_);
         return *this;
@@ -59,6 +62,12 @@ class MGHierarchy {
         c.pin_mode = pin == PinMode::On ? MF3D_PIN_ON : pin == PinMode::Off ? MF3D_PIN_OFF
                                                                             : MF3D_PIN_AUTO;
         setup(ctx, grid, bc, K, c);
+    }
+
+    // sizes the hierarchy and the solve vectors for grid (a no-op when they fit)
+    void reserve(const Grid3D& grid, int num_levels) {
+        const mf3d_grid g = to_c(grid);
+        mf3d_check(mf3d_workspace_reserve(ws_, &g, num_levels), "mf3d_workspace_reserve");
     }
 
     int num_levels() const { return mf3d_mg_num_levels(ws_); }

@@ -150,10 +150,27 @@ struct PaddedVelocityField {
 };
 
 // ---- workspace ------------------------------------------------------------------
-// Holds the MG hierarchy and the Krylov vectors of solve_head (all inside
-// mg_hierarchy's device workspace, sized lazily for the grid it last saw).
+// FlowWorkspace (src/physics/common/workspaces.cuh:111-167). The MG hierarchy,
+// the Krylov vectors and the right-hand side of solve_head all live in
+// mg_hierarchy's device workspace, sized for the grid it last saw: solve_head
+// sizes it on first use (solve_head.cu:89-94), allocate() does so up front.
 struct FlowWorkspace {
     multigrid::MGHierarchy mg_hierarchy;
+    size_t n_cells = 0;
+
+    FlowWorkspace() = default;
+    // records the grid; the device memory is taken by the first solve
+    void allocate(const Grid3D& grid) { n_cells = grid.num_cells(); }
+    // the hierarchy of mg_levels levels and every solve vector, now
+    void allocate(const Grid3D& grid, int mg_levels) {
+        allocate(grid);
+        if (mg_levels > 0) mg_hierarchy.reserve(grid, mg_levels);
+    }
+    bool is_allocated() const { return n_cells > 0; }
+    void clear() {
+        mg_hierarchy = multigrid::MGHierarchy();
+        n_cells = 0;
+    }
     size_t device_bytes() const { return mg_hierarchy.device_bytes(); }
     void set_tuning(int key, long value) { mg_hierarchy.set_tuning(key, value); }
 };
@@ -189,6 +206,29 @@ struct HeadSolveConfig {
     CoefFormat coef_format = CoefFormat::BF16;
     bool true_residual = true;
     bool flexible = false;
+
+    HeadSolveConfig() = default;
+
+    // HeadSolveConfig::from_yaml (solve_head.cuh:98-130): "cg" -> CG, "mg" ->
+    // MG, "pcg_mg" / "mg_cg" and any other string -> PCG_MG; the numbers and
+    // the pin mode are copied; the extensions keep their defaults.
+    static HeadSolveConfig from_yaml(const io::FlowYamlConfig& flow_cfg) {
+        HeadSolveConfig cfg;
+        if (flow_cfg.solver == "cg") cfg.solver_type = HeadSolverType::CG;
+        else if (flow_cfg.solver == "mg") cfg.solver_type = HeadSolverType::MG;
+        else cfg.solver_type = HeadSolverType::PCG_MG;
+        cfg.mg_levels = flow_cfg.mg_levels;
+        cfg.mg_pre_smooth = flow_cfg.mg_pre_smooth;
+        cfg.mg_post_smooth = flow_cfg.mg_post_smooth;
+        cfg.mg_max_cycles = flow_cfg.mg_max_cycles;
+        cfg.mg_coarse_iters = flow_cfg.mg_coarse_iters;
+        cfg.cg_max_iter = flow_cfg.cg_max_iter;
+        cfg.cg_rtol = flow_cfg.cg_rtol;
+        cfg.cg_check_every = flow_cfg.cg_check_every;
+        cfg.rtol = flow_cfg.rtol;
+        cfg.pin = flow_cfg.pin;
+        return cfg;
+    }
 
     // direct coarsest solve, ||b - A h|| <= rtol ||b|| tested every iteration, zero guess
     static HeadSolveConfig fp64_direct(real rtol = 1e-10, int levels = 6) {

@@ -122,6 +122,11 @@ int mf3d_workspace_create(mf3d_workspace** ws);
 int mf3d_workspace_destroy(mf3d_workspace* ws);
 /* Device bytes currently held by the workspace. */
 size_t mf3d_workspace_bytes(const mf3d_workspace* ws);
+/* FlowWorkspace::allocate(grid, mg_levels) (src/physics/common/workspaces.cuh:146-151)
+ * with MGHierarchy(grid, num_levels) (src/multigrid/mg_types.hpp:75-98): the
+ * hierarchy (halving rule) and every solve vector for this grid, allocated now
+ * rather than by the first solve; a no-op when the workspace already fits. */
+int mf3d_workspace_reserve(mf3d_workspace* ws, const mf3d_grid* grid, int mg_levels);
 
 /* ---- physics --------------------------------------------------------- */
 /* physics::solve_head (src/physics/flow/solve_head.cuh:158-160,
@@ -203,14 +208,6 @@ int mf3d_field_stats(mf3d_workspace* ws, mf3d_stream stream, const double* x, si
 int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, const double* b,
                         const double* K, size_t n, int num_iters, const mf3d_bc* bc, int pin);
 
-/* Extension of mf3d_gsrb_smooth_3d: with a caller scratch buffer of n doubles
- * the levels it supports (nx >= 32, ny >= 16, nz >= 8, z not periodic) run the
- * one-pass red-black kernel (red and black of one iteration in a single
- * sweep). zchunk: planes per workgroup (<= 0: default). Same results. */
-int mf3d_gsrb_smooth_3d_scratch(mf3d_stream stream, const mf3d_grid* grid, double* x,
-                                const double* b, const double* K, size_t n, int num_iters,
-                                const mf3d_bc* bc, int pin, double* scratch, int zchunk);
-
 /* multigrid::compute_residual_3d (residual_3d.cuh:30-32, residual_3d.cu:378-503). */
 int mf3d_compute_residual_3d(mf3d_stream stream, const mf3d_grid* grid, const double* x,
                              const double* b, const double* K, double* r, size_t n,
@@ -230,21 +227,13 @@ int mf3d_prolong_3d_add(mf3d_stream stream, const mf3d_grid* coarse, const mf3d_
 int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
                   const mf3d_bc* bc, const mf3d_head_solve_config* cfg, const double* K,
                   size_t n);
-/* Tuning knobs of a workspace (extension; the autotuner sets them). */
+/* Tuning knobs of a workspace (extension; the autotuner sets them). Keys
+ * 1-3 and 6-9 of earlier versions (one-pass march smoother, overlapped x
+ * step) no longer exist and are rejected as unknown. */
 enum {
-    MF3D_TUNE_MARCH_MIN_CELLS_FP64 = 1, /* levels with >= this many cells use the march kernel */
-    MF3D_TUNE_MARCH_MIN_CELLS_FP32 = 2,
-    MF3D_TUNE_ZCHUNK = 3,               /* planes per march workgroup (<= 0: default) */
     MF3D_TUNE_SPLIT = 4,                /* 1 (default): colour-split FP32 V-cycle vectors */
     MF3D_TUNE_SWEEP_BLOCK = 5,          /* threads per workgroup of the colour-split sweeps
                                            (64..512, multiple of 64; default 256) */
-    MF3D_TUNE_MARCH_MAX_CELLS_FP64 = 6, /* ... and <= this many (default: no limit) */
-    MF3D_TUNE_MARCH_MAX_CELLS_FP32 = 7,
-    MF3D_TUNE_X_OVERLAP = 8,            /* L > 0: PCG x update on a second stream from V-cycle
-                                           level L on (overlapping the coarse levels);
-                                           0 (default): fused with the p update */
-    MF3D_TUNE_X_STEP_BLOCKS = 9,        /* workgroups of that x update (0, default: one per
-                                           2048 cells) */
     MF3D_TUNE_FOLD_FIRST = 10,          /* 1 (default): a colour-split coarse level's first red
                                            half-sweep is computed by the restriction kernel */
     MF3D_TUNE_FOLD_R = 11,              /* 1 (default): the FP32-preconditioned PCG's residual

@@ -81,13 +81,13 @@ SIGNATURES = {
                                        _vp]),
     "mf3d_field_stats": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "mf3d_gsrb_smooth_3d": (_i, [_vp, _g, _vp, _vp, _vp, _sz, _i, _b, _i]),
-    "mf3d_gsrb_smooth_3d_scratch": (_i, [_vp, _g, _vp, _vp, _vp, _sz, _i, _b, _i, _vp, _i]),
     "mf3d_compute_residual_3d": (_i, [_vp, _g, _vp, _vp, _vp, _vp, _sz, _b, _i]),
     "mf3d_restrict_3d": (_i, [_vp, _g, _g, _vp, _sz, _vp, _sz]),
     "mf3d_prolong_3d_add": (_i, [_vp, _g, _g, _vp, _sz, _vp, _sz]),
     "mf3d_mg_setup": (_i, [_vp, _vp, _g, _b, _cfg, _vp, _sz]),
     "mf3d_mg_num_levels": (_i, [_vp]),
     "mf3d_workspace_set_tuning": (_i, [_vp, _i, C.c_long]),
+    "mf3d_workspace_reserve": (_i, [_vp, _g, _i]),
     "mf3d_mg_smooth": (_i, [_vp, _vp, _i, _i]),
     "mf3d_mg_precondition": (_i, [_vp, _vp, _vp, _vp, _sz]),
     "mf3d_varcoeff_apply": (_i, [_vp, _g, _b, _i, _vp, _vp, _vp, _sz]),

@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmf3d_hip.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["stencil.hip", "stencil_fast.hip", "faces.hip", "blas.hip", "velocity.hip", "solver.hip",
+SOURCES = ["stencil.hip", "faces.hip", "blas.hip", "velocity.hip", "solver.hip",
            "cnn.hip"]
 
 HIPCC_FLAGS = [
@@ -46,20 +46,27 @@ def _compile(src: str) -> str:
     return obj
 
 
-DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_solve.cpp")
+# C++ drivers over include/macroflow3d (host code only, linked to the library)
+DRIVERS = ["dropin_solve", "ensemble_dropin"]
 DROPIN_BIN = os.path.join(BUILD, "dropin_solve")
+ENSEMBLE_BIN = os.path.join(BUILD, "ensemble_dropin")
 
 
-def build_dropin() -> str | None:
-    """The C++ drop-in driver (tests/cpp/dropin_solve.cpp) over include/macroflow3d."""
-    if not os.path.exists(DROPIN_SRC):
-        return None
-    cmd = [hipcc(), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
-           DROPIN_SRC, "-L" + HERE, "-lmf3d_hip", "-Wl,-rpath,$ORIGIN/..", "-o", DROPIN_BIN]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed on dropin_solve.cpp:\n{r.stdout}\n{r.stderr}")
-    return DROPIN_BIN
+def build_dropin() -> list[str]:
+    """The C++ drop-in drivers (tests/cpp/*.cpp) over include/macroflow3d."""
+    out = []
+    for name in DRIVERS:
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        if not os.path.exists(src):
+            continue
+        dst = os.path.join(BUILD, name)
+        cmd = [hipcc(), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+               src, "-L" + HERE, "-lmf3d_hip", "-Wl,-rpath,$ORIGIN/..", "-o", dst]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {name}.cpp:\n{r.stdout}\n{r.stderr}")
+        out.append(dst)
+    return out
 
 
 def build(jobs: int = 8, verbose: bool = False) -> str:

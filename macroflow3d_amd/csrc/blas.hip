@@ -264,31 +264,6 @@ __global__ void __launch_bounds__(kBlock)
     if (i < n) x[i] = alpha * p[i] + x[i];
 }
 
-// x += xa p (xa = IterState::xa, zero when the iteration was skipped): the x
-// half of the PCG update, run apart from the p update so that it can overlap
-// other work. 8 cells per thread as the reductions.
-__global__ void __launch_bounds__(kBlock)
-    pcg_x_step_kernel(const IterState* __restrict__ st, const double* __restrict__ p,
-                      double* __restrict__ x, size_t n) {
-    const double a = st->xa;
-    if (a == 0.0) return;  // uniform across the grid
-    const long nb = (long)((n + kReduceCells - 1) / kReduceCells);
-    for (long blk = blockIdx.x; blk < nb; blk += gridDim.x) {  // grid may be smaller
-        double pv[kPer], xv[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const long i = red_load(blk, u, (long)n);
-            pv[u] = p[i];
-            xv[u] = x[i];
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const long i = red_cell(blk, u);
-            if (i < (long)n) x[i] = a * pv[u] + xv[u];
-        }
-    }
-}
-
 // PCG p = z + beta p (pcg.cuh:179, axpby(1, z, beta, p)).
 __global__ void __launch_bounds__(kBlock)
     pcg_update_p_kernel(const IterState* __restrict__ st, const double* __restrict__ z,
@@ -414,14 +389,6 @@ int pcg_update_r(hipStream_t s, const IterState* st, const double* Ap, double* r
 }
 int pcg_x_pending(hipStream_t s, const IterState* st, const double* p, double* x, size_t n) {
     pcg_x_pending_kernel<<<grid_for(n), kBlock, 0, s>>>(st, p, x, n);
-    MF3D_LAUNCH_CHECK();
-    return MF3D_OK;
-}
-int pcg_x_step(hipStream_t s, const IterState* st, const double* p, double* x, size_t n,
-               unsigned max_blocks) {
-    unsigned g = reduce_blocks(n);
-    if (max_blocks > 0 && g > max_blocks) g = max_blocks;
-    pcg_x_step_kernel<<<g, kBlock, 0, s>>>(st, p, x, n);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }

@@ -747,167 +747,6 @@ __global__ void __launch_bounds__(kSmallBlock)
     small_body<T, TC>(xs, d, bc, pin, bs, x, b, FX, FY, FZ, iters, zero_init, xc);
 }
 
-// ---- one-pass red-black march (fine levels) ----------------------------------
-constexpr int TX = 64, TY = 16, HALO = 2;
-constexpr int OX = TX - 2 * HALO, OY = TY - 2 * HALO;
-
-__device__ __forceinline__ void lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-struct MarchGeom {
-    int tiles_x, tiles_y, chunks, zchunk, nblocks, per_xcd;
-};
-
-// A workgroup owns a 64 x 16 column tile and marches a z-chunk: at plane k it
-// updates the red cells of plane k on the tile minus a 1-column rim, then the
-// black cells of plane k-1 on the tile minus a 2-column rim, and writes plane
-// k-1 (the reference's red pass then black pass, gsrb_3d.cu:392-575, in one
-// sweep: red reads only old black values, black only new red values). Reads
-// come from xin, writes go to xout, so overlapping tiles never see each
-// other's updates. x of planes k and k-1 are exchanged through LDS; the
-// column's own planes and face coefficients are in registers, prefetched two
-// planes ahead into compile-time slots (the march is unrolled by 5) so no
-// prefetch is waited for before its use. Workgroups are ordered so that each
-// XCD (blockIdx % 8) takes a contiguous run of neighbouring tiles.
-template <typename T, typename TC, bool ZERO, bool PROLONG>
-__global__ void __launch_bounds__(TX * TY)
-    march_kernel(Dims d, BcTypes bc, int pin, T bs, MarchGeom geo, const T* __restrict__ xin,
-                 T* __restrict__ xout, const T* __restrict__ b, const TC* __restrict__ FX,
-                 const TC* __restrict__ FY, const TC* __restrict__ FZ,
-                 const T* __restrict__ xc) {
-    __shared__ T sx[3][TY][TX];  // ring of planes k, k-1 and the one being refilled
-    const int nx = d.nx, ny = d.ny, nz = d.nz;
-    const int bid = blockIdx.x;
-    const int t = (bid & 7) * geo.per_xcd + (bid >> 3);
-    if (t >= geo.nblocks) return;  // padding blocks: uniform exit before any barrier
-    const int tile_x = t % geo.tiles_x;
-    const int rem = t / geo.tiles_x;
-    const int tile_y = rem % geo.tiles_y;
-    const int chunk = rem / geo.tiles_y;
-    const int z0 = chunk * geo.zchunk;
-    const int z1 = min(nz, z0 + geo.zchunk);
-
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    int i = tile_x * OX - HALO + tx;
-    int j = tile_y * OY - HALO + ty;
-    bool active = true;
-    if (i < 0 || i >= nx) {
-        if (bc.t[0] == MF3D_BC_PERIODIC) i = (i + nx) % nx; else active = false;
-    }
-    if (j < 0 || j >= ny) {
-        if (bc.t[2] == MF3D_BC_PERIODIC) j = (j + ny) % ny; else active = false;
-    }
-    const bool in_red = tx >= 1 && tx < TX - 1 && ty >= 1 && ty < TY - 1;
-    const bool in_out = tx >= HALO && tx < TX - HALO && ty >= HALO && ty < TY - HALO &&
-                        (tile_x * OX - HALO + tx) < nx && (tile_y * OY - HALO + ty) < ny;
-    // non-periodic domain edges: the neighbour value there is 0
-    const bool bxp = (i == nx - 1) && bc.t[1] != MF3D_BC_PERIODIC;
-    const bool bxm = (i == 0) && bc.t[0] != MF3D_BC_PERIODIC;
-    const bool byp = (j == ny - 1) && bc.t[3] != MF3D_BC_PERIODIC;
-    const bool bym = (j == 0) && bc.t[2] != MF3D_BC_PERIODIC;
-    const long plane = (long)nx * ny;
-    const long col = (long)i + (long)nx * j;
-    const int NXc = nx >> 1, NYc = ny >> 1;
-    const long ccol = (long)(i >> 1) + (long)NXc * (j >> 1);
-    const long cplane = (long)NXc * NYc;
-    const long fxcol = (long)i + (long)(nx + 1) * j, fxplane = (long)(nx + 1) * ny;
-    const long fycol = (long)i + (long)nx * j, fyplane = (long)nx * (ny + 1);
-
-    // plane p of this column: x, b and the faces x+, x-, y+, y-, z+ (z- is
-    // the z+ of plane p-1, except at p = 0)
-    auto load = [&](int p, T& xv, T& bv, T (&f)[5], T& fzm) {
-        if (active && p >= 0 && p < nz) {
-            const long o = col + plane * p;
-            bv = b[o];
-            T v = T(0);
-            if (!ZERO) v = xin[o];
-            if (PROLONG) v += xc[ccol + cplane * (p >> 1)];
-            xv = v;
-            f[0] = (T)cload(FX, fxcol + 1 + fxplane * p);
-            f[1] = (T)cload(FX, fxcol + fxplane * p);
-            f[2] = (T)cload(FY, fycol + nx + fyplane * p);
-            f[3] = (T)cload(FY, fycol + fyplane * p);
-            f[4] = (T)cload(FZ, col + plane * (p + 1));
-            fzm = (T)cload(FZ, col + plane * p);
-        } else {
-            xv = T(0);
-            bv = T(0);
-#pragma unroll
-            for (int q = 0; q < 5; ++q) f[q] = T(0);
-            fzm = T(0);
-        }
-    };
-
-    const int kstart = z0 - 1;
-    T X[5], B[5], F[5][5], FZM[5];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        X[c] = T(0);
-        B[c] = T(0);
-        FZM[c] = T(0);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) F[c][q] = T(0);
-    }
-    load(kstart - 1, X[4], B[4], F[4], FZM[4]);
-    load(kstart, X[0], B[0], F[0], FZM[0]);
-    load(kstart + 1, X[1], B[1], F[1], FZM[1]);
-
-    auto step = [&](auto cc, int k) {
-        constexpr int c = decltype(cc)::value;
-        constexpr int m2 = (c + 3) % 5, m1 = (c + 4) % 5, p1 = (c + 1) % 5, p2 = (c + 2) % 5;
-        load(k + 2, X[p2], B[p2], F[p2], FZM[p2]);  // consumed two steps later
-        // slot of plane k in the 3-plane ring; plane k-1 is in slot (s + 2) % 3.
-        // The slot written here was last read two steps ago, before this
-        // step's barriers' predecessors, so two barriers per plane suffice.
-        const int s = (k - kstart) % 3;
-        sx[s][ty][tx] = X[c];
-        lds_sync();
-        // red cells of plane k on the tile minus its outer ring
-        if (k >= 0 && k < nz && in_red && active && ((i + j + k) & 1) == 0) {
-            const T f[6] = {F[c][0], F[c][1], F[c][2], F[c][3], F[c][4], FZM[c]};
-            const T v[6] = {bxp ? T(0) : sx[s][ty][tx + 1], bxm ? T(0) : sx[s][ty][tx - 1],
-                            byp ? T(0) : sx[s][ty + 1][tx], bym ? T(0) : sx[s][ty - 1][tx],
-                            k == nz - 1 ? T(0) : X[p1], k == 0 ? T(0) : X[m1]};
-            X[c] = gs_value(f, v, B[c], bs, pin && i == 0 && j == 0 && k == 0);
-            sx[s][ty][tx] = X[c];
-        }
-        lds_sync();
-        // black cells of plane p = k-1 on the output tile, then write plane p
-        const int p = k - 1;
-        if (p >= z0 && p < z1 && active && in_out) {
-            if (((i + j + p) & 1) == 1) {
-                const int q = (s + 2) % 3;
-                const T f[6] = {F[m1][0], F[m1][1], F[m1][2], F[m1][3], F[m1][4], FZM[m1]};
-                const T v[6] = {bxp ? T(0) : sx[q][ty][tx + 1], bxm ? T(0) : sx[q][ty][tx - 1],
-                                byp ? T(0) : sx[q][ty + 1][tx], bym ? T(0) : sx[q][ty - 1][tx],
-                                p == nz - 1 ? T(0) : X[c], p == 0 ? T(0) : X[m2]};
-                X[m1] = gs_value(f, v, B[m1], bs, pin && i == 0 && j == 0 && p == 0);
-            }
-            xout[col + plane * p] = X[m1];
-        }
-    };
-
-    using I0 = std::integral_constant<int, 0>;
-    using I1 = std::integral_constant<int, 1>;
-    using I2 = std::integral_constant<int, 2>;
-    using I3 = std::integral_constant<int, 3>;
-    using I4 = std::integral_constant<int, 4>;
-    for (int k = kstart; k <= z1; k += 5) {
-        step(I0{}, k);
-        if (k + 1 > z1) break;
-        step(I1{}, k + 1);
-        if (k + 2 > z1) break;
-        step(I2{}, k + 2);
-        if (k + 3 > z1) break;
-        step(I3{}, k + 3);
-        if (k + 4 > z1) break;
-        step(I4{}, k + 4);
-    }
-}
-
 // ---- residual fused with restriction -----------------------------------------
 // r = b - A x (smoother convention, pin adds aC x_C at cell 0,
 // residual_3d.cu:365-368) of the 8 children of one coarse cell, averaged in
@@ -1125,7 +964,7 @@ __global__ void __launch_bounds__(kBlock)
     if (o < n) H[o] = 0.5 * (1.0 / K[o]);
 }
 
-// ---- prolongation for levels without the march kernel -----------------------
+// ---- prolongation for natural-order levels ------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     prolong_kernel(Dims d, const T* __restrict__ c, T* __restrict__ f) {
@@ -1991,16 +1830,6 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-MarchGeom geometry(const mf3d_grid& g, int zchunk) {
-    MarchGeom m;
-    m.tiles_x = (g.nx + OX - 1) / OX;
-    m.tiles_y = (g.ny + OY - 1) / OY;
-    m.zchunk = zchunk;
-    m.chunks = (g.nz + zchunk - 1) / zchunk;
-    m.nblocks = m.tiles_x * m.tiles_y * m.chunks;
-    m.per_xcd = (m.nblocks + 7) / 8;
-    return m;
-}
 
 inline Dims dims(const mf3d_grid& g) { return Dims{g.nx, g.ny, g.nz}; }
 
@@ -2098,26 +1927,12 @@ int half_recip(hipStream_t s, const double* K, double* H, size_t n) {
     return MF3D_OK;
 }
 
-bool march_ok(const mf3d_grid& g, const BcTypes& bc) {
-    if (g.nx < 32 || g.ny < 16 || g.nz < 8) return false;
-    if (bc.t[4] == MF3D_BC_PERIODIC) return false;  // the z march does not wrap
-    if ((bc.t[0] == MF3D_BC_PERIODIC && (g.nx & 1)) || (bc.t[2] == MF3D_BC_PERIODIC && (g.ny & 1)))
-        return false;  // odd periodic sizes break the two-colouring across the wrap
-    return true;
-}
-
-int march_default_zchunk(const mf3d_grid& g) {
-    const long tiles = (long)((g.nx + OX - 1) / OX) * ((g.ny + OY - 1) / OY);
-    int z = 32;
-    while (z > 4 && tiles * ((g.nz + z - 1) / z) < 1024) z >>= 1;
-    return z;
-}
 
 namespace {
 
 template <typename T, typename TC>
 int smooth_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
-             T* x, T* tmp, const T* b, const FaceSet& F, int iters, bool zero_init,
+             T* x, const T* b, const FaceSet& F, int iters, bool zero_init,
              const T* xc, const SmoothPlan& plan) {
     const Dims d = dims(g);
     const TC* FX = cptr<TC>(F, 0);
@@ -2138,31 +1953,6 @@ int smooth_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, dou
         small_kernel<T, TC><<<1, kSmallBlock, 0, s>>>(d, bc, pin ? 1 : 0, tbs, x, b, FX, FY, FZ,
                                                       iters, zero_init ? 1 : 0, xc);
         MF3D_HIP_TRY(hipGetLastError());
-        return MF3D_OK;
-    }
-    if (plan.kind == SMOOTH_MARCH && tmp && march_ok(g, bc)) {
-        const int zc = plan.zchunk > 0 ? plan.zchunk : march_default_zchunk(g);
-        const MarchGeom geo = geometry(g, zc);
-        const unsigned nb = (unsigned)geo.per_xcd * 8;
-        // outputs alternate so the last sweep writes x; a virtual (zero) first
-        // input lets an odd count start on x, otherwise an odd count ends in tmp
-        T* out = ((iters & 1) && zero_init) ? x : tmp;
-        const T* in = x;
-        for (int it = 0; it < iters; ++it) {
-            if (it == 0 && zero_init)
-                march_kernel<T, TC, true, false><<<nb, dim3(TX, TY), 0, s>>>(
-                    d, bc, pin ? 1 : 0, tbs, geo, nullptr, out, b, FX, FY, FZ, nullptr);
-            else if (it == 0 && xc)
-                march_kernel<T, TC, false, true><<<nb, dim3(TX, TY), 0, s>>>(
-                    d, bc, pin ? 1 : 0, tbs, geo, in, out, b, FX, FY, FZ, xc);
-            else
-                march_kernel<T, TC, false, false><<<nb, dim3(TX, TY), 0, s>>>(
-                    d, bc, pin ? 1 : 0, tbs, geo, in, out, b, FX, FY, FZ, nullptr);
-            in = out;
-            out = (out == x) ? tmp : x;
-        }
-        MF3D_HIP_TRY(hipGetLastError());
-        if (in != x) MF3D_HIP_TRY(hipMemcpyAsync(x, in, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         return MF3D_OK;
     }
     if (xc) prolong_kernel<T><<<xcd_grid(n), kBlock, 0, s>>>(d, xc, x);
@@ -2194,7 +1984,7 @@ int rr_t(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double 
 }  // namespace
 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
-                 double* x, double* tmp, const double* b, const FaceSet& F, int iters,
+                 double* x, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan, int order,
                  int corder, const double* r64, bool red_done, const RUpd* ru,
                  const RzDot* rz) {
@@ -2204,7 +1994,7 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
     if (r64) return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels read b directly");
     if (order != ORDER_NATURAL || corder != ORDER_NATURAL)
         return fail(MF3D_ERR_UNSUPPORTED, "FP64 levels are stored in natural order");
-    return smooth_t<double, double>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc, plan);
+    return smooth_t<double, double>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, plan);
 }
 
 namespace {
@@ -2325,7 +2115,7 @@ bool split_ok(const mf3d_grid& g) {
 }
 
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
-                 float* x, float* tmp, const float* b, const FaceSet& F, int iters,
+                 float* x, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan, int order, int corder,
                  const double* r64, bool red_done, const RUpd* ru, const RzDot* rz) {
     if (ru && !r64) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on the b-from-r sweep");
@@ -2361,12 +2151,12 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
         return fail(MF3D_ERR_UNSUPPORTED, "a natural-order level cannot take a split coarse level");
     switch (F.type) {
     case COEF_F32:
-        return smooth_t<float, float>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc, plan);
+        return smooth_t<float, float>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc, plan);
     case COEF_BF16:
-        return smooth_t<float, __hip_bfloat16>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init,
+        return smooth_t<float, __hip_bfloat16>(s, g, bc, pin, bs, x, b, F, iters, zero_init,
                                                xc, plan);
     case COEF_F16:
-        return smooth_t<float, __half>(s, g, bc, pin, bs, x, tmp, b, F, iters, zero_init, xc,
+        return smooth_t<float, __half>(s, g, bc, pin, bs, x, b, F, iters, zero_init, xc,
                                        plan);
     default:
         return fail(MF3D_ERR_UNSUPPORTED, "FP32 smoothing needs 32- or 16-bit faces");

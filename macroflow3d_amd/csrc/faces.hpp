@@ -28,12 +28,11 @@ struct FaceSet {
 // c = (i+j+k) & 1 at c*N/2 + (i >> 1) + (nx/2)(j + ny k).
 enum Order : int { ORDER_NATURAL = 0, ORDER_SPLIT = 1 };
 
-enum SmoothKind : int { SMOOTH_COLOUR = 0, SMOOTH_SMALL = 1, SMOOTH_MARCH = 2 };
+enum SmoothKind : int { SMOOTH_COLOUR = 0, SMOOTH_SMALL = 1 };
 
 // How one level is smoothed (chosen by defaults or the autotuner).
 struct SmoothPlan {
     int kind = SMOOTH_COLOUR;
-    int zchunk = 0;    // march kernel: planes per workgroup (<= 0: default)
     int block = 256;   // colour-split sweeps: threads per workgroup (64..512)
 };
 
@@ -50,13 +49,11 @@ int faces_build_split_h(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, do
                         const double* K, FaceSet& flo, double* H);
 // H = 1 / (2 K)
 int half_recip(hipStream_t s, const double* K, double* H, size_t n);
-bool march_ok(const mf3d_grid& g, const BcTypes& bc);
-int march_default_zchunk(const mf3d_grid& g);
 // iters GSRB iterations (red then black) with face coefficients; result in x.
-// bs = cs * dx^2. tmp: n-element scratch for the march kernel. zero_init: x is
-// zero on entry. xc (nullable): coarse correction prolonged and added first.
+// bs = cs * dx^2. zero_init: x is zero on entry. xc (nullable): coarse
+// correction prolonged and added first.
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
-                 double* x, double* tmp, const double* b, const FaceSet& F, int iters,
+                 double* x, const double* b, const FaceSet& F, int iters,
                  bool zero_init, const double* xc, const SmoothPlan& plan,
                  int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
                  const double* r64 = nullptr, bool red_done = false,
@@ -71,7 +68,7 @@ int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
 // rz (colour-split level, nx % 8 == 0): the last half-sweep also writes the
 // (r, x) partials of its natural-order FP64 r, one per sweep workgroup
 int smooth_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
-                 float* x, float* tmp, const float* b, const FaceSet& F, int iters,
+                 float* x, const float* b, const FaceSet& F, int iters,
                  bool zero_init, const float* xc, const SmoothPlan& plan,
                  int order = ORDER_NATURAL, int corder = ORDER_NATURAL,
                  const double* r64 = nullptr, bool red_done = false,

@@ -30,16 +30,6 @@ int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv);
 int band_inverse_batch(hipStream_t s, const BandDesc* D, const int* start, int m, int nblocks,
                        int nmax, int wmax);
 
-// ---- fast fine-level kernels (stencil_fast.hip) ----------------------------
-// True when the z-marching kernels cover this level (size and BC limits).
-bool fast_ok(const mf3d_grid& g, const BcTypes& bc);
-// iters one-pass red-black iterations; result in x, tmp = n-element scratch;
-// zero_init: x is zero on entry; xc (nullable): coarse correction added first;
-// zchunk <= 0 picks the default z-chunk.
-int gsrb_march(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double* x,
-               double* tmp, const double* b, const double* K, int iters, bool zero_init,
-               const double* xc, int zchunk);
-
 // ---- BLAS-1 and reductions (blas.hip) ---------------------------------------
 int fill(hipStream_t s, double* x, size_t n, double v);
 int copy(hipStream_t s, const double* x, double* y, size_t n);
@@ -63,10 +53,6 @@ int pcg_update_xr(hipStream_t s, const IterState* st, const double* p, const dou
 int pcg_update_r(hipStream_t s, const IterState* st, const double* Ap, double* r, size_t n,
                  double* partials);
 int pcg_x_pending(hipStream_t s, const IterState* st, const double* p, double* x, size_t n);
-// x += IterState::xa p (the x step of the iteration whose (r, r) was just formed);
-// max_blocks > 0 caps the grid (workgroups then loop over the vector)
-int pcg_x_step(hipStream_t s, const IterState* st, const double* p, double* x, size_t n,
-               unsigned max_blocks = 0);
 int pcg_update_p(hipStream_t s, const IterState* st, const double* z, double* p, size_t n);
 // CG on -A: if !st->stop { x += alpha p; r -= alpha (-Ap) } with partials of (r, r)
 int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,

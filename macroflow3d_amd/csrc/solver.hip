@@ -79,10 +79,8 @@ struct Level {
     double* x = nullptr;    // FP64 V-cycle correction (levels > 0)
     double* b = nullptr;    // FP64 V-cycle right-hand side (levels > 0)
     double* K = nullptr;    // coarsened conductivity (levels > 0)
-    double* t = nullptr;    // FP64 march scratch (level 0 uses the PCG's Ap)
     float* x32 = nullptr;   // FP32 V-cycle buffers
     float* b32 = nullptr;
-    float* t32 = nullptr;
     FaceSet f64;            // FP64 faces (operator on level 0, FP64 V-cycle)
     FaceSet flo;            // low-precision faces of the FP32 V-cycle
     bool has_f64 = false;
@@ -114,6 +112,9 @@ struct mf3d_workspace {
     hipEvent_t kc_ev = nullptr;
     hipEvent_t inv_ev = nullptr;   // the inverse is complete (recorded on coarse_stream)
     hipStream_t coarse_stream = nullptr;
+    // batched solves led by this workspace: their stream and phase events
+    hipStream_t batch_stream = nullptr;
+    hipEvent_t batch_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::thread coarse_worker;     // factorizes as soon as the coarsest K has arrived
     std::vector<double> coarse_band_h;
     int coarse_w = 0, coarse_status = 0;
@@ -149,20 +150,8 @@ struct mf3d_workspace {
     size_t hcopy_n = 0;
     bool ready = false;
     // tuning (mf3d_workspace_set_tuning / autotuner)
-    long march_min_cells[2] = {1L << 40, 1L << 40};  // FP64, FP32 V-cycle (off by default)
-    long march_max_cells[2] = {1L << 40, 1L << 40};
-    int zchunk = 0;
     bool split = true;  // colour-split FP32 levels where the sizes allow
     int sweep_block = 256;  // threads per workgroup of the colour-split sweeps
-    // PCG with z kept in x32, optionally: the x step of each iteration runs on
-    // x_stream while the main stream works through the coarse levels of the
-    // V-cycle (forked when the V-cycle reaches level x_overlap, joined before
-    // the p update, which reads the p the x step also reads). Off by default:
-    // at 256^3 it was slower than the fused x/p update at every fork level
-    // and grid size tried (the x step takes CU slots and bandwidth from the
-    // level kernels it runs beside, and the cross-stream waits cost more
-    // than the idle bandwidth it fills).
-    int x_overlap = 0;  // 0: off; else the level at which the x step forks
     bool fold_first = true;  // coarse first half-sweeps folded into the restrictions
     // PCG with z kept in x32: the r update (and its (r, r) partials, in
     // rr_partials) done by the V-cycle's first finest-level half-sweep
@@ -170,11 +159,6 @@ struct mf3d_workspace {
     double* rr_partials = nullptr;
     // ... and the (r, z) partials by its last finest-level half-sweep
     bool fold_rz = true;
-    unsigned x_step_blocks = 0;  // grid cap of the x step (0: one block per 2048 cells)
-    hipStream_t x_stream = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-    bool fork_pending = false;
-    double* fork_x = nullptr;
     size_t bytes = 0;
     std::vector<void*> owned;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -246,9 +230,9 @@ struct mf3d_workspace {
         if (kc_ev) (void)hipEventDestroy(kc_ev);
         if (inv_ev) (void)hipEventDestroy(inv_ev);
         if (coarse_stream) (void)hipStreamDestroy(coarse_stream);
-        if (x_stream) (void)hipStreamDestroy(x_stream);
-        if (fork_ev) (void)hipEventDestroy(fork_ev);
-        if (join_ev) (void)hipEventDestroy(join_ev);
+        if (batch_stream) (void)hipStreamDestroy(batch_stream);
+        for (auto& e : batch_ev)
+            if (e) (void)hipEventDestroy(e);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -579,10 +563,6 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     else
         ws->coarse_pending = false;
     if (!ws->h0) MF3D_TRY(ws->alloc(&ws->h0, cells(g)));
-    // the FP64 operator's h = 1 / (2 K); the finest mixed colour-split level
-    // builds it below, with its FP32/16-bit faces
-    if (!mixed || !(ws->split && kern::split_ok(g)))
-        MF3D_TRY(kern::half_recip(s, K, ws->h0, cells(g)));
     for (size_t l = 0; l < ws->lv.size(); ++l) {
         Level& L = ws->lv[l];
         if (!mixed) {  // FP64 faces of the FP64 V-cycle
@@ -592,13 +572,8 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
             }
             MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
         }
-        const long lo = ws->march_min_cells[mixed ? 1 : 0], hi = ws->march_max_cells[mixed ? 1 : 0];
-        const bool march = kern::march_ok(L.g, ws->bct) && (long)L.n >= lo && (long)L.n <= hi;
-        L.plan.kind = L.n <= 4096 ? SMOOTH_SMALL : march ? SMOOTH_MARCH : SMOOTH_COLOUR;
-        L.plan.zchunk = ws->zchunk;
+        L.plan.kind = L.n <= 4096 ? SMOOTH_SMALL : SMOOTH_COLOUR;
         L.plan.block = ws->sweep_block;
-        if (!mixed && L.plan.kind == SMOOTH_MARCH && l > 0 && !L.t) MF3D_TRY(ws->alloc(&L.t, L.n));
-        if (l == 0) L.t = ws->Ap;  // free while the V-cycle runs
     }
     // colour-split FP32 vectors and faces on a prefix of colour-swept levels
     // (never the coarsest, whose direct or small-level solve is natural order)
@@ -625,8 +600,13 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
         }
         if (!L.x32) MF3D_TRY(ws->alloc(&L.x32, L.n));
         if (!L.b32) MF3D_TRY(ws->alloc(&L.b32, L.n));
-        if (L.plan.kind == SMOOTH_MARCH && !L.t32) MF3D_TRY(ws->alloc(&L.t32, L.n));
     }
+    // the FP64 operator's h = 1 / (2 K): built together with the 16-bit faces
+    // when level 0 is the colour-split level of the FP32 V-cycle (decided
+    // above), on its own otherwise (FP64 V-cycle, a finest level too small
+    // for colour-split storage, a single level, odd sizes)
+    if (!(mixed && ws->lv[0].order == ORDER_SPLIT))
+        MF3D_TRY(kern::half_recip(s, K, ws->h0, cells(g)));
     return MF3D_OK;
 }
 
@@ -712,7 +692,6 @@ template <typename T> struct LevelView;
 template <> struct LevelView<double> {
     static double* x(Level& L) { return L.x; }
     static double* b(Level& L) { return L.b; }
-    static double* t(Level& L) { return L.t; }
     static const FaceSet& F(const Level& L) { return L.f64; }
     static double cs(const Level&) { return 1.0; }
     static int order(const Level&) { return ORDER_NATURAL; }
@@ -720,7 +699,6 @@ template <> struct LevelView<double> {
 template <> struct LevelView<float> {
     static float* x(Level& L) { return L.x32; }
     static float* b(Level& L) { return L.b32; }
-    static float* t(Level& L) { return L.t32; }
     static const FaceSet& F(const Level& L) { return L.flo; }
     static double cs(const Level& L) { return L.cs; }
     static int order(const Level& L) { return L.order; }
@@ -730,17 +708,6 @@ template <> struct LevelView<float> {
 // zero_init: x is zero on entry (the fills of v_cycle.cu:62 and
 // mg_preconditioner.cu:37 are folded into the first smoothing sweep); the
 // prolongation (v_cycle.cu:69) is folded into the first post-smoothing sweep.
-// enqueues the pending x step on x_stream after the work already on s
-int fork_x_step(mf3d_workspace* ws, hipStream_t s) {
-    ws->fork_pending = false;
-    MF3D_HIP_TRY(hipEventRecord(ws->fork_ev, s));
-    MF3D_HIP_TRY(hipStreamWaitEvent(ws->x_stream, ws->fork_ev, 0));
-    MF3D_TRY(kern::pcg_x_step(ws->x_stream, ws->st, ws->p, ws->fork_x, ws->nvec,
-                              ws->x_step_blocks));
-    MF3D_HIP_TRY(hipEventRecord(ws->join_ev, ws->x_stream));
-    return MF3D_OK;
-}
-
 // red_done: level l's first red half-sweep from x = 0 was folded into the
 // restriction from level l - 1
 template <typename T>
@@ -748,7 +715,6 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
            const double* r64 = nullptr, bool red_done = false, const RUpd* ru = nullptr,
            const RzDot* rz = nullptr) {
     using V = LevelView<T>;
-    if ((int)l == ws->x_overlap && ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
     const size_t nl = ws->lv.size();
     Level& L = ws->lv[l];
     const double dx2 = L.g.dx * L.g.dx;
@@ -759,12 +725,12 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
             MF3D_TRY(finish_coarse_inverse(ws, s));
             return kern::gemv(s, (int)L.n, ws->coarse_inv, b, x);
         }
-        return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F,
+        return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, b, F,
                                   ws->coarse_iters, zero_init, (const T*)nullptr, L.plan);
     }
     Level& C = ws->lv[l + 1];
     const int fo = V::order(L), co = V::order(C);
-    MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->pre,
+    MF3D_TRY(kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, b, F, ws->pre,
                                 zero_init, (const T*)nullptr, L.plan, fo, co, r64, red_done,
                                 ru));
     // after pre-smoothing that ends with a black half-sweep, the split
@@ -779,7 +745,7 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
                                            fold ? V::x(C) : (T*)nullptr, fold ? &V::F(C) : nullptr,
                                            bsc));
     MF3D_TRY(vcycle<T>(ws, s, l + 1, V::x(C), V::b(C), true, nullptr, fold));
-    return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, V::t(L), b, F, ws->post,
+    return kern::smooth_faces(s, L.g, ws->bct, ws->pin, cs * dx2, x, b, F, ws->post,
                               false, (const T*)V::x(C), L.plan, fo, co, nullptr, false, nullptr,
                               rz);
 }
@@ -933,19 +899,10 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     const bool flexible = cfg.flexible != 0;
     const bool zx = z_in_x32(ws, flexible);  // z kept in the finest level's x32
     double* z = zx ? nullptr : ws->z;
-    const bool ovl = zx && ws->x_overlap > 0;  // x step on x_stream, see mf3d_workspace
     // r update in the V-cycle's first sweep (its FIN_RR then follows the V-cycle)
-    const bool fuse_r = zx && !ovl && ws->fold_r && r_in_first_sweep(ws);
+    const bool fuse_r = zx && ws->fold_r && r_in_first_sweep(ws);
     const unsigned np_sweep = sweep_partials(ws);
     unsigned nprz = 0;  // (r, z) partials of the last preconditioner application
-    ws->fork_pending = false;
-    if (ovl) {
-        if (!ws->x_stream)
-            MF3D_HIP_TRY(hipStreamCreateWithFlags(&ws->x_stream, hipStreamNonBlocking));
-        if (!ws->fork_ev) MF3D_HIP_TRY(hipEventCreateWithFlags(&ws->fork_ev, hipEventDisableTiming));
-        if (!ws->join_ev) MF3D_HIP_TRY(hipEventCreateWithFlags(&ws->join_ev, hipEventDisableTiming));
-        ws->fork_x = x;
-    }
     Level& L0 = ws->lv[0];
     // the reference's own rule (||r||/||r0||, FP64) keeps its exact semantics
     const bool true_check =
@@ -990,14 +947,9 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                     MF3D_TRY(kern::dot_partials(s, ws->r, ws->z, n, ws->partials));
                     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_RZO, ws->st, nullptr));
                 }
-                ws->fork_pending = ovl;
                 MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, nullptr, &nprz));
                 MF3D_TRY(kern::finalize(s, ws->partials, nprz, FIN_RZ, ws->st, nullptr));
-                if (ovl) {
-                    if (ws->fork_pending) MF3D_TRY(fork_x_step(ws, s));
-                    MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->join_ev, 0));
-                    MF3D_TRY(kern::update_p_split8(s, ws->st, L0.x32, L0.g, ws->p, false));
-                } else if (zx) {
+                if (zx) {
                     MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
                 } else {
                     MF3D_TRY(kern::pcg_update_p(s, ws->st, ws->z, ws->p, n));
@@ -1006,8 +958,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             MF3D_TRY(read_state(ws, s));
         }
         // the x += alpha p of the iteration that stopped after its r update
-        // (with the overlapped x step every iteration has done its own)
-        if (zx && !ovl) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
+        if (zx) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
         if (ws->host_st->stop != 1 || restarts >= 3 || !true_check) break;
         // converged on the recursive residual: decide by the true one
         MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials, &npo));
@@ -1386,7 +1337,7 @@ int capture_iteration(const BPlan& P, hipStream_t s, GraphHolder& G) {
 // decided on the true residual, with up to 3 restarts)
 int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
                   const mf3d_head_solve_config& cfg, const BPlan& P, bool use_graph,
-                  int* restarts_out) {
+                  std::vector<int>& restarts) {
     const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
     const int m = P.vec.m;
     const IterState st0 = make_state(cfg, cfg.cg_rtol);
@@ -1402,7 +1353,8 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
     MF3D_TRY(kern::batch_finalize(s, V, m, FIN_R0, 0, true));
     GraphHolder G;
     if (use_graph) MF3D_TRY(capture_iteration(P, s, G));
-    int restarts = 0;
+    restarts.assign(count, 0);
+    int cycles = 0;
     for (;;) {
         // start of a cycle for the running realisations: z = M r, rz, p = z
         MF3D_TRY(precondition_batch(P, s));  // also the (r, z) partials
@@ -1424,12 +1376,15 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
         MF3D_TRY(kern::batch_op(s, 1, BO(P.vec), P.op_lds));
         MF3D_TRY(kern::batch_finalize(s, V, m, FIN_RESUME, 2, true));
         MF3D_TRY(read_states(ws, count, s));
+        // a realisation whose true residual misses the tolerance resumes from
+        // it (FIN_RESUME cleared its stop flag): that is its own restart
         bool again = false;
         for (int r = 0; r < count; ++r) again = again || !ws[r]->host_st->stop;
-        if (!again || restarts >= 3) break;
-        ++restarts;
+        if (!again || cycles >= 3) break;
+        ++cycles;
+        for (int r = 0; r < count; ++r)
+            if (!ws[r]->host_st->stop) ++restarts[r];
     }
-    *restarts_out = restarts;
     return MF3D_OK;
 }
 
@@ -1504,6 +1459,12 @@ static void clear_stale_error() { (void)hipGetLastError(); }
         if ((len) < (need))                                                     \
             return fail(MF3D_ERR_SIZE_MISMATCH, std::string(what) + " buffer too small"); \
     } while (0)
+
+int mf3d_workspace_reserve(mf3d_workspace* ws, const mf3d_grid* grid, int mg_levels) {
+    MF3D_CHECK_PTR(ws);
+    MF3D_TRY(validate_grid(grid));
+    return ensure_workspace(ws, *grid, mg_levels);
+}
 
 static int check_pair(const mf3d_grid* coarse, const mf3d_grid* fine) {
     MF3D_TRY(validate_grid(coarse));
@@ -1623,20 +1584,16 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
     }
     clear_stale_error();
     hipStream_t caller = as_stream(stream);
-    hipStream_t s = nullptr;
-    MF3D_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{s};
-    hipEvent_t ev[4];
-    for (auto& e : ev) MF3D_HIP_TRY(hipEventCreate(&e));
-    struct EventGuard {
-        hipEvent_t* e;
-        ~EventGuard() {
-            for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
-        }
-    } eguard{ev};
+    // the iteration is captured into a graph, which needs a stream of its own
+    // (not the null stream a caller may pass): kept by the first workspace
+    // and reused by its later batches, together with the phase events
+    mf3d_workspace* w0 = ws[0];
+    if (!w0->batch_stream)
+        MF3D_HIP_TRY(hipStreamCreateWithFlags(&w0->batch_stream, hipStreamNonBlocking));
+    for (auto& e : w0->batch_ev)
+        if (!e) MF3D_HIP_TRY(hipEventCreate(&e));
+    hipStream_t s = w0->batch_stream;
+    hipEvent_t* ev = w0->batch_ev;
     MF3D_HIP_TRY(hipEventRecord(ev[3], caller));
     MF3D_HIP_TRY(hipStreamWaitEvent(s, ev[3], 0));
     MF3D_HIP_TRY(hipEventRecord(ev[0], s));
@@ -1669,8 +1626,8 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
     MF3D_TRY(build_plan(P, ws, count, h));
     const auto t3 = now();
     MF3D_HIP_TRY(hipEventRecord(ev[1], s));
-    int restarts = 0;
-    MF3D_TRY(run_pcg_batch(ws, count, s, *cfg, P, !(flags & MF3D_BATCH_NO_GRAPH), &restarts));
+    std::vector<int> restarts;
+    MF3D_TRY(run_pcg_batch(ws, count, s, *cfg, P, !(flags & MF3D_BATCH_NO_GRAPH), restarts));
     if (timing) {
         MF3D_HIP_TRY(hipStreamSynchronize(s));
         std::fprintf(stderr,
@@ -1697,7 +1654,7 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
         results[r].initial_residual = st.r0;
         results[r].final_residual = std::sqrt(st.rr);
         results[r].bnorm = st.bnorm;
-        results[r].restarts = restarts;
+        results[r].restarts = restarts[r];
         results[r].setup_ms = setup;
         results[r].solve_ms = solve;
         if (st.stop == 2) status = fail(MF3D_ERR_BREAKDOWN, "PCG breakdown: (p, Ap) not finite");
@@ -1825,25 +1782,6 @@ int mf3d_gsrb_smooth_3d(mf3d_stream stream, const mf3d_grid* grid, double* x, co
                       false);
 }
 
-int mf3d_gsrb_smooth_3d_scratch(mf3d_stream stream, const mf3d_grid* grid, double* x,
-                                const double* b, const double* K, size_t n, int num_iters,
-                                const mf3d_bc* bc, int pin, double* scratch, int zchunk) {
-    MF3D_TRY(validate_grid(grid));
-    MF3D_TRY(validate_bc(bc));
-    MF3D_CHECK_PTR(x);
-    MF3D_CHECK_PTR(b);
-    MF3D_CHECK_PTR(K);
-    MF3D_CHECK_PTR(scratch);
-    MF3D_CHECK_LEN(n, cells(*grid), "x/b/K/scratch");
-    if (num_iters < 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "num_iters < 0");
-    clear_stale_error();
-    const BcTypes bt = bc_types(*bc);
-    if (kern::fast_ok(*grid, bt))
-        return kern::gsrb_march(as_stream(stream), *grid, bt, pin != 0, x, scratch, b, K,
-                                num_iters, false, nullptr, zchunk);
-    return kern::gsrb(as_stream(stream), *grid, bt, pin != 0, x, b, K, num_iters, false);
-}
-
 int mf3d_compute_residual_3d(mf3d_stream stream, const mf3d_grid* grid, const double* x,
                              const double* b, const double* K, double* r, size_t n,
                              const mf3d_bc* bc, int pin) {
@@ -1896,17 +1834,7 @@ int mf3d_mg_setup(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* grid,
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     MF3D_CHECK_PTR(ws);
     switch (key) {
-    case MF3D_TUNE_MARCH_MIN_CELLS_FP64: ws->march_min_cells[0] = value; break;
-    case MF3D_TUNE_MARCH_MIN_CELLS_FP32: ws->march_min_cells[1] = value; break;
-    case MF3D_TUNE_MARCH_MAX_CELLS_FP64: ws->march_max_cells[0] = value; break;
-    case MF3D_TUNE_MARCH_MAX_CELLS_FP32: ws->march_max_cells[1] = value; break;
-    case MF3D_TUNE_ZCHUNK: ws->zchunk = (int)value; break;
     case MF3D_TUNE_SPLIT: ws->split = value != 0; break;
-    case MF3D_TUNE_X_OVERLAP:
-        if (value < 0) return fail(MF3D_ERR_INVALID_ARGUMENT, "x overlap level must be >= 0");
-        ws->x_overlap = (int)value;
-        break;
-    case MF3D_TUNE_X_STEP_BLOCKS: ws->x_step_blocks = (unsigned)std::max(0L, value); break;
     case MF3D_TUNE_FOLD_FIRST: ws->fold_first = value != 0; break;
     case MF3D_TUNE_FOLD_R: ws->fold_r = value != 0; break;
     case MF3D_TUNE_FOLD_RZ: ws->fold_rz = value != 0; break;
@@ -1933,12 +1861,11 @@ int mf3d_mg_smooth(mf3d_workspace* ws, mf3d_stream stream, int level, int num_it
     Level& L = ws->lv[level];
     const double dx2 = L.g.dx * L.g.dx;
     if (ws->precision == MF3D_PRECOND_FP32)
-        return kern::smooth_faces(s, L.g, ws->bct, ws->pin, L.cs * dx2, L.x32, L.t32, L.b32,
-                                  L.flo, num_iters, false, (const float*)nullptr, L.plan,
-                                  L.order);
+        return kern::smooth_faces(s, L.g, ws->bct, ws->pin, L.cs * dx2, L.x32, L.b32, L.flo,
+                                  num_iters, false, (const float*)nullptr, L.plan, L.order);
     double* x = level == 0 ? ws->z : L.x;
     const double* b = level == 0 ? ws->r : L.b;
-    return kern::smooth_faces(s, L.g, ws->bct, ws->pin, dx2, x, L.t, b, L.f64, num_iters, false,
+    return kern::smooth_faces(s, L.g, ws->bct, ws->pin, dx2, x, b, L.f64, num_iters, false,
                               (const double*)nullptr, L.plan);
 }
 

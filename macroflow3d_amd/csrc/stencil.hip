@@ -4,8 +4,8 @@
 // all boundary types handled in the same launch: the reference spends 27
 // launches per colour on interior/face/edge/vertex kernels
 // (src/multigrid/smoothers/gsrb_3d.cu:392-575); here a colour is one launch.
-// These kernels serve every level and every boundary mix; the z-marching
-// fine-level kernels in stencil_fast.hip take over on large levels.
+// These kernels serve every level and every boundary mix of the
+// reference-semantics entry points (mf3d_gsrb_smooth_3d, ...).
 
 #include "kernels.hpp"
 

@@ -53,20 +53,6 @@ def gsrb_smooth_3d(ctx: HipContext, grid: Grid3D, x: torch.Tensor, b: torch.Tens
               1 if pin.enabled else 0)
 
 
-def gsrb_smooth_3d_scratch(ctx: HipContext, grid: Grid3D, x: torch.Tensor, b: torch.Tensor,
-                           K: torch.Tensor, num_iters: int, bc: BCSpec, pin: PinSpec = PinSpec(),
-                           scratch: torch.Tensor | None = None, zchunk: int = 0) -> None:
-    """gsrb_smooth_3d with an n-element scratch: eligible levels run the one-pass
-    red-black kernel (same result as the two colour passes)."""
-    bc.validate()
-    n = grid.num_cells()
-    if scratch is None:
-        scratch = ctx.empty(n)
-    _lib.call("mf3d_gsrb_smooth_3d_scratch", ctx.handle, C.byref(grid.c()), dptr(x, n, "x", ctx),
-              dptr(b, n, "b", ctx), dptr(K, n, "K", ctx), n, int(num_iters), C.byref(bc.c()),
-              1 if pin.enabled else 0, dptr(scratch, n, "scratch", ctx), int(zchunk))
-
-
 def compute_residual_3d(ctx: HipContext, grid: Grid3D, x: torch.Tensor, b: torch.Tensor,
                         K: torch.Tensor, r: torch.Tensor, bc: BCSpec,
                         pin: PinSpec = PinSpec()) -> None:

@@ -179,23 +179,18 @@ class FlowWorkspace:
     def bytes(self) -> int:
         return int(_lib.lib().mf3d_workspace_bytes(self._h))
 
-    TUNE_MARCH_MIN_CELLS_FP64 = 1
-    TUNE_MARCH_MIN_CELLS_FP32 = 2
-    TUNE_ZCHUNK = 3
     TUNE_SPLIT = 4
     TUNE_SWEEP_BLOCK = 5
-    TUNE_MARCH_MAX_CELLS_FP64 = 6
-    TUNE_MARCH_MAX_CELLS_FP32 = 7
-    TUNE_X_OVERLAP = 8
-    TUNE_X_STEP_BLOCKS = 9
     TUNE_FOLD_FIRST = 10
     TUNE_FOLD_R = 11
     TUNE_FOLD_RZ = 12
     TUNE_OP_TILE = 13
 
     def set_tuning(self, key: int, value: int) -> None:
-        """Smoother selection knobs (levels with >= value cells use the one-pass
-        red-black kernel; z-chunk of that kernel); applied at the next set-up."""
+        """Kernel-layout knobs (colour-split storage, sweep workgroup size,
+        folded PCG steps, plane-tile operator); applied at the next set-up.
+        None changes the result beyond rounding; the sweep block size and the
+        operator tile leave it bit-identical."""
         _lib.call("mf3d_workspace_set_tuning", self._h, int(key), int(value))
 
     def clear(self) -> None:

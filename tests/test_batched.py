@@ -85,26 +85,3 @@ def test_batched_second_call_reuses_workspaces(ctx):
     for w in wss:
         w.close()
 
-
-@pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(64, 64, 64), (96, 48, 40)])
-def test_overlapped_x_step_matches_fused_update(ctx, shape):
-    """The PCG x update on a second stream (tuning option) and the fused x/p
-    update (default) give the same head bit for bit and the same iterations."""
-    (g, K), = _realisations(ctx, [shape], seed0=900)
-    bc = BCSpec.darcy_x()
-    cfg = ph.HeadSolveConfig.mixed(1e-10, 4)
-    heads, iters = [], []
-    for ovl, blocks in ((1, 0), (0, 0), (2, 256)):
-        print(f"x overlap {ovl} {blocks} {shape}", flush=True)
-        ws = ph.FlowWorkspace()
-        ws.set_tuning(ph.FlowWorkspace.TUNE_X_OVERLAP, ovl)
-        ws.set_tuning(ph.FlowWorkspace.TUNE_X_STEP_BLOCKS, blocks)
-        h = ctx.zeros(g.num_cells())
-        r = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
-        assert r.converged and r.true_residual <= 1e-10 * r.bnorm
-        heads.append(h.clone())
-        iters.append(r.num_iterations)
-        ws.close()
-    assert iters[0] == iters[1] == iters[2]
-    assert torch.equal(heads[0], heads[1]) and torch.equal(heads[0], heads[2])

@@ -97,3 +97,91 @@ def test_cpp_dropin_matches_oracle(tmp_path):
     ref = CO.output(y_o, 1.0, 0.0, (n, n, n))
     assert np.abs(rd("h0").reshape(n, n, n) - ref).max() < 2e-4 / n
     assert info["warm_converged"] and info["warm_iters"] > 0
+
+
+@pytest.mark.gpu
+def test_pipeline_sequence_matches_oracle(tmp_path):
+    """tests/cpp/ensemble_dropin.cpp: the realisation loop's calls
+    (EnsembleRunner.cu:146-309) through the C++ surface, from_yaml("mg_cg")
+    included: each realisation (seed base + r) reaches the oracle's iteration
+    count and head in reference semantics, and its K statistics and velocity
+    divergence are reported."""
+    assert os.path.exists(B.ENSEMBLE_BIN), "build the package first (macroflow3d_amd.build)"
+    n, nr, base = 32, 3, 700
+    prefix = str(tmp_path / "ens")
+    print(f"ensemble_dropin n={n} realisations={nr}", flush=True)
+    r = subprocess.run([B.ENSEMBLE_BIN, str(n), str(nr), str(base), prefix], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(s) for s in r.stdout.splitlines() if s.startswith("{")]
+    assert [x["r"] for x in lines] == list(range(nr))
+    g, bc = O.grid(n, n, n), O.darcy_bc()
+    seen = set()
+    for x in lines:
+        assert x["seed"] == base + x["r"]
+        assert x["solver_type"] == 2 and x["mg_levels"] == 3 and x["cg_check_every"] == 10
+        rd = lambda s: np.fromfile(f"{prefix}.r{x['r']}.{s}.bin", dtype=np.float64)  # noqa: E731
+        K, h = rd("K"), rd("h")
+        seen.add(float(K.sum()))
+        ho, ro = O.solve_head(g, bc, O.mode_r(mg_levels=3, cg_rtol=1e-10), K)
+        assert x["converged"] and x["iters"] == ro.num_iterations
+        assert _rel(h, ho) < 1e-10
+        assert abs(x["r0"] - ro.initial_residual) <= 1e-12 * ro.initial_residual
+        assert x["kmin"] == K.min() and x["kmax"] == K.max()
+        assert abs(x["kmean"] - K.mean()) <= 1e-12 * K.mean()
+        # the padded velocity's divergence is the solve's residual: tiny
+        assert max(abs(x["div_min"]), abs(x["div_max"])) < 1e-6
+    assert len(seen) == nr  # distinct fields per realisation
+
+
+def test_from_yaml_solver_strings():
+    """HeadSolveConfig::from_yaml's solver mapping (solve_head.cuh:98-130) and
+    the FlowYamlConfig defaults (Config.hpp:86-114), compiled and run on the
+    host (no GPU call)."""
+    import tempfile
+    src = r'''
+#include "macroflow3d/physics.hpp"
+#include <cstdio>
+using namespace macroflow3d;
+int main() {
+    io::FlowYamlConfig f;
+    int bad = 0;
+    bad += physics::HeadSolveConfig::from_yaml(f).solver_type != physics::HeadSolverType::MG;
+    const char* s[5] = {"cg", "mg", "mg_cg", "pcg_mg", "other"};
+    int want[5] = {0, 1, 2, 2, 2};
+    for (int i = 0; i < 5; ++i) {
+        f.solver = s[i];
+        bad += (int)physics::HeadSolveConfig::from_yaml(f).solver_type != want[i];
+    }
+    f.mg_levels = 6; f.mg_pre_smooth = 4; f.mg_post_smooth = 3; f.mg_max_cycles = 7;
+    f.mg_coarse_iters = 9; f.cg_max_iter = 11; f.cg_rtol = 1e-9; f.cg_check_every = 5;
+    f.rtol = 1e-7; f.pin.mode = PinMode::On;
+    const physics::HeadSolveConfig c = physics::HeadSolveConfig::from_yaml(f);
+    bad += c.mg_levels != 6 || c.mg_pre_smooth != 4 || c.mg_post_smooth != 3 ||
+           c.mg_max_cycles != 7 || c.mg_coarse_iters != 9 || c.cg_max_iter != 11 ||
+           c.cg_rtol != 1e-9 || c.cg_check_every != 5 || c.rtol != 1e-7 ||
+           c.pin.mode != PinMode::On;
+    // extensions keep the reference behaviour
+    bad += c.init_guess != physics::InitGuess::Reference ||
+           c.coarse_solver != physics::CoarseSolver::GSRB ||
+           c.precond_precision != physics::PrecondPrecision::FP64;
+    io::FlowYamlConfig d;
+    bad += d.mg_levels != 4 || d.mg_pre_smooth != 2 || d.mg_post_smooth != 2 ||
+           d.mg_coarse_iters != 50 || d.mg_max_cycles != 20 || d.cg_max_iter != 1000 ||
+           d.cg_rtol != 1e-8 || d.cg_check_every != 10 || d.rtol != 1e-6 || d.verify_velocity;
+    std::printf("%d\n", bad);
+    return bad;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        cpp = os.path.join(d, "t.cpp")
+        exe = os.path.join(d, "t")
+        with open(cpp, "w") as f:
+            f.write(src)
+        lib = os.path.dirname(B.LIB)
+        r = subprocess.run([B.hipcc(), "-std=c++17", "-O1", "-Wall", "-I" + os.path.join(ROOT, "include"),
+                            cpp, "-L" + lib, "-lmf3d_hip", "-Wl,-rpath," + lib, "-o", exe],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr
