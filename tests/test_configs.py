@@ -48,6 +48,18 @@ def _solve(ctx, K, g, bc, cfg):
     return h, v, res
 
 
+def _checksum_dev(v, ref_uvw):
+    """Largest deviation of the (sum, sum of squares, max |.|) checksums of the
+    three components; the sum is taken on the scale of the component's L1 mass
+    (the V and W sums are ~0: no net transverse flux)."""
+    dev = np.zeros(3)
+    for comp, ref in zip(("U", "V", "W"), ref_uvw):
+        cg, co = checksums(getattr(v, comp).cpu().numpy()), checksums(ref)
+        scale = np.array([np.abs(ref).sum(), co[1], co[2]])
+        dev = np.maximum(dev, np.abs(cg - co) / scale)
+    return dev
+
+
 def test_config2_fp64_and_mixed_heads_match_cpu_oracle_256cubed(ctx):
     n = 256
     print(f"config 2: {n}^3 K field", flush=True)
@@ -64,25 +76,37 @@ def test_config2_fp64_and_mixed_heads_match_cpu_oracle_256cubed(ctx):
     assert r64.converged and r64.true_residual <= 1e-10 * r64.bnorm
     assert rmx.converged and rmx.true_residual <= 1e-10 * rmx.bnorm
     print(f"config 2: GPU solves done ({r64.num_iterations} / {rmx.num_iterations} iterations); "
-          f"CPU oracle solve to 1e-13", flush=True)
+          f"CPU oracle solves to 1e-10 and 1e-13", flush=True)
     Kh = K.cpu().numpy()
     og, obc = O.grid(n, n, n), O.darcy_bc()
+    # the same algorithm as the GPU FP64 solve: linear head, direct 8^3
+    # coarsest solve, ||b - A h|| <= 1e-10 ||b|| checked every iteration
+    h10, r10 = O.solve_head(og, obc, O.mode_n(mg_levels=6, cg_rtol=1e-10, init_guess=0), Kh)
+    assert r10.converged
+    v10 = O.velocity_compact(og, obc, h10, Kh)
+    e_same = _rel(h64, torch.as_tensor(h10).to(ctx.device))
+    dev_same = _checksum_dev(v64, v10)
+    print(f"config 2: GPU FP64 vs oracle, same algorithm: {r64.num_iterations} vs "
+          f"{r10.num_iterations} iterations, head rel L2 {e_same:.3e}, velocity checksums "
+          f"(sum, sum of squares, max) max rel dev {dev_same}", flush=True)
+    assert r64.num_iterations == r10.num_iterations
+    assert e_same < 1e-10
+    assert np.all(dev_same < 1e-8), dev_same
+    # accuracy reference: the oracle solved to 1e-13
     ho, ro = O.solve_head(og, obc, O.mode_n(mg_levels=6, cg_rtol=1e-13), Kh)
     assert ro.converged
-    Uo, Vo, Wo = O.velocity_compact(og, obc, ho, Kh)
+    vo = O.velocity_compact(og, obc, ho, Kh)
     hot = torch.as_tensor(ho).to(ctx.device)
     e64, emx = _rel(h64, hot), _rel(hmx, hot)
-    print(f"config 2: head rel L2 vs oracle: fp64 {e64:.3e}, mixed {emx:.3e}", flush=True)
+    d64, dmx = _checksum_dev(v64, vo), _checksum_dev(vmx, vo)
+    print(f"config 2: vs the 1e-13 oracle head: fp64 {e64:.3e}, mixed {emx:.3e}; velocity "
+          f"checksum devs fp64 {d64}, mixed {dmx}", flush=True)
     assert e64 < 1e-8 and emx < 1e-8
-    for name, v in (("fp64", v64), ("mixed", vmx)):
-        for comp, ref in (("U", Uo), ("V", Vo), ("W", Wo)):
-            cg = checksums(getattr(v, comp).cpu().numpy())
-            co = checksums(ref)
-            # sum of V and W is ~0 (no net transverse flux): compare it on the
-            # scale of the component's L1 mass
-            scale = np.array([np.abs(ref).sum(), co[1], co[2]])
-            err = np.abs(cg - co) / scale
-            assert np.all(err < 1e-8), (name, comp, cg, co, err)
+    # sums and sums of squares to 1e-8; the max |.| is one face value, whose
+    # error is the 1e-10 solve's head error times the largest face K
+    # (sigma2 = 4: K up to ~e^8), so it is held to 1e-7
+    for d in (d64, dmx):
+        assert d[0] < 1e-8 and d[1] < 1e-8 and d[2] < 1e-7, d
 
 
 def test_config3_mixed_512cubed_matches_fp64(ctx):
