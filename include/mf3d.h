@@ -269,6 +269,17 @@ int mf3d_varcoeff_apply(mf3d_stream stream, const mf3d_grid* grid, const mf3d_bc
  * (device pointer). Deterministic two-stage reduction; scratch from ws. */
 int mf3d_dot(mf3d_workspace* ws, mf3d_stream stream, const double* x, const double* y, size_t n,
              double* d_result);
+/* No counterpart in the reference: mf3d_dot without a workspace, for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). Same kernels and summation order as
+ * mf3d_dot, so the result is bitwise the same. Its partial sums live in a
+ * fixed 512 KiB device array inside the library, one per GPU, allocated with
+ * the library's code and held until the process ends: n is limited to
+ * 2^27 (= 512^3) elements, larger n returns MF3D_ERR_INVALID_ARGUMENT. Calls
+ * that overlap on one GPU (two streams, two threads) share that array and
+ * race; serialise them. Allocates nothing, ever. */
+int mf3d_dot_noalloc(mf3d_stream stream, const double* x, const double* y, size_t n,
+                     double* d_result);
 /* blas::nrm2_device (nrm2.cu:9-36) */
 int mf3d_nrm2(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
               double* d_result);

@@ -93,6 +93,7 @@ SIGNATURES = {
     "mf3d_varcoeff_apply": (_i, [_vp, _g, _b, _i, _vp, _vp, _vp, _sz]),
     "mf3d_dot": (_i, [_vp, _vp, _vp, _vp, _sz, _vp]),
     "mf3d_nrm2": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "mf3d_dot_noalloc": (_i, [_vp, _vp, _vp, _sz, _vp]),
     "mf3d_axpy": (_i, [_vp, _d, _vp, _vp, _sz]),
     "mf3d_axpby": (_i, [_vp, _d, _vp, _d, _vp, _sz]),
     "mf3d_copy": (_i, [_vp, _vp, _vp, _sz]),

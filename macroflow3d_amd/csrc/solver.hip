@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -1397,6 +1398,40 @@ int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// ---- dot product without allocation -----------------------------------
+// The first-stage partials of mf3d_dot_noalloc live in this file-scope device
+// array (one copy per GPU, held until the process ends). Host code never
+// passes the array's name to a kernel: its device address is looked up once
+// per device with hipGetSymbolAddress and kept in noalloc_ptr.
+namespace {
+constexpr size_t kNoAllocPartials = size_t(1) << 16;  // 512 KiB of doubles
+constexpr size_t kNoAllocMaxCells = kNoAllocPartials * kReduceCells;  // 2^27 = 512^3
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex noalloc_mutex;
+double* noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// device address of this GPU's scratch; the first call on a device resolves it
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    MF3D_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(MF3D_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MF3D_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                          hipGetErrorString(e));
+        noalloc_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = noalloc_ptr[dev];
+    return MF3D_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int mf3d_abi_version(void) { return MF3D_ABI_VERSION; }
@@ -1912,6 +1947,25 @@ int mf3d_dot(mf3d_workspace* ws, mf3d_stream stream, const double* x, const doub
     MF3D_TRY(scratch_partials(ws, n));
     MF3D_TRY(kern::dot_partials(s, x, y, n, ws->partials));
     return kern::finalize(s, ws->partials, kern::reduce_blocks(n), FIN_STORE, nullptr, d_result);
+}
+
+int mf3d_dot_noalloc(mf3d_stream stream, const double* x, const double* y, size_t n,
+                     double* d_result) {
+    MF3D_CHECK_PTR(d_result);
+    if (n > kNoAllocMaxCells)
+        return fail(MF3D_ERR_INVALID_ARGUMENT,
+                    "mf3d_dot_noalloc: n = " + std::to_string(n) + " exceeds the maximum " +
+                        std::to_string(kNoAllocMaxCells) + " (use mf3d_dot)");
+    clear_stale_error();
+    hipStream_t s = as_stream(stream);
+    if (n == 0) return kern::fill(s, d_result, 1, 0.0);  // dot.cu:13-18
+    MF3D_CHECK_PTR(x);
+    MF3D_CHECK_PTR(y);
+    double* part = nullptr;
+    MF3D_TRY(noalloc_scratch(&part));
+    // every partial the finalisation reads is written by this call's first stage
+    MF3D_TRY(kern::dot_partials(s, x, y, n, part));
+    return kern::finalize(s, part, kern::reduce_blocks(n), FIN_STORE, nullptr, d_result);
 }
 
 int mf3d_nrm2(mf3d_workspace* ws, mf3d_stream stream, const double* x, size_t n,
