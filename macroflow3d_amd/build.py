@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmf3d_hip.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["stencil.hip", "faces.hip", "blas.hip", "velocity.hip", "solver.hip",
+SOURCES = ["stencil.hip", "faces.hip", "split64.hip", "blas.hip", "velocity.hip", "solver.hip",
            "cnn.hip"]
 
 HIPCC_FLAGS = [

@@ -141,6 +141,35 @@ int fill_f32(hipStream_t s, float* x, size_t n, float v);
 // partials of sum(log K) for the coefficient scale
 int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials);
 
+// ---- FP64 V-cycle levels in colour-split order, faces from h = 1 / (2 K) -------
+// (split64.hip) x, b and h in the split order of ORDER_SPLIT; sizes: nx % 4 == 0,
+// ny and nz even
+bool split64_ok(const mf3d_grid& g);
+// (r, z) partials written by the finest level's last half-sweep (tb threads
+// per workgroup)
+unsigned split64_partials(const mf3d_grid& g, int tb);
+// hs = 1 / (2 K) in split order, and in natural order into hn when not null
+int split_h(hipStream_t s, const mf3d_grid& g, const double* K, double* hs, double* hn);
+int to_split64(hipStream_t s, const mf3d_grid& g, const double* nat, double* sp);
+// split -> natural; with r and partials also the (r, nat) partials (np of them)
+int from_split64(hipStream_t s, const mf3d_grid& g, const double* sp, double* nat,
+                 const double* r, double* partials, unsigned* np);
+// iters GSRB iterations (red then black), bs = dx^2, tb threads per workgroup.
+// zero_init: x is zero on entry; xc: coarse correction (order corder) prolonged
+// and added (folded into the first half-sweep); r64 (zero_init): b is taken
+// from this natural-order vector and written to b in split order; z: the
+// last half-sweep writes x to z in natural order instead, with the (rdot, z)
+// partials when rdot is given (np: their count)
+int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
+                   double* x, double* b, const double* hs, int iters, bool zero_init,
+                   const double* xc, int corder, int tb, const double* r64, double* z,
+                   const double* rdot, double* partials, unsigned* np);
+// coarse b = restriction of (b - A x) (rs = 1 / dx^2) into a coarse level of
+// order corder; red_only after a sweep that ended with the black half-sweep
+int residual_restrict_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,
+                              double rs, const double* x, const double* b, const double* hs,
+                              double* bcoarse, int corder, bool red_only);
+
 // ---- batched launches (descriptors and block starts in device memory) --------
 // kBlock-thread blocks needed for `threads` threads
 long batch_blocks(long threads);

@@ -88,6 +88,10 @@ struct Level {
     bool has_flo = false;
     double cs = 1.0;        // storage scale of flo
     int order = ORDER_NATURAL;  // storage order of x32/b32
+    // FP64 V-cycle: colour-split x, b and h = 1 / (2 K) (no stored faces), or
+    // natural order with the FP64 faces f64
+    int order64 = ORDER_NATURAL;
+    double* hs = nullptr;
     SmoothPlan plan;
 };
 
@@ -292,9 +296,10 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     MF3D_TRY(ws->alloc(&ws->z, n));
     MF3D_TRY(ws->alloc(&ws->p, n));
     MF3D_TRY(ws->alloc(&ws->Ap, n));
-    // (the finest split4 sweeps write one per workgroup of >= 64 threads)
+    // (the finest split4 sweeps write one per workgroup of >= 64 threads, the
+    // FP64 split sweeps too, with 4 cells per thread)
     ws->partials_cap =
-        std::max<size_t>({kern::reduce_blocks(n), kern::op_partials(g), n / 512}) + 64;
+        std::max<size_t>({kern::reduce_blocks(n), kern::op_partials(g), n / 256}) + 64;
     MF3D_TRY(ws->alloc(&ws->partials, ws->partials_cap));
     // one per workgroup of the finest sweep (8 cells per thread, >= 64 threads)
     MF3D_TRY(ws->alloc(&ws->rr_partials, n / 512 + 64));
@@ -564,17 +569,33 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     else
         ws->coarse_pending = false;
     if (!ws->h0) MF3D_TRY(ws->alloc(&ws->h0, cells(g)));
+    // FP64 V-cycle: colour-split levels (x, b, h = 1 / (2 K) in split order,
+    // no stored faces) on a prefix of colour-swept levels, the rest natural
+    // with stored FP64 faces; the finest split level also writes the
+    // operator's natural-order h in the same pass
+    bool prefix64 = !mixed && ws->split;
     for (size_t l = 0; l < ws->lv.size(); ++l) {
         Level& L = ws->lv[l];
-        if (!mixed) {  // FP64 faces of the FP64 V-cycle
-            if (!L.has_f64) {
-                MF3D_TRY(ws->alloc_faces(L.f64, L.g, COEF_F64));
-                L.has_f64 = true;
-            }
-            MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
-        }
         L.plan.kind = L.n <= 4096 ? SMOOTH_SMALL : SMOOTH_COLOUR;
         L.plan.block = ws->sweep_block;
+        prefix64 = prefix64 && l + 1 < ws->lv.size() && L.plan.kind == SMOOTH_COLOUR &&
+                   kern::split64_ok(L.g);
+        L.order64 = prefix64 ? ORDER_SPLIT : ORDER_NATURAL;
+        if (mixed) continue;
+        if (L.order64 == ORDER_SPLIT) {
+            if (!L.hs) MF3D_TRY(ws->alloc(&L.hs, L.n));
+            if (l == 0) {  // level 0's x and b (the others have theirs)
+                if (!L.x) MF3D_TRY(ws->alloc(&L.x, L.n));
+                if (!L.b) MF3D_TRY(ws->alloc(&L.b, L.n));
+            }
+            MF3D_TRY(kern::split_h(s, L.g, level_K(ws, l), L.hs, l == 0 ? ws->h0 : nullptr));
+            continue;
+        }
+        if (!L.has_f64) {  // FP64 faces of a natural-order level
+            MF3D_TRY(ws->alloc_faces(L.f64, L.g, COEF_F64));
+            L.has_f64 = true;
+        }
+        MF3D_TRY(kern::faces_build(s, L.g, ws->bct, 1.0, level_K(ws, l), L.f64));
     }
     // colour-split FP32 vectors and faces on a prefix of colour-swept levels
     // (never the coarsest, whose direct or small-level solve is natural order)
@@ -606,8 +627,9 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     // when level 0 is the colour-split level of the FP32 V-cycle (decided
     // above), on its own otherwise (FP64 V-cycle, a finest level too small
     // for colour-split storage, a single level, odd sizes)
-    if (!(mixed && ws->lv[0].order == ORDER_SPLIT))
-        MF3D_TRY(kern::half_recip(s, K, ws->h0, cells(g)));
+    const bool h0_built = mixed ? ws->lv[0].order == ORDER_SPLIT
+                                : ws->lv[0].order64 == ORDER_SPLIT;
+    if (!h0_built) MF3D_TRY(kern::half_recip(s, K, ws->h0, cells(g)));
     return MF3D_OK;
 }
 
@@ -751,6 +773,32 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
                               rz);
 }
 
+// The FP64 V-cycle from colour-split level l (v_cycle.cu:36-74): pre-smoothing
+// (zero start; on the finest level b taken from the natural-order r64 by the
+// first half-sweep), residual of the red children + restriction, the coarser
+// levels (split while the sizes allow, then natural with stored faces),
+// post-smoothing with the prolongation folded into its first half-sweep. z:
+// the last half-sweep writes the result there in natural order (with the
+// (rdot, z) partials when rdot is given) instead of into the split x.
+int vcycle64(mf3d_workspace* ws, hipStream_t s, size_t l, bool zero_init, const double* r64,
+             double* z, const double* rdot, double* partials, unsigned* np) {
+    Level& L = ws->lv[l];
+    Level& C = ws->lv[l + 1];  // a split level is never the coarsest
+    const double dx2 = L.g.dx * L.g.dx;
+    const int tb = L.plan.block;
+    MF3D_TRY(kern::smooth_split64(s, L.g, ws->bct, ws->pin, dx2, L.x, L.b, L.hs, ws->pre,
+                                  zero_init, nullptr, ORDER_NATURAL, tb, r64, nullptr, nullptr,
+                                  nullptr, nullptr));
+    MF3D_TRY(kern::residual_restrict_split64(s, L.g, ws->bct, ws->pin, 1.0 / dx2, L.x, L.b, L.hs,
+                                             C.b, C.order64, ws->pre > 0));
+    if (C.order64 == ORDER_SPLIT)
+        MF3D_TRY(vcycle64(ws, s, l + 1, true, nullptr, nullptr, nullptr, nullptr, nullptr));
+    else
+        MF3D_TRY(vcycle<double>(ws, s, l + 1, C.x, C.b, true));
+    return kern::smooth_split64(s, L.g, ws->bct, ws->pin, dx2, L.x, L.b, L.hs, ws->post, false,
+                                C.x, C.order64, tb, nullptr, z, rdot, partials, np);
+}
+
 // PCG may leave z = x32 unstored (the (r, z) partials and the p update read the
 // FP32 V-cycle result directly): FP32 V-cycle, colour-split finest level with
 // nx % 8 == 0, and not flexible CG (which keeps z_old).
@@ -792,6 +840,12 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
     if (np_rz) *np_rz = kern::reduce_blocks(L0.n);
     if (ws->precision == MF3D_PRECOND_FP64) {
         if (ru) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on the FP32 V-cycle");
+        if (L0.order64 == ORDER_SPLIT) {  // z and the (r, z) partials from the last sweep
+            unsigned np = 0;
+            MF3D_TRY(vcycle64(ws, s, 0, true, r, z, rz_partials ? r : nullptr, rz_partials, &np));
+            if (np_rz) *np_rz = np;
+            return MF3D_OK;
+        }
         MF3D_TRY(vcycle<double>(ws, s, 0, z, r, true));
         return rz_partials ? kern::dot_partials(s, r, z, L0.n, rz_partials) : MF3D_OK;
     }
@@ -1044,9 +1098,20 @@ int run_mg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     const double den = cfg.stop_norm == MF3D_STOP_RHS_NORM ? bnorm : r0;
     res->initial_residual = r0;
     res->bnorm = bnorm;
+    Level& L0 = ws->lv[0];
+    const bool split0 = L0.order64 == ORDER_SPLIT;
+    if (split0) {  // the iterate lives in split order between the cycles
+        MF3D_TRY(kern::to_split64(s, g, x, L0.x));
+        MF3D_TRY(kern::to_split64(s, g, ws->rhs, L0.b));
+    }
     for (int c = 0; c < cfg.mg_max_cycles; ++c) {
         res->num_iterations = c + 1;
-        MF3D_TRY(vcycle<double>(ws, s, 0, x, ws->rhs, false));
+        if (split0) {
+            MF3D_TRY(vcycle64(ws, s, 0, false, nullptr, nullptr, nullptr, nullptr, nullptr));
+            MF3D_TRY(kern::from_split64(s, g, L0.x, x, nullptr, nullptr, nullptr));
+        } else {
+            MF3D_TRY(vcycle<double>(ws, s, 0, x, ws->rhs, false));
+        }
         MF3D_TRY(kern::residual(s, g, ws->bct, ws->pin, x, ws->rhs, K, ws->r));
         MF3D_TRY(host_nrm2(ws, s, ws->r, n, &rn));
         res->final_residual = rn;
@@ -1898,6 +1963,10 @@ int mf3d_mg_smooth(mf3d_workspace* ws, mf3d_stream stream, int level, int num_it
     if (ws->precision == MF3D_PRECOND_FP32)
         return kern::smooth_faces(s, L.g, ws->bct, ws->pin, L.cs * dx2, L.x32, L.b32, L.flo,
                                   num_iters, false, (const float*)nullptr, L.plan, L.order);
+    if (L.order64 == ORDER_SPLIT)  // in place on the level's split x and b
+        return kern::smooth_split64(s, L.g, ws->bct, ws->pin, dx2, L.x, L.b, L.hs, num_iters,
+                                    false, nullptr, ORDER_NATURAL, L.plan.block, nullptr,
+                                    nullptr, nullptr, nullptr, nullptr);
     double* x = level == 0 ? ws->z : L.x;
     const double* b = level == 0 ? ws->r : L.b;
     return kern::smooth_faces(s, L.g, ws->bct, ws->pin, dx2, x, b, L.f64, num_iters, false,
