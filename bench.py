@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"))
     ap.add_argument("--summarize-trace", metavar="DIR")
+    ap.add_argument("--sweep-only", choices=["fp64", "mixed"],
+                    help="set up one hierarchy and time only its finest-level half-sweeps "
+                         "(--smoother-reps red-black iterations; for kernel counters)")
     ap.add_argument("--no-warm-start", action="store_true",
                     help="headline from the linear head instead of the learned warm start")
     ap.add_argument("--weights", default=None, help="warm-start parameters (.npy)")
@@ -127,16 +130,24 @@ def summarize_trace(d: str, tail: int = 0) -> int:
     for (name, grid), (cnt, tot, _) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
         print(f"{tot / cnt / 1e3:10.2f} us avg\tcalls={cnt}\tgrid={grid}\ttotal={tot / 1e6:.3f} ms"
               f"\t{name[:110]}")
-    # the finest-level half-sweep's last launches are the timed smoother
-    # repetitions (2 per mg_smooth iteration), the launches bench.py's HIP
-    # events cover; the averages above also hold the in-solve launches
+    # the finest-level half-sweeps' last launches at each grid are the timed
+    # smoother repetitions (2 per mg_smooth iteration), the launches bench.py's
+    # HIP events cover; the averages above also hold the in-solve launches.
+    # Mixed V-cycle: colour_split4_kernel<., 0, false>; FP64: sweep64_kernel<0, ...>
+    # (the FP64 march kernel launches the same grid on every level it runs
+    # on, so its rows above mix levels; its last launches are the finest
+    # level's timed repetitions)
     sweeps = [(k, v) for k, v in acc.items()
-              if "colour_split4_kernel<" in k[0] and ", 0, false>" in k[0]]
-    if sweeps and tail > 0:
-        (name, grid), (_, _, launches) = max(sweeps, key=lambda kv: kv[0][1])
-        last = [d for _, d in sorted(launches)[-tail:]]
-        print(f"{sum(last) / len(last) / 1e3:10.2f} us avg\tcalls={len(last)}\tgrid={grid}"
-              f"\tlast {len(last)} launches (timed smoother repetitions)\t{name[:80]}")
+              if ("colour_split4_kernel<" in k[0] and ", 0, false>" in k[0]) or
+              "march64_kernel<0, false, false>" in k[0]]
+    if tail > 0:
+        for kind in ("colour_split4_kernel<", "march64_kernel<"):
+            own = [kv for kv in sweeps if kind in kv[0][0]]
+            # the two largest grids: the finest levels of the 256^3 and 512^3 runs
+            for (name, grid), (_, _, launches) in sorted(own, key=lambda kv: -kv[0][1])[:2]:
+                last = [d for _, d in sorted(launches)[-tail:]]
+                print(f"{sum(last) / len(last) / 1e3:10.2f} us avg\tcalls={len(last)}\tgrid={grid}"
+                      f"\tlast {len(last)} launches (timed smoother repetitions)\t{name[:80]}")
     return 0
 
 
@@ -232,12 +243,79 @@ def committed_traffic(n: int, coef: str):
     return None
 
 
+def time_smoother(ctx, ws, reps):
+    """Average duration (ms) of one finest-level half-sweep of ws's V-cycle:
+    reps red-black iterations in place (mf3d_mg_smooth), HIP events on the
+    solve stream; one untimed iteration first."""
+    import torch
+
+    from macroflow3d_amd import multigrid as mg
+    mg.smooth_level(ctx, ws, 0, 1)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(ctx.stream)
+    mg.smooth_level(ctx, ws, 0, reps)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / (2 * reps)
+
+
+def sweep_only(args):
+    """One hierarchy (FP64 or mixed) on the workload's K, then --smoother-reps
+    timed red-black iterations of its finest level: the only launches of the
+    dominant half-sweep kernel in the run, for kernel traces and counters."""
+    import torch
+
+    from macroflow3d_amd import multigrid as mg
+    from macroflow3d_amd import physics as ph
+    from macroflow3d_amd.core import BCSpec, Grid3D
+    from macroflow3d_amd.kfield import lognormal_lnK
+    from macroflow3d_amd.runtime import HipContext
+    ctx = HipContext(0)
+    n = args.n
+    g = Grid3D(n, n, n)
+    K = torch.exp(lognormal_lnK(n, n, n, args.sigma2, args.corr, args.seed, device=ctx.device)[0])
+    cfg = (ph.HeadSolveConfig.fp64_direct(args.rtol, args.levels) if args.sweep_only == "fp64"
+           else ph.HeadSolveConfig.mixed(args.rtol, args.levels))
+    # counter calibration: a plain copy of 2^27 doubles (1 GiB read, 1 GiB write)
+    from macroflow3d_amd import numerics as nm
+    ca = torch.ones(1 << 27, dtype=torch.float64, device=ctx.device)
+    cb = torch.empty_like(ca)
+    nm.blas.copy(ctx, ca, cb)
+    torch.cuda.synchronize()
+    del ca, cb
+    ws = ph.FlowWorkspace()
+    mg.mg_setup(ctx, ws, g, BCSpec.darcy_x(), cfg, K)
+    t = time_smoother(ctx, ws, args.smoother_reps)
+    ws.close()
+    N = g.num_cells()
+    bpc = 20.0 if args.sweep_only == "fp64" else 12.0
+    gbs = bpc * N / (t * 1e-3) / 1e9
+    print(json.dumps({"sweep": args.sweep_only, "cells": N, "avg_launch_us": round(t * 1e3, 2),
+                      "algorithmic_bytes_per_launch": bpc * N, "achieved_GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+    return 0
+
+
+def fp64_smoother_line(t_ms, n):
+    """The FP64 V-cycle's finest half-sweep (colour-split, faces from
+    h = 1/(2K)): 20 B/cell algorithmic (h 8, other-colour x 4, own b 4, own x
+    out 4)."""
+    alg = 20.0 * n
+    gbs = alg / (t_ms * 1e-3) / 1e9
+    return {"kernel": "march64_kernel<0> (FP64 colour-split half-sweep)",
+            "avg_launch_us": round(t_ms * 1e3, 2), "algorithmic_bytes_per_launch": alg,
+            "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def main():
     args = parse()
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc)
     if args.summarize_trace:
         return summarize_trace(args.summarize_trace, 2 * args.smoother_reps)
+    if args.sweep_only:
+        return sweep_only(args)
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU only: one realisation's head "
                                    "solve is not sharded across GPUs",
@@ -313,9 +391,11 @@ def main():
         r64, ms64 = timed(cfg64, args.fp64_steps, 1, h64, v64, ws64)
     else:
         r64, ms64 = step(cfg64, h64, v64, ws64, False), float("nan")
+    t64 = time_smoother(ctx, ws64, args.smoother_reps)
     ws64.close()
     log(args, f"fp64: {r64.num_iterations} iterations, rel true residual "
-              f"{r64.true_residual / r64.bnorm:.3e}, {ms64:.2f} ms per solve")
+              f"{r64.true_residual / r64.bnorm:.3e}, {ms64:.2f} ms per solve; half-sweep "
+              f"{t64 * 1e3:.1f} us")
 
     # mixed precision from the linear head
     ws = ph.FlowWorkspace()
@@ -407,6 +487,7 @@ def main():
         "head_rel_l2_vs_fp64": head_err,
         "velocity_rel_l2_vs_fp64": vel_err,
         "velocity_diagnostics": diagnostics,
+        "fp64_smoother": fp64_smoother_line(t64, N),
         "roofline": {
             "kernel": f"colour_split4_kernel<{args.coef}> (finest level, one colour)",
             "bound": "hbm",
@@ -482,6 +563,16 @@ def bench_512(args, ctx, coef):
             ph.compute_velocity_from_head(v, h, K, g, bc, ctx)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) * 1e3 / reps
+        ts = time_smoother(ctx, ws, args.smoother_reps)
+        if name == "fp64":
+            out["fp64_smoother_512"] = fp64_smoother_line(ts, g.num_cells())
+        else:  # 12 B/cell: BF16 faces 6, other-colour x 2, own b 2, own x out 2
+            gbs = 12.0 * g.num_cells() / (ts * 1e-3) / 1e9
+            out["smoother_512"] = {"kernel": "colour_split4_kernel<bf16> (finest level, one colour)",
+                                   "avg_launch_us": round(ts * 1e3, 2),
+                                   "achieved_GBs": round(gbs, 1),
+                                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(args, f"512^3 {name} half-sweep {ts * 1e3:.1f} us")
         ws.close()
         out[f"{name}_ms_512"] = round(ms, 3)
         out[f"{name}_iterations_512"] = res.num_iterations

@@ -35,6 +35,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "faces.hpp"
 #include "hface.hpp"
 #include "reduce.hpp"
@@ -83,213 +85,264 @@ __device__ __forceinline__ double gs64(const double (&f)[6], const double (&v)[6
     return (result - b * bs) / aC;
 }
 
-// Other-colour values at the x neighbours of the thread's two cells (array O
-// of the other colour, places base and base + 1 of row r0): xp[e] / xm[e] at
-// x+ / x- of cell e; a missing neighbour (non-periodic boundary) reads 0.
-__device__ __forceinline__ void x_pair(const double* __restrict__ O, long base, long r0, int h,
-                                       int p, bool xlo, bool xhi, int lane, bool per_lo,
-                                       bool per_hi, double (&a)[2], double (&xp)[2],
-                                       double (&xm)[2]) {
-    const d2 v = ld2(O + base);
-    a[0] = v.x, a[1] = v.y;
-    double prv = __shfl_up(v.y, 1, 64);
-    double nxt = __shfl_down(v.x, 1, 64);
+// Other-colour values at the x neighbours of the thread's two cells, from the
+// other colour's pair a = (places base, base + 1) of row r0 (array O):
+// xp[e] / xm[e] at x+ / x- of cell e; the places outside the pair come from
+// the adjacent lanes, the first and last lane of a wave load them; a missing
+// neighbour (non-periodic boundary) reads 0.
+__device__ __forceinline__ void x_nb(const double* __restrict__ O, const d2 a, long base, long r0,
+                                     int h, int p, bool xlo, bool xhi, int lane, bool per_lo,
+                                     bool per_hi, double (&xp)[2], double (&xm)[2]) {
+    double prv = __shfl_up(a.y, 1, 64);
+    double nxt = __shfl_down(a.x, 1, 64);
     if (lane == 0 && !xlo) prv = O[base - 1];
     if (lane == 63 && !xhi) nxt = O[base + 2];
     if (p == 0) {  // i = 4q + 2e: x+ at place 2q + e, x- at 2q + e - 1
-        xp[0] = a[0], xp[1] = a[1];
-        xm[1] = a[0];
+        xp[0] = a.x, xp[1] = a.y;
+        xm[1] = a.x;
         xm[0] = xlo ? (per_lo ? O[r0 + h - 1] : 0.0) : prv;
     } else {       // i = 4q + 2e + 1: x- at place 2q + e, x+ at 2q + e + 1
-        xm[0] = a[0], xm[1] = a[1];
-        xp[0] = a[1];
+        xm[0] = a.x, xm[1] = a.y;
+        xp[0] = a.y;
         xp[1] = xhi ? (per_hi ? O[r0] : 0.0) : nxt;
     }
 }
 
+// Chunks of planes a sweep marches through: a workgroup owns `tiles`-th of a
+// plane's thread rows and kz consecutive planes.
+struct March {
+    int tiles;   // workgroups per plane
+    int kz;      // planes per chunk
+    int chunks;  // ceil(nz / kz)
+};
+// at least kMarchBlocks workgroups where the level allows, else one plane per chunk
+constexpr long kMarchBlocks = 2048;
+March march_of(const Geo& G, int tb) {
+    March m;
+    const long P = (long)G.h2 * G.ny;
+    m.tiles = (int)((P + tb - 1) / tb);
+    const long want = (kMarchBlocks + m.tiles - 1) / m.tiles;
+    const int chunks = (int)std::max(1L, std::min((long)G.nz, want));
+    m.kz = (G.nz + chunks - 1) / chunks;
+    m.chunks = (G.nz + m.kz - 1) / m.kz;
+    return m;
+}
+
+// What one plane of the march loads ahead (issued before the previous plane
+// is computed): the streams that come from HBM, i.e. the other colour's pair
+// one plane up (z+; it becomes the x neighbours of the next plane and the z-
+// of the one after) and the own colour's h and b. The y-neighbour rows (the
+// other colour's rows j +- 1, loaded one plane earlier as other threads' z+
+// pairs, so L2 hits) are loaded when the plane is computed.
+struct PlaneLd {
+    d2 xzp, hzp, hown, bown;
+};
+
+// One colour of GSRB on a colour-split level, marching in z. A thread owns
+// places (2q, 2q + 1) of row j in every plane of its chunk; the cells it
+// updates in plane k are i = 4q + 2e + p (p = (j + k + colour) & 1). The
+// other colour's values at those places in plane k +- 1 are exactly its z
+// neighbours, and in plane k its x neighbours (with the adjacent places from
+// the adjacent lanes), so each other-colour pair is loaded once (as z+) and
+// reused from registers twice.
 // MODE 0: plain; 1: x = 0 on entry; 2: coarse correction xc (split coarse
 // order if CS) prolonged into the neighbour values; 3: as 1 with b taken from
 // the natural-order r64 and both colours of the split b written to bw.
-// RZ (MODE 0, last half-sweep): z (natural order) of the thread's 4 natural
-// cells written, and the workgroup's (r, z) partial.
+// RZ (MODE 0, last half-sweep): z of the thread's 4 natural cells written
+// instead of x, and the workgroup's (r64, z) partial when partials is given.
 template <int MODE, bool CS, bool RZ>
-__device__ __forceinline__ void sweep_body(const Geo& G, const BcTypes& bc, int pin, double bs,
+__device__ __forceinline__ void march_body(const Geo& G, const BcTypes& bc, int pin, double bs,
                                            double* __restrict__ x, const double* __restrict__ b,
                                            const double* __restrict__ hs, int colour,
                                            const double* __restrict__ xc,
                                            const double* __restrict__ r64,
                                            double* __restrict__ bw, double* __restrict__ z,
-                                           double* __restrict__ partials, long blk) {
-    const long total = G.half >> 1;
-    const long t = blk * (long)blockDim.x + threadIdx.x;
-    const bool valid = t < total;
-    const long tt = valid ? t : total - 1;  // lanes past the end mirror the last one
+                                           double* __restrict__ partials, long blk,
+                                           const March& M) {
+    constexpr bool XR = MODE == 0 || MODE == 2;  // reads the other colour's x
+    const long P = (long)G.h2 * G.ny;
+    const int chunk = (int)(blk / M.tiles);
+    const long tp = (blk - (long)chunk * M.tiles) * (long)blockDim.x + threadIdx.x;
+    const bool valid = tp < P;
+    const long tq = valid ? tp : P - 1;  // lanes past the plane mirror its last thread
     const int lane = threadIdx.x & 63;
-    const int q = (int)(tt % G.h2);
-    const long row = tt / G.h2;
-    const int j = (int)(row % G.ny);
-    const int k = (int)(row / G.ny);
-    const int p = (j + k + colour) & 1;
-    const long r0 = row * G.h;
-    const long base = r0 + 2L * q;
-    const long own = (long)colour * G.half + base;
-    const long othc = (long)(colour ^ 1) * G.half;
-    const long oth = othc + base;
+    const int q = (int)(tq % G.h2);
+    const int j = (int)(tq / G.h2);
+    const int k0 = chunk * M.kz, k1 = min(G.nz, k0 + M.kz);
     const long sy = G.h, sz = (long)G.h * G.ny;
+    const long ownc = (long)colour * G.half, othc = (long)(colour ^ 1) * G.half;
     const bool xlo = q == 0, xhi = q == G.h2 - 1;
     const bool per_lo = bc.t[0] == MF3D_BC_PERIODIC, per_hi = bc.t[1] == MF3D_BC_PERIODIC;
-
     const Side Yp = side(j < G.ny - 1, sy, bc.t[3], -(long)(G.ny - 1) * sy);
     const Side Ym = side(j > 0, -sy, bc.t[2], (long)(G.ny - 1) * sy);
-    const Side Zp = side(k < G.nz - 1, sz, bc.t[5], -(long)(G.nz - 1) * sz);
-    const Side Zm = side(k > 0, -sz, bc.t[4], (long)(G.nz - 1) * sz);
-    // x sides of the two cells (offsets unused: the values come from x_pair)
-    Side Ex[2], Wx[2];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        Ex[e] = side(!(p == 1 && e == 1 && xhi), 0, bc.t[1], 0);
-        Wx[e] = side(!(p == 0 && e == 0 && xlo), 0, bc.t[0], 0);
-    }
+    const Side Zb = side(k0 > 0, -sz, bc.t[4], (long)(G.nz - 1) * sz);  // z- of plane k0
 
-    // h: own colour, and the other colour at the six sides
-    const d2 hcv = ld2(hs + own);
-    const double hc[2] = {hcv.x, hcv.y};
-    double ha[2], hxp[2], hxm[2];
-    x_pair(hs + othc, base, r0, G.h, p, xlo, xhi, lane, per_lo, per_hi, ha, hxp, hxm);
-    const d2 hyp = ld2(hs + oth + Yp.off), hym = ld2(hs + oth + Ym.off);
-    const d2 hzp = ld2(hs + oth + Zp.off), hzm = ld2(hs + oth + Zm.off);
+    auto load = [&](int k, long pb) {
+        const Side Zp = side(k < G.nz - 1, sz, bc.t[5], -(long)(G.nz - 1) * sz);
+        PlaneLd l;
+        l.hzp = ld2(hs + othc + pb + Zp.off);
+        l.hown = ld2(hs + ownc + pb);
+        l.xzp = XR ? ld2(x + othc + pb + Zp.off) : d2{0.0, 0.0};
+        l.bown = MODE == 3 ? d2{0.0, 0.0} : ld2(b + ownc + pb);
+        return l;
+    };
 
-    double vxp[2], vxm[2], vyp[2], vym[2], vzp[2], vzm[2], xa[2];
-    if (MODE == 1 || MODE == 3) {
-#pragma unroll
-        for (int e = 0; e < 2; ++e) vxp[e] = vxm[e] = vyp[e] = vym[e] = vzp[e] = vzm[e] = 0.0;
-    } else {
-        x_pair(x + othc, base, r0, G.h, p, xlo, xhi, lane, per_lo, per_hi, xa, vxp, vxm);
-        const d2 a = ld2(x + oth + Yp.off), c = ld2(x + oth + Ym.off);
-        const d2 u = ld2(x + oth + Zp.off), w = ld2(x + oth + Zm.off);
-        vyp[0] = Yp.nb ? a.x : 0.0, vyp[1] = Yp.nb ? a.y : 0.0;
-        vym[0] = Ym.nb ? c.x : 0.0, vym[1] = Ym.nb ? c.y : 0.0;
-        vzp[0] = Zp.nb ? u.x : 0.0, vzp[1] = Zp.nb ? u.y : 0.0;
-        vzm[0] = Zm.nb ? w.x : 0.0, vzm[1] = Zm.nb ? w.y : 0.0;
-    }
-
-    double bb[2];
-    if (MODE == 3) {
-        // natural cells 4q .. 4q + 3 of the row: own colour at 2e + p
-        const double* rp = r64 + row * G.nx + 4L * q;
-        const d2 n01 = ld2(rp), n23 = ld2(rp + 2);
-        const double nv[4] = {n01.x, n01.y, n23.x, n23.y};
-        bb[0] = nv[p], bb[1] = nv[2 + p];
-        if (valid) {
-            st2(bw + own, bb[0], bb[1]);
-            st2(bw + oth, nv[1 - p], nv[3 - p]);
-        }
-    } else {
-        const d2 bv = ld2(b + own);
-        bb[0] = bv.x, bb[1] = bv.y;
-    }
-
-    if (MODE == 2) {
-        // each in-domain (or periodic) neighbour gains its coarse parent's
-        // value: own cells' parents I = 2q + e of parent row (J0, K0); the y/z
-        // neighbours' parent rows; the x neighbours' parents I - 1 .. I + 1
-        const int NXc = G.nx >> 1, NYc = G.ny >> 1, NZc = G.nz >> 1;
-        const int J0 = j >> 1, K0 = k >> 1;
-        const long hcw = NXc >> 1, HC = hcw * NYc * NZc;
-        auto cval = [&](int I, int J, int K) -> double {
-            if (CS) return xc[(long)((I + J + K) & 1) * HC + (I >> 1) + hcw * ((long)J + (long)NYc * K)];
-            return xc[(long)I + (long)NXc * ((long)J + (long)NYc * K)];
-        };
-        auto rowpair = [&](int J, int K, double (&o)[2]) {
-            if (CS) {
-                const long rb = (long)q + hcw * ((long)J + (long)NYc * K);
-                o[0] = xc[(long)((J + K) & 1) * HC + rb];
-                o[1] = xc[(long)((J + K + 1) & 1) * HC + rb];
-            } else {
-                const d2 A = ld2(xc + 2L * q + (long)NXc * ((long)J + (long)NYc * K));
-                o[0] = A.x, o[1] = A.y;
-            }
-        };
-        double R[2], cy[2], cz[2];
-        rowpair(J0, K0, R);
-        // even sizes: an even j shares its parent row with j + 1, an odd j
-        // with j - 1; only the other y neighbour's parent row is loaded (z alike)
-        const bool jodd = j & 1, kodd = k & 1;
-        const int Jp = j + 1 < G.ny ? (j + 1) >> 1 : 0, Jm = j > 0 ? (j - 1) >> 1 : NYc - 1;
-        const int Kp = k + 1 < G.nz ? (k + 1) >> 1 : 0, Km = k > 0 ? (k - 1) >> 1 : NZc - 1;
-        rowpair(jodd ? Jp : Jm, K0, cy);
-        rowpair(J0, kodd ? Kp : Km, cz);
+    long pb = ((long)j + (long)G.ny * k0) * G.h + 2L * q;  // the pair's place in plane k0
+    // the other colour in plane k0 (x neighbours) and below it (z-)
+    d2 xo = XR ? ld2(x + othc + pb) : d2{0.0, 0.0};
+    d2 ho = ld2(hs + othc + pb);
+    d2 xzm = XR ? ld2(x + othc + pb + Zb.off) : d2{0.0, 0.0};
+    d2 hzm = ld2(hs + othc + pb + Zb.off);
+    PlaneLd cur = load(k0, pb);
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        PlaneLd nxt = cur;
+        if (k + 1 < k1) nxt = load(k + 1, pb + sz);
+        const Side Zp = side(k < G.nz - 1, sz, bc.t[5], -(long)(G.nz - 1) * sz);
+        const Side Zm = k == k0 ? Zb : side(true, -sz, bc.t[4], 0);
+        const int p = (j + k + colour) & 1;
+        const long r0 = ((long)j + (long)G.ny * k) * G.h;
+        const long row = (long)j + (long)G.ny * k;
+        Side Ex[2], Wx[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            if (Yp.nb) vyp[e] += jodd ? cy[e] : R[e];
-            if (Ym.nb) vym[e] += jodd ? R[e] : cy[e];
-            if (Zp.nb) vzp[e] += kodd ? cz[e] : R[e];
-            if (Zm.nb) vzm[e] += kodd ? R[e] : cz[e];
+            Ex[e] = side(!(p == 1 && e == 1 && xhi), 0, bc.t[1], 0);
+            Wx[e] = side(!(p == 0 && e == 0 && xlo), 0, bc.t[0], 0);
         }
-        double rm = __shfl_up(R[1], 1, 64), rpn = __shfl_down(R[0], 1, 64);
-        if (lane == 0 && !xlo) rm = cval(2 * q - 1, J0, K0);
-        if (lane == 63 && !xhi) rpn = cval(2 * q + 2, J0, K0);
-        if (p == 0) {  // x+ of cell e: parent 2q + e; x-: 2q + e - 1
-            vxp[0] += R[0], vxp[1] += R[1];
-            vxm[1] += R[0];
-            if (!xlo) vxm[0] += rm;
-            else if (per_lo) vxm[0] += cval(NXc - 1, J0, K0);
-        } else {       // x- of cell e: parent 2q + e; x+: 2q + e + 1
-            vxm[0] += R[0], vxm[1] += R[1];
-            vxp[0] += R[1];
-            if (!xhi) vxp[1] += rpn;
-            else if (per_hi) vxp[1] += cval(0, J0, K0);
+        const d2 hyp = ld2(hs + othc + pb + Yp.off), hym = ld2(hs + othc + pb + Ym.off);
+        double hxp[2], hxm[2];
+        x_nb(hs + othc, ho, pb, r0, G.h, p, xlo, xhi, lane, per_lo, per_hi, hxp, hxm);
+        double vxp[2] = {0.0, 0.0}, vxm[2] = {0.0, 0.0}, vyp[2] = {0.0, 0.0}, vym[2] = {0.0, 0.0};
+        double vzp[2] = {0.0, 0.0}, vzm[2] = {0.0, 0.0};
+        if (XR) {
+            const d2 xyp = ld2(x + othc + pb + Yp.off), xym = ld2(x + othc + pb + Ym.off);
+            x_nb(x + othc, xo, pb, r0, G.h, p, xlo, xhi, lane, per_lo, per_hi, vxp, vxm);
+            vyp[0] = Yp.nb ? xyp.x : 0.0, vyp[1] = Yp.nb ? xyp.y : 0.0;
+            vym[0] = Ym.nb ? xym.x : 0.0, vym[1] = Ym.nb ? xym.y : 0.0;
+            vzp[0] = Zp.nb ? cur.xzp.x : 0.0, vzp[1] = Zp.nb ? cur.xzp.y : 0.0;
+            vzm[0] = Zm.nb ? xzm.x : 0.0, vzm[1] = Zm.nb ? xzm.y : 0.0;
         }
-    }
-
-    double res[2];
-    const double hy[2][2] = {{hyp.x, hyp.y}, {hym.x, hym.y}};
-    const double hz[2][2] = {{hzp.x, hzp.y}, {hzm.x, hzm.y}};
+        double bb[2];
+        if (MODE == 3) {
+            // natural cells 4q .. 4q + 3 of the row: own colour at 2e + p
+            const double* rp = r64 + row * G.nx + 4L * q;
+            const d2 n01 = ld2(rp), n23 = ld2(rp + 2);
+            const double nv[4] = {n01.x, n01.y, n23.x, n23.y};
+            bb[0] = nv[p], bb[1] = nv[2 + p];
+            if (valid) {
+                st2(bw + ownc + pb, bb[0], bb[1]);
+                st2(bw + othc + pb, nv[1 - p], nv[3 - p]);
+            }
+        } else {
+            bb[0] = cur.bown.x, bb[1] = cur.bown.y;
+        }
+        if (MODE == 2) {
+            // each in-domain (or periodic) neighbour gains its coarse parent's
+            // value: own cells' parents I = 2q + e of parent row (J0, K0); the
+            // y/z neighbours' parent rows; the x neighbours' parents I - 1 .. I + 1
+            const int NXc = G.nx >> 1, NYc = G.ny >> 1, NZc = G.nz >> 1;
+            const int J0 = j >> 1, K0 = k >> 1;
+            const long hcw = NXc >> 1, HC = hcw * NYc * NZc;
+            auto cval = [&](int I, int J, int K) -> double {
+                if (CS) return xc[(long)((I + J + K) & 1) * HC + (I >> 1) + hcw * ((long)J + (long)NYc * K)];
+                return xc[(long)I + (long)NXc * ((long)J + (long)NYc * K)];
+            };
+            auto rowpair = [&](int J, int K, double (&o)[2]) {
+                if (CS) {
+                    const long rb = (long)q + hcw * ((long)J + (long)NYc * K);
+                    o[0] = xc[(long)((J + K) & 1) * HC + rb];
+                    o[1] = xc[(long)((J + K + 1) & 1) * HC + rb];
+                } else {
+                    const d2 A = ld2(xc + 2L * q + (long)NXc * ((long)J + (long)NYc * K));
+                    o[0] = A.x, o[1] = A.y;
+                }
+            };
+            double R[2], cy[2], cz[2];
+            rowpair(J0, K0, R);
+            // even sizes: an even j shares its parent row with j + 1, an odd j
+            // with j - 1; only the other y neighbour's parent row is loaded (z alike)
+            const bool jodd = j & 1, kodd = k & 1;
+            const int Jp = j + 1 < G.ny ? (j + 1) >> 1 : 0, Jm = j > 0 ? (j - 1) >> 1 : NYc - 1;
+            const int Kp = k + 1 < G.nz ? (k + 1) >> 1 : 0, Km = k > 0 ? (k - 1) >> 1 : NZc - 1;
+            rowpair(jodd ? Jp : Jm, K0, cy);
+            rowpair(J0, kodd ? Kp : Km, cz);
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const double f[6] = {hface(Ex[e], hc[e], hxp[e]), hface(Wx[e], hc[e], hxm[e]),
-                             hface(Yp, hc[e], hy[0][e]),  hface(Ym, hc[e], hy[1][e]),
-                             hface(Zp, hc[e], hz[0][e]),  hface(Zm, hc[e], hz[1][e])};
-        const double v[6] = {vxp[e], vxm[e], vyp[e], vym[e], vzp[e], vzm[e]};
-        res[e] = gs64(f, v, bb[e], bs, pin && colour == 0 && row == 0 && q == 0 && e == 0);
+            for (int e = 0; e < 2; ++e) {
+                if (Yp.nb) vyp[e] += jodd ? cy[e] : R[e];
+                if (Ym.nb) vym[e] += jodd ? R[e] : cy[e];
+                if (Zp.nb) vzp[e] += kodd ? cz[e] : R[e];
+                if (Zm.nb) vzm[e] += kodd ? R[e] : cz[e];
+            }
+            double rm = __shfl_up(R[1], 1, 64), rpn = __shfl_down(R[0], 1, 64);
+            if (lane == 0 && !xlo) rm = cval(2 * q - 1, J0, K0);
+            if (lane == 63 && !xhi) rpn = cval(2 * q + 2, J0, K0);
+            if (p == 0) {  // x+ of cell e: parent 2q + e; x-: 2q + e - 1
+                vxp[0] += R[0], vxp[1] += R[1];
+                vxm[1] += R[0];
+                if (!xlo) vxm[0] += rm;
+                else if (per_lo) vxm[0] += cval(NXc - 1, J0, K0);
+            } else {       // x- of cell e: parent 2q + e; x+: 2q + e + 1
+                vxm[0] += R[0], vxm[1] += R[1];
+                vxp[0] += R[1];
+                if (!xhi) vxp[1] += rpn;
+                else if (per_hi) vxp[1] += cval(0, J0, K0);
+            }
+        }
+        const double hc[2] = {cur.hown.x, cur.hown.y};
+        const double hy[2][2] = {{hyp.x, hyp.y}, {hym.x, hym.y}};
+        const double hz[2][2] = {{cur.hzp.x, cur.hzp.y}, {hzm.x, hzm.y}};
+        double res[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const double f[6] = {hface(Ex[e], hc[e], hxp[e]), hface(Wx[e], hc[e], hxm[e]),
+                                 hface(Yp, hc[e], hy[0][e]),  hface(Ym, hc[e], hy[1][e]),
+                                 hface(Zp, hc[e], hz[0][e]),  hface(Zm, hc[e], hz[1][e])};
+            const double v[6] = {vxp[e], vxm[e], vyp[e], vym[e], vzp[e], vzm[e]};
+            res[e] = gs64(f, v, bb[e], bs, pin && colour == 0 && row == 0 && q == 0 && e == 0);
+        }
+        if (!RZ) {
+            if (valid) st2(x + ownc + pb, res[0], res[1]);
+        } else {
+            // z of natural cells 4q .. 4q + 3: own at 2e + p, the other colour
+            // (final after the previous half-sweep) at 2e + 1 - p
+            const long zn = row * G.nx + 4L * q;
+            const double z0 = p ? xo.x : res[0], z1 = p ? res[0] : xo.x;
+            const double z2 = p ? xo.y : res[1], z3 = p ? res[1] : xo.y;
+            if (valid) {
+                st2(z + zn, z0, z1);
+                st2(z + zn + 2, z2, z3);
+                if (partials) {  // r64 is the r of the (r, z) partials here
+                    const d2 ra = ld2(r64 + zn), rb = ld2(r64 + zn + 2);
+                    acc += ra.x * z0;
+                    acc += ra.y * z1;
+                    acc += rb.x * z2;
+                    acc += rb.y * z3;
+                }
+            }
+        }
+        // march: this plane's other colour becomes the next plane's z-, the
+        // z+ pair the next plane's x neighbours
+        xzm = xo, hzm = ho;
+        xo = cur.xzp, ho = cur.hzp;
+        pb += sz;
+        cur = nxt;
     }
-    if (!RZ) {
-        if (valid) st2(x + own, res[0], res[1]);
-        return;
+    if (RZ && partials) {  // uniform
+        const double sb = block_sum_dyn(acc);
+        if (threadIdx.x == 0) partials[blk] = sb;
     }
-    // z of natural cells 4q .. 4q + 3: own at 2e + p, the other colour (read
-    // above as x neighbours, final after the previous half-sweep) at 2e + 1 - p
-    const long zn = row * G.nx + 4L * q;
-    const double z0 = p ? xa[0] : res[0], z1 = p ? res[0] : xa[0];
-    const double z2 = p ? xa[1] : res[1], z3 = p ? res[1] : xa[1];
-    if (valid) {
-        st2(z + zn, z0, z1);
-        st2(z + zn + 2, z2, z3);
-    }
-    if (!partials) return;  // uniform: no (r, z) wanted
-    double acc = 0.0;
-    if (valid) {  // r64 is the r of the (r, z) partials here
-        const d2 ra = ld2(r64 + zn), rb = ld2(r64 + zn + 2);
-        acc = ra.x * z0;
-        acc += ra.y * z1;
-        acc += rb.x * z2;
-        acc += rb.y * z3;
-    }
-    const double sb = block_sum_dyn(acc);
-    if (threadIdx.x == 0) partials[blk] = sb;
 }
 
 template <int MODE, bool CS, bool RZ>
 __global__ void __launch_bounds__(kMaxBlock)
-    sweep64_kernel(Geo G, BcTypes bc, int pin, double bs, double* __restrict__ x,
+    march64_kernel(Geo G, BcTypes bc, int pin, double bs, double* __restrict__ x,
                    const double* __restrict__ b, const double* __restrict__ hs, int colour,
                    const double* __restrict__ xc, const double* __restrict__ r64,
                    double* __restrict__ bw, double* __restrict__ z,
-                   double* __restrict__ partials) {
-    const long total = G.half >> 1;
-    const long blk = xcd_block((total + blockDim.x - 1) / blockDim.x);
+                   double* __restrict__ partials, March M) {
+    const long blk = xcd_block((long)M.tiles * M.chunks);
     if (blk < 0) return;  // whole workgroup
-    sweep_body<MODE, CS, RZ>(G, bc, pin, bs, x, b, hs, colour, xc, r64, bw, z, partials, blk);
+    march_body<MODE, CS, RZ>(G, bc, pin, bs, x, b, hs, colour, xc, r64, bw, z, partials, blk, M);
 }
 
 // residual r = b - A x of one cell of a split level (compute_residual_3d's
@@ -486,8 +539,9 @@ template <int MODE, bool CS, bool RZ>
 int launch_sweep(hipStream_t s, const Geo& G, const BcTypes& bc, bool pin, double bs, double* x,
                  const double* b, const double* hs, int colour, const double* xc,
                  const double* r64, double* bw, double* z, double* partials, int tb) {
-    sweep64_kernel<MODE, CS, RZ><<<blocks_for(G.half >> 1, tb), tb, 0, s>>>(
-        G, bc, pin ? 1 : 0, bs, x, b, hs, colour, xc, r64, bw, z, partials);
+    const March M = march_of(G, tb);
+    march64_kernel<MODE, CS, RZ><<<xcd_round((size_t)M.tiles * M.chunks), tb, 0, s>>>(
+        G, bc, pin ? 1 : 0, bs, x, b, hs, colour, xc, r64, bw, z, partials, M);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -520,8 +574,8 @@ bool split64_ok(const mf3d_grid& g) {
 }
 
 unsigned split64_partials(const mf3d_grid& g, int tb) {
-    const long total = (long)cells(g) / 4;
-    return (unsigned)((total + tb - 1) / tb);
+    const March M = march_of(geo_of(g), tb);
+    return (unsigned)M.tiles * (unsigned)M.chunks;
 }
 
 int split_h(hipStream_t s, const mf3d_grid& g, const double* K, double* hs, double* hn) {

@@ -1,9 +1,11 @@
 """CPU restatement of the index maps of the FP64 colour-split V-cycle kernels
 (split64.hip), checked against the oracle (no GPU).
 
-The emulation follows the kernels thread by thread: `sweep_body` (two places
+The emulation follows the kernels thread by thread: `march_body` (two places
 per thread, the x neighbours outside the pair taken from lane - 1 / lane + 1
-of the same wave or loaded by the first and last lane, MODE 0 plain, 1 zero
+of the same wave or loaded by the first and last lane; every value is loaded
+here from its array, where the kernel keeps the other colour's z-neighbour
+pairs in registers while it marches up the planes; MODE 0 plain, 1 zero
 start, 2 prolongation folded in with a natural or colour-split coarse level,
 3 zero start with b from the natural-order r, RZ writing z in natural order
 with the (r, z) partials), `rr64_kernel` (red children only or all eight),
@@ -101,36 +103,37 @@ def x_pair(O_, othc, base, r0, h, p, xlo, xhi, prv_lane, nxt_lane, per_lo, per_h
 
 def sweep(G, types, pin, bs, x, b, hs, colour, mode, tb=64, xc=None, cs=False, r64=None,
           z=None, rdot=None):
-    """One half-sweep as sweep_body computes it; returns the partials (RZ)."""
-    total = G.half >> 1
-    nblk = -(-total // tb)
+    """One half-sweep as march_body computes it (each plane of each column on
+    its own); returns the per-thread-row partial sums (RZ)."""
+    P = G.h2 * G.ny                 # threads of one plane
+    tiles = -(-P // tb)             # workgroups per plane
     X, B, H = Arr(x), Arr(b), Arr(hs)
     per_lo, per_hi = types[0] == PERIODIC, types[1] == PERIODIC
     sy, sz = G.h, G.h * G.ny
-    # lane values for the shuffles: per thread, the other colour's pair of h, x, R
-    def thread_geo(t):
-        tt = min(t, total - 1)
-        q = tt % G.h2
-        row = tt // G.h2
-        j, k = row % G.ny, row // G.ny
-        return tt, q, row, j, k
-    def pairs(t):
-        tt, q, row, j, k = thread_geo(t)
+    # a workgroup's lanes cover consecutive threads of one plane; lanes past
+    # the plane mirror its last thread
+    def thread_geo(k, tp):
+        tq = min(tp, P - 1)
+        q, j = tq % G.h2, tq // G.h2
+        return tq, q, j + G.ny * k, j, k
+    def pairs(k, tp):
+        tq, q, row, j, _ = thread_geo(k, tp)
         base = row * G.h + 2 * q
         othc = (colour ^ 1) * G.half
         hp = H.ld2(othc + base)
         xp_ = X.ld2(othc + base) if mode in (0, 2) else (0.0, 0.0)
         R = None
         if mode == 2:
-            R = rowpair(G, xc, cs, q, row % G.ny >> 1, row // G.ny >> 1)
+            R = rowpair(G, xc, cs, q, j >> 1, k >> 1)
         return hp, xp_, R
-    cache = {t: pairs(t) for t in range(nblk * tb)}
-    partials = np.zeros(nblk)
+    cache = {(k, tp): pairs(k, tp) for k in range(G.nz) for tp in range(tiles * tb)}
+    partials = np.zeros(tiles * G.nz)
     writes = []
-    for t in range(nblk * tb):
-        tt, q, row, j, k = thread_geo(t)
-        valid = t < total
-        lane = t % 64
+    for k, tp in sorted(cache):
+        tq, q, row, j, k = thread_geo(k, tp)
+        t = (k, tp)
+        valid = tp < P
+        lane = tp % 64
         p = (j + k + colour) & 1
         r0 = row * G.h
         base = r0 + 2 * q
@@ -145,8 +148,8 @@ def sweep(G, types, pin, bs, x, b, hs, colour, mode, tb=64, xc=None, cs=False, r
         Ex = [side(not (p == 1 and e == 1 and xhi), 0, types[1], 0) for e in range(2)]
         Wx = [side(not (p == 0 and e == 0 and xlo), 0, types[0], 0) for e in range(2)]
         hc = H.ld2(own)
-        prv = cache[t - 1][0][1] if lane > 0 else None
-        nxt = cache[t + 1][0][0] if lane < 63 else None
+        prv = cache[(k, tp - 1)][0][1] if lane > 0 else None
+        nxt = cache[(k, tp + 1)][0][0] if lane < 63 and (k, tp + 1) in cache else None
         _, hxp, hxm = x_pair(H, othc, base, r0, G.h, p, xlo, xhi, prv, nxt, per_lo, per_hi)
         hy = [H.ld2(oth + Yp[2]), H.ld2(oth + Ym[2])]
         hz = [H.ld2(oth + Zp[2]), H.ld2(oth + Zm[2])]
@@ -154,8 +157,8 @@ def sweep(G, types, pin, bs, x, b, hs, colour, mode, tb=64, xc=None, cs=False, r
             vxp = vxm = vyp = vym = vzp = vzm = [0.0, 0.0]
             xa = None
         else:
-            prv = cache[t - 1][1][1] if lane > 0 else None
-            nxt = cache[t + 1][1][0] if lane < 63 else None
+            prv = cache[(k, tp - 1)][1][1] if lane > 0 else None
+            nxt = cache[(k, tp + 1)][1][0] if lane < 63 and (k, tp + 1) in cache else None
             xa, vxp, vxm = x_pair(X, othc, base, r0, G.h, p, xlo, xhi, prv, nxt, per_lo, per_hi)
             a, c = X.ld2(oth + Yp[2]), X.ld2(oth + Ym[2])
             u, w = X.ld2(oth + Zp[2]), X.ld2(oth + Zm[2])
@@ -191,8 +194,9 @@ def sweep(G, types, pin, bs, x, b, hs, colour, mode, tb=64, xc=None, cs=False, r
                 if Ym[0]: vym[e] += R[e] if jodd else cy[e]
                 if Zp[0]: vzp[e] += cz[e] if kodd else R[e]
                 if Zm[0]: vzm[e] += R[e] if kodd else cz[e]
-            rm = cache[t - 1][2][1] if lane > 0 else (cval(G, xc, cs, 2 * q - 1, J0, K0) if not xlo else None)
-            rpn = cache[t + 1][2][0] if lane < 63 else (cval(G, xc, cs, 2 * q + 2, J0, K0) if not xhi else None)
+            rm = cache[(k, tp - 1)][2][1] if lane > 0 else (cval(G, xc, cs, 2 * q - 1, J0, K0) if not xlo else None)
+            rpn = (cache[(k, tp + 1)][2][0] if lane < 63 and (k, tp + 1) in cache else
+                   (cval(G, xc, cs, 2 * q + 2, J0, K0) if not xhi else None))
             if p == 0:
                 vxp[0] += R[0]; vxp[1] += R[1]
                 vxm[1] += R[0]
@@ -226,7 +230,7 @@ def sweep(G, types, pin, bs, x, b, hs, colour, mode, tb=64, xc=None, cs=False, r
             writes.append(("z", zn, zz[0], zz[1]))
             writes.append(("z", zn + 2, zz[2], zz[3]))
             ra, rb = Arr(rdot).ld2(zn), Arr(rdot).ld2(zn + 2)
-            partials[t // tb] += ra[0] * zz[0] + ra[1] * zz[1] + rb[0] * zz[2] + rb[1] * zz[3]
+            partials[k * tiles + tp // tb] += ra[0] * zz[0] + ra[1] * zz[1] + rb[0] * zz[2] + rb[1] * zz[3]
     # writes land after the reads of the whole half-sweep (no cell reads its own colour)
     for name, i, u, v in writes:
         {"x": X, "b": B, "z": Arr(z) if z is not None else None}[name].st2(i, u, v)
