@@ -239,8 +239,10 @@ enum {
     MF3D_TUNE_FOLD_R = 11,              /* 1 (default): the FP32-preconditioned PCG's residual
                                            update runs in the first finest-level half-sweep */
     MF3D_TUNE_FOLD_RZ = 12,             /* 1 (default): ... and its (r, z) in the last one */
-    MF3D_TUNE_OP_TILE = 13              /* 1 (default): the FP64 operator stages plane tiles in
+    MF3D_TUNE_OP_TILE = 13,             /* 1 (default): the FP64 operator stages plane tiles in
                                            LDS when nx <= 512 (even, 16-byte aligned vectors) */
+    MF3D_TUNE_GRAPH = 14                /* 1 (default): a mixed-precision single solve captures
+                                           one PCG iteration in a hipGraph and replays it */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

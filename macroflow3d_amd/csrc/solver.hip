@@ -150,6 +150,10 @@ struct mf3d_workspace {
     // alignment, the same way for single and batched solves)
     bool op_pairs = true;
     bool op_tile = true;  // plane-tile operator where it applies (MF3D_TUNE_OP_TILE)
+    // single solves replay one captured PCG iteration (MF3D_TUNE_GRAPH); the
+    // capture runs on this private stream (created on first use)
+    bool graph = true;
+    hipStream_t capture_stream = nullptr;
     // aligned stand-in for a head vector that is not 16-byte aligned
     double* hcopy = nullptr;
     size_t hcopy_n = 0;
@@ -235,6 +239,7 @@ struct mf3d_workspace {
         if (kc_ev) (void)hipEventDestroy(kc_ev);
         if (inv_ev) (void)hipEventDestroy(inv_ev);
         if (coarse_stream) (void)hipStreamDestroy(coarse_stream);
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
         if (batch_stream) (void)hipStreamDestroy(batch_stream);
         for (auto& e : batch_ev)
             if (e) (void)hipEventDestroy(e);
@@ -931,6 +936,15 @@ struct Chunker {
     }
 };
 
+struct GraphHolder {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~GraphHolder() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
 int state_upload(mf3d_workspace* ws, hipStream_t s, const IterState& st) {
     *ws->host_st = st;
     MF3D_HIP_TRY(hipMemcpyAsync(ws->st, ws->host_st, sizeof(IterState), hipMemcpyHostToDevice, s));
@@ -968,10 +982,44 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     MF3D_TRY(kern::finalize(s, ws->partials, np, FIN_BNORM, ws->st, nullptr));
     MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials, &npo));
     MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_R0, ws->st, nullptr));
+    // one iteration of the folded form: A p with (p, Ap), the V-cycle whose
+    // first half-sweep updates r (with (r, r)) and whose last forms (r, z),
+    // both finalisations, then x and p. Every scalar it needs lives in the
+    // device state, so the sequence can be captured once and replayed.
+    auto iteration = [&](hipStream_t q) -> int {
+        unsigned npa = 0, nrz = 0;
+        MF3D_TRY(apply_op(ws, q, ws->p, ws->Ap, ws->partials, &npa));
+        MF3D_TRY(kern::finalize(q, ws->partials, npa, FIN_PAP, ws->st, nullptr));
+        const RUpd ru{ws->st, ws->Ap, ws->r, ws->rr_partials};
+        MF3D_TRY(precondition(ws, q, ws->r, z, ws->partials, &ru, &nrz));
+        MF3D_TRY(kern::finalize2(q, ws->rr_partials, np_sweep, FIN_RR, ws->partials, nrz, FIN_RZ,
+                                 ws->st));
+        return kern::update_xp_split8(q, ws->st, L0.x32, L0.g, x, ws->p);
+    };
+    GraphHolder G;
     int restarts = 0;
     for (;;) {
         MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, nullptr, &nprz));
         MF3D_TRY(kern::finalize(s, ws->partials, nprz, FIN_RZ0, ws->st, nullptr));
+        if (fuse_r && ws->graph && !G.x) {
+            // captured on the workspace's private stream (the caller's may be
+            // the null stream, which cannot capture), replayed on the caller's;
+            // the coarsest inverse has joined in the V-cycle above
+            if (!ws->capture_stream)
+                MF3D_HIP_TRY(hipStreamCreateWithFlags(&ws->capture_stream, hipStreamNonBlocking));
+            MF3D_HIP_TRY(hipStreamBeginCapture(ws->capture_stream, hipStreamCaptureModeThreadLocal));
+            const int cst = iteration(ws->capture_stream);
+            hipGraph_t gr = nullptr;
+            const hipError_t e = hipStreamEndCapture(ws->capture_stream, &gr);
+            if (cst != MF3D_OK) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return cst;
+            }
+            if (e != hipSuccess)
+                return fail(MF3D_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            G.g = gr;
+            MF3D_HIP_TRY(hipGraphInstantiate(&G.x, gr, nullptr, nullptr, 0));
+        }
         if (zx) MF3D_TRY(kern::update_p_split8(s, ws->st, L0.x32, L0.g, ws->p, true));
         else MF3D_TRY(kern::copy(s, ws->z, ws->p, n));
         MF3D_TRY(read_state(ws, s));
@@ -985,11 +1033,8 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                 MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials, &npo));
                 MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_PAP, ws->st, nullptr));
                 if (fuse_r) {
-                    const RUpd ru{ws->st, ws->Ap, ws->r, ws->rr_partials};
-                    MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, &ru, &nprz));
-                    MF3D_TRY(kern::finalize2(s, ws->rr_partials, np_sweep, FIN_RR, ws->partials,
-                                             nprz, FIN_RZ, ws->st));
-                    MF3D_TRY(kern::update_xp_split8(s, ws->st, L0.x32, L0.g, x, ws->p));
+                    if (G.x) MF3D_HIP_TRY(hipGraphLaunch(G.x, s));
+                    else MF3D_TRY(iteration(s));
                     continue;
                 }
                 if (zx)  // x is updated together with p below
@@ -1374,14 +1419,6 @@ int read_states(mf3d_workspace* const* ws, int count, hipStream_t s) {
     return MF3D_OK;
 }
 
-struct GraphHolder {
-    hipGraph_t g = nullptr;
-    hipGraphExec_t x = nullptr;
-    ~GraphHolder() {
-        if (x) (void)hipGraphExecDestroy(x);
-        if (g) (void)hipGraphDestroy(g);
-    }
-};
 
 int capture_iteration(const BPlan& P, hipStream_t s, GraphHolder& G) {
     MF3D_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1939,6 +1976,7 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     case MF3D_TUNE_FOLD_R: ws->fold_r = value != 0; break;
     case MF3D_TUNE_FOLD_RZ: ws->fold_rz = value != 0; break;
     case MF3D_TUNE_OP_TILE: ws->op_tile = value != 0; break;
+    case MF3D_TUNE_GRAPH: ws->graph = value != 0; break;
     case MF3D_TUNE_SWEEP_BLOCK:
         if (value < 64 || value > 512 || (value & 63))
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");

@@ -185,6 +185,7 @@ class FlowWorkspace:
     TUNE_FOLD_R = 11
     TUNE_FOLD_RZ = 12
     TUNE_OP_TILE = 13
+    TUNE_GRAPH = 14
 
     def set_tuning(self, key: int, value: int) -> None:
         """Kernel-layout knobs (colour-split storage, sweep workgroup size,

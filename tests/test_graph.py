@@ -1,0 +1,44 @@
+"""The mixed-precision single solve replays one captured PCG iteration
+(hipGraph) between its host checks: the same iterations and bitwise the same
+head as launching the iteration's kernels one by one, for two solves in a row
+on one workspace (each captures again and replays its graph many times, with
+host synchronisations between the replays) and over the boundary mixes."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import bc_pair, grid_pair, random_K
+from macroflow3d_amd import physics as ph
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, ctx):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to(ctx.device)
+
+
+@pytest.mark.parametrize("mix", ["darcy_x", "periodic_yz", "all_neumann"])
+def test_graph_replay_equals_plain_launches(ctx, mix):
+    g, _ = grid_pair(64, 48, 32)
+    bc, _ = bc_pair(mix)
+    n = g.num_cells()
+    cfg = ph.HeadSolveConfig.mixed(1e-10, 4)
+    heads = {}
+    for graph in (0, 1):
+        ws = ph.FlowWorkspace()
+        ws.set_tuning(ph.FlowWorkspace.TUNE_GRAPH, graph)
+        out = []
+        for seed in (91, 92):  # two solves in a row on one workspace
+            K = dev(random_K(n, seed, 1.5), ctx)
+            h = ctx.zeros(n)
+            res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+            assert res.converged and res.true_residual <= 1e-10 * res.bnorm, (mix, graph, seed)
+            torch.cuda.synchronize()
+            out.append((h.cpu().numpy(), res.num_iterations, res.restarts))
+        heads[graph] = out
+        ws.close()
+    for (h0, it0, rs0), (h1, it1, rs1) in zip(heads[0], heads[1]):
+        assert it0 == it1 and rs0 == rs1
+        assert np.array_equal(h0, h1)
