@@ -1,15 +1,28 @@
 """Autotuner of the solve's launch shapes (no counterpart in MacroFlow3D).
 
-Every choice it makes leaves the results bitwise unchanged (each cell's
-arithmetic is the same for every workgroup size, and the reductions keep their
-fixed 256-thread first stage), so tuning only moves time:
+Its choices change where and when the work runs, not what is computed:
 
 * ``sweep_block``: threads per workgroup of the colour-split red-black sweeps
-  (64, 128, 256, 512), timed on the finest level of the V-cycle in place;
-* ``graph``: for batched solves, replaying the captured PCG iteration or
-  launching its kernels one by one, timed on the whole batch.
+  of the mixed V-cycle (64, 128, 256, 512): a runtime launch parameter; each
+  cell's arithmetic and every partial sum are the same for each size, so the
+  head is bitwise unchanged;
+* ``graph``: replaying one captured PCG iteration or launching its kernels
+  one by one (single mixed-precision solves and batches): bitwise unchanged;
+* ``op_tile``: the finest-level FP64 operator on plane tiles staged in LDS or
+  on register cell pairs: the same faces and per-cell sums, but the (p, Ap)
+  partials are grouped by a different tile geometry, so the head agrees to
+  the rounding of that reduction (iteration counts may differ by one).
 
-Results are cached per device and problem shape in a JSON file (default
+The FP64 V-cycle's sweeps also take ``sweep_block``; there the (r, z)
+partials are one per workgroup, so a different size regroups that sum.
+
+``tune_sweep_block`` times the finest-level smoothing in place;
+``tune_solve`` times whole single solves over (sweep_block, graph, op_tile);
+``tune_batch`` times whole batched solves over (sweep_block, graph). Each
+candidate gets one untimed run, then the median of ``reps`` timed runs.
+
+Results are cached per device, problem shape, solver configuration and
+boundary types in a JSON file (default
 ``~/.cache/macroflow3d_amd/autotune.json``, or ``$MF3D_AUTOTUNE_CACHE``);
 reads and writes hold a process lock and an exclusive file lock.
 """
@@ -18,6 +31,7 @@ from __future__ import annotations
 import fcntl
 import json
 import os
+import statistics
 import threading
 import time
 
@@ -53,7 +67,7 @@ def _load(path: str) -> dict:
 
 
 def _store(path: str, key: str, value: dict) -> None:
-    os.makedirs(os.path.dirname(path), exist_ok=True)
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     with open(path, "a+") as f:
         fcntl.flock(f, fcntl.LOCK_EX)
         try:
@@ -66,76 +80,126 @@ def _store(path: str, key: str, value: dict) -> None:
             f.seek(0)
             f.truncate()
             json.dump(data, f, indent=1, sort_keys=True)
+            f.flush()
         finally:
             fcntl.flock(f, fcntl.LOCK_UN)
 
 
-def _key(kind: str, ctx, shapes, cfg) -> str:
+def key_of(kind: str, device: str, shapes, cfg, bc) -> str:
+    """Cache key: device, shapes, the configuration fields that change the
+    launch sequence, and the boundary types (they select kernel variants)."""
     sh = ";".join("x".join(str(v) for v in s) for s in sorted(shapes))
-    return (f"{kind}|{_device_name(ctx)}|{sh}|levels={cfg.mg_levels}|"
-            f"prec={int(cfg.precond_precision)}|coef={int(cfg.coef_format)}")
+    types = "".join(str(int(f.type)) for f in bc.faces())
+    return (f"{kind}|{device}|{sh}|levels={cfg.mg_levels}|pre={cfg.mg_pre_smooth}|"
+            f"post={cfg.mg_post_smooth}|prec={int(cfg.precond_precision)}|"
+            f"coef={int(cfg.coef_format)}|bc={types}")
 
 
-def _time_events(ctx, fn, reps: int) -> float:
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    fn()  # warm
-    e0.record(ctx.stream)
-    for _ in range(reps):
-        fn()
-    e1.record(ctx.stream)
-    e1.synchronize()
-    return e0.elapsed_time(e1) / reps
+def _key(kind: str, ctx, shapes, cfg, bc) -> str:
+    return key_of(kind, _device_name(ctx), shapes, cfg, bc)
 
 
-def tune_sweep_block(ctx, grid, bc, cfg, K, reps: int = 10, use_cache: bool = True) -> dict:
-    """Fastest sweep workgroup size for this grid (finest-level smoothing, in place)."""
-    key = _key("sweep", ctx, [(grid.nx, grid.ny, grid.nz)], cfg)
+def cached_or_tune(key: str, tune, use_cache: bool = True) -> dict:
+    """The cached entry for key, or tune() stored under it (process and file
+    locks held while tuning, so concurrent tuners do not repeat the work)."""
     path = cache_path()
     with _LOCK:
         if use_cache:
             hit = _load(path).get(key)
             if hit:
                 return hit
+        out = tune()
+        _store(path, key, out)
+        return out
+
+
+def _median_ms(ctx, fn, reps: int, events: bool) -> float:
+    """fn() once untimed, then the median of reps timed runs (HIP events on
+    the context stream, or the host clock around a synchronised call)."""
+    fn()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(max(1, reps)):
+        if events:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(ctx.stream)
+            fn()
+            e1.record(ctx.stream)
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1))
+        else:
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+    return statistics.median(times)
+
+
+def tune_sweep_block(ctx, grid, bc, cfg, K, reps: int = 5, use_cache: bool = True) -> dict:
+    """Fastest sweep workgroup size for this grid (finest-level smoothing, in place)."""
+    def tune():
         times = {}
         for b in BLOCKS:
             ws = ph.FlowWorkspace()
             ws.set_tuning(ph.FlowWorkspace.TUNE_SWEEP_BLOCK, b)
             mg.mg_setup(ctx, ws, grid, bc, cfg, K)
-            times[b] = _time_events(ctx, lambda: mg.smooth_level(ctx, ws, 0, 1), reps)
+            times[b] = _median_ms(ctx, lambda: mg.smooth_level(ctx, ws, 0, 1), reps, True)
             ws.close()
         best = min(times, key=times.get)
-        out = {"sweep_block": best, "ms_per_iteration": {str(b): t for b, t in times.items()}}
-        _store(path, key, out)
-        return out
+        return {"sweep_block": best, "ms_per_iteration": {str(b): t for b, t in times.items()}}
+    return cached_or_tune(_key("sweep", ctx, [(grid.nx, grid.ny, grid.nz)], cfg, bc), tune,
+                          use_cache)
 
 
-def tune_batch(ctx, hs, Ks, grids, bc, cfg, workspaces, use_cache: bool = True) -> dict:
-    """Fastest (sweep_block, graph) for a batch: one timed batched solve per candidate."""
-    key = _key("batch", ctx, [(g.nx, g.ny, g.nz) for g in grids], cfg)
-    path = cache_path()
-    with _LOCK:
-        if use_cache:
-            hit = _load(path).get(key)
-            if hit:
-                return hit
+def tune_solve(ctx, grid, bc, cfg, K, reps: int = 3, use_cache: bool = True,
+               blocks=(128, 256, 512)) -> dict:
+    """Fastest (sweep_block, graph, op_tile) for single solves of this grid:
+    whole solves into a scratch head, one untimed then the median of reps."""
+    def tune():
+        times = {}
+        h = ctx.zeros(grid.num_cells())
+        for b in blocks:
+            for graph in (True, False):
+                for tile in (True, False):
+                    ws = ph.FlowWorkspace()
+                    apply([ws], {"sweep_block": b, "graph": graph, "op_tile": tile})
+                    times[(b, graph, tile)] = _median_ms(
+                        ctx, lambda: ph.solve_head(h, K, grid, bc, cfg, ctx, ws), reps, False)
+                    ws.close()
+        b, graph, tile = min(times, key=times.get)
+        return {"sweep_block": b, "graph": graph, "op_tile": tile,
+                "ms": {f"{k[0]}/{'graph' if k[1] else 'launches'}/"
+                       f"{'tile' if k[2] else 'pairs'}": v for k, v in times.items()}}
+    return cached_or_tune(_key("solve", ctx, [(grid.nx, grid.ny, grid.nz)], cfg, bc), tune,
+                          use_cache)
+
+
+def tune_batch(ctx, hs, Ks, grids, bc, cfg, workspaces, reps: int = 3,
+               use_cache: bool = True) -> dict:
+    """Fastest (sweep_block, graph) for a batch: whole batched solves per candidate."""
+    def tune():
         times = {}
         for b in (128, 256, 512):
             for graph in (True, False):
-                for w in workspaces:
-                    w.set_tuning(ph.FlowWorkspace.TUNE_SWEEP_BLOCK, b)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                ph.solve_head_batched(hs, Ks, grids, bc, cfg, ctx, workspaces, use_graph=graph)
-                times[(b, graph)] = (time.perf_counter() - t0) * 1e3
+                apply(workspaces, {"sweep_block": b})
+                times[(b, graph)] = _median_ms(
+                    ctx, lambda: ph.solve_head_batched(hs, Ks, grids, bc, cfg, ctx, workspaces,
+                                                       use_graph=graph), reps, False)
         (b, graph) = min(times, key=times.get)
-        out = {"sweep_block": b, "graph": graph,
-               "ms": {f"{k[0]}/{'graph' if k[1] else 'launches'}": v for k, v in times.items()}}
-        _store(path, key, out)
-        return out
+        return {"sweep_block": b, "graph": graph,
+                "ms": {f"{k[0]}/{'graph' if k[1] else 'launches'}": v for k, v in times.items()}}
+    return cached_or_tune(_key("batch", ctx, [(g.nx, g.ny, g.nz) for g in grids], cfg, bc),
+                          tune, use_cache)
 
 
 def apply(workspaces, params: dict) -> None:
-    """Sets the tuned workgroup size on the workspaces (takes effect at set-up)."""
+    """Sets tuned choices on the workspaces (they take effect at the next set-up
+    or solve); keys absent from params are left as they are."""
     for w in workspaces:
-        w.set_tuning(ph.FlowWorkspace.TUNE_SWEEP_BLOCK, int(params["sweep_block"]))
+        if "sweep_block" in params:
+            w.set_tuning(ph.FlowWorkspace.TUNE_SWEEP_BLOCK, int(params["sweep_block"]))
+        if "graph" in params:
+            w.set_tuning(ph.FlowWorkspace.TUNE_GRAPH, int(bool(params["graph"])))
+        if "op_tile" in params:
+            w.set_tuning(ph.FlowWorkspace.TUNE_OP_TILE, int(bool(params["op_tile"])))
