@@ -697,6 +697,12 @@ __global__ void __launch_bounds__(kBlock)
 constexpr int kSmallBlock = 1024;
 constexpr int kSmallPer = 4;
 
+// Thread t owns cells o = 2t + (m & 1) + 2048 (m >> 1), m = 0..3: two
+// x-adjacent pairs, one cell of each colour per pair when nx is even, so every
+// lane has work in every half-sweep. The neighbours' LDS slots (-1 for a
+// non-periodic boundary, whose ghost value is 0), the faces and b are
+// gathered once; the iterations then touch LDS only. Per cell the same sums
+// as colour_kernel (neighbours x+, x-, y+, y-, z+, z-; gs_value).
 template <typename T, typename TC>
 __device__ __forceinline__ void small_body(T* xs, const Dims& d, const BcTypes& bc, int pin, T bs,
                                            T* __restrict__ x, const T* __restrict__ b,
@@ -705,32 +711,46 @@ __device__ __forceinline__ void small_body(T* xs, const Dims& d, const BcTypes& 
                                            const T* __restrict__ xc) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const int n = nx * ny * nz;
+    const long sy = nx, sz = (long)nx * ny;
     T f[kSmallPer][6], bl[kSmallPer];
+    int nb[kSmallPer][6];
+    int colour[kSmallPer];  // -1: no cell
     const int NXc = nx >> 1, NYc = ny >> 1;
 #pragma unroll
     for (int m = 0; m < kSmallPer; ++m) {
-        const int o = threadIdx.x + m * kSmallBlock;
+        const int o = 2 * (int)threadIdx.x + (m & 1) + 2 * kSmallBlock * (m >> 1);
         bl[m] = T(0);
+        colour[m] = -1;
         if (o < n) {
             const int i = o % nx, q = o / nx, j = q % ny, k = q / ny;
+            colour[m] = (i + j + k) & 1;
             T v = zero_init ? T(0) : x[o];
             if (xc) v += xc[(i >> 1) + NXc * ((j >> 1) + NYc * (k >> 1))];
             xs[o] = v;
             cell_faces(d, FX, FY, FZ, i, j, k, f[m]);
             bl[m] = b[o];
+            nb[m][0] = i < nx - 1 ? o + 1 : (bc.t[1] == MF3D_BC_PERIODIC ? o - (nx - 1) : -1);
+            nb[m][1] = i > 0 ? o - 1 : (bc.t[0] == MF3D_BC_PERIODIC ? o + (nx - 1) : -1);
+            nb[m][2] = j < ny - 1 ? (int)(o + sy)
+                                  : (bc.t[3] == MF3D_BC_PERIODIC ? (int)(o - (ny - 1) * sy) : -1);
+            nb[m][3] = j > 0 ? (int)(o - sy)
+                             : (bc.t[2] == MF3D_BC_PERIODIC ? (int)(o + (ny - 1) * sy) : -1);
+            nb[m][4] = k < nz - 1 ? (int)(o + sz)
+                                  : (bc.t[5] == MF3D_BC_PERIODIC ? (int)(o - (nz - 1) * sz) : -1);
+            nb[m][5] = k > 0 ? (int)(o - sz)
+                             : (bc.t[4] == MF3D_BC_PERIODIC ? (int)(o + (nz - 1) * sz) : -1);
         }
     }
     __syncthreads();
     for (int it = 0; it < iters; ++it) {
-        for (int colour = 0; colour < 2; ++colour) {
+        for (int c = 0; c < 2; ++c) {
 #pragma unroll
             for (int m = 0; m < kSmallPer; ++m) {
-                const int o = threadIdx.x + m * kSmallBlock;
-                if (o >= n) continue;
-                const int i = o % nx, q = o / nx, j = q % ny, k = q / ny;
-                if (((i + j + k) & 1) != colour) continue;
+                if (colour[m] != c) continue;
+                const int o = 2 * (int)threadIdx.x + (m & 1) + 2 * kSmallBlock * (m >> 1);
                 T v[6];
-                neighbours(d, bc, xs, o, i, j, k, v);
+#pragma unroll
+                for (int q = 0; q < 6; ++q) v[q] = nb[m][q] >= 0 ? xs[nb[m][q]] : T(0);
                 xs[o] = gs_value(f[m], v, bl[m], bs, pin && o == 0);
             }
             __syncthreads();
