@@ -41,7 +41,8 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kVT = 4;  // 16-voxel tiles per wave in the convolution
+constexpr int kVT = 4;  // 16-voxel tiles per wave in the convolution (large grids)
+constexpr long kConvWaves = 2048;  // fewer tiles per wave below this many waves
 
 __device__ __forceinline__ float bf2f(__hip_bfloat16 v) { return __bfloat162float(v); }
 __device__ __forceinline__ __hip_bfloat16 f2bf(float v) { return __float2bfloat16(v); }
@@ -62,7 +63,7 @@ __device__ __forceinline__ bf16x8 load8(const __hip_bfloat16* p) {
 // data with the forward tap order in W). Epilogue, per output (v, c):
 //   s = acc + bias[c] (if bias) + R[v*Cs + c] (if R); s = relu(s) (if relu);
 //   s *= (M[v*Cs + c] > 0) (if M); stored to out (BF16 or FP32) for c < Cs.
-template <int CT, bool F32OUT>
+template <int CT, int VT, bool F32OUT>
 __global__ void __launch_bounds__(256)
     conv_kernel(Geo g, int cin, int kpad, int cs, const __hip_bfloat16* __restrict__ X,
                 const __hip_bfloat16* __restrict__ W, const float* __restrict__ bias,
@@ -71,15 +72,15 @@ __global__ void __launch_bounds__(256)
     const long nvox = g.n();
     const int lane = threadIdx.x & 63;
     const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const long vbase = wave * (16L * kVT);
+    const long vbase = wave * (16L * VT);
     if (vbase >= nvox) return;  // whole wave
     const int col = lane & 15;   // voxel within a tile / output channel row of A
     const int grp = lane >> 4;   // k group: k = 8 grp .. 8 grp + 7 of each 32-chunk
 
-    int vi[kVT], vj[kVT], vk[kVT];
-    bool live[kVT];
+    int vi[VT], vj[VT], vk[VT];
+    bool live[VT];
 #pragma unroll
-    for (int t = 0; t < kVT; ++t) {
+    for (int t = 0; t < VT; ++t) {
         const long v = vbase + 16 * t + col;
         live[t] = v < nvox;
         const long vv = live[t] ? v : 0;
@@ -88,27 +89,27 @@ __global__ void __launch_bounds__(256)
         vj[t] = (int)(q % g.ny);
         vk[t] = (int)(q / g.ny);
     }
-    f32x4 acc[kVT][CT];
+    f32x4 acc[VT][CT];
 #pragma unroll
-    for (int t = 0; t < kVT; ++t)
+    for (int t = 0; t < VT; ++t)
 #pragma unroll
         for (int c = 0; c < CT; ++c) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int K = 27 * cin;
     const bf16x8 zero8 = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
-    for (int k0 = 0; k0 < kpad; k0 += 32) {
+    // the operands of one 32-wide k step: weight rows (A) and the tap's
+    // input channels at each voxel (B, zero outside the grid or past K)
+    auto load_step = [&](int k0, bf16x8 (&a)[CT], bf16x8 (&b)[VT]) {
         const int k = k0 + 8 * grp;
-        bf16x8 a[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) a[c] = load8(W + (long)(c * 16 + col) * kpad + k);
-        bf16x8 b[kVT];
         if (k < K) {
             int tap = k / cin;
             const int c0 = k - tap * cin;
             if (flip) tap = 26 - tap;
             const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
 #pragma unroll
-            for (int t = 0; t < kVT; ++t) {
+            for (int t = 0; t < VT; ++t) {
                 const int ii = vi[t] + dx, jj = vj[t] + dy, kk = vk[t] + dz;
                 const bool in = live[t] && ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny &&
                                 kk >= 0 && kk < g.nz;
@@ -117,17 +118,31 @@ __global__ void __launch_bounds__(256)
             }
         } else {
 #pragma unroll
-            for (int t = 0; t < kVT; ++t) b[t] = zero8;
+            for (int t = 0; t < VT; ++t) b[t] = zero8;
         }
+    };
+    // the next step's operands are loaded before this step's products are
+    // issued, so the load latency overlaps the MFMAs (same accumulation order)
+    bf16x8 a[CT], b[VT];
+    load_step(0, a, b);
+    for (int k0 = 0; k0 < kpad; k0 += 32) {
+        bf16x8 an[CT], bn[VT];
+        if (k0 + 32 < kpad) load_step(k0 + 32, an, bn);
 #pragma unroll
-        for (int t = 0; t < kVT; ++t)
+        for (int t = 0; t < VT; ++t)
 #pragma unroll
             for (int c = 0; c < CT; ++c)
                 acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c], b[t], acc[t][c], 0, 0, 0);
+        if (k0 + 32 < kpad) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) a[c] = an[c];
+#pragma unroll
+            for (int t = 0; t < VT; ++t) b[t] = bn[t];
+        }
     }
     // acc[t][c][r] = out(channel c*16 + 4 grp + r, voxel vbase + 16 t + col)
 #pragma unroll
-    for (int t = 0; t < kVT; ++t) {
+    for (int t = 0; t < VT; ++t) {
         if (!live[t]) continue;
         const long v = vbase + 16 * t + col;
 #pragma unroll
@@ -411,48 +426,49 @@ __global__ void __launch_bounds__(256)
     if (v < n) yd[v] = y[v * ych];
 }
 
-// h0 = h_lin + J trilinear(y) on the fine grid g (y: coarse, ych channels, channel 0).
-// Workgroup b covers segment b % segs (256 cells) of fine row b / segs, so j, k
-// and the y/z weights are uniform in a workgroup; the segment's coarse values
-// (at most 66 along x for each of the 4 y/z corners) are staged in LDS first,
-// so each cell makes one coalesced store and no gathers from global memory.
+// h0 = h_lin + J trilinear(yd) on the fine grid g (yd: the coarse field's
+// channel 0, stored densely). A thread writes 4 consecutive cells of a row
+// (two 16-byte stores; the network's sizes are multiples of 16); the coarse
+// values it interpolates, at most 3 along x for each of the 4 y/z corner rows,
+// are read directly (the coarse field is 1/64 of the fine one and stays in L2).
 __global__ void __launch_bounds__(256)
-    output_kernel(Geo g, double hw, double he, double inv_nx, const float* __restrict__ y,
-                  int ych, double* __restrict__ h) {
-    constexpr int kSpan = 256 / kS + 2;
-    __shared__ float cy[4][kSpan];
-    const int segs = (g.nx + 255) / 256;
-    const long row = (long)blockIdx.x / segs;
-    const int i0 = (int)((long)blockIdx.x - row * segs) * 256;
-    const int i = i0 + (int)threadIdx.x;
+    output_kernel(Geo g, double hw, double he, double inv_nx, const float* __restrict__ yd,
+                  double* __restrict__ h) {
+    const int q4 = g.nx / 4;
+    const long total = (long)q4 * g.ny * g.nz;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const int q = (int)(t % q4);
+    const long row = t / q4;
     const int j = (int)(row % g.ny), k = (int)(row / g.ny);
     const int NX = g.nx / kS, NY = g.ny / kS, NZ = g.nz / kS;
     int y0, y1, z0, z1;
     float wy, wz;
     lin_axis(j, NY, y0, y1, wy);
     lin_axis(k, NZ, z0, z1, wz);
-    const int Ib = max(i0 / kS - 1, 0);  // first staged coarse x
-    const int Ie = min((i0 + 255) / kS + 1, NX - 1);
-    const int span = Ie - Ib + 1;  // <= kSpan
-    for (int t = threadIdx.x; t < 4 * span; t += 256) {
-        const int c = t / span, a = Ib + (t - c * span);
-        const int b = (c & 1) ? y1 : y0, z = (c & 2) ? z1 : z0;
-        cy[c][t - c * span] = y[((long)a + (long)NX * (b + (long)NY * z)) * ych];
-    }
-    __syncthreads();
-    if (i >= g.nx) return;
-    int x0, x1;
-    float wx;
-    lin_axis(i, NX, x0, x1, wx);
-    const int a0 = x0 - Ib, a1 = x1 - Ib;
-    const float c00 = cy[0][a0] * (1.f - wx) + cy[0][a1] * wx;
-    const float c10 = cy[1][a0] * (1.f - wx) + cy[1][a1] * wx;
-    const float c01 = cy[2][a0] * (1.f - wx) + cy[2][a1] * wx;
-    const float c11 = cy[3][a0] * (1.f - wx) + cy[3][a1] * wx;
-    const float c0 = c00 * (1.f - wy) + c10 * wy, c1 = c01 * (1.f - wy) + c11 * wy;
-    const float yi = c0 * (1.f - wz) + c1 * wz;
+    const float* r00 = yd + (long)NX * (y0 + (long)NY * z0);
+    const float* r10 = yd + (long)NX * (y1 + (long)NY * z0);
+    const float* r01 = yd + (long)NX * (y0 + (long)NY * z1);
+    const float* r11 = yd + (long)NX * (y1 + (long)NY * z1);
     const double J = (hw - he) * inv_nx;
-    h[row * g.nx + i] = hw + (he - hw) * ((i + 0.5) * inv_nx) + J * (double)yi;
+    double out[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = 4 * q + e;
+        int x0, x1;
+        float wx;
+        lin_axis(i, NX, x0, x1, wx);
+        const float c00 = r00[x0] * (1.f - wx) + r00[x1] * wx;
+        const float c10 = r10[x0] * (1.f - wx) + r10[x1] * wx;
+        const float c01 = r01[x0] * (1.f - wx) + r01[x1] * wx;
+        const float c11 = r11[x0] * (1.f - wx) + r11[x1] * wx;
+        const float c0 = c00 * (1.f - wy) + c10 * wy, c1 = c01 * (1.f - wy) + c11 * wy;
+        const float yi = c0 * (1.f - wz) + c1 * wz;
+        out[e] = hw + (he - hw) * ((i + 0.5) * inv_nx) + J * (double)yi;
+    }
+    double* o = h + row * g.nx + 4L * q;
+    *reinterpret_cast<double2*>(o) = make_double2(out[0], out[1]);
+    *reinterpret_cast<double2*>(o + 2) = make_double2(out[2], out[3]);
 }
 
 // per coarse cell V: target t = mean over its block of (h - h_lin) / J;
@@ -715,19 +731,33 @@ int conv(hipStream_t s, const mf3d_grid& g, int cin_ch, int cout_pad, int kp, in
          const __hip_bfloat16* R, const __hip_bfloat16* M, bool relu, bool flip, void* out,
          bool f32out) {
     const Geo G{g.nx, g.ny, g.nz};
-    const long waves = (long)((cells(g) + 16 * kVT - 1) / (16 * kVT));
+    // 16-voxel tiles per wave: kVT on large grids, fewer where that would
+    // leave the launch with fewer than kConvWaves waves (the small grids of
+    // the deeper layers; same products and order per output either way)
+    const long nv = (long)cells(g);
+    const int vt = (nv + 16 * kVT - 1) / (16 * kVT) >= kConvWaves ? kVT
+                   : (nv + 31) / 32 >= kConvWaves                   ? 2
+                                                                    : 1;
+    const long waves = (nv + 16L * vt - 1) / (16L * vt);
     const unsigned nb = (unsigned)((waves + 3) / 4);
     const int ct = cout_pad / 16;
-#define MF3D_CONV(CTV, F32)                                                                    \
-    conv_kernel<CTV, F32><<<nb, 256, 0, s>>>(G, cin_ch, kp, cs, X, W, bias, R, M, relu ? 1 : 0, \
-                                             flip ? 1 : 0, out)
+#define MF3D_CONV(CTV, VTV, F32)                                                               \
+    conv_kernel<CTV, VTV, F32><<<nb, 256, 0, s>>>(G, cin_ch, kp, cs, X, W, bias, R, M,        \
+                                                  relu ? 1 : 0, flip ? 1 : 0, out)
+#define MF3D_CONV_VT(CTV, F32)                                                                 \
+    do {                                                                                       \
+        if (vt == kVT) MF3D_CONV(CTV, kVT, F32);                                               \
+        else if (vt == 2) MF3D_CONV(CTV, 2, F32);                                              \
+        else MF3D_CONV(CTV, 1, F32);                                                           \
+    } while (0)
     if (f32out) {
         if (ct != 1) return fail(MF3D_ERR_UNSUPPORTED, "FP32 output only for 16 channels");
-        MF3D_CONV(1, true);
-    } else if (ct == 1) MF3D_CONV(1, false);
-    else if (ct == 2) MF3D_CONV(2, false);
-    else if (ct == 4) MF3D_CONV(4, false);
+        MF3D_CONV_VT(1, true);
+    } else if (ct == 1) MF3D_CONV_VT(1, false);
+    else if (ct == 2) MF3D_CONV_VT(2, false);
+    else if (ct == 4) MF3D_CONV_VT(4, false);
     else return fail(MF3D_ERR_UNSUPPORTED, "convolution: unsupported channel count");
+#undef MF3D_CONV_VT
 #undef MF3D_CONV
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
@@ -979,9 +1009,13 @@ int mf3d_wsnet_predict(mf3d_wsnet* net, mf3d_stream stream, const mf3d_grid* gri
     // share cache lines between neighbouring cells
     const long n0 = cells(*grid) / ((long)kS * kS * kS);
     channel0_kernel<<<blocks(n0), 256, 0, s>>>(net->y, 16, n0, net->yd);
-    const long rows = (long)grid->ny * grid->nz, segs = (grid->nx + 255) / 256;
-    output_kernel<<<(unsigned)(rows * segs), 256, 0, s>>>(Geo{grid->nx, grid->ny, grid->nz}, hw,
-                                                          he, 1.0 / grid->nx, net->yd, 1, h0);
+    // h0 must be 16-byte aligned for the paired stores (the network's sizes
+    // make every row start so)
+    if (reinterpret_cast<uintptr_t>(h0) & 15)
+        return fail(MF3D_ERR_UNSUPPORTED, "warm-start output needs a 16-byte aligned head");
+    const long quads = cells(*grid) / 4;
+    output_kernel<<<(unsigned)((quads + 255) / 256), 256, 0, s>>>(
+        Geo{grid->nx, grid->ny, grid->nz}, hw, he, 1.0 / grid->nx, net->yd, h0);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
