@@ -248,9 +248,57 @@ def compare_guesses(ctx: HipContext, net: WarmStartNet, samples, n: int, levels:
     return rows
 
 
+def wall_time_gain(ctx: HipContext, net: WarmStartNet, n: int, params, levels: int,
+                   rtol: float = 1e-10, reps: int = 3, precond=ph.PrecondPrecision.FP32):
+    """Per field: iterations and median wall time (reps timed runs after one
+    untimed) of the solve from the linear head, and of prediction + solve from
+    the predicted head (mixed-precision solve by default, as in the headline
+    step). Fields are generated on the device from params."""
+    g = Grid3D(n, n, n)
+    bc = BCSpec.darcy_x()
+    ws = ph.FlowWorkspace()
+    h = ctx.zeros(g.num_cells())
+    rows = []
+    for s2, lam, seed in params:
+        lnK, _ = lognormal_lnK(n, n, n, s2, lam, seed, device=ctx.device)
+        K = torch.exp(lnK)
+        row = {}
+        for name in ("linear", "predicted"):
+            cfg = ph.HeadSolveConfig.fp64_direct(rtol, levels)
+            cfg.precond_precision = precond
+            if precond == ph.PrecondPrecision.FP32:
+                cfg = ph.HeadSolveConfig.mixed(rtol, levels)
+            cfg.init_guess = (ph.InitGuess.REFERENCE if name == "linear"
+                              else ph.InitGuess.PROVIDED)
+
+            def run():
+                if name == "predicted":
+                    net.predict(ctx, K, g, bc, h)
+                return ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+            run()
+            times = []
+            for _ in range(max(1, reps)):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                res = run()
+                torch.cuda.synchronize()
+                times.append((time.perf_counter() - t0) * 1e3)
+            row[name] = {"iterations": res.num_iterations, "ms": float(np.median(times)),
+                         "converged": bool(res.converged and res.true_residual <= rtol * res.bnorm)}
+        rows.append(row)
+        del lnK, K
+    ws.close()
+    return rows
+
+
 def _main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("compare", help="wall time from the predicted vs the linear head")
+    c.add_argument("--sizes", type=int, nargs="+", default=[128, 256, 512])
+    c.add_argument("--fields", type=int, default=4)
+    c.add_argument("--seed", type=int, default=88_000_000)
+    c.add_argument("--weights", default=None)
     t = sub.add_parser("train")
     t.add_argument("--out", required=True)
     t.add_argument("--n", type=int, default=128)
@@ -269,6 +317,20 @@ def _main(argv=None) -> int:
         print(msg, flush=True)
 
     ctx = HipContext(0)
+    if a.cmd == "compare":
+        net = WarmStartNet.load(a.weights)
+        out = {}
+        for n in a.sizes:
+            levels = {128: 5, 256: 6, 512: 7}.get(n, 5)
+            rows = wall_time_gain(ctx, net, n, field_params(a.seed + n, a.fields), levels)
+            out[n] = {k: {"mean_iterations": float(np.mean([r[k]["iterations"] for r in rows])),
+                          "mean_ms": float(np.mean([r[k]["ms"] for r in rows])),
+                          "all_converged": all(r[k]["converged"] for r in rows)}
+                      for k in ("linear", "predicted")}
+            log(f"{n}^3: {json.dumps(out[n])}")
+        log(json.dumps(out))
+        net.close()
+        return 0
     ptrain = field_params(a.seed, a.train)
     pheld = field_params(a.seed + 10 ** 6, a.held)
     log(f"making {a.train} training + {a.held} held-out fields at {a.n}^3")

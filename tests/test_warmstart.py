@@ -179,3 +179,23 @@ def test_trained_network_beats_zero_and_linear_guess(ctx):
     print(f"mean PCG iterations: {it}", flush=True)
     assert it["predicted"] < it["linear"] < it["zero"], it
     net.close()
+
+
+@gpu
+@pytest.mark.skipif(not os.path.exists(W.DEFAULT_WEIGHTS), reason="no trained weights")
+def test_warm_start_pays_in_wall_time_at_256cubed(ctx):
+    """The headline step at 256^3 (mixed-precision solve to 1e-10): inference
+    plus the solve from the predicted head takes less wall time, on average
+    over held-out fields (sigma2 2-4, correlation 4-12 cells, seeds not used in
+    training), than the solve from the linear head; every solve converges."""
+    net = W.WarmStartNet.load()
+    print("warm start wall time 256^3", flush=True)
+    rows = W.wall_time_gain(ctx, net, 256, W.field_params(88_000_256, 4), levels=6, reps=5)
+    for r in rows:
+        assert r["linear"]["converged"] and r["predicted"]["converged"]
+    it = {k: np.mean([r[k]["iterations"] for r in rows]) for k in ("linear", "predicted")}
+    ms = {k: np.mean([r[k]["ms"] for r in rows]) for k in ("linear", "predicted")}
+    print(f"256^3 iterations {it}, ms {ms}", flush=True)
+    assert it["predicted"] < it["linear"], it
+    assert ms["predicted"] < ms["linear"], ms
+    net.close()
