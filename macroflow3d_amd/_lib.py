@@ -87,6 +87,7 @@ SIGNATURES = {
     "mf3d_mg_setup": (_i, [_vp, _vp, _g, _b, _cfg, _vp, _sz]),
     "mf3d_mg_num_levels": (_i, [_vp]),
     "mf3d_workspace_set_tuning": (_i, [_vp, _i, C.c_long]),
+    "mf3d_workspace_solve_counts": (_i, [_vp, _vp, _vp]),
     "mf3d_workspace_reserve": (_i, [_vp, _g, _i]),
     "mf3d_mg_smooth": (_i, [_vp, _vp, _i, _i]),
     "mf3d_mg_precondition": (_i, [_vp, _vp, _vp, _vp, _sz]),

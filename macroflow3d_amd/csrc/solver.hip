@@ -154,6 +154,9 @@ struct mf3d_workspace {
     // capture runs on this private stream (created on first use)
     bool graph = true;
     hipStream_t capture_stream = nullptr;
+    // finest-level operator and preconditioner applications enqueued since
+    // the last single solve began (replays of a captured iteration included)
+    long n_op = 0, n_pc = 0;
     // aligned stand-in for a head vector that is not 16-byte aligned
     double* hcopy = nullptr;
     size_t hcopy_n = 0;
@@ -840,6 +843,7 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
                  double* rz_partials = nullptr, const RUpd* ru = nullptr,
                  unsigned* np_rz = nullptr) {
     Level& L0 = ws->lv[0];
+    ++ws->n_pc;
     if (!z && !z_in_x32(ws, false))
         return fail(MF3D_ERR_UNSUPPORTED, "z can stay in x32 only on a colour-split FP32 level");
     if (np_rz) *np_rz = kern::reduce_blocks(L0.n);
@@ -881,12 +885,14 @@ bool op_pairs_for(const mf3d_workspace* ws, const double* x) {
 int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, double* partials,
              unsigned* np = nullptr) {
     Level& L0 = ws->lv[0];
+    ++ws->n_op;
     return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 0, x, nullptr, y, partials, np,
                           ws->op_pairs, ws->op_tile);
 }
 int residual_op(mf3d_workspace* ws, hipStream_t s, const double* x, const double* b, double* r,
                 double* partials, unsigned* np = nullptr) {
     Level& L0 = ws->lv[0];
+    ++ws->n_op;
     return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 1, x, b, r, partials, np,
                           ws->op_pairs, ws->op_tile);
 }
@@ -997,6 +1003,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
         return kern::update_xp_split8(q, ws->st, L0.x32, L0.g, x, ws->p);
     };
     GraphHolder G;
+    long g_op = 0, g_pc = 0;  // applications in one replay of G
     int restarts = 0;
     for (;;) {
         MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, nullptr, &nprz));
@@ -1008,7 +1015,12 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             if (!ws->capture_stream)
                 MF3D_HIP_TRY(hipStreamCreateWithFlags(&ws->capture_stream, hipStreamNonBlocking));
             MF3D_HIP_TRY(hipStreamBeginCapture(ws->capture_stream, hipStreamCaptureModeThreadLocal));
+            const long op0 = ws->n_op, pc0 = ws->n_pc;
             const int cst = iteration(ws->capture_stream);
+            g_op = ws->n_op - op0;  // captured, not run
+            g_pc = ws->n_pc - pc0;
+            ws->n_op = op0;
+            ws->n_pc = pc0;
             hipGraph_t gr = nullptr;
             const hipError_t e = hipStreamEndCapture(ws->capture_stream, &gr);
             if (cst != MF3D_OK) {
@@ -1030,13 +1042,18 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             const double target = cfg.cg_rtol * ws->host_st->den;
             const int chunk = ch.next(*ws->host_st, target);
             for (int c = 0; c < chunk; ++c) {
-                MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials, &npo));
-                MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_PAP, ws->st, nullptr));
-                if (fuse_r) {
-                    if (G.x) MF3D_HIP_TRY(hipGraphLaunch(G.x, s));
-                    else MF3D_TRY(iteration(s));
+                if (fuse_r) {  // the whole iteration, A p included
+                    if (G.x) {
+                        MF3D_HIP_TRY(hipGraphLaunch(G.x, s));
+                        ws->n_op += g_op;
+                        ws->n_pc += g_pc;
+                    } else {
+                        MF3D_TRY(iteration(s));
+                    }
                     continue;
                 }
+                MF3D_TRY(apply_op(ws, s, ws->p, ws->Ap, ws->partials, &npo));
+                MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_PAP, ws->st, nullptr));
                 if (zx)  // x is updated together with p below
                     MF3D_TRY(kern::pcg_update_r(s, ws->st, ws->Ap, ws->r, n, ws->partials));
                 else
@@ -1645,6 +1662,7 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     clear_stale_error();
     hipStream_t s = as_stream(stream);
     MF3D_TRY(ensure_workspace(ws, *grid, cfg->mg_levels));
+    ws->n_op = ws->n_pc = 0;
     MF3D_HIP_TRY(hipEventRecord(ws->ev[0], s));
     MF3D_TRY(mg_setup(ws, s, *grid, *bc, *cfg, K));
     MF3D_TRY(kern::rhs(s, *grid, *bc, K, ws->rhs));
@@ -1984,6 +2002,16 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
         break;
     default: return fail(MF3D_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
+    return MF3D_OK;
+}
+
+int mf3d_workspace_solve_counts(const mf3d_workspace* ws, long* op_applications,
+                                long* precond_applications) {
+    MF3D_CHECK_PTR(ws);
+    MF3D_CHECK_PTR(op_applications);
+    MF3D_CHECK_PTR(precond_applications);
+    *op_applications = ws->n_op;
+    *precond_applications = ws->n_pc;
     return MF3D_OK;
 }
 

@@ -189,10 +189,18 @@ class FlowWorkspace:
 
     def set_tuning(self, key: int, value: int) -> None:
         """Kernel-layout knobs (colour-split storage, sweep workgroup size,
-        folded PCG steps, plane-tile operator); applied at the next set-up.
-        None changes the result beyond rounding; the sweep block size and the
-        operator tile leave it bit-identical."""
+        folded PCG steps, plane-tile operator, graph replay); applied at the
+        next set-up or solve. None changes the result beyond rounding; the
+        sweep block size of the mixed V-cycle and the graph replay leave it
+        bit-identical, the operator tile regroups the (p, Ap) partials."""
         _lib.call("mf3d_workspace_set_tuning", self._h, int(key), int(value))
+
+    def solve_counts(self) -> tuple[int, int]:
+        """(operator applications, preconditioner applications) that the last
+        single solve on this workspace enqueued, graph replays included."""
+        op, pc = C.c_long(), C.c_long()
+        _lib.call("mf3d_workspace_solve_counts", self._h, C.byref(op), C.byref(pc))
+        return op.value, pc.value
 
     def clear(self) -> None:
         self.close()

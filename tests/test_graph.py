@@ -42,3 +42,32 @@ def test_graph_replay_equals_plain_launches(ctx, mix):
     for (h0, it0, rs0), (h1, it1, rs1) in zip(heads[0], heads[1]):
         assert it0 == it1 and rs0 == rs1
         assert np.array_equal(h0, h1)
+
+
+@pytest.mark.parametrize("graph", [1, 0])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_one_operator_application_per_iteration(ctx, graph, precision):
+    """The PCG solve enqueues one operator application per preconditioner
+    application; besides them only the initial residual, the true-residual
+    checks and the final true residual apply the operator (a doubled A p in
+    the iteration would add one per iteration). Every iteration the solve
+    reports was enqueued."""
+    g, _ = grid_pair(64, 64, 64)
+    bc, _ = bc_pair("darcy_x")
+    n = g.num_cells()
+    K = dev(random_K(n, 93, 1.5), ctx)
+    cfg = (ph.HeadSolveConfig.mixed(1e-10, 4) if precision == "mixed"
+           else ph.HeadSolveConfig.fp64_direct(1e-10, 4))
+    ws = ph.FlowWorkspace()
+    ws.set_tuning(ph.FlowWorkspace.TUNE_GRAPH, graph)
+    for _ in range(2):  # the counts restart with each solve
+        h = ctx.zeros(n)
+        res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+        assert res.converged
+        n_op, n_pc = ws.solve_counts()
+        passes = res.restarts + 1
+        print(f"{precision} graph={graph}: {res.num_iterations} iterations, "
+              f"{n_op} operator, {n_pc} preconditioner applications", flush=True)
+        assert n_pc - passes >= res.num_iterations
+        assert 0 <= n_op - n_pc <= 2
+    ws.close()
