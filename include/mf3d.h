@@ -241,8 +241,16 @@ enum {
     MF3D_TUNE_FOLD_RZ = 12,             /* 1 (default): ... and its (r, z) in the last one */
     MF3D_TUNE_OP_TILE = 13,             /* 1 (default): the FP64 operator stages plane tiles in
                                            LDS when nx <= 512 (even, 16-byte aligned vectors) */
-    MF3D_TUNE_GRAPH = 14                /* 1 (default): a mixed-precision single solve captures
-                                           one PCG iteration in a hipGraph and replays it */
+    MF3D_TUNE_GRAPH = 14,               /* 1: a mixed-precision single solve captures one PCG
+                                           iteration in a hipGraph and replays it; 0: it
+                                           launches the kernels; -1 (default): replay when the
+                                           grid has at most 2^21 cells */
+    MF3D_TUNE_OP_CHUNK = 15,            /* 16 (default): the FP64 operator's z-planes per
+                                           workgroup, at most (1..64; fewer where the launch
+                                           would have too few workgroups) */
+    MF3D_TUNE_MARCH_BLOCKS = 16         /* 2048 (default): workgroups an FP64 colour-split
+                                           sweep aims for (256..8192) by shortening its chunks
+                                           of planes */
 };
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 

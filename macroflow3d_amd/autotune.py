@@ -11,15 +11,23 @@ Its choices change where and when the work runs, not what is computed:
 * ``op_tile``: the finest-level FP64 operator on plane tiles staged in LDS or
   on register cell pairs: the same faces and per-cell sums, but the (p, Ap)
   partials are grouped by a different tile geometry, so the head agrees to
-  the rounding of that reduction (iteration counts may differ by one).
+  the rounding of that reduction (iteration counts may differ by one);
+* ``op_chunk``: the operator's z-planes per workgroup (at most; fewer where
+  the launch would have too few workgroups): the same per-cell sums, the
+  (p, Ap) partials regrouped;
+* ``march_blocks``: the workgroup count the FP64 V-cycle's colour-split
+  sweeps aim for by shortening their chunks of planes: the same per-cell
+  values, the (r, z) partials regrouped.
 
 The FP64 V-cycle's sweeps also take ``sweep_block``; there the (r, z)
 partials are one per workgroup, so a different size regroups that sum.
 
 ``tune_sweep_block`` times the finest-level smoothing in place;
-``tune_solve`` times whole single solves over (sweep_block, graph, op_tile);
-``tune_batch`` times whole batched solves over (sweep_block, graph). Each
-candidate gets one untimed run, then the median of ``reps`` timed runs.
+``tune_solve`` times whole single solves over (sweep_block, graph, op_tile),
+then, from the fastest of those, over ``op_chunk`` and, when the V-cycle is
+FP64, ``march_blocks`` (one knob at a time); ``tune_batch`` times whole
+batched solves over (sweep_block, graph). Each candidate gets one untimed
+run, then the median of ``reps`` timed runs.
 
 Results are cached per device, problem shape, solver configuration and
 boundary types in a JSON file (default
@@ -152,25 +160,43 @@ def tune_sweep_block(ctx, grid, bc, cfg, K, reps: int = 5, use_cache: bool = Tru
                           use_cache)
 
 
+OP_CHUNKS = (4, 8, 16, 32)
+MARCH_BLOCKS = (1024, 2048, 4096)
+
+
 def tune_solve(ctx, grid, bc, cfg, K, reps: int = 3, use_cache: bool = True,
-               blocks=(128, 256, 512)) -> dict:
-    """Fastest (sweep_block, graph, op_tile) for single solves of this grid:
-    whole solves into a scratch head, one untimed then the median of reps."""
+               blocks=(128, 256, 512), op_chunks=OP_CHUNKS, march_blocks=MARCH_BLOCKS) -> dict:
+    """Fastest (sweep_block, graph, op_tile), then op_chunk and (FP64
+    V-cycle) march_blocks, for single solves of this grid: whole solves into
+    a scratch head, one untimed then the median of reps."""
+    h = ctx.zeros(grid.num_cells())
+
+    def time_params(params):
+        ws = ph.FlowWorkspace()
+        apply([ws], params)
+        t = _median_ms(ctx, lambda: ph.solve_head(h, K, grid, bc, cfg, ctx, ws), reps, False)
+        ws.close()
+        return t
+
     def tune():
         times = {}
-        h = ctx.zeros(grid.num_cells())
         for b in blocks:
             for graph in (True, False):
                 for tile in (True, False):
-                    ws = ph.FlowWorkspace()
-                    apply([ws], {"sweep_block": b, "graph": graph, "op_tile": tile})
-                    times[(b, graph, tile)] = _median_ms(
-                        ctx, lambda: ph.solve_head(h, K, grid, bc, cfg, ctx, ws), reps, False)
-                    ws.close()
+                    times[(b, graph, tile)] = time_params(
+                        {"sweep_block": b, "graph": graph, "op_tile": tile})
         b, graph, tile = min(times, key=times.get)
-        return {"sweep_block": b, "graph": graph, "op_tile": tile,
-                "ms": {f"{k[0]}/{'graph' if k[1] else 'launches'}/"
-                       f"{'tile' if k[2] else 'pairs'}": v for k, v in times.items()}}
+        best = {"sweep_block": b, "graph": graph, "op_tile": tile}
+        ms = {f"{k[0]}/{'graph' if k[1] else 'launches'}/{'tile' if k[2] else 'pairs'}": v
+              for k, v in times.items()}
+        stages = [("op_chunk", op_chunks)]
+        if cfg.precond_precision == ph.PrecondPrecision.FP64:
+            stages.append(("march_blocks", march_blocks))
+        for knob, values in stages:
+            cand = {v: time_params({**best, knob: v}) for v in values}
+            best[knob] = min(cand, key=cand.get)
+            ms.update({f"{knob}={v}": t for v, t in cand.items()})
+        return {**best, "ms": ms}
     return cached_or_tune(_key("solve", ctx, [(grid.nx, grid.ny, grid.nz)], cfg, bc), tune,
                           use_cache)
 
@@ -203,3 +229,7 @@ def apply(workspaces, params: dict) -> None:
             w.set_tuning(ph.FlowWorkspace.TUNE_GRAPH, int(bool(params["graph"])))
         if "op_tile" in params:
             w.set_tuning(ph.FlowWorkspace.TUNE_OP_TILE, int(bool(params["op_tile"])))
+        if "op_chunk" in params:
+            w.set_tuning(ph.FlowWorkspace.TUNE_OP_CHUNK, int(params["op_chunk"]))
+        if "march_blocks" in params:
+            w.set_tuning(ph.FlowWorkspace.TUNE_MARCH_BLOCKS, int(params["march_blocks"]))

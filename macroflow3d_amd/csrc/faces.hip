@@ -1785,7 +1785,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // MODE 0: Ap = A p with (p, Ap) partials, skipped once stopped. MODE 1:
 // r = rhs - A x with (r, r) partials, always (the true-residual check).
-// start[] counts op_partials() workgroups per realisation.
+// start[] counts each realisation's operator workgroups (its op_plan geometry).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
     op_batch_kernel(const BatchVec* __restrict__ Vs, const int* __restrict__ start, int m,
@@ -2199,37 +2199,37 @@ int residual_restrict_faces(hipStream_t s, const mf3d_grid& g, const BcTypes& bc
     }
 }
 
-OpGeom op_geom(const mf3d_grid& g, bool pairs) {
+OpGeom op_geom(const mf3d_grid& g, bool pairs, int kz_max) {
     OpGeom o;
     o.pairs = pairs && !(g.nx & 1);
     const long P = (long)g.nx * g.ny;
     const long per = o.pairs ? 2 * kBlock : kBlock;
     o.tiles = (int)((P + per - 1) / per);
-    // z-chunks of up to 16 planes, shortened while the launch has fewer than
-    // 2048 workgroups (8 per CU)
-    o.kz = (int)std::max(1L, std::min(16L, (long)o.tiles * g.nz / 2048));
+    // z-chunks of up to kz_max planes, shortened while the launch has fewer
+    // than 2048 workgroups (8 per CU)
+    o.kz = (int)std::max(1L, std::min((long)kz_max, (long)o.tiles * g.nz / 2048));
     o.chunks = (g.nz + o.kz - 1) / o.kz;
     return o;
 }
 
 // the plane-tile variant: kOpTile cells per workgroup
-static OpGeom op_tile_geom(const mf3d_grid& g) {
+static OpGeom op_tile_geom(const mf3d_grid& g, int kz_max) {
     OpGeom o;
     o.pairs = true;
     const long P = (long)g.nx * g.ny;
     o.tiles = (int)((P + kOpTile - 1) / kOpTile);
-    o.kz = (int)std::max(1L, std::min(16L, (long)o.tiles * g.nz / 1024));
+    o.kz = (int)std::max(1L, std::min((long)kz_max, (long)o.tiles * g.nz / 1024));
     o.chunks = (g.nz + o.kz - 1) / o.kz;
     return o;
 }
 
-OpGeom op_plan(const mf3d_grid& g, bool pairs, bool allow_tile) {
+OpGeom op_plan(const mf3d_grid& g, bool pairs, bool allow_tile, int kz_max) {
     if (pairs && allow_tile && !(g.nx & 1) && g.nx <= kOpLdsMaxNx) {
-        OpGeom o = op_tile_geom(g);
+        OpGeom o = op_tile_geom(g, kz_max);
         o.tile = true;
         return o;
     }
-    return op_geom(g, pairs);
+    return op_geom(g, pairs, kz_max);
 }
 
 size_t op_tile_lds(const mf3d_grid& g) {
@@ -2237,7 +2237,7 @@ size_t op_tile_lds(const mf3d_grid& g) {
 }
 
 unsigned op_partials(const mf3d_grid& g) {
-    const OpGeom a = op_geom(g, false), b = op_geom(g, true), c = op_tile_geom(g);
+    const OpGeom a = op_geom(g, false, 1), b = op_geom(g, true, 1), c = op_tile_geom(g, 1);
     return (unsigned)std::max({(long)a.tiles * a.chunks, (long)b.tiles * b.chunks,
                                (long)c.tiles * c.chunks});
 }
@@ -2250,12 +2250,14 @@ bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y
 
 int op_apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* H,
              int mode, const double* x, const double* b, double* y, double* partials,
-             unsigned* np, bool allow_pairs, bool allow_tile) {
+             unsigned* np, bool allow_pairs, bool allow_tile, int kz_max) {
+    if (kz_max < 1 || kz_max > kOpChunkMax)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "operator chunk depth out of range");
     const double inv_dx2 = 1.0 / (g.dx * g.dx);
     const bool pairs = allow_pairs && op_pairs_ok(g, x, b, y, H);
     const Dims d = dims(g);
     const int pn = pin ? 1 : 0;
-    const OpGeom o = op_plan(g, pairs, allow_tile);
+    const OpGeom o = op_plan(g, pairs, allow_tile, kz_max);
     const long nb = (long)o.tiles * o.chunks;
     const unsigned grid = xcd_round((size_t)nb);
     if (o.tile) {

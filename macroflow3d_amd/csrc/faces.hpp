@@ -103,12 +103,15 @@ struct OpGeom {
     bool pairs;
     bool tile = false;  // plane tiles staged in LDS (pairs too)
 };
-OpGeom op_geom(const mf3d_grid& g, bool pairs);
+// z-chunks of at most kz_max planes per workgroup (kOpChunkDefault unless tuned)
+constexpr int kOpChunkDefault = 16, kOpChunkMax = 64;
+OpGeom op_geom(const mf3d_grid& g, bool pairs, int kz_max = kOpChunkDefault);
 // the variant op_apply runs: plane tiles with pairs when allowed and nx <= 512
-OpGeom op_plan(const mf3d_grid& g, bool pairs, bool allow_tile);
+OpGeom op_plan(const mf3d_grid& g, bool pairs, bool allow_tile, int kz_max = kOpChunkDefault);
 // dynamic LDS bytes of the plane-tile variant
 size_t op_tile_lds(const mf3d_grid& g);
-unsigned op_partials(const mf3d_grid& g);  // the larger count of the two variants
+// the most partials any variant writes, at any chunk depth (one plane per chunk)
+unsigned op_partials(const mf3d_grid& g);
 bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y,
                  const void* H);
 // mode 0: y = A x with (x, y) partials; mode 1: y = b - A x (y nullable) with
@@ -117,7 +120,8 @@ bool op_pairs_ok(const mf3d_grid& g, const void* x, const void* b, const void* y
 // plane-tile variant with the y neighbours staged in LDS
 int op_apply(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, const double* H,
              int mode, const double* x, const double* b, double* y, double* partials,
-             unsigned* np, bool allow_pairs = true, bool allow_tile = true);
+             unsigned* np, bool allow_pairs = true, bool allow_tile = true,
+             int kz_max = kOpChunkDefault);
 int to_f32(hipStream_t s, const double* a, float* o, size_t n);
 int to_f64(hipStream_t s, const float* a, double* o, size_t n);
 // conversions between a natural FP64 vector and an FP32 vector of `order`
@@ -147,7 +151,10 @@ int sum_log_partials(hipStream_t s, const double* K, size_t n, double* partials)
 bool split64_ok(const mf3d_grid& g);
 // (r, z) partials written by the finest level's last half-sweep (tb threads
 // per workgroup)
-unsigned split64_partials(const mf3d_grid& g, int tb);
+// blocks: the workgroup count a sweep aims for (planes per chunk shortened
+// until the launch reaches it, where the level allows)
+constexpr long kMarchDefaultBlocks = 2048, kMarchMinBlocks = 256, kMarchMaxBlocks = 8192;
+unsigned split64_partials(const mf3d_grid& g, int tb, long blocks = kMarchDefaultBlocks);
 // hs = 1 / (2 K) in split order, and in natural order into hn when not null
 int split_h(hipStream_t s, const mf3d_grid& g, const double* K, double* hs, double* hn);
 int to_split64(hipStream_t s, const mf3d_grid& g, const double* nat, double* sp);
@@ -163,7 +170,8 @@ int from_split64(hipStream_t s, const mf3d_grid& g, const double* sp, double* na
 int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                    double* x, double* b, const double* hs, int iters, bool zero_init,
                    const double* xc, int corder, int tb, const double* r64, double* z,
-                   const double* rdot, double* partials, unsigned* np);
+                   const double* rdot, double* partials, unsigned* np,
+                   long blocks = kMarchDefaultBlocks);
 // coarse b = restriction of (b - A x) (rs = 1 / dx^2) into a coarse level of
 // order corder; red_only after a sweep that ended with the black half-sweep
 int residual_restrict_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,

@@ -150,9 +150,12 @@ struct mf3d_workspace {
     // alignment, the same way for single and batched solves)
     bool op_pairs = true;
     bool op_tile = true;  // plane-tile operator where it applies (MF3D_TUNE_OP_TILE)
-    // single solves replay one captured PCG iteration (MF3D_TUNE_GRAPH); the
-    // capture runs on this private stream (created on first use)
-    bool graph = true;
+    int op_chunk = kern::kOpChunkDefault;        // operator planes per workgroup, at most
+    long march_blocks = kern::kMarchDefaultBlocks;  // FP64 split sweeps' workgroup target
+    // single solves replay one captured PCG iteration (MF3D_TUNE_GRAPH: 1
+    // always, 0 never, -1 when the finest level has at most kGraphMaxCells
+    // cells); the capture runs on this private stream (created on first use)
+    int graph = -1;
     hipStream_t capture_stream = nullptr;
     // finest-level operator and preconditioner applications enqueued since
     // the last single solve began (replays of a captured iteration included)
@@ -305,9 +308,12 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     MF3D_TRY(ws->alloc(&ws->p, n));
     MF3D_TRY(ws->alloc(&ws->Ap, n));
     // (the finest split4 sweeps write one per workgroup of >= 64 threads, the
-    // FP64 split sweeps too, with 4 cells per thread)
-    ws->partials_cap =
-        std::max<size_t>({kern::reduce_blocks(n), kern::op_partials(g), n / 256}) + 64;
+    // FP64 split sweeps too, with 4 cells per thread, in at most
+    // kMarchMaxBlocks + one plane's workgroups where chunks are shortened)
+    const size_t march_cap =
+        (size_t)kern::kMarchMaxBlocks + ((size_t)g.nx * g.ny / 4 + 63) / 64;
+    ws->partials_cap = std::max<size_t>({kern::reduce_blocks(n), kern::op_partials(g), n / 256,
+                                         march_cap}) + 64;
     MF3D_TRY(ws->alloc(&ws->partials, ws->partials_cap));
     // one per workgroup of the finest sweep (8 cells per thread, >= 64 threads)
     MF3D_TRY(ws->alloc(&ws->rr_partials, n / 512 + 64));
@@ -796,7 +802,7 @@ int vcycle64(mf3d_workspace* ws, hipStream_t s, size_t l, bool zero_init, const 
     const int tb = L.plan.block;
     MF3D_TRY(kern::smooth_split64(s, L.g, ws->bct, ws->pin, dx2, L.x, L.b, L.hs, ws->pre,
                                   zero_init, nullptr, ORDER_NATURAL, tb, r64, nullptr, nullptr,
-                                  nullptr, nullptr));
+                                  nullptr, nullptr, ws->march_blocks));
     MF3D_TRY(kern::residual_restrict_split64(s, L.g, ws->bct, ws->pin, 1.0 / dx2, L.x, L.b, L.hs,
                                              C.b, C.order64, ws->pre > 0));
     if (C.order64 == ORDER_SPLIT)
@@ -804,7 +810,8 @@ int vcycle64(mf3d_workspace* ws, hipStream_t s, size_t l, bool zero_init, const 
     else
         MF3D_TRY(vcycle<double>(ws, s, l + 1, C.x, C.b, true));
     return kern::smooth_split64(s, L.g, ws->bct, ws->pin, dx2, L.x, L.b, L.hs, ws->post, false,
-                                C.x, C.order64, tb, nullptr, z, rdot, partials, np);
+                                C.x, C.order64, tb, nullptr, z, rdot, partials, np,
+                                ws->march_blocks);
 }
 
 // PCG may leave z = x32 unstored (the (r, z) partials and the p update read the
@@ -887,14 +894,14 @@ int apply_op(mf3d_workspace* ws, hipStream_t s, const double* x, double* y, doub
     Level& L0 = ws->lv[0];
     ++ws->n_op;
     return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 0, x, nullptr, y, partials, np,
-                          ws->op_pairs, ws->op_tile);
+                          ws->op_pairs, ws->op_tile, ws->op_chunk);
 }
 int residual_op(mf3d_workspace* ws, hipStream_t s, const double* x, const double* b, double* r,
                 double* partials, unsigned* np = nullptr) {
     Level& L0 = ws->lv[0];
     ++ws->n_op;
     return kern::op_apply(s, L0.g, ws->bct, ws->pin, ws->h0, 1, x, b, r, partials, np,
-                          ws->op_pairs, ws->op_tile);
+                          ws->op_pairs, ws->op_tile, ws->op_chunk);
 }
 
 IterState make_state(const mf3d_head_solve_config& cfg, double rtol) {
@@ -941,6 +948,8 @@ struct Chunker {
         return std::max(1, std::min(c, 32));
     }
 };
+
+constexpr size_t kGraphMaxCells = size_t(1) << 21;  // graph replay by default up to 128^3
 
 struct GraphHolder {
     hipGraph_t g = nullptr;
@@ -1002,13 +1011,18 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
                                  ws->st));
         return kern::update_xp_split8(q, ws->st, L0.x32, L0.g, x, ws->p);
     };
+    // replaying the captured iteration saves host launch time, which bounds
+    // small grids; from 128^3 up the device is the bound and plain launches
+    // are as fast or faster (64^3: 164 vs 172 us per iteration, 256^3: 844 vs
+    // 831 us on MI355X)
+    const bool use_graph = ws->graph == 1 || (ws->graph == -1 && n <= kGraphMaxCells);
     GraphHolder G;
     long g_op = 0, g_pc = 0;  // applications in one replay of G
     int restarts = 0;
     for (;;) {
         MF3D_TRY(precondition(ws, s, ws->r, z, ws->partials, nullptr, &nprz));
         MF3D_TRY(kern::finalize(s, ws->partials, nprz, FIN_RZ0, ws->st, nullptr));
-        if (fuse_r && ws->graph && !G.x) {
+        if (fuse_r && use_graph && !G.x) {
             // captured on the workspace's private stream (the caller's may be
             // the null stream, which cannot capture), replayed on the caller's;
             // the coarsest inverse has joined in the V-cycle above
@@ -1297,7 +1311,8 @@ int build_plan(BPlan& P, mf3d_workspace* const* ws, int count, double* const* h)
         v.partials = w->partials;
         v.np = kern::reduce_blocks(L0.n);
         // the variant and geometry single solves use for the same vectors (op_apply)
-        const kern::OpGeom og = kern::op_plan(L0.g, op_pairs_for(w, v.x), w->op_tile);
+        const kern::OpGeom og =
+            kern::op_plan(L0.g, op_pairs_for(w, v.x), w->op_tile, w->op_chunk);
         v.np_op = (unsigned)og.tiles * (unsigned)og.chunks;
         v.otiles = og.tiles;
         v.okz = og.kz;
@@ -1994,7 +2009,20 @@ int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value) {
     case MF3D_TUNE_FOLD_R: ws->fold_r = value != 0; break;
     case MF3D_TUNE_FOLD_RZ: ws->fold_rz = value != 0; break;
     case MF3D_TUNE_OP_TILE: ws->op_tile = value != 0; break;
-    case MF3D_TUNE_GRAPH: ws->graph = value != 0; break;
+    case MF3D_TUNE_OP_CHUNK:
+        if (value < 1 || value > kern::kOpChunkMax)
+            return fail(MF3D_ERR_INVALID_ARGUMENT, "operator chunk depth must be 1..64 planes");
+        ws->op_chunk = (int)value;
+        break;
+    case MF3D_TUNE_MARCH_BLOCKS:
+        if (value < kern::kMarchMinBlocks || value > kern::kMarchMaxBlocks)
+            return fail(MF3D_ERR_INVALID_ARGUMENT, "march workgroup target must be 256..8192");
+        ws->march_blocks = value;
+        break;
+    case MF3D_TUNE_GRAPH:
+        if (value < -1 || value > 1) return fail(MF3D_ERR_INVALID_ARGUMENT, "graph must be -1, 0 or 1");
+        ws->graph = (int)value;
+        break;
     case MF3D_TUNE_SWEEP_BLOCK:
         if (value < 64 || value > 512 || (value & 63))
             return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep block must be 64..512, a multiple of 64");
@@ -2032,7 +2060,7 @@ int mf3d_mg_smooth(mf3d_workspace* ws, mf3d_stream stream, int level, int num_it
     if (L.order64 == ORDER_SPLIT)  // in place on the level's split x and b
         return kern::smooth_split64(s, L.g, ws->bct, ws->pin, dx2, L.x, L.b, L.hs, num_iters,
                                     false, nullptr, ORDER_NATURAL, L.plan.block, nullptr,
-                                    nullptr, nullptr, nullptr, nullptr);
+                                    nullptr, nullptr, nullptr, nullptr, ws->march_blocks);
     double* x = level == 0 ? ws->z : L.x;
     const double* b = level == 0 ? ws->r : L.b;
     return kern::smooth_faces(s, L.g, ws->bct, ws->pin, dx2, x, b, L.f64, num_iters, false,

@@ -115,13 +115,12 @@ struct March {
     int kz;      // planes per chunk
     int chunks;  // ceil(nz / kz)
 };
-// at least kMarchBlocks workgroups where the level allows, else one plane per chunk
-constexpr long kMarchBlocks = 2048;
-March march_of(const Geo& G, int tb) {
+// at least `blocks` workgroups where the level allows, else one plane per chunk
+March march_of(const Geo& G, int tb, long blocks) {
     March m;
     const long P = (long)G.h2 * G.ny;
     m.tiles = (int)((P + tb - 1) / tb);
-    const long want = (kMarchBlocks + m.tiles - 1) / m.tiles;
+    const long want = (blocks + m.tiles - 1) / m.tiles;
     const int chunks = (int)std::max(1L, std::min((long)G.nz, want));
     m.kz = (G.nz + chunks - 1) / chunks;
     m.chunks = (G.nz + m.kz - 1) / m.kz;
@@ -538,8 +537,9 @@ inline unsigned blocks_for(long threads, int tb) {
 template <int MODE, bool CS, bool RZ>
 int launch_sweep(hipStream_t s, const Geo& G, const BcTypes& bc, bool pin, double bs, double* x,
                  const double* b, const double* hs, int colour, const double* xc,
-                 const double* r64, double* bw, double* z, double* partials, int tb) {
-    const March M = march_of(G, tb);
+                 const double* r64, double* bw, double* z, double* partials, int tb,
+                 long blocks) {
+    const March M = march_of(G, tb, blocks);
     march64_kernel<MODE, CS, RZ><<<xcd_round((size_t)M.tiles * M.chunks), tb, 0, s>>>(
         G, bc, pin ? 1 : 0, bs, x, b, hs, colour, xc, r64, bw, z, partials, M);
     MF3D_HIP_TRY(hipGetLastError());
@@ -548,19 +548,19 @@ int launch_sweep(hipStream_t s, const Geo& G, const BcTypes& bc, bool pin, doubl
 
 int sweep(hipStream_t s, const Geo& G, const BcTypes& bc, bool pin, double bs, double* x,
           const double* b, const double* hs, int colour, int mode, bool cs, const double* xc,
-          const double* r64, double* bw, int tb) {
+          const double* r64, double* bw, int tb, long blocks) {
     switch (mode) {
     case 0: return launch_sweep<0, false, false>(s, G, bc, pin, bs, x, b, hs, colour, nullptr,
-                                                 nullptr, nullptr, nullptr, nullptr, tb);
+                                                 nullptr, nullptr, nullptr, nullptr, tb, blocks);
     case 1: return launch_sweep<1, false, false>(s, G, bc, pin, bs, x, b, hs, colour, nullptr,
-                                                 nullptr, nullptr, nullptr, nullptr, tb);
+                                                 nullptr, nullptr, nullptr, nullptr, tb, blocks);
     case 3: return launch_sweep<3, false, false>(s, G, bc, pin, bs, x, bw, hs, colour, nullptr,
-                                                 r64, bw, nullptr, nullptr, tb);
+                                                 r64, bw, nullptr, nullptr, tb, blocks);
     default:
         return cs ? launch_sweep<2, true, false>(s, G, bc, pin, bs, x, b, hs, colour, xc, nullptr,
-                                                 nullptr, nullptr, nullptr, tb)
+                                                 nullptr, nullptr, nullptr, tb, blocks)
                   : launch_sweep<2, false, false>(s, G, bc, pin, bs, x, b, hs, colour, xc,
-                                                  nullptr, nullptr, nullptr, nullptr, tb);
+                                                  nullptr, nullptr, nullptr, nullptr, tb, blocks);
     }
 }
 
@@ -573,8 +573,8 @@ bool split64_ok(const mf3d_grid& g) {
            g.nz >= 2;
 }
 
-unsigned split64_partials(const mf3d_grid& g, int tb) {
-    const March M = march_of(geo_of(g), tb);
+unsigned split64_partials(const mf3d_grid& g, int tb, long blocks) {
+    const March M = march_of(geo_of(g), tb, blocks);
     return (unsigned)M.tiles * (unsigned)M.chunks;
 }
 
@@ -607,10 +607,12 @@ int from_split64(hipStream_t s, const mf3d_grid& g, const double* sp, double* na
 int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                    double* x, double* b, const double* hs, int iters, bool zero_init,
                    const double* xc, int corder, int tb, const double* r64, double* z,
-                   const double* rdot, double* partials, unsigned* np) {
+                   const double* rdot, double* partials, unsigned* np, long blocks) {
     if (!split64_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "smooth_split64: sizes not split-capable");
     if (tb < 64 || tb > kMaxBlock || tb % 64)
         return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep workgroup size must be 64..512, a multiple of 64");
+    if (blocks < kMarchMinBlocks || blocks > kMarchMaxBlocks)
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "march workgroup target out of range");
     if (zero_init && xc) return fail(MF3D_ERR_INVALID_ARGUMENT, "zero start with a coarse correction");
     if (r64 && !zero_init) return fail(MF3D_ERR_INVALID_ARGUMENT, "b from r64 needs a zero start");
     const Geo G = geo_of(g);
@@ -642,11 +644,12 @@ int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pi
                 // iters == 1, where it ran just above)
                 MF3D_TRY((launch_sweep<0, false, true>(s, G, bc, pin, bs, x, b, hs, 1, nullptr,
                                                        rdot, nullptr, z,
-                                                       rdot ? partials : nullptr, tb)));
-                if (np) *np = split64_partials(g, tb);
+                                                       rdot ? partials : nullptr, tb,
+                                                       blocks)));
+                if (np) *np = split64_partials(g, tb, blocks);
                 continue;
             }
-            MF3D_TRY(sweep(s, G, bc, pin, bs, x, b, hs, colour, mode, cs, xc, r64, b, tb));
+            MF3D_TRY(sweep(s, G, bc, pin, bs, x, b, hs, colour, mode, cs, xc, r64, b, tb, blocks));
         }
     }
     return MF3D_OK;

@@ -186,13 +186,17 @@ class FlowWorkspace:
     TUNE_FOLD_RZ = 12
     TUNE_OP_TILE = 13
     TUNE_GRAPH = 14
+    TUNE_OP_CHUNK = 15
+    TUNE_MARCH_BLOCKS = 16
 
     def set_tuning(self, key: int, value: int) -> None:
         """Kernel-layout knobs (colour-split storage, sweep workgroup size,
-        folded PCG steps, plane-tile operator, graph replay); applied at the
-        next set-up or solve. None changes the result beyond rounding; the
-        sweep block size of the mixed V-cycle and the graph replay leave it
-        bit-identical, the operator tile regroups the (p, Ap) partials."""
+        folded PCG steps, plane-tile operator and its chunk depth, the FP64
+        sweeps' workgroup target, graph replay); applied at the next set-up
+        or solve. None changes the result beyond rounding; the sweep block
+        size of the mixed V-cycle and the graph replay leave it bit-identical,
+        the operator tile and chunk depth regroup the (p, Ap) partials, the
+        FP64 sweeps' block size and workgroup target the (r, z) partials."""
         _lib.call("mf3d_workspace_set_tuning", self._h, int(key), int(value))
 
     def solve_counts(self) -> tuple[int, int]:

@@ -114,3 +114,25 @@ def test_python_layer_validates_like_reference():
     assert cfg.pin == PinMode.On and cfg.cg_rtol == 1e-6
     assert HeadSolveConfig.from_yaml({"solver": "cg"}).solver_type == HeadSolverType.CG
     assert HeadSolveConfig.from_yaml({"solver": "mg"}).solver_type == HeadSolverType.MG
+
+
+def test_tuning_keys_validate_their_ranges():
+    """Workspace tuning keys accept their documented ranges and reject the
+    rest without device work; a fresh workspace counts no applications."""
+    L = _lib.lib()
+    ws = C.c_void_p()
+    assert L.mf3d_workspace_create(C.byref(ws)) == 0
+    cases = {14: ([-1, 0, 1], [-2, 2]),           # graph replay
+             15: ([1, 16, 64], [0, 65]),          # operator chunk depth
+             16: ([256, 2048, 8192], [255, 8193]),  # FP64 sweep workgroup target
+             5: ([64, 256, 512], [32, 100, 576])}  # sweep workgroup size
+    for key, (good, bad) in cases.items():
+        for v in good:
+            assert L.mf3d_workspace_set_tuning(ws, key, v) == 0, (key, v)
+        for v in bad:
+            assert L.mf3d_workspace_set_tuning(ws, key, v) != 0, (key, v)
+    assert L.mf3d_workspace_set_tuning(ws, 99, 1) != 0
+    op, pc = C.c_long(-1), C.c_long(-1)
+    assert L.mf3d_workspace_solve_counts(ws, C.byref(op), C.byref(pc)) == 0
+    assert (op.value, pc.value) == (0, 0)
+    assert L.mf3d_workspace_destroy(ws) == 0

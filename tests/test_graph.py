@@ -26,7 +26,7 @@ def test_graph_replay_equals_plain_launches(ctx, mix):
     n = g.num_cells()
     cfg = ph.HeadSolveConfig.mixed(1e-10, 4)
     heads = {}
-    for graph in (0, 1):
+    for graph in (0, 1, -1):  # -1: the default, replay up to 2^21 cells
         ws = ph.FlowWorkspace()
         ws.set_tuning(ph.FlowWorkspace.TUNE_GRAPH, graph)
         out = []
@@ -39,9 +39,10 @@ def test_graph_replay_equals_plain_launches(ctx, mix):
             out.append((h.cpu().numpy(), res.num_iterations, res.restarts))
         heads[graph] = out
         ws.close()
-    for (h0, it0, rs0), (h1, it1, rs1) in zip(heads[0], heads[1]):
-        assert it0 == it1 and rs0 == rs1
-        assert np.array_equal(h0, h1)
+    for other in (1, -1):
+        for (h0, it0, rs0), (h1, it1, rs1) in zip(heads[0], heads[other]):
+            assert it0 == it1 and rs0 == rs1
+            assert np.array_equal(h0, h1)
 
 
 @pytest.mark.parametrize("graph", [1, 0])

@@ -94,7 +94,8 @@ def test_autotuner_choices_leave_the_head_unchanged(ctx, tmp_path, monkeypatch):
     assert np.array_equal(out[0], out[1])
     # whole-solve tuning: the tuned solve reaches the head of the default one
     t2 = autotune.tune_solve(ctx, g, bc, cfg, K, reps=1, blocks=(128, 256))
-    assert {"sweep_block", "graph", "op_tile", "ms"} <= set(t2)
+    assert {"sweep_block", "graph", "op_tile", "op_chunk", "ms"} <= set(t2)
+    assert t2["op_chunk"] in autotune.OP_CHUNKS
     heads = []
     for params in ({}, t2):
         ws = ph.FlowWorkspace()
