@@ -274,6 +274,23 @@ __global__ void __launch_bounds__(kBlock)
     if (i < n) p[i] = 1.0 * z[i] + beta * p[i];
 }
 
+// x += alpha p and p = z + beta p in one pass (the x half of
+// pcg_update_xr_kernel and pcg_update_p_kernel), for the iteration whose r
+// update rode on the preconditioner's first half-sweep; skipped once stopped
+// (pcg_x_pending_kernel then applies the last x step).
+__global__ void __launch_bounds__(kBlock)
+    pcg_update_xp_kernel(const IterState* __restrict__ st, const double* __restrict__ z,
+                         double* __restrict__ x, double* __restrict__ p, size_t n) {
+    if (st->stop) return;  // uniform across the grid
+    const double alpha = st->alpha, beta = st->beta;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double pv = p[i];
+        x[i] = alpha * pv + x[i];
+        p[i] = 1.0 * z[i] + beta * pv;
+    }
+}
+
 // CG on the negated operator (update_x_and_r_kernel, src/numerics/blas/cg_kernels.cu:19-31):
 // Ap holds A p, the CG operator is -A, so r - alpha (-Ap) = r + alpha Ap.
 __global__ void __launch_bounds__(kBlock)
@@ -389,6 +406,12 @@ int pcg_update_r(hipStream_t s, const IterState* st, const double* Ap, double* r
 }
 int pcg_x_pending(hipStream_t s, const IterState* st, const double* p, double* x, size_t n) {
     pcg_x_pending_kernel<<<grid_for(n), kBlock, 0, s>>>(st, p, x, n);
+    MF3D_LAUNCH_CHECK();
+    return MF3D_OK;
+}
+int pcg_update_xp(hipStream_t s, const IterState* st, const double* z, double* x, double* p,
+                  size_t n) {
+    pcg_update_xp_kernel<<<grid_for(n), kBlock, 0, s>>>(st, z, x, p, n);
     MF3D_LAUNCH_CHECK();
     return MF3D_OK;
 }

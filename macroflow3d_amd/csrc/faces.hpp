@@ -166,12 +166,14 @@ int from_split64(hipStream_t s, const mf3d_grid& g, const double* sp, double* na
 // and added (folded into the first half-sweep); r64 (zero_init): b is taken
 // from this natural-order vector and written to b in split order; z: the
 // last half-sweep writes x to z in natural order instead, with the (rdot, z)
-// partials when rdot is given (np: their count)
+// partials when rdot is given (np: their count); ru (r64 == ru->r, iters > 0):
+// the first half-sweep first updates r -= alpha Ap in place and writes the
+// (r, r) partials, split64_partials(g, tb, blocks) of them
 int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                    double* x, double* b, const double* hs, int iters, bool zero_init,
                    const double* xc, int corder, int tb, const double* r64, double* z,
                    const double* rdot, double* partials, unsigned* np,
-                   long blocks = kMarchDefaultBlocks);
+                   long blocks = kMarchDefaultBlocks, const RUpd* ru = nullptr);
 // coarse b = restriction of (b - A x) (rs = 1 / dx^2) into a coarse level of
 // order corder; red_only after a sweep that ended with the black half-sweep
 int residual_restrict_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin,

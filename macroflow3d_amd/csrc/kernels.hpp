@@ -54,6 +54,8 @@ int pcg_update_r(hipStream_t s, const IterState* st, const double* Ap, double* r
                  double* partials);
 int pcg_x_pending(hipStream_t s, const IterState* st, const double* p, double* x, size_t n);
 int pcg_update_p(hipStream_t s, const IterState* st, const double* z, double* p, size_t n);
+int pcg_update_xp(hipStream_t s, const IterState* st, const double* z, double* x, double* p,
+                  size_t n);
 // CG on -A: if !st->stop { x += alpha p; r -= alpha (-Ap) } with partials of (r, r)
 int cg_update_xr(hipStream_t s, const IterState* st, const double* p, const double* Ap,
                  double* x, double* r, size_t n, double* partials);

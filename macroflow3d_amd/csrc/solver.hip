@@ -315,8 +315,9 @@ int ensure_workspace(mf3d_workspace* ws, const mf3d_grid& g, int levels) {
     ws->partials_cap = std::max<size_t>({kern::reduce_blocks(n), kern::op_partials(g), n / 256,
                                          march_cap}) + 64;
     MF3D_TRY(ws->alloc(&ws->partials, ws->partials_cap));
-    // one per workgroup of the finest sweep (8 cells per thread, >= 64 threads)
-    MF3D_TRY(ws->alloc(&ws->rr_partials, n / 512 + 64));
+    // one per workgroup of the finest sweep (8 cells per thread, >= 64 threads;
+    // FP64: one per march workgroup)
+    MF3D_TRY(ws->alloc(&ws->rr_partials, std::max<size_t>(n / 512, march_cap) + 64));
     double* stp = nullptr;
     MF3D_TRY(ws->alloc(&stp, (sizeof(IterState) + 7) / 8));
     ws->st = reinterpret_cast<IterState*>(stp);
@@ -795,14 +796,15 @@ int vcycle(mf3d_workspace* ws, hipStream_t s, size_t l, T* x, const T* b, bool z
 // the last half-sweep writes the result there in natural order (with the
 // (rdot, z) partials when rdot is given) instead of into the split x.
 int vcycle64(mf3d_workspace* ws, hipStream_t s, size_t l, bool zero_init, const double* r64,
-             double* z, const double* rdot, double* partials, unsigned* np) {
+             double* z, const double* rdot, double* partials, unsigned* np,
+             const RUpd* ru = nullptr) {
     Level& L = ws->lv[l];
     Level& C = ws->lv[l + 1];  // a split level is never the coarsest
     const double dx2 = L.g.dx * L.g.dx;
     const int tb = L.plan.block;
     MF3D_TRY(kern::smooth_split64(s, L.g, ws->bct, ws->pin, dx2, L.x, L.b, L.hs, ws->pre,
                                   zero_init, nullptr, ORDER_NATURAL, tb, r64, nullptr, nullptr,
-                                  nullptr, nullptr, ws->march_blocks));
+                                  nullptr, nullptr, ws->march_blocks, ru));
     MF3D_TRY(kern::residual_restrict_split64(s, L.g, ws->bct, ws->pin, 1.0 / dx2, L.x, L.b, L.hs,
                                              C.b, C.order64, ws->pre > 0));
     if (C.order64 == ORDER_SPLIT)
@@ -827,6 +829,13 @@ bool z_in_x32(const mf3d_workspace* ws, bool flexible) {
 unsigned sweep_partials(const mf3d_workspace* ws) {
     const Level& L0 = ws->lv[0];
     return (unsigned)((L0.n / 8 + L0.plan.block - 1) / L0.plan.block);
+}
+
+// the FP64 V-cycle's colour-split finest level takes r directly in its first
+// half-sweep (b from r), which can update r first
+bool r_in_first_sweep64(const mf3d_workspace* ws) {
+    return ws->precision == MF3D_PRECOND_FP64 && ws->lv.size() > 1 &&
+           ws->lv[0].order64 == ORDER_SPLIT && ws->pre > 0;
 }
 
 // the FP32 V-cycle's colour-split finest level takes r directly in its first
@@ -855,13 +864,16 @@ int precondition(mf3d_workspace* ws, hipStream_t s, const double* r, double* z,
         return fail(MF3D_ERR_UNSUPPORTED, "z can stay in x32 only on a colour-split FP32 level");
     if (np_rz) *np_rz = kern::reduce_blocks(L0.n);
     if (ws->precision == MF3D_PRECOND_FP64) {
-        if (ru) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on the FP32 V-cycle");
         if (L0.order64 == ORDER_SPLIT) {  // z and the (r, z) partials from the last sweep
+            if (ru && !r_in_first_sweep64(ws))
+                return fail(MF3D_ERR_UNSUPPORTED, "the r update needs a pre-smoothing half-sweep");
             unsigned np = 0;
-            MF3D_TRY(vcycle64(ws, s, 0, true, r, z, rz_partials ? r : nullptr, rz_partials, &np));
+            MF3D_TRY(vcycle64(ws, s, 0, true, r, z, rz_partials ? r : nullptr, rz_partials, &np,
+                              ru));
             if (np_rz) *np_rz = np;
             return MF3D_OK;
         }
+        if (ru) return fail(MF3D_ERR_UNSUPPORTED, "the r update rides on a colour-split sweep");
         MF3D_TRY(vcycle<double>(ws, s, 0, z, r, true));
         return rz_partials ? kern::dot_partials(s, r, z, L0.n, rz_partials) : MF3D_OK;
     }
@@ -983,11 +995,16 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
     const bool flexible = cfg.flexible != 0;
     const bool zx = z_in_x32(ws, flexible);  // z kept in the finest level's x32
     double* z = zx ? nullptr : ws->z;
-    // r update in the V-cycle's first sweep (its FIN_RR then follows the V-cycle)
-    const bool fuse_r = zx && ws->fold_r && r_in_first_sweep(ws);
-    const unsigned np_sweep = sweep_partials(ws);
-    unsigned nprz = 0;  // (r, z) partials of the last preconditioner application
     Level& L0 = ws->lv[0];
+    // r update in the V-cycle's first sweep (its FIN_RR then follows the V-cycle):
+    // the FP32 V-cycle with z in x32, or the FP64 one on a colour-split finest
+    // level (z natural; flexible CG needs (r_new, z_old) before the V-cycle)
+    const bool fuse64 = !flexible && ws->fold_r && r_in_first_sweep64(ws);
+    const bool fuse_r = (zx && ws->fold_r && r_in_first_sweep(ws)) || fuse64;
+    const unsigned np_sweep =
+        fuse64 ? kern::split64_partials(L0.g, L0.plan.block, ws->march_blocks)
+               : sweep_partials(ws);
+    unsigned nprz = 0;  // (r, z) partials of the last preconditioner application
     // the reference's own rule (||r||/||r0||, FP64) keeps its exact semantics
     const bool true_check =
         cfg.stop_norm == MF3D_STOP_RHS_NORM || ws->precision == MF3D_PRECOND_FP32;
@@ -1009,6 +1026,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
         MF3D_TRY(precondition(ws, q, ws->r, z, ws->partials, &ru, &nrz));
         MF3D_TRY(kern::finalize2(q, ws->rr_partials, np_sweep, FIN_RR, ws->partials, nrz, FIN_RZ,
                                  ws->st));
+        if (fuse64) return kern::pcg_update_xp(q, ws->st, ws->z, x, ws->p, n);
         return kern::update_xp_split8(q, ws->st, L0.x32, L0.g, x, ws->p);
     };
     // replaying the captured iteration saves host launch time, which bounds
@@ -1089,7 +1107,7 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
             MF3D_TRY(read_state(ws, s));
         }
         // the x += alpha p of the iteration that stopped after its r update
-        if (zx) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
+        if (zx || fuse64) MF3D_TRY(kern::pcg_x_pending(s, ws->st, ws->p, x, n));
         if (ws->host_st->stop != 1 || restarts >= 3 || !true_check) break;
         // converged on the recursive residual: decide by the true one
         MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials, &npo));

@@ -146,18 +146,21 @@ struct PlaneLd {
 // reused from registers twice.
 // MODE 0: plain; 1: x = 0 on entry; 2: coarse correction xc (split coarse
 // order if CS) prolonged into the neighbour values; 3: as 1 with b taken from
-// the natural-order r64 and both colours of the split b written to bw.
+// the natural-order r64 and both colours of the split b written to bw; with
+// ru.st set, the PCG residual update r -= alpha Ap (pcg_update_xr_kernel's
+// form, skipped once stopped) comes first, in place in ru.r (then r64), b
+// is taken from the new r and the workgroup's (r, r) partial goes to
+// ru.partials.
 // RZ (MODE 0, last half-sweep): z of the thread's 4 natural cells written
 // instead of x, and the workgroup's (r64, z) partial when partials is given.
 template <int MODE, bool CS, bool RZ>
 __device__ __forceinline__ void march_body(const Geo& G, const BcTypes& bc, int pin, double bs,
                                            double* __restrict__ x, const double* __restrict__ b,
                                            const double* __restrict__ hs, int colour,
-                                           const double* __restrict__ xc,
-                                           const double* __restrict__ r64,
+                                           const double* __restrict__ xc, const double* r64,
                                            double* __restrict__ bw, double* __restrict__ z,
                                            double* __restrict__ partials, long blk,
-                                           const March& M) {
+                                           const March& M, const RUpd& ru) {
     constexpr bool XR = MODE == 0 || MODE == 2;  // reads the other colour's x
     const long P = (long)G.h2 * G.ny;
     const int chunk = (int)(blk / M.tiles);
@@ -194,6 +197,9 @@ __device__ __forceinline__ void march_body(const Geo& G, const BcTypes& bc, int 
     d2 hzm = ld2(hs + othc + pb + Zb.off);
     PlaneLd cur = load(k0, pb);
     double acc = 0.0;
+    const bool upd = MODE == 3 && ru.st != nullptr;  // uniform
+    const bool go = upd && !ru.st->stop;
+    const double alpha = upd ? ru.st->alpha : 0.0;
     for (int k = k0; k < k1; ++k) {
         PlaneLd nxt = cur;
         if (k + 1 < k1) nxt = load(k + 1, pb + sz);
@@ -224,8 +230,30 @@ __device__ __forceinline__ void march_body(const Geo& G, const BcTypes& bc, int 
         double bb[2];
         if (MODE == 3) {
             // natural cells 4q .. 4q + 3 of the row: own colour at 2e + p
-            const double* rp = r64 + row * G.nx + 4L * q;
-            const d2 n01 = ld2(rp), n23 = ld2(rp + 2);
+            const long rn = row * G.nx + 4L * q;
+            d2 n01, n23;
+            if (upd) {
+                n01 = ld2(ru.r + rn), n23 = ld2(ru.r + rn + 2);
+                const d2 a01 = ld2(ru.Ap + rn), a23 = ld2(ru.Ap + rn + 2);
+                if (go) {
+                    n01.x = -alpha * a01.x + n01.x;
+                    n01.y = -alpha * a01.y + n01.y;
+                    n23.x = -alpha * a23.x + n23.x;
+                    n23.y = -alpha * a23.y + n23.y;
+                    if (valid) {
+                        st2(ru.r + rn, n01.x, n01.y);
+                        st2(ru.r + rn + 2, n23.x, n23.y);
+                    }
+                }
+                if (valid) {
+                    acc += n01.x * n01.x;
+                    acc += n01.y * n01.y;
+                    acc += n23.x * n23.x;
+                    acc += n23.y * n23.y;
+                }
+            } else {
+                n01 = ld2(r64 + rn), n23 = ld2(r64 + rn + 2);
+            }
             const double nv[4] = {n01.x, n01.y, n23.x, n23.y};
             bb[0] = nv[p], bb[1] = nv[2 + p];
             if (valid) {
@@ -330,18 +358,23 @@ __device__ __forceinline__ void march_body(const Geo& G, const BcTypes& bc, int 
         const double sb = block_sum_dyn(acc);
         if (threadIdx.x == 0) partials[blk] = sb;
     }
+    if (upd) {  // uniform
+        const double sb = block_sum_dyn(acc);
+        if (threadIdx.x == 0) ru.partials[blk] = sb;
+    }
 }
 
 template <int MODE, bool CS, bool RZ>
 __global__ void __launch_bounds__(kMaxBlock)
     march64_kernel(Geo G, BcTypes bc, int pin, double bs, double* __restrict__ x,
                    const double* __restrict__ b, const double* __restrict__ hs, int colour,
-                   const double* __restrict__ xc, const double* __restrict__ r64,
+                   const double* __restrict__ xc, const double* r64,
                    double* __restrict__ bw, double* __restrict__ z,
-                   double* __restrict__ partials, March M) {
+                   double* __restrict__ partials, March M, RUpd ru) {
     const long blk = xcd_block((long)M.tiles * M.chunks);
     if (blk < 0) return;  // whole workgroup
-    march_body<MODE, CS, RZ>(G, bc, pin, bs, x, b, hs, colour, xc, r64, bw, z, partials, blk, M);
+    march_body<MODE, CS, RZ>(G, bc, pin, bs, x, b, hs, colour, xc, r64, bw, z, partials, blk, M,
+                             ru);
 }
 
 // residual r = b - A x of one cell of a split level (compute_residual_3d's
@@ -538,24 +571,25 @@ template <int MODE, bool CS, bool RZ>
 int launch_sweep(hipStream_t s, const Geo& G, const BcTypes& bc, bool pin, double bs, double* x,
                  const double* b, const double* hs, int colour, const double* xc,
                  const double* r64, double* bw, double* z, double* partials, int tb,
-                 long blocks) {
+                 long blocks, const RUpd* ru = nullptr) {
     const March M = march_of(G, tb, blocks);
     march64_kernel<MODE, CS, RZ><<<xcd_round((size_t)M.tiles * M.chunks), tb, 0, s>>>(
-        G, bc, pin ? 1 : 0, bs, x, b, hs, colour, xc, r64, bw, z, partials, M);
+        G, bc, pin ? 1 : 0, bs, x, b, hs, colour, xc, r64, bw, z, partials, M,
+        ru ? *ru : RUpd{nullptr, nullptr, nullptr, nullptr});
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
 
 int sweep(hipStream_t s, const Geo& G, const BcTypes& bc, bool pin, double bs, double* x,
           const double* b, const double* hs, int colour, int mode, bool cs, const double* xc,
-          const double* r64, double* bw, int tb, long blocks) {
+          const double* r64, double* bw, int tb, long blocks, const RUpd* ru) {
     switch (mode) {
     case 0: return launch_sweep<0, false, false>(s, G, bc, pin, bs, x, b, hs, colour, nullptr,
                                                  nullptr, nullptr, nullptr, nullptr, tb, blocks);
     case 1: return launch_sweep<1, false, false>(s, G, bc, pin, bs, x, b, hs, colour, nullptr,
                                                  nullptr, nullptr, nullptr, nullptr, tb, blocks);
     case 3: return launch_sweep<3, false, false>(s, G, bc, pin, bs, x, bw, hs, colour, nullptr,
-                                                 r64, bw, nullptr, nullptr, tb, blocks);
+                                                 r64, bw, nullptr, nullptr, tb, blocks, ru);
     default:
         return cs ? launch_sweep<2, true, false>(s, G, bc, pin, bs, x, b, hs, colour, xc, nullptr,
                                                  nullptr, nullptr, nullptr, tb, blocks)
@@ -607,7 +641,8 @@ int from_split64(hipStream_t s, const mf3d_grid& g, const double* sp, double* na
 int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pin, double bs,
                    double* x, double* b, const double* hs, int iters, bool zero_init,
                    const double* xc, int corder, int tb, const double* r64, double* z,
-                   const double* rdot, double* partials, unsigned* np, long blocks) {
+                   const double* rdot, double* partials, unsigned* np, long blocks,
+                   const RUpd* ru) {
     if (!split64_ok(g)) return fail(MF3D_ERR_UNSUPPORTED, "smooth_split64: sizes not split-capable");
     if (tb < 64 || tb > kMaxBlock || tb % 64)
         return fail(MF3D_ERR_INVALID_ARGUMENT, "sweep workgroup size must be 64..512, a multiple of 64");
@@ -615,6 +650,8 @@ int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pi
         return fail(MF3D_ERR_INVALID_ARGUMENT, "march workgroup target out of range");
     if (zero_init && xc) return fail(MF3D_ERR_INVALID_ARGUMENT, "zero start with a coarse correction");
     if (r64 && !zero_init) return fail(MF3D_ERR_INVALID_ARGUMENT, "b from r64 needs a zero start");
+    if (ru && !(r64 == ru->r && iters > 0))
+        return fail(MF3D_ERR_INVALID_ARGUMENT, "the r update rides on the b-from-r half-sweep");
     const Geo G = geo_of(g);
     const bool cs = corder == ORDER_SPLIT;
     const size_t n = cells(g);
@@ -649,7 +686,8 @@ int smooth_split64(hipStream_t s, const mf3d_grid& g, const BcTypes& bc, bool pi
                 if (np) *np = split64_partials(g, tb, blocks);
                 continue;
             }
-            MF3D_TRY(sweep(s, G, bc, pin, bs, x, b, hs, colour, mode, cs, xc, r64, b, tb, blocks));
+            MF3D_TRY(sweep(s, G, bc, pin, bs, x, b, hs, colour, mode, cs, xc, r64, b, tb, blocks,
+                           mode == 3 ? ru : nullptr));
         }
     }
     return MF3D_OK;

@@ -126,3 +126,29 @@ def test_split_fp64_256cubed_matches_natural_order(ctx):
     assert out[0][1] == out[1][1]
     assert float(torch.linalg.vector_norm(out[0][0] - out[1][0]) /
                  torch.linalg.vector_norm(out[1][0])) < 1e-12
+
+
+@pytest.mark.parametrize("mix", ["darcy_x", "all_periodic", "periodic_x_dirichlet_z"])
+@pytest.mark.parametrize("semantics", ["reference", "fp64_direct"])
+def test_split_fp64_folded_r_update_equals_separate_kernels(ctx, mix, semantics):
+    """The FP64 PCG with r -= alpha Ap in the V-cycle's first half-sweep and
+    x, p updated together after it takes the same iterations to bitwise the
+    same head as the separate r/x and p kernels: every per-cell formula is
+    the same, only the (r, r) sum is grouped by march workgroup."""
+    g, _ = grid_pair(64, 48, 32)
+    bc, _ = bc_pair(mix)
+    n = g.num_cells()
+    K = dev(random_K(n, 37, 1.5), ctx)
+    cfg = (ph.HeadSolveConfig(mg_levels=4, cg_rtol=1e-10) if semantics == "reference"
+           else ph.HeadSolveConfig.fp64_direct(1e-10, 4))
+    out = []
+    for fold in (1, 0):
+        ws = ph.FlowWorkspace()
+        ws.set_tuning(ph.FlowWorkspace.TUNE_FOLD_R, fold)
+        h = ctx.zeros(n)
+        res = ph.solve_head(h, K, g, bc, cfg, ctx, ws)
+        assert res.converged, (mix, semantics, fold)
+        out.append((host(h), res.num_iterations, res.restarts))
+        ws.close()
+    assert out[0][1:] == out[1][1:], (out[0][1:], out[1][1:])
+    assert np.array_equal(out[0][0], out[1][0])
