@@ -144,18 +144,22 @@ __device__ __forceinline__ void cell_faces(const Dims& d, const TC* FX, const TC
 
 // GSRB cell value (sum f x_nb - b cs dx^2) / sum f, summed x+, x-, y+, y-,
 // z+, z- as in gsrb_interior_kernel (gsrb_3d.cu:53-69); pin doubles the
-// diagonal of cell 0 (gsrb_3d.cu:360-362).
+// diagonal of cell 0 (gsrb_3d.cu:360-362). The multiply-adds are written as
+// fma: the first product is rounded, each later one is fused into the running
+// sum, and b bs into the last step. Left to contraction, the compiler may
+// fuse either product of a pair depending on the code around a call, and two
+// kernels calling this would then differ in the last bit.
 template <typename T>
 __device__ __forceinline__ T gs_value(const T (&f)[6], const T (&v)[6], T b, T bs, bool pin0) {
     T result = v[0] * f[0];
-    result += v[1] * f[1];
-    result += v[2] * f[2];
-    result += v[3] * f[3];
-    result += v[4] * f[4];
-    result += v[5] * f[5];
+    result = fma(v[1], f[1], result);
+    result = fma(v[2], f[2], result);
+    result = fma(v[3], f[3], result);
+    result = fma(v[4], f[4], result);
+    result = fma(v[5], f[5], result);
     T aC = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
     if (pin0) aC *= T(2);
-    return (result - b * bs) / aC;
+    return fma(-b, bs, result) / aC;
 }
 
 // ---- one colour of GSRB on any level -----------------------------------------
@@ -409,28 +413,88 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     const long sz = (long)sp.h * d.ny;
     const bool xlo = q == 0, xhi = q == h4 - 1;  // thread holds i = 0 / i = nx - 1
 
-    // faces: lower ones at the own index, upper ones at the +x/+y/+z neighbours
-    float fxm[4], fym[4], fzm[4], fxp[4], fyp[4], fzp[4];
-    load4(F.lx + own, fxm);
-    load4(F.ly + own, fym);
-    load4(F.lz + own, fzm);
-    if (j < d.ny - 1) {
-        load4(F.ly + oth + sp.h, fyp);
-    } else {
+    // MODE 3 with the r update: the r and Ap loads of its unit-stride pass
+    // (below) are issued before the face loads, so that their latency
+    // overlaps; gfx9's in-order vmcnt would otherwise serialize each pair's
+    // loads behind the previous pair's stores. Indices past the end read the
+    // last pair (in bounds) and are not used.
+    double2 rpre[4], apre[4];
+    long epre[4];
+    int cpre[4];
+    if constexpr (MODE == 3 && RU) {
+        const long n = (long)d.nx * d.ny * d.nz;
+        const long w0 = 8L * (blk * (long)blockDim.x + (long)(threadIdx.x & ~63));
+        CellWalk cw(d, min(w0 + 2L * lane, n - 2));
 #pragma unroll
-        for (int e = 0; e < 4; ++e) fyp[e] = cvt(F.uy[2 * (4 * q + e) + p + (long)d.nx * k]);
+        for (int u = 0; u < 4; ++u) {
+            const long e = w0 + 128L * u + 2L * lane;
+            if (u > 0) cw.advance(d, 128);
+            const long ec = e < n ? e : n - 2;
+            epre[u] = e;
+            cpre[u] = (cw.j + cw.k) & 1;  // colour of cell 2m
+            rpre[u] = *reinterpret_cast<const double2*>(r64 + ec);
+            apre[u] = *reinterpret_cast<const double2*>(ru->Ap + ec);
+        }
     }
-    if (k < d.nz - 1) {
-        load4(F.lz + oth + sz, fzp);
-    } else {
+
+    // Every global load of the thread is issued before the first value is
+    // used: addresses are chosen without branches (a lane at a boundary reads
+    // an in-bounds stand-in, replaced or zeroed afterwards), so that a wave
+    // waits for its memory once instead of once per branch (gfx9's vmcnt
+    // counts in order).
+    constexpr bool XR = !(MODE == 1 || MODE == 3);  // reads the other colour
+    const bool ytop = j == d.ny - 1, ztop = k == d.nz - 1;
+    // faces: lower ones at the own index, upper ones at the +x/+y/+z neighbours
+    // (the last row / plane reads its own other-colour row and takes the
+    // boundary faces below)
+    const Pack4<TC> pxm = *reinterpret_cast<const Pack4<TC>*>(F.lx + own);
+    const Pack4<TC> pym = *reinterpret_cast<const Pack4<TC>*>(F.ly + own);
+    const Pack4<TC> pzm = *reinterpret_cast<const Pack4<TC>*>(F.lz + own);
+    const Pack4<TC> pyp = *reinterpret_cast<const Pack4<TC>*>(F.ly + oth + (ytop ? 0 : sp.h));
+    const Pack4<TC> pzp = *reinterpret_cast<const Pack4<TC>*>(F.lz + oth + (ztop ? 0 : sz));
+    const Pack4<TC> plo = *reinterpret_cast<const Pack4<TC>*>(F.lx + oth);  // other colour
+    const TC fnext = F.lx[oth + (xhi ? 0 : 4)];  // the next thread's first place (lane 63)
+    const TC fux = F.ux[row];                    // upper x boundary face of the row
+    // the other colour's values: own places, the places either side of the
+    // thread's 4 (x edges and periodic wraps) and the y/z neighbour rows (the
+    // wrapped row at a boundary: periodic, or zeroed below)
+    const float* O = x + (long)(colour ^ 1) * sp.half;
+    const long sy = sp.h;
+    float a[4], oyp[4], oym[4], ozp[4], ozm[4], aprv = 0.0f, anxt = 0.0f;
+    if (XR) {
+        load4(O + base, a);
+        aprv = O[xlo ? r0 + sp.h - 1 : base - 1];
+        anxt = O[xhi ? r0 : base + 4];
+        load4(O + base + (j < d.ny - 1 ? sy : -(long)(d.ny - 1) * sy), oyp);
+        load4(O + base + (j > 0 ? -sy : (long)(d.ny - 1) * sy), oym);
+        load4(O + base + (k < d.nz - 1 ? sz : -(long)(d.nz - 1) * sz), ozp);
+        load4(O + base + (k > 0 ? -sz : (long)(d.nz - 1) * sz), ozm);
+    }
+    float bpre[4];  // own b (MODE 3 takes it from r below)
+    if constexpr (MODE != 3) load4(b + own, bpre);
+    // RZ: the r of the thread's 8 natural cells for the (r, z) partial
+    double2 rzr[4];
+    if constexpr (RZ) {
+        const double2* rp = reinterpret_cast<const double2*>(rz->r + row * d.nx + 8L * q);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) fzp[e] = cvt(F.uz[2 * (4 * q + e) + p + (long)d.nx * j]);
+        for (int e = 0; e < 4; ++e) rzr[e] = rp[e];
+    }
+
+    float fxm[4], fym[4], fzm[4], fxp[4], fyp[4], fzp[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        fxm[e] = cvt(pxm.v[e]);
+        fym[e] = cvt(pym.v[e]);
+        fzm[e] = cvt(pzm.v[e]);
+        fyp[e] = cvt(pyp.v[e]);
+        fzp[e] = cvt(pzp.v[e]);
     }
     {
         float lo[4];
-        load4(F.lx + oth, lo);  // lower x faces of the other colour, same places
+#pragma unroll
+        for (int e = 0; e < 4; ++e) lo[e] = cvt(plo.v[e]);
         float nxt = __shfl_down(lo[0], 1, 64);
-        if (lane == 63 && !xhi) nxt = cvt(F.lx[oth + 4]);
+        if (lane == 63 && !xhi) nxt = cvt(fnext);
         if (p == 0) {  // i = 2m: the +x neighbour is other-colour place m
 #pragma unroll
             for (int e = 0; e < 4; ++e) fxp[e] = lo[e];
@@ -438,47 +502,52 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
             fxp[0] = lo[1];
             fxp[1] = lo[2];
             fxp[2] = lo[3];
-            fxp[3] = xhi ? cvt(F.ux[row]) : nxt;
+            fxp[3] = xhi ? cvt(fux) : nxt;
         }
+    }
+    if (ytop) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) fyp[e] = cvt(F.uy[2 * (4 * q + e) + p + (long)d.nx * k]);
+    }
+    if (ztop) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) fzp[e] = cvt(F.uz[2 * (4 * q + e) + p + (long)d.nx * j]);
     }
 
     float vxp[4], vxm[4], vyp[4], vym[4], vzp[4], vzm[4];
     float oc[4];  // the other colour at the thread's 4 places (MODE 0 and 2)
-    if (MODE == 1 || MODE == 3) {
+    if (!XR) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) vxp[e] = vxm[e] = vyp[e] = vym[e] = vzp[e] = vzm[e] = 0.0f;
     } else {
-        const float* O = x + (long)(colour ^ 1) * sp.half;
-        float a[4];
-        load4(O + base, a);
 #pragma unroll
         for (int e = 0; e < 4; ++e) oc[e] = a[e];
         float prv = __shfl_up(a[3], 1, 64);
         float nxt = __shfl_down(a[0], 1, 64);
-        if (lane == 0 && !xlo) prv = O[base - 1];
-        if (lane == 63 && !xhi) nxt = O[base + 4];
+        if (lane == 0 && !xlo) prv = aprv;   // O[base - 1]
+        if (lane == 63 && !xhi) nxt = anxt;  // O[base + 4]
         if (p == 0) {  // x+ at place m, x- at m - 1
             vxp[0] = a[0], vxp[1] = a[1], vxp[2] = a[2], vxp[3] = a[3];
             vxm[1] = a[0], vxm[2] = a[1], vxm[3] = a[2];
-            vxm[0] = xlo ? (bc.t[0] == MF3D_BC_PERIODIC ? O[r0 + sp.h - 1] : 0.0f) : prv;
+            // (at xlo, aprv is O[r0 + h - 1])
+            vxm[0] = xlo ? (bc.t[0] == MF3D_BC_PERIODIC ? aprv : 0.0f) : prv;
         } else {       // x- at place m, x+ at m + 1
             vxm[0] = a[0], vxm[1] = a[1], vxm[2] = a[2], vxm[3] = a[3];
             vxp[0] = a[1], vxp[1] = a[2], vxp[2] = a[3];
-            vxp[3] = xhi ? (bc.t[1] == MF3D_BC_PERIODIC ? O[r0] : 0.0f) : nxt;
+            // (at xhi, anxt is O[r0])
+            vxp[3] = xhi ? (bc.t[1] == MF3D_BC_PERIODIC ? anxt : 0.0f) : nxt;
         }
-        const long sy = sp.h;
-        if (j < d.ny - 1) load4(O + base + sy, vyp);
-        else if (bc.t[3] == MF3D_BC_PERIODIC) load4(O + base - (long)(d.ny - 1) * sy, vyp);
-        else vyp[0] = vyp[1] = vyp[2] = vyp[3] = 0.0f;
-        if (j > 0) load4(O + base - sy, vym);
-        else if (bc.t[2] == MF3D_BC_PERIODIC) load4(O + base + (long)(d.ny - 1) * sy, vym);
-        else vym[0] = vym[1] = vym[2] = vym[3] = 0.0f;
-        if (k < d.nz - 1) load4(O + base + sz, vzp);
-        else if (bc.t[5] == MF3D_BC_PERIODIC) load4(O + base - (long)(d.nz - 1) * sz, vzp);
-        else vzp[0] = vzp[1] = vzp[2] = vzp[3] = 0.0f;
-        if (k > 0) load4(O + base - sz, vzm);
-        else if (bc.t[4] == MF3D_BC_PERIODIC) load4(O + base + (long)(d.nz - 1) * sz, vzm);
-        else vzm[0] = vzm[1] = vzm[2] = vzm[3] = 0.0f;
+        const bool yp_in = j < d.ny - 1 || bc.t[3] == MF3D_BC_PERIODIC;
+        const bool ym_in = j > 0 || bc.t[2] == MF3D_BC_PERIODIC;
+        const bool zp_in = k < d.nz - 1 || bc.t[5] == MF3D_BC_PERIODIC;
+        const bool zm_in = k > 0 || bc.t[4] == MF3D_BC_PERIODIC;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            vyp[e] = yp_in ? oyp[e] : 0.0f;
+            vym[e] = ym_in ? oym[e] : 0.0f;
+            vzp[e] = zp_in ? ozp[e] : 0.0f;
+            vzm[e] = zm_in ? ozm[e] : 0.0f;
+        }
     }
     float bb[4];
     if (MODE == 3 && RU) {
@@ -491,21 +560,18 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
         // (pairs 4 lane .. 4 lane + 3) as one float4.
         __shared__ float bown[kMaxSweepBlock * 4];
         const long n = (long)d.nx * d.ny * d.nz;
-        const long w0 = 8L * (blk * (long)blockDim.x + (long)(threadIdx.x & ~63));
         const bool go = !ru->st->stop;  // uniform across the grid
         const double alpha = ru->st->alpha;
         float* bw = const_cast<float*>(b);
         double acc = 0.0;
-        CellWalk cw(d, min(w0 + 2L * lane, n - 2));
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const long e = w0 + 128L * u + 2L * lane;
-            if (u > 0) cw.advance(d, 128);
+            const long e = epre[u];
             if (e < n) {
-                const int c0 = (cw.j + cw.k) & 1;  // colour of cell 2m
+                const int c0 = cpre[u];
                 const long pl = e >> 1;
-                double2 rv = *reinterpret_cast<const double2*>(r64 + e);
-                const double2 av = *reinterpret_cast<const double2*>(ru->Ap + e);
+                double2 rv = rpre[u];
+                const double2 av = apre[u];
                 if (go) {
                     rv.x = -alpha * av.x + rv.x;
                     rv.y = -alpha * av.y + rv.y;
@@ -546,7 +612,8 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
             *reinterpret_cast<float4*>(bw + oth) = make_float4(bo[0], bo[1], bo[2], bo[3]);
         }
     } else {
-        load4(b + own, bb);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bb[e] = bpre[e];
     }
     if (MODE == 2) {
         // prolongation: each in-domain neighbour gains its coarse parent's value.
@@ -626,11 +693,10 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     }
     if (valid) *reinterpret_cast<float4*>(x + own) = make_float4(res[0], res[1], res[2], res[3]);
     if (RZ) {  // own cells at natural 2e + p, the other colour at 2e + 1 - p
-        const double2* rp = reinterpret_cast<const double2*>(rz->r + row * d.nx + 8L * q);
         double acc = 0.0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const double2 v2 = rp[e];
+            const double2 v2 = rzr[e];
             const double ro = p ? v2.y : v2.x, rt = p ? v2.x : v2.y;
             acc += ro * (double)res[e];
             acc += rt * (double)oc[e];
@@ -781,14 +847,14 @@ __device__ __forceinline__ T residual_cell(const Dims& d, const BcTypes& bc, boo
     cell_faces(d, FX, FY, FZ, i, j, k, f);
     neighbours(d, bc, x, o, i, j, k, v);
     const T xc = x[o];
-    T result = f[0] * (xc - v[0]);
-    result += f[1] * (xc - v[1]);
-    result += f[2] * (xc - v[2]);
-    result += f[3] * (xc - v[3]);
-    result += f[4] * (xc - v[4]);
-    result += f[5] * (xc - v[5]);
-    if (pin && o == 0) result += (f[0] + f[1] + f[2] + f[3] + f[4] + f[5]) * xc;
-    return b[o] + result * rs;
+    T result = f[0] * (xc - v[0]);  // fma as in gs_value
+    result = fma(f[1], xc - v[1], result);
+    result = fma(f[2], xc - v[2], result);
+    result = fma(f[3], xc - v[3], result);
+    result = fma(f[4], xc - v[4], result);
+    result = fma(f[5], xc - v[5], result);
+    if (pin && o == 0) result = fma(f[0] + f[1] + f[2] + f[3] + f[4] + f[5], xc, result);
+    return fma(result, rs, b[o]);
 }
 
 template <typename T, typename TC>
@@ -844,14 +910,14 @@ __device__ __forceinline__ float residual_cell_split(const Dims& d, const BcType
     cell_faces_split(d, sp, F, i, j, k, f);
     neighbours_split(d, bc, x + (long)(c ^ 1) * sp.half, r0, sp.h, i, j, k, v);
     const float xc = x[o];
-    float result = f[0] * (xc - v[0]);
-    result += f[1] * (xc - v[1]);
-    result += f[2] * (xc - v[2]);
-    result += f[3] * (xc - v[3]);
-    result += f[4] * (xc - v[4]);
-    result += f[5] * (xc - v[5]);
-    if (pin && (i | j | k) == 0) result += (f[0] + f[1] + f[2] + f[3] + f[4] + f[5]) * xc;
-    return b[o] + result * rs;
+    float result = f[0] * (xc - v[0]);  // fma as in gs_value
+    result = fma(f[1], xc - v[1], result);
+    result = fma(f[2], xc - v[2], result);
+    result = fma(f[3], xc - v[3], result);
+    result = fma(f[4], xc - v[4], result);
+    result = fma(f[5], xc - v[5], result);
+    if (pin && (i | j | k) == 0) result = fma(f[0] + f[1] + f[2] + f[3] + f[4] + f[5], xc, result);
+    return fma(result, rs, b[o]);
 }
 
 // residual + restriction of a colour-split fine level into a coarse level of
