@@ -413,6 +413,10 @@ def main():
     else:
         res, ms = rlin, mslin
         step(cfglin, h, vel, ws, False)
+    # iterations the last timed solve enqueued (one preconditioner application
+    # each, plus one per restart pass): those past convergence cost a V-cycle
+    _, n_pc = ws.solve_counts()
+    enqueued = n_pc - res.restarts - 1
 
     def rel(a, b):
         return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
@@ -475,6 +479,7 @@ def main():
                            "the timed step)" if net is not None else "linear head"),
         },
         "iterations": res.num_iterations,
+        "iterations_enqueued": enqueued,
         "mixed_linear_guess_ms": round(mslin, 3),
         "mixed_linear_guess_iterations": rlin.num_iterations,
         "rel_true_residual": res.true_residual / res.bnorm,

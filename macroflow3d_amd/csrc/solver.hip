@@ -950,8 +950,12 @@ struct Chunker {
         if (have_prev && st.iter > prev_iter) {
             const double rate = (cur - prev_log) / (st.iter - prev_iter);
             if (rate < -1e-3) {
+                // iterations enqueued past convergence still run their
+                // V-cycle (a whole iteration), while stopping short costs one
+                // more host check; PCG tends to speed up as it goes, so the
+                // extrapolated count is scaled down
                 const double left = (std::log(target) - cur) / rate;
-                c = (int)std::ceil(left);
+                c = (int)std::ceil(0.8 * left);
             }
         }
         prev_log = cur;
