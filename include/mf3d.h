@@ -255,12 +255,12 @@ enum {
 int mf3d_workspace_set_tuning(mf3d_workspace* ws, int key, long value);
 
 /* Extension, no counterpart in MacroFlow3D: the finest-level FP64 operator
- * applications (A p and residuals) and the preconditioner applications that
- * the last mf3d_solve_head on ws enqueued, replays of a captured PCG iteration
- * included, iterations enqueued past convergence (which return at once on the
- * device) too. A check on the kernel sequence for tests and tuning: a PCG
- * solve applies the operator once per preconditioner application, plus the
- * initial and true residuals. */
+ * applications (A p and residuals) and the preconditioner applications
+ * enqueued on ws since the last mf3d_solve_head began (it resets both; later
+ * mf3d_mg_precondition calls on ws add to them), replays of a captured PCG
+ * iteration included, iterations enqueued past convergence too. A check on
+ * the kernel sequence for tests and tuning: a PCG solve applies the operator
+ * once per preconditioner application, plus the initial and true residuals. */
 int mf3d_workspace_solve_counts(const mf3d_workspace* ws, long* op_applications,
                                 long* precond_applications);
 

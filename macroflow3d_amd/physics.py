@@ -200,8 +200,9 @@ class FlowWorkspace:
         _lib.call("mf3d_workspace_set_tuning", self._h, int(key), int(value))
 
     def solve_counts(self) -> tuple[int, int]:
-        """(operator applications, preconditioner applications) that the last
-        single solve on this workspace enqueued, graph replays included."""
+        """(operator applications, preconditioner applications) enqueued on
+        this workspace since the last single solve began, graph replays
+        included."""
         op, pc = C.c_long(), C.c_long()
         _lib.call("mf3d_workspace_solve_counts", self._h, C.byref(op), C.byref(pc))
         return op.value, pc.value
