@@ -77,9 +77,10 @@ inline bool pin_enabled(int mode, const mf3d_bc& bc) {
 // dir: 0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+. Returns the neighbour's linear
 // index (or -1 for a Neumann/Dirichlet face) and its coefficient.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline long neighbour_coef(int nx, int ny, int nz, const BcTypes& bc,
-                                               const double* K, int i, int j, int k, int dir,
-                                               double Kc, double& coef) {
+// neighbour of cell (i, j, k) across face dir, or -1 where the face is a
+// non-periodic boundary
+__host__ __device__ inline long neighbour_index(int nx, int ny, int nz, const BcTypes& bc, int i,
+                                                int j, int k, int dir) {
     int ni = i, nj = j, nk = k;
     bool at_bnd;
     switch (dir) {
@@ -90,11 +91,18 @@ __host__ __device__ inline long neighbour_coef(int nx, int ny, int nz, const BcT
     case 4: at_bnd = (k == 0); nk = at_bnd ? nz - 1 : k - 1; break;
     default: at_bnd = (k == nz - 1); nk = at_bnd ? 0 : k + 1; break;
     }
-    if (at_bnd && bc.t[dir] != MF3D_BC_PERIODIC) {
+    if (at_bnd && bc.t[dir] != MF3D_BC_PERIODIC) return -1;
+    return (long)ni + (long)nx * ((long)nj + (long)ny * (long)nk);
+}
+
+__host__ __device__ inline long neighbour_coef(int nx, int ny, int nz, const BcTypes& bc,
+                                               const double* K, int i, int j, int k, int dir,
+                                               double Kc, double& coef) {
+    const long n = neighbour_index(nx, ny, nz, bc, i, j, k, dir);
+    if (n < 0) {
         coef = (bc.t[dir] == MF3D_BC_DIRICHLET) ? 2.0 * Kc : 0.0;
         return -1;
     }
-    const long n = (long)ni + (long)nx * ((long)nj + (long)ny * (long)nk);
     coef = 2.0 / (1.0 / Kc + 1.0 / K[n]);
     return n;
 }
@@ -221,7 +229,21 @@ struct BatchVec {  // FP64 finest level and Krylov vectors of one realisation
 struct BandDesc {  // banded Cholesky factor of one coarsest operator and its inverse
     int n, w;
     const double* Lb;
-    double* Ainv;
+    double* Y;     // n * n: row c = column c of L^{-1}
+    double* Ainv;  // n * n
+};
+
+// One coarsest operator M = -A to assemble from K and factor on the device
+// (natural order, band width w)
+struct FactorDesc {
+    int nx, ny, nz, w;
+    double dx2;
+    BcTypes bc;
+    int pin;
+    const double* K;
+    double* C;   // n * (w + 1) scratch: M in column storage, C[j*(w+1) + (i-j)] = M[i][j]
+    double* Lb;  // n * (w + 1) factor, row-major band: Lb[i*(w+1) + (i-j)] = L[i][j]
+    int* flag;   // written last: 0, or j + 1 for the first pivot j that is not positive
 };
 
 __device__ __forceinline__ int batch_entry(const int* __restrict__ start, int m, long blk) {

@@ -8,13 +8,13 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <cstdlib>
 #include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
-#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -108,8 +108,10 @@ struct mf3d_workspace {
     double* coarse_inv = nullptr;  // n*n inverse followed by the n*(w+1) band factor
     int coarse_n = 0;
     size_t coarse_cap = 0;         // elements allocated at coarse_inv
-    // the coarsest K travels to the host when the set-up is enqueued; the
-    // factorization waits for it only before the first coarsest solve
+    // single solves: the coarsest K travels to the host when the set-up is
+    // enqueued, a host thread factors it, and the inverse is built on
+    // coarse_stream; the solve's stream waits for it (inv_ev) only before the
+    // first coarsest solve
     bool coarse_pending = false;
     double* host_kc = nullptr;     // pinned coarsest K
     double* host_band = nullptr;   // pinned band factor (source of the upload)
@@ -117,15 +119,23 @@ struct mf3d_workspace {
     hipEvent_t kc_ev = nullptr;
     hipEvent_t inv_ev = nullptr;   // the inverse is complete (recorded on coarse_stream)
     hipStream_t coarse_stream = nullptr;
+    std::thread coarse_worker;     // factors as soon as the coarsest K has arrived
+    std::vector<double> coarse_band_h;
+    int coarse_status = 0;
+    std::string coarse_msg;
+    // batched solves factor on the device: pinned host int the factor kernel
+    // writes (0: definite, j + 1: first pivot j that is not positive)
+    int* coarse_flag = nullptr;
     // batched solves led by this workspace: their stream and phase events
     hipStream_t batch_stream = nullptr;
     hipEvent_t batch_ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::thread coarse_worker;     // factorizes as soon as the coarsest K has arrived
-    std::vector<double> coarse_band_h;
-    int coarse_w = 0, coarse_status = 0;
-    std::string coarse_msg;
     void join_coarse() {
         if (coarse_worker.joinable()) coarse_worker.join();
+    }
+    // the coarse stream's work reads and writes workspace memory
+    void drain_coarse() {
+        join_coarse();
+        if (coarse_stream) (void)hipStreamSynchronize(coarse_stream);
     }
     // Krylov vectors (PCGWorkspace, src/numerics/solvers/pcg.cuh:45-62) + rhs
     size_t nvec = 0;
@@ -212,7 +222,7 @@ struct mf3d_workspace {
         return MF3D_OK;
     }
     void release() {
-        join_coarse();
+        drain_coarse();
         for (void* q : owned) (void)hipFree(q);
         owned.clear();
         bytes = 0;
@@ -236,12 +246,12 @@ struct mf3d_workspace {
         fine = mf3d_grid{};
     }
     ~mf3d_workspace() {
-        join_coarse();
         release();
         if (host_st) (void)hipHostFree(host_st);
         if (host_scalar) (void)hipHostFree(host_scalar);
         if (host_kc) (void)hipHostFree(host_kc);
         if (host_band) (void)hipHostFree(host_band);
+        if (coarse_flag) (void)hipHostFree(coarse_flag);
         if (kc_ev) (void)hipEventDestroy(kc_ev);
         if (inv_ev) (void)hipEventDestroy(inv_ev);
         if (coarse_stream) (void)hipStreamDestroy(coarse_stream);
@@ -355,10 +365,35 @@ int host_scalar_sync(mf3d_workspace* ws, hipStream_t s, double* out) {
 }
 
 // ---------------------------------------------------------------------------
-// Dense coarsest solve: assemble M = -A of the coarsest level (smoother /
-// residual operator, pin included), banded Cholesky on the host, explicit
-// inverse on the device (band_inverse_kernel).
+// Dense coarsest solve: M = -A of the coarsest level (smoother / residual
+// operator, pin included), its banded Cholesky factor, and the explicit
+// inverse -L^{-T} L^{-1} on the device (band_lower_kernel, syrk_neg_kernel);
+// the V-cycle's coarsest solve is then one GEMV. A single solve factors on a
+// host thread while its set-up runs on the GPU; a batched solve factors every
+// realisation on the device (band_factor_kernel, one workgroup each), with no
+// host round trip.
 // ---------------------------------------------------------------------------
+const double* coarsest_K(const mf3d_workspace* ws) {
+    return ws->lv.size() == 1 ? ws->K0 : ws->lv.back().K;
+}
+
+// band width of the coarsest operator in natural order: the largest index
+// distance between neighbouring cells (periodic faces included)
+int coarse_bandwidth(const mf3d_workspace* ws) {
+    const mf3d_grid& g = ws->lv.back().g;
+    int w = 0;
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+                const int c = i + g.nx * (j + g.ny * k);
+                for (int d = 0; d < 6; ++d) {
+                    const long nb = neighbour_index(g.nx, g.ny, g.nz, ws->bct, i, j, k, d);
+                    if (nb >= 0) w = std::max(w, std::abs((int)nb - c));
+                }
+            }
+    return w;
+}
+
 // Banded Cholesky factor of M = -A (coarsest level) from its host K: returns
 // the band width w and fills band (row-major, band[i*(w+1) + (i-j)] = L[i][j]).
 // The factorization runs right-looking on column storage (C[j*(w+1) + (i-j)] =
@@ -371,19 +406,7 @@ int coarse_band(const mf3d_workspace* ws, const double* Kc, std::vector<double>&
     const int n = (int)L.n;
     const mf3d_grid& g = L.g;
     const double dx2 = g.dx * g.dx;
-    // band width from the neighbour offsets actually present
-    int w = 0;
-    for (int k = 0; k < g.nz; ++k)
-        for (int j = 0; j < g.ny; ++j)
-            for (int i = 0; i < g.nx; ++i) {
-                const int c = i + g.nx * (j + g.ny * k);
-                for (int d = 0; d < 6; ++d) {
-                    double coef;
-                    const long nb = neighbour_coef(g.nx, g.ny, g.nz, ws->bct, Kc, i, j, k, d, Kc[c],
-                                                   coef);
-                    if (nb >= 0) w = std::max(w, std::abs((int)nb - c));
-                }
-            }
+    const int w = coarse_bandwidth(ws);
     const int W1 = w + 1;
     std::vector<double> C((size_t)n * W1, 0.0);
     for (int k = 0; k < g.nz; ++k)
@@ -412,8 +435,9 @@ int coarse_band(const mf3d_workspace* ws, const double* Kc, std::vector<double>&
     for (int j = 0; j < n; ++j) {
         double* cj = cm + (size_t)j * W1;
         if (!(cj[0] > 0.0))
-            return fail(MF3D_ERR_SINGULAR, "coarsest operator is not definite (pure Neumann/"
-                                           "periodic problem without pin?)");
+            return fail(MF3D_ERR_SINGULAR, "coarsest operator is not definite at pivot " +
+                                               std::to_string(j) +
+                                               " (pure Neumann/periodic problem without pin?)");
         const double Ljj = std::sqrt(cj[0]);
         cj[0] = Ljj;
         const int m = std::min(w, n - 1 - j);
@@ -433,13 +457,10 @@ int coarse_band(const mf3d_workspace* ws, const double* Kc, std::vector<double>&
     return MF3D_OK;
 }
 
-const double* coarsest_K(const mf3d_workspace* ws) {
-    return ws->lv.size() == 1 ? ws->K0 : ws->lv.back().K;
-}
-
-// room for the n*n inverse followed by the n*(w+1) band
+// the n*n inverse, the n*(w+1) band factor and n*n for L^{-1}, whose room
+// holds the assembled operator while a device factor is computed (w < n)
 int reserve_coarse(mf3d_workspace* ws, int n, int w) {
-    const size_t need = (size_t)n * n + (size_t)n * (w + 1);
+    const size_t need = 2 * (size_t)n * n + (size_t)n * (w + 1);
     if (!ws->coarse_inv || ws->coarse_cap < need) {
         MF3D_TRY(ws->alloc(&ws->coarse_inv, need));
         ws->coarse_cap = need;
@@ -448,11 +469,11 @@ int reserve_coarse(mf3d_workspace* ws, int n, int w) {
     return MF3D_OK;
 }
 
-// Dense coarsest inverse: coarsest K to the host, banded Cholesky there, the
-// columns of the inverse on the GPU (band_inverse_kernel). start_coarse_inverse
-// only enqueues the download (after the coarsening kernels); finish_coarse_inverse
-// runs before the first coarsest solve, so the host factorization overlaps the
-// rest of the set-up on the GPU.
+double* coarse_band_dev(mf3d_workspace* ws, int n) { return ws->coarse_inv + (size_t)n * n; }
+double* coarse_lower(mf3d_workspace* ws, int n, int w) {
+    return ws->coarse_inv + (size_t)n * n + (size_t)n * (w + 1);
+}
+
 int grow_pinned(double** p, size_t* cap, size_t need) {
     if (*p && *cap >= need) return MF3D_OK;
     if (*p) MF3D_HIP_TRY(hipHostFree(*p));
@@ -463,29 +484,13 @@ int grow_pinned(double** p, size_t* cap, size_t need) {
     return MF3D_OK;
 }
 
-int coarse_bandwidth(const mf3d_workspace* ws) {
-    const mf3d_grid& g = ws->lv.back().g;
-    std::vector<double> ones((size_t)g.nx * g.ny * g.nz, 1.0);
-    int w = 0;
-    for (int k = 0; k < g.nz; ++k)
-        for (int j = 0; j < g.ny; ++j)
-            for (int i = 0; i < g.nx; ++i) {
-                const int c = i + g.nx * (j + g.ny * k);
-                for (int d = 0; d < 6; ++d) {
-                    double coef;
-                    const long nb = neighbour_coef(g.nx, g.ny, g.nz, ws->bct, ones.data(), i, j, k,
-                                                   d, 1.0, coef);
-                    if (nb >= 0) w = std::max(w, std::abs((int)nb - c));
-                }
-            }
-    return w;
-}
-
-// The worker thread waits for the download, factorizes, uploads the factor
-// and launches the inverse on the workspace's second stream; the solve's
-// stream waits for that (finish_coarse_inverse) only before its first
-// coarsest solve, so the factorization and the inverse overlap the set-up
-// and the first descent of the V-cycle. Allocation stays on this thread.
+// Single solve: start_coarse_inverse enqueues the download of the coarsest K
+// (after the coarsening kernels) and starts the worker thread, which waits for
+// the download, factors, uploads the factor and launches the inverse on the
+// workspace's second stream; the solve's stream waits for that
+// (finish_coarse_inverse) only before its first coarsest solve, so the
+// factorization and the inverse overlap the set-up and the first descent of
+// the V-cycle. Allocation stays on this thread.
 int start_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
     const int n = (int)ws->lv.back().n;
     if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
@@ -518,11 +523,12 @@ int start_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
         if (wb != w) return failed(MF3D_ERR_HIP, "coarsest band width changed");
         std::memcpy(ws->host_band, ws->coarse_band_h.data(),
                     ws->coarse_band_h.size() * sizeof(double));
-        double* dband = ws->coarse_inv + (size_t)n * n;
+        double* dband = coarse_band_dev(ws, n);
         if (hipMemcpyAsync(dband, ws->host_band, ws->coarse_band_h.size() * sizeof(double),
                            hipMemcpyHostToDevice, ws->coarse_stream) != hipSuccess)
             return failed(MF3D_ERR_HIP, "band factor upload failed");
-        const int bs = kern::band_inverse(ws->coarse_stream, n, w, dband, ws->coarse_inv);
+        const int bs = kern::band_inverse(ws->coarse_stream, n, w, dband, coarse_lower(ws, n, w),
+                                          ws->coarse_inv);
         if (bs != MF3D_OK) return failed(bs, last_error_cstr());
         if (hipEventRecord(ws->inv_ev, ws->coarse_stream) != hipSuccess)
             return failed(MF3D_ERR_HIP, "event record failed");
@@ -536,6 +542,29 @@ int finish_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
     ws->join_coarse();
     if (ws->coarse_status != MF3D_OK) return fail(ws->coarse_status, ws->coarse_msg);
     MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->inv_ev, 0));
+    return MF3D_OK;
+}
+
+// a pending inverse that no solve will use: s waits for the coarse stream's
+// writes to the workspace before it reuses the memory
+int discard_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
+    if (!ws->coarse_pending) return MF3D_OK;
+    ws->coarse_pending = false;
+    ws->join_coarse();
+    if (ws->inv_ev) MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->inv_ev, 0));
+    return MF3D_OK;
+}
+
+// Batched solves: the factor descriptor of ws's coarsest operator (after
+// reserve_coarse)
+int coarse_desc(mf3d_workspace* ws, int w, FactorDesc* D) {
+    if (!ws->coarse_flag)
+        MF3D_HIP_TRY(hipHostMalloc((void**)&ws->coarse_flag, sizeof(int), hipHostMallocCoherent));
+    const Level& L = ws->lv.back();
+    const int n = (int)L.n;
+    *D = FactorDesc{L.g.nx, L.g.ny, L.g.nz, w, L.g.dx * L.g.dx, ws->bct, ws->pin ? 1 : 0,
+                    coarsest_K(ws), coarse_lower(ws, n, w), coarse_band_dev(ws, n),
+                    ws->coarse_flag};
     return MF3D_OK;
 }
 
@@ -579,10 +608,11 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     }
     for (size_t l = 1; l < ws->lv.size(); ++l)
         MF3D_TRY(kern::coarsen(s, ws->lv[l].g, level_K(ws, l - 1), ws->lv[l].K));
-    if (cfg.coarse_solver == MF3D_COARSE_DIRECT && !defer_coarse)
+    if (cfg.coarse_solver == MF3D_COARSE_DIRECT && !defer_coarse) {
         MF3D_TRY(start_coarse_inverse(ws, s));
-    else
-        ws->coarse_pending = false;
+    } else {
+        MF3D_TRY(discard_coarse_inverse(ws, s));
+    }
     if (!ws->h0) MF3D_TRY(ws->alloc(&ws->h0, cells(g)));
     // FP64 V-cycle: colour-split levels (x, b, h = 1 / (2 K) in split order,
     // no stored faces) on a prefix of colour-swept levels, the rest natural
@@ -648,80 +678,40 @@ int mg_setup(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g, const mf3d_b
     return MF3D_OK;
 }
 
-// build_coarse_inverse for many workspaces with one host round trip and one
-// band-inverse launch; the pinned host staging (keep_host) and the device
-// descriptors (keep_dev) must live until s completes
+// The coarsest factors and inverses of many workspaces, two launches on s
+// with no host round trip; the pinned host descriptors (keep_host) and their
+// device copy (keep_dev) must live until s completes
 int build_coarse_inverses(mf3d_workspace* const* ws, int count, hipStream_t s,
                           std::vector<void*>& keep_host, std::vector<void*>& keep_dev) {
-    std::vector<size_t> off(count + 1, 0);
-    for (int r = 0; r < count; ++r) {
-        const size_t n = ws[r]->lv.back().n;
-        if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
-        off[r + 1] = off[r] + n;
-    }
-    double* kh = nullptr;
-    MF3D_HIP_TRY(hipHostMalloc((void**)&kh, off[count] * sizeof(double), hipHostMallocDefault));
-    keep_host.push_back(kh);
-    for (int r = 0; r < count; ++r)
-        MF3D_HIP_TRY(hipMemcpyAsync(kh + off[r], coarsest_K(ws[r]),
-                                    (off[r + 1] - off[r]) * sizeof(double),
-                                    hipMemcpyDeviceToHost, s));
-    MF3D_HIP_TRY(hipStreamSynchronize(s));
-    const auto t_sync = std::chrono::steady_clock::now();
-    std::vector<std::vector<double>> bands(count);
-    std::vector<int> wv(count);
-    std::vector<int> fst(count, MF3D_OK);
-    // independent factorizations: spread over host threads
-    const int nth = std::max(1, std::min(count, (int)std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nth; ++t)
-        pool.emplace_back([&, t] {
-            for (int r = t; r < count; r += nth)
-                fst[r] = coarse_band(ws[r], kh + off[r], bands[r], &wv[r]);
-        });
-    for (auto& th : pool) th.join();
-    size_t tot = 0;
-    for (int r = 0; r < count; ++r) {
-        if (fst[r] != MF3D_OK)  // the worker's message stayed on its thread
-            return fail(fst[r], "coarsest factorization of realisation " + std::to_string(r) +
-                                   " failed (operator not definite?)");
-        tot += bands[r].size();
-    }
-    if (std::getenv("MF3D_BATCH_TIMING"))
-        std::fprintf(stderr, "[batch] coarse factorizations %.2f ms on %d threads\n",
-                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
-                                                               t_sync).count(),
-                     nth);
-    double* bh = nullptr;
-    MF3D_HIP_TRY(hipHostMalloc((void**)&bh, tot * sizeof(double) + sizeof(BandDesc) * count +
-                                                sizeof(int) * (count + 1),
-                               hipHostMallocDefault));
-    keep_host.push_back(bh);
-    BandDesc* dh = reinterpret_cast<BandDesc*>(bh + tot);
+    const size_t bytes = sizeof(FactorDesc) * count + sizeof(BandDesc) * count +
+                         sizeof(int) * (count + 1);
+    void* hbuf = nullptr;
+    MF3D_HIP_TRY(hipHostMalloc(&hbuf, bytes, hipHostMallocDefault));
+    keep_host.push_back(hbuf);
+    FactorDesc* fh = static_cast<FactorDesc*>(hbuf);
+    BandDesc* dh = reinterpret_cast<BandDesc*>(fh + count);
     int* sh = reinterpret_cast<int*>(dh + count);
-    size_t bo = 0;
     int nmax = 0, wmax = 0;
     sh[0] = 0;
     for (int r = 0; r < count; ++r) {
         const int n = (int)ws[r]->lv.back().n;
-        wmax = std::max(wmax, wv[r]);
-        MF3D_TRY(reserve_coarse(ws[r], n, wv[r]));
-        double* dband = ws[r]->coarse_inv + (size_t)n * n;
-        std::memcpy(bh + bo, bands[r].data(), bands[r].size() * sizeof(double));
-        MF3D_HIP_TRY(hipMemcpyAsync(dband, bh + bo, bands[r].size() * sizeof(double),
-                                    hipMemcpyHostToDevice, s));
-        bo += bands[r].size();
-        dh[r] = BandDesc{n, wv[r], dband, ws[r]->coarse_inv};
+        if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
+        const int w = coarse_bandwidth(ws[r]);
+        MF3D_TRY(reserve_coarse(ws[r], n, w));
+        MF3D_TRY(coarse_desc(ws[r], w, &fh[r]));
+        dh[r] = BandDesc{n, w, fh[r].Lb, fh[r].C, ws[r]->coarse_inv};
         sh[r + 1] = sh[r] + n;
         nmax = std::max(nmax, n);
+        wmax = std::max(wmax, w);
     }
-    void* dd = nullptr;
-    MF3D_HIP_TRY(hipMalloc(&dd, sizeof(BandDesc) * count + sizeof(int) * (count + 1)));
-    keep_dev.push_back(dd);
-    MF3D_HIP_TRY(hipMemcpyAsync(dd, dh, sizeof(BandDesc) * count + sizeof(int) * (count + 1),
-                                hipMemcpyHostToDevice, s));
-    const BandDesc* D = static_cast<const BandDesc*>(dd);
+    void* dbuf = nullptr;
+    MF3D_HIP_TRY(hipMalloc(&dbuf, bytes));
+    keep_dev.push_back(dbuf);
+    MF3D_HIP_TRY(hipMemcpyAsync(dbuf, hbuf, bytes, hipMemcpyHostToDevice, s));
+    const FactorDesc* F = static_cast<const FactorDesc*>(dbuf);
+    const BandDesc* D = reinterpret_cast<const BandDesc*>(F + count);
     const int* st = reinterpret_cast<const int*>(D + count);
+    MF3D_TRY(kern::band_factor_batch(s, F, count, wmax));
     return kern::band_inverse_batch(s, D, st, count, sh[count], nmax, wmax);
 }
 
@@ -1837,6 +1827,11 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
                                         hipMemcpyDeviceToHost, s));
     }
     MF3D_HIP_TRY(hipStreamSynchronize(s));
+    for (int r = 0; r < count; ++r)
+        if (const int bad = *ws[r]->coarse_flag)
+            return fail(MF3D_ERR_SINGULAR, "realisation " + std::to_string(r) +
+                                               ": coarsest operator is not definite at pivot " +
+                                               std::to_string(bad - 1));
     const float setup = elapsed(ev[0], ev[1]), solve = elapsed(ev[1], ev[2]);
     int status = MF3D_OK;
     for (int r = 0; r < count; ++r) {

@@ -431,23 +431,23 @@ __global__ void __launch_bounds__(256)
     if (lane == 0) x[row] = s;
 }
 
-// Columns of A^{-1} from a banded Cholesky factor of M = -A = L L^T
-// (row-major band storage, Lb[i*(w+1) + (i-j)] = L[i][j] for i-w <= j <= i).
-// One wave per column c: solves L y = e_c then L^T z = y and writes row c of
-// -M^{-1}, which is column c of the symmetric A^{-1}; y lives in LDS.
+// Inverse of the coarsest operator from its banded Cholesky factor M = -A =
+// L L^T (row-major band storage, Lb[i*(w+1) + (i-j)] = L[i][j] for
+// i-w <= j <= i), as LAPACK's potri does it: the columns of L^{-1}, then
+// A^{-1} = -M^{-1} = -L^{-T} L^{-1} as one symmetric product.
 //
-// band_inverse_slab (w <= 64): the substitutions go in blocks of 32 rows, one
+// band_lower_inverse: one wave per column c solves L y = e_c and writes
+// Y[c][i] = y[i] = L^{-1}[i][c] (zero for i < c); y lives in LDS.
+// band_lower_slab (w <= 64): the substitution goes in blocks of 32 rows, one
 // row per lane. The band rows of a block are first copied to LDS with
 // coalesced loads (transposed: S[d * 32 + r] = L[R0 + r][R0 + r - d]); a lane
 // then gathers its row's terms from the blocks already solved, and the block
 // is solved row by row, each new value broadcast from its lane with
 // v_readlane and folded into the later rows' sums, so a sequential step is a
-// multiply, a broadcast and a fused multiply-add. The backward pass keeps the
-// previous two blocks' rows too: they hold the terms from the rows already
-// solved.
-// LDS: n + 3 * kSlab doubles.
-// band_inverse_row (any w): one row per step, the band dot product spread
-// over the lanes and reduced across the wave. LDS: n doubles.
+// multiply, a broadcast and a fused multiply-add. LDS: n + kSlab doubles.
+// band_lower_row (any w): one row per step, the band dot product spread over
+// the lanes and reduced across the wave. LDS: n doubles.
+// syrk_neg_kernel: A^{-1}[i][j] = -sum_{k >= max(i, j)} Y[i][k] Y[j][k].
 __device__ __forceinline__ double readlane_f64(double v, int l) {
     const long long u = __builtin_bit_cast(long long, v);
     const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffLL), l);
@@ -481,10 +481,9 @@ __device__ __forceinline__ void stage_band(const double* __restrict__ Lb, int W1
     if (W1 > 64 && lane < kSlabRows) S[64 * kSlabRows + lane] = v64;
 }
 
-__device__ void band_inverse_slab(int n, int w, const double* __restrict__ Lb,
-                                  double* __restrict__ Ainv, int c, double* __restrict__ y,
-                                  double* __restrict__ S0, double* __restrict__ S1,
-                                  double* __restrict__ S2) {
+__device__ void band_lower_slab(int n, int w, const double* __restrict__ Lb,
+                                double* __restrict__ Y, int c, double* __restrict__ y,
+                                double* __restrict__ S0) {
     constexpr int B = kSlabRows;
     const int lane = threadIdx.x;
     const bool part = lane < B;  // lanes B..63 only help with the copies
@@ -524,59 +523,13 @@ __device__ void band_inverse_slab(int n, int w, const double* __restrict__ Lb,
         }
         if (act) y[i] = yi;
     }
-    // backward: L^T z = y (y = 0 before c); lane l of a block holds row i1 - 1 - l.
-    // cur: rows i1 - B .. i1 - 1 (row i1 - 1 - l at r = B - 1 - l); p1, p2: the
-    // two blocks staged before (rows i1 .. i1 + B - 1 and i1 + B .. i1 + 2B - 1)
-    double* cur = S0;
-    double* p1 = S1;
-    double* p2 = S2;
-    for (int i1 = n; i1 > 0; i1 -= B) {
-        const int i = i1 - 1 - lane;
-        const bool act = part && i >= 0;
-        __syncthreads();
-        stage_band(Lb, W1, n, i1 - B, cur);
-        __syncthreads();
-        const double rhs = act && i >= c ? y[i] : 0.0;
-        // terms j = i1 + t (d = j - i = t + 1 + lane <= w) from the rows already solved
-        const int tmax = act ? min(w - 1 - lane, n - 1 - i1) : -1;
-        double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int t0 = 0; t0 < w; t0 += 8) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int t = t0 + k;
-                const bool ok = t <= tmax;
-                const int d = ok ? t + 1 + lane : 0;
-                const double L = t < B ? p1[d * B + (t & (B - 1))] : p2[d * B + (t & (B - 1))];
-                const double yv = y[ok ? i1 + t : 0];
-                p[k] += ok ? L * yv : 0.0;
-            }
-        }
-        double acc = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
-        const double rd = act ? 1.0 / cur[B - 1 - (lane & (B - 1))] : 0.0;
-        double zi = 0.0;
-        const int m = min(B, i1);
-#pragma unroll 4
-        for (int s = 0; s < m; ++s) {
-            const int d = lane - s;  // L[i1 - 1 - s][i] = cur[d * B + B - 1 - s]
-            const double L = cur[(d > 0 && d <= w ? d : 0) * B + B - 1 - s];
-            const double zs = readlane_f64((rhs - acc) * rd, s);
-            if (lane == s) zi = zs;
-            if (act && d > 0 && d <= w) acc += L * zs;
-        }
-        __syncthreads();  // every lane has read its y[i] and the solved rows above
-        if (act) y[i] = zi;
-        double* t = p2;
-        p2 = p1;
-        p1 = cur;
-        cur = t;
-    }
     __syncthreads();
-    double* out = Ainv + (long)c * n;
-    for (int i = lane; i < n; i += 64) out[i] = -y[i];
+    double* out = Y + (long)c * n;
+    for (int i = lane; i < n; i += 64) out[i] = i < c ? 0.0 : y[i];
 }
 
-__device__ void band_inverse_row(int n, int w, const double* __restrict__ Lb,
-                                 double* __restrict__ Ainv, int c, double* __restrict__ y) {
+__device__ void band_lower_row(int n, int w, const double* __restrict__ Lb,
+                               double* __restrict__ Y, int c, double* __restrict__ y) {
     const int lane = threadIdx.x;
     const int W1 = w + 1;
     for (int i = lane; i < n; i += 64) y[i] = 0.0;
@@ -593,49 +546,339 @@ __device__ void band_inverse_row(int n, int w, const double* __restrict__ Lb,
         if (lane == 0) y[i] = yi;
         __syncthreads();
     }
-    for (int i = n - 1; i >= 0; --i) {
-        double s = 0.0;
-        for (int t = lane; t < w; t += 64) {
-            const int j = i + 1 + t;
-            if (j < n) s += Lb[(long)j * W1 + (j - i)] * y[j];
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        const double zi = ((i >= c ? y[i] : 0.0) - s) / Lb[(long)i * W1];
-        __syncthreads();
-        if (lane == 0) y[i] = zi;
-        __syncthreads();
-    }
-    double* out = Ainv + (long)c * n;
-    for (int i = lane; i < n; i += 64) out[i] = -y[i];
+    double* out = Y + (long)c * n;
+    for (int i = lane; i < n; i += 64) out[i] = y[i];
 }
 
-// LDS doubles per column for (n, w): the slab variant when it fits in 64 KB
+// the slab variant when its LDS fits in 64 KB
 __host__ __device__ inline bool band_slab_ok(int n, int w) {
-    return w <= 64 && (long)(n + 3 * kSlab) * 8 <= 65536;
+    return w <= 64 && (long)(n + kSlab) * 8 <= 65536;
 }
 
-__device__ __forceinline__ void band_inverse_body(int n, int w, const double* __restrict__ Lb,
-                                  double* __restrict__ Ainv, int c, int nmax, bool slab) {
+__device__ __forceinline__ void band_lower_body(int n, int w, const double* __restrict__ Lb,
+                                                double* __restrict__ Y, int c, int nmax,
+                                                bool slab) {
     extern __shared__ double y[];
-    if (slab) band_inverse_slab(n, w, Lb, Ainv, c, y, y + nmax, y + nmax + kSlab,
-                                y + nmax + 2 * kSlab);
-    else band_inverse_row(n, w, Lb, Ainv, c, y);
+    if (slab) band_lower_slab(n, w, Lb, Y, c, y, y + nmax);
+    else band_lower_row(n, w, Lb, Y, c, y);
 }
 
 __global__ void __launch_bounds__(64)
-    band_inverse_kernel(int n, int w, const double* __restrict__ Lb, double* __restrict__ Ainv,
-                        int slab) {
-    band_inverse_body(n, w, Lb, Ainv, blockIdx.x, n, slab != 0);
+    band_lower_kernel(int n, int w, const double* __restrict__ Lb, double* __restrict__ Y,
+                      int slab) {
+    band_lower_body(n, w, Lb, Y, blockIdx.x, n, slab != 0);
 }
 
 // many factors at once: entry e owns the blocks [start[e], start[e+1]), one per column
 __global__ void __launch_bounds__(64)
-    band_inverse_batch_kernel(const BandDesc* __restrict__ D, const int* __restrict__ start,
-                              int m, int nmax, int slab) {
+    band_lower_batch_kernel(const BandDesc* __restrict__ D, const int* __restrict__ start, int m,
+                            int nmax, int slab) {
     const int e = batch_entry(start, m, blockIdx.x);
     const BandDesc& B = D[e];
-    band_inverse_body(B.n, B.w, B.Lb, B.Ainv, (int)blockIdx.x - start[e], nmax, slab != 0);
+    band_lower_body(B.n, B.w, B.Lb, B.Y, (int)blockIdx.x - start[e], nmax, slab != 0);
+}
+
+// Ainv = -Y Y^T in 64 x 64 tiles, 4 x 4 entries per thread; k runs from the
+// tile's first nonzero column (Y[i][k] = 0 for k < i) in steps of 16 through
+// LDS. Entry (i, j) and (j, i) take the same products in the same order, so
+// the result is exactly symmetric.
+constexpr int kSyrkTile = 64, kSyrkK = 16;
+
+__device__ __forceinline__ void syrk_neg_tile(int n, const double* __restrict__ Y,
+                                              double* __restrict__ Ainv, int I0, int J0) {
+    __shared__ double As[kSyrkK][kSyrkTile + 1], Bs[kSyrkK][kSyrkTile + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int kb = max(I0, J0) & ~(kSyrkK - 1); kb < n; kb += kSyrkK) {
+        for (int q = threadIdx.x; q < kSyrkTile * kSyrkK; q += 256) {
+            const int r = q / kSyrkK, kk = q % kSyrkK;
+            const int gk = kb + kk;
+            As[kk][r] = I0 + r < n && gk < n ? Y[(long)(I0 + r) * n + gk] : 0.0;
+            Bs[kk][r] = J0 + r < n && gk < n ? Y[(long)(J0 + r) * n + gk] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kSyrkK; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = As[kk][ty + 16 * u];
+                b[u] = Bs[kk][tx + 16 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = I0 + ty + 16 * u, j = J0 + tx + 16 * v;
+            if (i < n && j < n) Ainv[(long)i * n + j] = -acc[u][v];
+        }
+}
+
+__global__ void __launch_bounds__(256)
+    syrk_neg_kernel(int n, const double* __restrict__ Y, double* __restrict__ Ainv) {
+    syrk_neg_tile(n, Y, Ainv, blockIdx.y * kSyrkTile, blockIdx.x * kSyrkTile);
+}
+
+// one realisation per blockIdx.z; tiles beyond its n return
+__global__ void __launch_bounds__(256) syrk_neg_batch_kernel(const BandDesc* __restrict__ D) {
+    const BandDesc& B = D[blockIdx.z];
+    const int I0 = blockIdx.y * kSyrkTile, J0 = blockIdx.x * kSyrkTile;
+    if (I0 >= B.n || J0 >= B.n) return;
+    syrk_neg_tile(B.n, B.Y, B.Ainv, I0, J0);
+}
+
+// ---------------------------------------------------------------------------
+// Banded Cholesky factor M = L L^T of the coarsest operator M = -A, assembled
+// from K on the device. One workgroup per factor:
+//  1. every thread assembles entries of M into column storage (C, global
+//     scratch) and zeroes the factor's band entries above the matrix;
+//  2. right-looking factorization, one column j per step: column j is scaled
+//     by 1 / sqrt(M[j][j]) and written to the row-major factor, and its outer
+//     product is subtracted from the next w columns, each entry with one fused
+//     multiply-add (entry (j + dk + t, j + dk) -= L[j + dk + t][j] L[j + dk][j]),
+//     steps in ascending j.
+// Bands of w + 1 <= kFactorWin entries per column factor in an LDS window of
+// w + 2 columns (column k in slot k mod (w + 2)) with one barrier per step:
+// during step j, wave 0 updates column j + 1 and scales it at once (its pivot
+// broadcast within the wave), waves 1..15 update columns j + 2 .. j + w, and
+// the last two waves store the column entering the window (loaded a step
+// earlier, into the slot column j - 1 left) and load the next one; barriers
+// wait for LDS only, so that load stays in flight, and no wave waits for the
+// factor's global stores. Wider bands factor in place in C, two barriers per
+// step. Both variants round identically (the same operations in the same
+// order).
+// ---------------------------------------------------------------------------
+constexpr int kFactorThreads = 1024;
+constexpr int kFactorWin = 90;    // (w + 2)(w + 1) doubles of LDS: 65,520 bytes at w + 1 = 90
+constexpr int kFactorPairs = 5;   // update pairs per thread of waves 1..15: w (w - 1) / 2 <= 5 * 960
+
+// barrier for LDS data only: outstanding global loads and stores stay in flight
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// M into C (and columns 0..w into window slots 0..w when win is given). C and
+// the window are cleared first; then the thread of cell c writes column c:
+// the diagonal sum_f K_f / dx^2 (doubled at a pinned cell 0) plus -K_f / dx^2
+// for each face that wraps onto the cell itself, and -K_f / dx^2 at offset
+// q - c for each face joining c to a cell q > c (faces to the same q
+// accumulate in face order). The face coefficients are symmetric, so column
+// q's entry at c is the same sum seen from q.
+__device__ void band_assemble(const FactorDesc& D, int n, int W1, double* __restrict__ win) {
+    constexpr int nt = kFactorThreads;
+    const int tid = threadIdx.x;
+    const long nW = (long)n * W1;
+    for (long e = tid; e < nW; e += nt) D.C[e] = 0.0;
+    if (win)
+        for (int e = tid; e < W1 * W1; e += nt) win[e] = 0.0;
+    for (int e = tid; e < W1 * W1; e += nt) {  // L[i][i - d] with i - d < 0
+        const int i = e / W1, d = e - i * W1;
+        if (d > i && i < n) D.Lb[e] = 0.0;
+    }
+    __syncthreads();
+    const int nx = D.nx, ny = D.ny, nz = D.nz;
+    for (int c = tid; c < n; c += nt) {
+        const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        const double Kc = D.K[c];
+        double coef[6], asum = 0.0;
+        long nb[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            nb[d] = neighbour_coef(nx, ny, nz, D.bc, D.K, i, j, k, d, Kc, coef[d]);
+            asum += coef[d];
+        }
+        if (D.pin && c == 0) asum *= 2.0;
+        double e[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) e[d] = -coef[d] / D.dx2;
+        double diag = asum / D.dx2;
+#pragma unroll
+        for (int d = 0; d < 6; ++d)
+            if (nb[d] == c) diag += e[d];
+        double* col = D.C + (long)c * W1;
+        double* wcol = win && c < W1 ? win + c * W1 : nullptr;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            bool first = nb[d] > c;  // the first face to this neighbour writes its entry
+            for (int d2 = 0; d2 < d; ++d2) first = first && nb[d2] != nb[d];
+            if (!first) continue;
+            double acc = e[d];
+            for (int d2 = d + 1; d2 < 6; ++d2)
+                if (nb[d2] == nb[d]) acc += e[d2];
+            col[nb[d] - c] = acc;
+            if (wcol) wcol[nb[d] - c] = acc;
+        }
+        col[0] = diag;
+        if (wcol) wcol[0] = diag;
+    }
+}
+
+__device__ void band_factor_window(const FactorDesc& D, double* __restrict__ win) {
+    constexpr int nt = kFactorThreads;
+    const int n = D.nx * D.ny * D.nz;
+    const int w = D.w, W1 = w + 1, WS = w + 2;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    band_assemble(D, n, W1, win);
+    __syncthreads();
+    // waves 1..15: this thread's update pairs (dk >= 2, t), dk = 0 for none
+    int pdk[kFactorPairs], pt[kFactorPairs];
+#pragma unroll
+    for (int r = 0; r < kFactorPairs; ++r) pdk[r] = pt[r] = 0;
+    if (wave > 0) {
+        int e0 = 0;
+        for (int dk = 2; dk <= w; ++dk) {
+            const int len = W1 - dk;
+#pragma unroll
+            for (int r = 0; r < kFactorPairs; ++r) {
+                const int idx = tid - 64 + r * (nt - 64) - e0;
+                if (idx >= 0 && idx < len) {
+                    pdk[r] = dk;
+                    pt[r] = idx;
+                }
+            }
+            e0 += len;
+        }
+    }
+    // the last two waves: entry pf of the column entering the window next
+    const int pf = tid - (nt - 128);
+    const bool pfl = pf >= 0 && pf < W1;
+    double pre = 0.0;
+    if (pfl && w + 1 < n) pre = D.C[(long)(w + 1) * W1 + pf];
+    int bad = 0;
+    if (wave == 0) {  // column 0
+        const int m0 = min(w, n - 1);
+        const double d0 = win[0];
+        const double L0 = sqrt(d0);
+        if (!(d0 > 0.0)) bad = 1;
+        for (int t = lane; t <= m0; t += 64) {
+            const double v = t == 0 ? L0 : win[t] / L0;
+            if (t > 0) win[t] = v;
+            D.Lb[(long)t * W1 + t] = v;
+        }
+    }
+    lds_barrier();
+    int sj = 0;  // slot of column j
+    for (int j = 0; j < n; ++j) {
+        const int m = min(w, n - 1 - j);
+        const double* cj = win + sj * W1;
+        const int s1 = sj + 1 == WS ? 0 : sj + 1;
+        if (wave == 0) {
+            if (m >= 1) {  // column j + 1: entries t = lane and lane + 64
+                const int m1 = min(w, n - 2 - j);
+                double* c1 = win + s1 * W1;
+                const double l1 = cj[1];
+                const int t0 = lane, t1 = lane + 64;
+                double v0 = 0.0, v1 = 0.0;
+                if (t0 <= m1) {
+                    v0 = c1[t0];
+                    if (t0 <= m - 1) v0 = fma(-cj[1 + t0], l1, v0);
+                }
+                if (t1 <= m1) {
+                    v1 = c1[t1];
+                    if (t1 <= m - 1) v1 = fma(-cj[1 + t1], l1, v1);
+                }
+                const double piv = readlane_f64(v0, 0);
+                const double L = sqrt(piv);
+                if (!(piv > 0.0) && bad == 0) bad = j + 2;
+                if (t0 <= m1) {
+                    const double v = t0 == 0 ? L : v0 / L;
+                    if (t0 > 0) c1[t0] = v;
+                    D.Lb[(long)(j + 1 + t0) * W1 + t0] = v;
+                }
+                if (t1 <= m1) {
+                    const double v = v1 / L;
+                    c1[t1] = v;
+                    D.Lb[(long)(j + 1 + t1) * W1 + t1] = v;
+                }
+            }
+        } else {
+            if (pfl) {  // column j + 1 + w into the slot of column j - 1
+                if (j + 1 + w < n) win[(sj == 0 ? WS - 1 : sj - 1) * W1 + pf] = pre;
+                if (j + 2 + w < n) pre = D.C[(long)(j + 2 + w) * W1 + pf];
+            }
+            double a[kFactorPairs], l[kFactorPairs], c[kFactorPairs];
+            int at[kFactorPairs];
+            bool ok[kFactorPairs];
+#pragma unroll
+            for (int r = 0; r < kFactorPairs; ++r) {
+                const int dk = pdk[r], t = pt[r];
+                ok[r] = dk > 0 && dk + t <= m;
+                const int s = sj + dk < WS ? sj + dk : sj + dk - WS;
+                at[r] = s * W1 + t;
+                a[r] = ok[r] ? cj[dk + t] : 0.0;
+                l[r] = ok[r] ? cj[dk] : 0.0;
+                c[r] = ok[r] ? win[at[r]] : 0.0;
+            }
+#pragma unroll
+            for (int r = 0; r < kFactorPairs; ++r)
+                if (ok[r]) win[at[r]] = fma(-a[r], l[r], c[r]);
+        }
+        lds_barrier();
+        sj = s1;
+    }
+    if (tid == 0) *D.flag = bad;
+}
+
+__device__ void band_factor_global(const FactorDesc& D) {
+    constexpr int nt = kFactorThreads;
+    const int n = D.nx * D.ny * D.nz;
+    const int w = D.w, W1 = w + 1;
+    const int tid = threadIdx.x;
+    band_assemble(D, n, W1, nullptr);
+    __syncthreads();
+    int bad = 0;
+    for (int j = 0; j < n; ++j) {
+        const int m = min(w, n - 1 - j);
+        double* cj = D.C + (long)j * W1;
+        const double djj = cj[0];
+        const double Ljj = sqrt(djj);
+        if (!(djj > 0.0) && bad == 0) bad = j + 1;
+        for (int t = tid; t <= m; t += nt) {
+            const double v = t == 0 ? Ljj : cj[t] / Ljj;
+            if (t > 0) cj[t] = v;  // the pivot entry is read by every thread and stays
+            D.Lb[(long)(j + t) * W1 + t] = v;
+        }
+        __syncthreads();
+        for (int dk = 1 + tid / 64; dk <= m; dk += nt / 64) {
+            double* ck = D.C + (long)(j + dk) * W1;
+            const double l = cj[dk];
+            for (int t = tid & 63; t <= m - dk; t += 64) ck[t] = fma(-cj[dk + t], l, ck[t]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *D.flag = bad;
+}
+
+template <bool WIN>
+__device__ __forceinline__ void band_factor_body(const FactorDesc& D, double* win) {
+    if (WIN) band_factor_window(D, win);
+    else band_factor_global(D);
+}
+
+template <bool WIN>
+__global__ void __launch_bounds__(kFactorThreads) band_factor_kernel(FactorDesc D) {
+    extern __shared__ double win[];
+    band_factor_body<WIN>(D, win);
+}
+
+template <bool WIN>
+__global__ void __launch_bounds__(kFactorThreads)
+    band_factor_batch_kernel(const FactorDesc* __restrict__ D) {
+    extern __shared__ double win[];
+    band_factor_body<WIN>(D[blockIdx.x], win);
 }
 
 }  // namespace
@@ -747,10 +990,35 @@ int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double
     return MF3D_OK;
 }
 
-int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv) {
+int band_factor(hipStream_t s, const FactorDesc& D) {
+    const int W1 = D.w + 1;
+    if (W1 <= kFactorWin)
+        band_factor_kernel<true>
+            <<<1, kFactorThreads, (size_t)(W1 + 1) * W1 * sizeof(double), s>>>(D);
+    else
+        band_factor_kernel<false><<<1, kFactorThreads, 0, s>>>(D);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int band_factor_batch(hipStream_t s, const FactorDesc* D, int m, int wmax) {
+    const int W1 = wmax + 1;
+    if (W1 <= kFactorWin)
+        band_factor_batch_kernel<true>
+            <<<m, kFactorThreads, (size_t)(W1 + 1) * W1 * sizeof(double), s>>>(D);
+    else
+        band_factor_batch_kernel<false><<<m, kFactorThreads, 0, s>>>(D);
+    MF3D_HIP_TRY(hipGetLastError());
+    return MF3D_OK;
+}
+
+int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Y, double* Ainv) {
     const bool slab = band_slab_ok(n, w);
-    const size_t lds = ((size_t)n + (slab ? 3 * kSlab : 0)) * sizeof(double);
-    band_inverse_kernel<<<n, 64, lds, s>>>(n, w, Lb, Ainv, slab ? 1 : 0);
+    const size_t lds = ((size_t)n + (slab ? kSlab : 0)) * sizeof(double);
+    band_lower_kernel<<<n, 64, lds, s>>>(n, w, Lb, Y, slab ? 1 : 0);
+    MF3D_HIP_TRY(hipGetLastError());
+    const unsigned t = (unsigned)((n + kSyrkTile - 1) / kSyrkTile);
+    syrk_neg_kernel<<<dim3(t, t), 256, 0, s>>>(n, Y, Ainv);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
@@ -759,8 +1027,11 @@ int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Ainv) {
 int band_inverse_batch(hipStream_t s, const BandDesc* D, const int* start, int m, int nblocks,
                        int nmax, int wmax) {
     const bool slab = band_slab_ok(nmax, wmax);
-    const size_t lds = ((size_t)nmax + (slab ? 3 * kSlab : 0)) * sizeof(double);
-    band_inverse_batch_kernel<<<nblocks, 64, lds, s>>>(D, start, m, nmax, slab ? 1 : 0);
+    const size_t lds = ((size_t)nmax + (slab ? kSlab : 0)) * sizeof(double);
+    band_lower_batch_kernel<<<nblocks, 64, lds, s>>>(D, start, m, nmax, slab ? 1 : 0);
+    MF3D_HIP_TRY(hipGetLastError());
+    const unsigned t = (unsigned)((nmax + kSyrkTile - 1) / kSyrkTile);
+    syrk_neg_batch_kernel<<<dim3(t, t, (unsigned)m), 256, 0, s>>>(D);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }
