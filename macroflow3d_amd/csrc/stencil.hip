@@ -578,22 +578,18 @@ __global__ void __launch_bounds__(64)
     band_lower_body(B.n, B.w, B.Lb, B.Y, (int)blockIdx.x - start[e], nmax, slab != 0);
 }
 
-// Ainv = -Y Y^T in 64 x 64 tiles, 4 x 4 entries per thread; k runs from the
-// tile's first nonzero column (Y[i][k] = 0 for k < i) in steps of 16 through
-// LDS. Entry (i, j) and (j, i) take the same products in the same order, so
-// the result is exactly symmetric.
-constexpr int kSyrkTile = 64, kSyrkK = 16;
+// Ainv = -Y Y^T in 32 x 32 tiles, 2 x 2 entries per thread: only tiles with
+// I >= J are computed, each entry written to (i, j) and (j, i), so the result
+// is exactly symmetric. k runs from the tile's first nonzero column
+// (Y[i][k] = 0 for k < i) in steps of 32 through LDS.
+constexpr int kSyrkTile = 32, kSyrkK = 32;
 
 __device__ __forceinline__ void syrk_neg_tile(int n, const double* __restrict__ Y,
                                               double* __restrict__ Ainv, int I0, int J0) {
     __shared__ double As[kSyrkK][kSyrkTile + 1], Bs[kSyrkK][kSyrkTile + 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-    for (int kb = max(I0, J0) & ~(kSyrkK - 1); kb < n; kb += kSyrkK) {
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int kb = I0 & ~(kSyrkK - 1); kb < n; kb += kSyrkK) {  // I0 >= J0
         for (int q = threadIdx.x; q < kSyrkTile * kSyrkK; q += 256) {
             const int r = q / kSyrkK, kk = q % kSyrkK;
             const int gk = kb + kk;
@@ -601,32 +597,32 @@ __device__ __forceinline__ void syrk_neg_tile(int n, const double* __restrict__ 
             Bs[kk][r] = J0 + r < n && gk < n ? Y[(long)(J0 + r) * n + gk] : 0.0;
         }
         __syncthreads();
-#pragma unroll 4
+#pragma unroll 8
         for (int kk = 0; kk < kSyrkK; ++kk) {
-            double a[4], b[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                a[u] = As[kk][ty + 16 * u];
-                b[u] = Bs[kk][tx + 16 * u];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+            const double a0 = As[kk][ty], a1 = As[kk][ty + 16];
+            const double b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+            acc[0][0] = fma(a0, b0, acc[0][0]);
+            acc[0][1] = fma(a0, b1, acc[0][1]);
+            acc[1][0] = fma(a1, b0, acc[1][0]);
+            acc[1][1] = fma(a1, b1, acc[1][1]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < 2; ++v) {
             const int i = I0 + ty + 16 * u, j = J0 + tx + 16 * v;
-            if (i < n && j < n) Ainv[(long)i * n + j] = -acc[u][v];
+            if (i < n && j < n && j <= i) {
+                Ainv[(long)i * n + j] = -acc[u][v];
+                Ainv[(long)j * n + i] = -acc[u][v];
+            }
         }
 }
 
 __global__ void __launch_bounds__(256)
     syrk_neg_kernel(int n, const double* __restrict__ Y, double* __restrict__ Ainv) {
+    if (blockIdx.x > blockIdx.y) return;
     syrk_neg_tile(n, Y, Ainv, blockIdx.y * kSyrkTile, blockIdx.x * kSyrkTile);
 }
 
@@ -634,7 +630,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256) syrk_neg_batch_kernel(const BandDesc* __restrict__ D) {
     const BandDesc& B = D[blockIdx.z];
     const int I0 = blockIdx.y * kSyrkTile, J0 = blockIdx.x * kSyrkTile;
-    if (I0 >= B.n || J0 >= B.n) return;
+    if (blockIdx.x > blockIdx.y || I0 >= B.n) return;
     syrk_neg_tile(B.n, B.Y, B.Ainv, I0, J0);
 }
 
