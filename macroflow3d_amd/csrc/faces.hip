@@ -1306,10 +1306,12 @@ __device__ __forceinline__ void opz2_body(const Dims& d, const BcTypes& bc, int 
 // nx*ny, so that every x and y neighbour of a tile cell, periodic wraps
 // included, is the slot at a fixed distance: one unit-stride pass per array
 // and plane replaces the three row loads of the paired kernel. Thread tid
-// owns the tile's cell pairs tid and tid + kBlock and loads them itself (their
-// plane-k+1 values are the up neighbours); the halo pairs are dealt to the
-// threads in order. Per cell: the faces and the sum order of opz2_body.
-constexpr int kOpTile = 4 * kBlock;  // cells: two pairs per thread
+// owns the tile's cell pairs tid + m kBlock (m < kOpPairs) and loads them
+// itself (their plane-k+1 values are the up neighbours); the halo pairs are
+// dealt to the threads in order. Per cell: the faces and the sum order of
+// opz2_body.
+constexpr int kOpPairs = 2;                     // cell pairs per thread
+constexpr int kOpTile = 2 * kOpPairs * kBlock;  // cells
 constexpr int kOpLdsMaxNx = 512;
 constexpr int kOpHalo = kOpLdsMaxNx / kBlock;  // halo pairs per thread, at most
 
@@ -1337,12 +1339,12 @@ __device__ __forceinline__ void opz_lds_body(const Dims& d, const BcTypes& bc, i
     // plane cell of slot L (even L: the pair's left cell, same row)
     auto cell_of = [&](int L) { return wrapP(t0 - nx + L); };
     auto plane_off = [&](int k) { return (long)((k % nz + nz) % nz) * P; };
-    // own pairs: tile pairs tid and tid + kBlock (slots nx + 2 tid, + 2 kBlock)
-    int oi[2], oj[2];
-    bool act[2];
-    long oc[2];
+    // own pairs: tile pairs tid + m kBlock (slots nx + 2 tid + 2 m kBlock)
+    int oi[kOpPairs], oj[kOpPairs];
+    bool act[kOpPairs];
+    long oc[kOpPairs];
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
+    for (int m = 0; m < kOpPairs; ++m) {
         const long c = t0 + 2L * (tid + m * kBlock);
         act[m] = c < P;
         // pairs past the plane's end stage their slot's wrapped cells (the
@@ -1361,13 +1363,13 @@ __device__ __forceinline__ void opz_lds_body(const Dims& d, const BcTypes& bc, i
         hc_[m] = q < nh ? cell_of(L) : 0;
     }
     struct Plane {
-        dbl2 x[2], h[2], hx[kOpHalo], hh[kOpHalo], bo[2];
+        dbl2 x[kOpPairs], h[kOpPairs], hx[kOpHalo], hh[kOpHalo], bo[kOpPairs];
     };
     auto load = [&](int k, bool halo) {
         const long po = plane_off(k);
         Plane l;
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
+        for (int m = 0; m < kOpPairs; ++m) {
             l.x[m] = ld2(x, po + oc[m]);
             l.h[m] = ld2(H, po + oc[m]);
             l.bo[m] = MODE == 1 ? ld2(b, po + oc[m]) : dbl2{0.0, 0.0};
@@ -1383,7 +1385,7 @@ __device__ __forceinline__ void opz_lds_body(const Dims& d, const BcTypes& bc, i
     };
     auto stage = [&](const Plane& l) {
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
+        for (int m = 0; m < kOpPairs; ++m) {
             const int L = nx + 2 * (tid + m * kBlock);
             *reinterpret_cast<dbl2*>(lx + L) = l.x[m];
             *reinterpret_cast<dbl2*>(lh + L) = l.h[m];
@@ -1398,10 +1400,10 @@ __device__ __forceinline__ void opz_lds_body(const Dims& d, const BcTypes& bc, i
     };
     // the plane below k0 (own pairs) and the first plane
     const Side D0 = side(k0 > 0, 0, bc.t[4], 0);
-    dbl2 xm[2], fzl[2];
+    dbl2 xm[kOpPairs], fzl[kOpPairs];
     Plane cur = load(k0, true);
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
+    for (int m = 0; m < kOpPairs; ++m) {
         const long po = plane_off(k0 - 1);
         const dbl2 xd = ld2(x, po + oc[m]), hd = ld2(H, po + oc[m]);
         xm[m] = D0.nb ? xd : dbl2{0.0, 0.0};
@@ -1416,7 +1418,7 @@ __device__ __forceinline__ void opz_lds_body(const Dims& d, const BcTypes& bc, i
         __syncthreads();          // plane k staged
         const Side U = side(k < nz - 1, 0, bc.t[5], 0);
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
+        for (int m = 0; m < kOpPairs; ++m) {
             const int i = oi[m], j = oj[m];
             const int L = nx + 2 * (tid + m * kBlock);
             const Side Wt = side(i > 0, 0, bc.t[0], 0);
