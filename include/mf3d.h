@@ -33,7 +33,8 @@ enum {
     MF3D_ERR_BREAKDOWN = 3,        /* (p, Ap) zero or not finite */
     MF3D_ERR_HIP = 4,              /* a HIP runtime call failed */
     MF3D_ERR_ALLOC = 5,            /* device allocation failed */
-    MF3D_ERR_SINGULAR = 6,         /* coarsest operator not definite (direct solve) */
+    MF3D_ERR_SINGULAR = 6,         /* coarsest operator not definite (direct solve); a
+                                      single solve reports it at its first coarsest solve */
     MF3D_ERR_UNSUPPORTED = 7       /* combination not implemented */
 };
 
@@ -147,7 +148,9 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
  * colour-swept level and 16-byte aligned heads (other cases: status
  * MF3D_ERR_UNSUPPORTED). Each realisation stops on its own tolerance; results[r]
  * holds its iterations and residuals (setup_ms/solve_ms: the whole batch).
- * Synchronises. */
+ * Every coarsest operator is factored on the device; one that is not
+ * definite makes the call return MF3D_ERR_SINGULAR after the solve, with the
+ * realisation and pivot in mf3d_last_error. Synchronises. */
 enum { MF3D_BATCH_NO_GRAPH = 1 };
 int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream stream,
                             const mf3d_grid* grids, const mf3d_bc* bc,
