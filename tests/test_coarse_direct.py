@@ -3,7 +3,8 @@
 
 A one-level hierarchy makes the V-cycle exactly the coarsest solve, so the
 preconditioned vector z must solve A z = r: checked by the oracle's residual
-(||r - A z|| <= 1e-12 ||r||, backward stability of a Cholesky solve) and,
+(||r - A z|| <= 1e-13 ||r||, backward stability of a Cholesky solve; measured
+2e-14 at most) and,
 up to 1024 cells, against the oracle's own dense coarsest solve. The shapes cover both factor
 variants (band width w + 1 <= 90 in an LDS window, wider bands in place in
 global memory), the largest dense level (16^3 = 4096 cells), periodic wraps
@@ -69,7 +70,7 @@ def test_coarsest_direct_solve_solves_the_operator(ctx, shape, mix):
     M.apply(ctx, dev(r, ctx), z)
     zg = host(z)
     res = O.compute_residual_3d(og, obc, pin_for(mix), zg, r, K)
-    assert np.linalg.norm(res) <= 1e-12 * np.linalg.norm(r), (shape, mix)
+    assert np.linalg.norm(res) <= 1e-13 * np.linalg.norm(r), (shape, mix)
     if n <= 1024:  # the oracle's dense solve is O(n^3) on one core
         zo = O.v_cycle_precond(og, obc, O.SolveParams(mg_levels=1, coarse_solver=1), K, r)
         assert np.linalg.norm(zg - zo) <= 1e-10 * np.linalg.norm(zo), (shape, mix)
@@ -78,7 +79,7 @@ def test_coarsest_direct_solve_solves_the_operator(ctx, shape, mix):
     M = nm.MultigridPreconditioner.build(ctx, ws, g, bc, direct_cfg(), dev(K2, ctx))
     M.apply(ctx, dev(r, ctx), z)
     res = O.compute_residual_3d(og, obc, pin_for(mix), host(z), r, K2)
-    assert np.linalg.norm(res) <= 1e-12 * np.linalg.norm(r), (shape, mix)
+    assert np.linalg.norm(res) <= 1e-13 * np.linalg.norm(r), (shape, mix)
     ws.close()
 
 
