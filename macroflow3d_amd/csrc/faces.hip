@@ -267,6 +267,24 @@ __device__ __forceinline__ bool nb_coord(const Dims& d, const BcTypes& bc, int q
     return true;
 }
 
+
+// The FP64 Krylov vectors that the mixed-precision iteration's split kernels
+// touch (r, A p, x, p: 128 MiB each at 256^3) are each read or written once
+// per kernel, so they move with non-temporal loads and stores: that keeps
+// them from displacing the FP32 V-cycle's x, b and faces (224 MiB at 256^3)
+// from the 256 MB Infinity Cache between that V-cycle's half-sweeps.
+typedef double __attribute__((ext_vector_type(2))) ntd2;
+__device__ __forceinline__ double2 ldnt(const double* p) {
+    const ntd2 v = __builtin_nontemporal_load(reinterpret_cast<const ntd2*>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void stnt(double* p, double a, double b) {
+    ntd2 v;
+    v.x = a;
+    v.y = b;
+    __builtin_nontemporal_store(v, reinterpret_cast<ntd2*>(p));
+}
+
 template <typename TC>
 struct SFaces {
     const TC* lx;
@@ -432,8 +450,8 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
             const long ec = e < n ? e : n - 2;
             epre[u] = e;
             cpre[u] = (cw.j + cw.k) & 1;  // colour of cell 2m
-            rpre[u] = *reinterpret_cast<const double2*>(r64 + ec);
-            apre[u] = *reinterpret_cast<const double2*>(ru->Ap + ec);
+            rpre[u] = ldnt(r64 + ec);
+            apre[u] = ldnt(ru->Ap + ec);
         }
     }
 
@@ -475,9 +493,9 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     // RZ: the r of the thread's 8 natural cells for the (r, z) partial
     double2 rzr[4];
     if constexpr (RZ) {
-        const double2* rp = reinterpret_cast<const double2*>(rz->r + row * d.nx + 8L * q);
+        const double* rp = rz->r + row * d.nx + 8L * q;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) rzr[e] = rp[e];
+        for (int e = 0; e < 4; ++e) rzr[e] = ldnt(rp + 2 * e);
     }
 
     float fxm[4], fym[4], fzm[4], fxp[4], fyp[4], fzp[4];
@@ -575,7 +593,7 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
                 if (go) {
                     rv.x = -alpha * av.x + rv.x;
                     rv.y = -alpha * av.y + rv.y;
-                    *reinterpret_cast<double2*>(ru->r + e) = rv;
+                    stnt(ru->r + e, rv.x, rv.y);
                 }
                 acc += rv.x * rv.x;
                 acc += rv.y * rv.y;
@@ -1663,11 +1681,9 @@ __device__ __forceinline__ void xp_split8_body(const Dims& d, const float* __res
         const long pl = e >> 1;  // (nx / 2) row + m
         const int c0 = (cw.j + cw.k) & 1;
         const double z0 = a[(long)c0 * half + pl], z1 = a[(long)(c0 ^ 1) * half + pl];
-        double2* pp = reinterpret_cast<double2*>(p + e);
-        double2* xp = reinterpret_cast<double2*>(x + e);
-        const double2 pv = *pp, xv = *xp;
-        *xp = make_double2(alpha * pv.x + xv.x, alpha * pv.y + xv.y);
-        *pp = make_double2(1.0 * z0 + beta * pv.x, 1.0 * z1 + beta * pv.y);
+        const double2 pv = ldnt(p + e), xv = ldnt(x + e);
+        stnt(x + e, alpha * pv.x + xv.x, alpha * pv.y + xv.y);
+        stnt(p + e, 1.0 * z0 + beta * pv.x, 1.0 * z1 + beta * pv.y);
     }
 }
 __global__ void __launch_bounds__(kBlock)
