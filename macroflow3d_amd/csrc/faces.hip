@@ -268,11 +268,12 @@ __device__ __forceinline__ bool nb_coord(const Dims& d, const BcTypes& bc, int q
 }
 
 
-// The FP64 Krylov vectors that the mixed-precision iteration's split kernels
-// touch (r, A p, x, p: 128 MiB each at 256^3) are each read or written once
-// per kernel, so they move with non-temporal loads and stores: that keeps
-// them from displacing the FP32 V-cycle's x, b and faces (224 MiB at 256^3)
-// from the 256 MB Infinity Cache between that V-cycle's half-sweeps.
+// The FP64 Krylov vectors that the mixed-precision iteration's r-update sweep
+// and x/p update touch (r, A p, x, p: 128 MiB each at 256^3) are each read or
+// written once per kernel, so they move with non-temporal loads and stores:
+// that keeps them from displacing the FP32 V-cycle's x, b and faces (224 MiB
+// at 256^3) from the 256 MB Infinity Cache between that V-cycle's
+// half-sweeps. (The (r, z) sweep's read of r measured faster without it.)
 typedef double __attribute__((ext_vector_type(2))) ntd2;
 __device__ __forceinline__ double2 ldnt(const double* p) {
     const ntd2 v = __builtin_nontemporal_load(reinterpret_cast<const ntd2*>(p));
@@ -495,7 +496,7 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     if constexpr (RZ) {
         const double* rp = rz->r + row * d.nx + 8L * q;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) rzr[e] = ldnt(rp + 2 * e);
+        for (int e = 0; e < 4; ++e) rzr[e] = *reinterpret_cast<const double2*>(rp + 2 * e);
     }
 
     float fxm[4], fym[4], fzm[4], fxp[4], fyp[4], fzp[4];
