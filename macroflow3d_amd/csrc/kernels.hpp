@@ -24,9 +24,8 @@ int coarsen(hipStream_t s, const mf3d_grid& coarse, const double* Kf, double* Kc
 int rhs(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, const double* K, double* out);
 int guess(hipStream_t s, const mf3d_grid& g, const mf3d_bc& bc, int mode, double* h);
 int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double* x);
-// banded Cholesky factor of the coarsest operator, assembled from K: one
-// workgroup per factor (band_factor_batch: m factors, wmax = max band width)
-int band_factor(hipStream_t s, const FactorDesc& D);
+// banded Cholesky factors of m coarsest operators, assembled from K: one
+// workgroup per factor (wmax = max band width)
 int band_factor_batch(hipStream_t s, const FactorDesc* D, int m, int wmax);
 // A^{-1} = -L^{-T} L^{-1} from the band factor (Y: n * n scratch for L^{-1})
 int band_inverse(hipStream_t s, int n, int w, const double* Lb, double* Y, double* Ainv);

@@ -865,14 +865,8 @@ __device__ __forceinline__ void band_factor_body(const FactorDesc& D, double* wi
 }
 
 template <bool WIN>
-__global__ void __launch_bounds__(kFactorThreads) band_factor_kernel(FactorDesc D) {
-    extern __shared__ double win[];
-    band_factor_body<WIN>(D, win);
-}
-
-template <bool WIN>
 __global__ void __launch_bounds__(kFactorThreads)
-    band_factor_batch_kernel(const FactorDesc* __restrict__ D) {
+    band_factor_kernel(const FactorDesc* __restrict__ D) {
     extern __shared__ double win[];
     band_factor_body<WIN>(D[blockIdx.x], win);
 }
@@ -986,24 +980,13 @@ int dense_gemv(hipStream_t s, int n, const double* Ainv, const double* b, double
     return MF3D_OK;
 }
 
-int band_factor(hipStream_t s, const FactorDesc& D) {
-    const int W1 = D.w + 1;
-    if (W1 <= kFactorWin)
-        band_factor_kernel<true>
-            <<<1, kFactorThreads, (size_t)(W1 + 1) * W1 * sizeof(double), s>>>(D);
-    else
-        band_factor_kernel<false><<<1, kFactorThreads, 0, s>>>(D);
-    MF3D_HIP_TRY(hipGetLastError());
-    return MF3D_OK;
-}
-
 int band_factor_batch(hipStream_t s, const FactorDesc* D, int m, int wmax) {
     const int W1 = wmax + 1;
     if (W1 <= kFactorWin)
-        band_factor_batch_kernel<true>
+        band_factor_kernel<true>
             <<<m, kFactorThreads, (size_t)(W1 + 1) * W1 * sizeof(double), s>>>(D);
     else
-        band_factor_batch_kernel<false><<<m, kFactorThreads, 0, s>>>(D);
+        band_factor_kernel<false><<<m, kFactorThreads, 0, s>>>(D);
     MF3D_HIP_TRY(hipGetLastError());
     return MF3D_OK;
 }

@@ -1,17 +1,19 @@
-"""Dense coarsest solve on the device: the operator is assembled from K, factored
-(banded Cholesky, one workgroup) and inverted on the GPU.
+"""Dense coarsest solve: the band factor of the coarsest operator (host
+thread for a single solve, device kernel for a batched one) and its explicit
+inverse -L^{-T} L^{-1} on the GPU.
 
 A one-level hierarchy makes the V-cycle exactly the coarsest solve, so the
 preconditioned vector z must solve A z = r: checked by the oracle's residual
 (||r - A z|| <= 1e-13 ||r||, backward stability of a Cholesky solve; measured
-2e-14 at most) and,
-up to 1024 cells, against the oracle's own dense coarsest solve. The shapes cover both factor
-variants (band width w + 1 <= 90 in an LDS window, wider bands in place in
-global memory), the largest dense level (16^3 = 4096 cells), periodic wraps
-whose neighbours coincide (a dimension of 2) or are the cell itself (a
-dimension of 1), and the pinned singular mixes. An operator that is not
-definite (negative K) is reported as MF3D_ERR_SINGULAR by the solve, by the
-first mf3d_mg_precondition and by the batched solve."""
+2e-14 at most) and, up to 1024 cells, against the oracle's own dense coarsest
+solve. The shapes cover both inverse variants (band width w <= 64 in blocks
+staged in LDS, wider bands row by row), the largest dense level (16^3 = 4096
+cells), periodic wraps whose neighbours coincide (a dimension of 2) or are the
+cell itself (a dimension of 1), and the pinned singular mixes. The batched
+solve's device factor is checked against single solves, in its global-memory
+variant here (band 256) and its LDS window in test_batched.py. An operator
+that is not definite (negative K) is reported as MF3D_ERR_SINGULAR by the
+solve, by the first mf3d_mg_precondition and by the batched solve."""
 from __future__ import annotations
 
 import numpy as np
@@ -28,12 +30,12 @@ from macroflow3d_amd.core import BCSpec, Grid3D
 pytestmark = pytest.mark.gpu
 
 CASES = [
-    ((8, 8, 8), "darcy_x"),                 # w = 64: LDS window
+    ((8, 8, 8), "darcy_x"),                 # w = 64: inverse in LDS-staged blocks
     ((8, 8, 8), "periodic_x_dirichlet_z"),  # w = 64
     ((2, 3, 4), "all_neumann"),             # pinned, w = 6
     ((2, 2, 2), "all_periodic"),            # pinned, both x-neighbours the same cell
     ((5, 4, 1), "all_periodic"),            # z-faces wrap onto the cell itself
-    ((12, 10, 9), "all_dirichlet"),         # w = 120: factored in global memory
+    ((12, 10, 9), "all_dirichlet"),         # w = 120: inverse row by row
     ((6, 5, 7), "periodic_yz"),             # w = 180 (z wrap)
     ((16, 16, 16), "darcy_x"),              # 4096 cells, w = 256
 ]
@@ -115,4 +117,32 @@ def test_operator_that_is_not_definite_is_reported(ctx):
         ph.solve_head_batched(hs, Ks, grids, bc, bcfg, ctx, wss)
     assert e.value.status == 6 and "realisation 1" in str(e.value)
     for w in wss + [ws]:
+        w.close()
+
+
+def test_batched_wide_band_coarsest_factor_matches_single_solves(ctx):
+    """Batched solves factor their coarsest operators on the device: with a
+    16^3 coarsest level (band width 256) that is the variant working in
+    global memory (bands up to 89 use the LDS window, covered by
+    test_batched.py). Each realisation reproduces its single solve, whose
+    factor is computed on the host: same iterations, head within 1e-12."""
+    from macroflow3d_amd.kfield import lognormal_lnK
+    grids = [Grid3D(64, 64, 64), Grid3D(64, 64, 64)]
+    Ks = [torch.exp(lognormal_lnK(64, 64, 64, 2.0, 6.0, 91 + r, device=ctx.device)[0])
+          for r in range(2)]
+    bc = BCSpec.darcy_x()
+    cfg = ph.HeadSolveConfig.mixed(1e-10, 3)
+    print("batched 2 x 64^3, 3 levels (coarsest 16^3, band 256)", flush=True)
+    hs = [ctx.zeros(g.num_cells()) for g in grids]
+    wss = [ph.FlowWorkspace() for _ in grids]
+    res = ph.solve_head_batched(hs, Ks, grids, bc, cfg, ctx, wss)
+    ws1 = ph.FlowWorkspace()
+    for g, K, h, r in zip(grids, Ks, hs, res):
+        h1 = ctx.zeros(g.num_cells())
+        r1 = ph.solve_head(h1, K, g, bc, cfg, ctx, ws1)
+        assert r.converged and r.true_residual <= 1e-10 * r.bnorm
+        assert r.num_iterations == r1.num_iterations
+        d = float(torch.linalg.vector_norm(h - h1) / torch.linalg.vector_norm(h1))
+        assert d <= 1e-12, d
+    for w in wss + [ws1]:
         w.close()
