@@ -285,9 +285,11 @@ __global__ void __launch_bounds__(kBlock)
     const double alpha = st->alpha, beta = st->beta;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        const double pv = p[i];
-        x[i] = alpha * pv + x[i];
-        p[i] = 1.0 * z[i] + beta * pv;
+        // x and p are touched once here: non-temporal, to keep the
+        // V-cycle's arrays in the Infinity Cache
+        const double pv = __builtin_nontemporal_load(p + i);
+        __builtin_nontemporal_store(alpha * pv + __builtin_nontemporal_load(x + i), x + i);
+        __builtin_nontemporal_store(1.0 * z[i] + beta * pv, p + i);
     }
 }
 

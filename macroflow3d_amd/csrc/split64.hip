@@ -54,6 +54,15 @@ __device__ __forceinline__ d2 ld2(const double* __restrict__ p) {
 __device__ __forceinline__ void st2(double* __restrict__ p, double a, double b) {
     *reinterpret_cast<d2*>(p) = d2{a, b};
 }
+// r and A p of the PCG update folded into the first half-sweep are each
+// touched once per launch: non-temporal loads and stores keep them from
+// displacing the V-cycle's own arrays from the Infinity Cache
+__device__ __forceinline__ d2 ldnt2(const double* __restrict__ p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
+}
+__device__ __forceinline__ void stnt2(double* __restrict__ p, double a, double b) {
+    __builtin_nontemporal_store(d2{a, b}, reinterpret_cast<d2*>(p));
+}
 
 // one colour-split level
 struct Geo {
@@ -233,16 +242,16 @@ __device__ __forceinline__ void march_body(const Geo& G, const BcTypes& bc, int 
             const long rn = row * G.nx + 4L * q;
             d2 n01, n23;
             if (upd) {
-                n01 = ld2(ru.r + rn), n23 = ld2(ru.r + rn + 2);
-                const d2 a01 = ld2(ru.Ap + rn), a23 = ld2(ru.Ap + rn + 2);
+                n01 = ldnt2(ru.r + rn), n23 = ldnt2(ru.r + rn + 2);
+                const d2 a01 = ldnt2(ru.Ap + rn), a23 = ldnt2(ru.Ap + rn + 2);
                 if (go) {
                     n01.x = -alpha * a01.x + n01.x;
                     n01.y = -alpha * a01.y + n01.y;
                     n23.x = -alpha * a23.x + n23.x;
                     n23.y = -alpha * a23.y + n23.y;
                     if (valid) {
-                        st2(ru.r + rn, n01.x, n01.y);
-                        st2(ru.r + rn + 2, n23.x, n23.y);
+                        stnt2(ru.r + rn, n01.x, n01.y);
+                        stnt2(ru.r + rn + 2, n23.x, n23.y);
                     }
                 }
                 if (valid) {
