@@ -513,7 +513,6 @@ __device__ void band_lower_slab(int n, int w, const double* __restrict__ Lb,
         const double e = i == c ? 1.0 : 0.0;
         double yi = 0.0;
         const int m = min(B, n - i0);
-#pragma unroll 4
         for (int s = 0; s < m; ++s) {
             const int d = lane - s;
             const double L = S0[(d > 0 && d <= w ? d : 0) * B + (lane & (B - 1))];
