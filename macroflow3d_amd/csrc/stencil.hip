@@ -634,10 +634,11 @@ __global__ void __launch_bounds__(256) syrk_neg_batch_kernel(const BandDesc* __r
 }
 
 // ---------------------------------------------------------------------------
-// Banded Cholesky factor M = L L^T of the coarsest operator M = -A, assembled
-// from K on the device. One workgroup per factor:
-//  1. every thread assembles entries of M into column storage (C, global
-//     scratch) and zeroes the factor's band entries above the matrix;
+// Banded Cholesky factors M = L L^T of the coarsest operators M = -A of a
+// batched solve, assembled from K on the device. One workgroup per factor:
+//  1. the thread of each cell writes that cell's column of M into column
+//     storage (C, global scratch); the factor's band entries above the matrix
+//     are zeroed;
 //  2. right-looking factorization, one column j per step: column j is scaled
 //     by 1 / sqrt(M[j][j]) and written to the row-major factor, and its outer
 //     product is subtracted from the next w columns, each entry with one fused
