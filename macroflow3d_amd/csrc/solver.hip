@@ -8,13 +8,13 @@
 
 #include <algorithm>
 #include <chrono>
-#include <thread>
 #include <cstdlib>
 #include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -105,7 +105,7 @@ struct mf3d_workspace {
     int req_levels = 0;
     std::vector<Level> lv;
     // dense coarsest inverse
-    double* coarse_inv = nullptr;  // n*n inverse followed by the n*(w+1) band factor
+    double* coarse_inv = nullptr;  // n*n inverse, the n*(w+1) band factor, n*n for L^{-1}
     int coarse_n = 0;
     size_t coarse_cap = 0;         // elements allocated at coarse_inv
     // single solves: the coarsest K travels to the host when the set-up is
