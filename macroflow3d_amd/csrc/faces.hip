@@ -267,7 +267,6 @@ __device__ __forceinline__ bool nb_coord(const Dims& d, const BcTypes& bc, int q
     return true;
 }
 
-
 // The FP64 Krylov vectors that the mixed-precision iteration's r-update sweep
 // and x/p update touch (r, A p, x, p: 128 MiB each at 256^3) are each read or
 // written once per kernel, so they move with non-temporal loads and stores:
@@ -494,9 +493,9 @@ __device__ __forceinline__ void split4_body(const Dims& d, const BcTypes& bc, in
     // RZ: the r of the thread's 8 natural cells for the (r, z) partial
     double2 rzr[4];
     if constexpr (RZ) {
-        const double* rp = rz->r + row * d.nx + 8L * q;
+        const double2* rp = reinterpret_cast<const double2*>(rz->r + row * d.nx + 8L * q);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) rzr[e] = *reinterpret_cast<const double2*>(rp + 2 * e);
+        for (int e = 0; e < 4; ++e) rzr[e] = rp[e];
     }
 
     float fxm[4], fym[4], fzm[4], fxp[4], fyp[4], fzp[4];
