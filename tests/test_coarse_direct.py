@@ -11,7 +11,7 @@ staged in LDS, wider bands row by row), the largest dense level (16^3 = 4096
 cells), periodic wraps whose neighbours coincide (a dimension of 2) or are the
 cell itself (a dimension of 1), and the pinned singular mixes. The batched
 solve's device factor is checked against single solves, in its global-memory
-variant here (band 256) and its LDS window in test_batched.py. An operator
+variant (band 256) and its LDS window at its largest (band 88). An operator
 that is not definite (negative K) is reported as MF3D_ERR_SINGULAR by the
 solve, by the first mf3d_mg_precondition and by the batched solve."""
 from __future__ import annotations
@@ -120,19 +120,25 @@ def test_operator_that_is_not_definite_is_reported(ctx):
         w.close()
 
 
-def test_batched_wide_band_coarsest_factor_matches_single_solves(ctx):
-    """Batched solves factor their coarsest operators on the device: with a
-    16^3 coarsest level (band width 256) that is the variant working in
-    global memory (bands up to 89 use the LDS window, covered by
-    test_batched.py). Each realisation reproduces its single solve, whose
-    factor is computed on the host: same iterations, head within 1e-12."""
+@pytest.mark.parametrize("shape,levels", [((64, 64, 64), 3), ((64, 88, 24), 4)],
+                         ids=["coarsest16cubed_band256", "coarsest8x11x3_band88"])
+def test_batched_coarsest_factor_matches_single_solves(ctx, shape, levels):
+    """Batched solves factor their coarsest operators on the device: a 16^3
+    coarsest level (band width 256) takes the variant working in global
+    memory, an 8 x 11 x 3 one (band 88) the LDS window at its largest
+    (w + 1 = 89 of at most 90, 64,080 bytes). Each realisation reproduces its
+    single solve, whose factor is computed on the host: same iterations,
+    head within 1e-12."""
     from macroflow3d_amd.kfield import lognormal_lnK
-    grids = [Grid3D(64, 64, 64), Grid3D(64, 64, 64)]
-    Ks = [torch.exp(lognormal_lnK(64, 64, 64, 2.0, 6.0, 91 + r, device=ctx.device)[0])
+    from macroflow3d_amd.multigrid import hierarchy_grids
+    grids = [Grid3D(*shape), Grid3D(*shape)]
+    cg = hierarchy_grids(grids[0], levels)[-1]
+    assert cg.nx * cg.ny == (256 if levels == 3 else 88)  # the band width (no wraps)
+    Ks = [torch.exp(lognormal_lnK(*shape, 2.0, 6.0, 91 + r, device=ctx.device)[0])
           for r in range(2)]
     bc = BCSpec.darcy_x()
-    cfg = ph.HeadSolveConfig.mixed(1e-10, 3)
-    print("batched 2 x 64^3, 3 levels (coarsest 16^3, band 256)", flush=True)
+    cfg = ph.HeadSolveConfig.mixed(1e-10, levels)
+    print(f"batched 2 x {shape}, {levels} levels", flush=True)
     hs = [ctx.zeros(g.num_cells()) for g in grids]
     wss = [ph.FlowWorkspace() for _ in grids]
     res = ph.solve_head_batched(hs, Ks, grids, bc, cfg, ctx, wss)
