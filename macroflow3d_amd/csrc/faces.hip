@@ -1031,33 +1031,39 @@ __global__ void __launch_bounds__(kBlock)
     const int k = (int)(q / ny);
     const long sy = nx, sz = (long)nx * ny;
     const long so = split_of(d).at(d, i, j, k);
+    // every load first, from indices that are in range whether or not the
+    // face uses them (the lower neighbour or the periodic wrap; the upper wrap
+    // at the last cell of a line, the cell itself elsewhere), so that they
+    // are in flight together
     const double Kc = K[o];
+    const double Kw = K[i > 0 ? o - 1 : o + nx - 1];
+    const double Ks = K[j > 0 ? o - sy : o + (long)(ny - 1) * sy];
+    const double Kd = K[k > 0 ? o - sz : o + (long)(nz - 1) * sz];
+    const double Ke = K[i == nx - 1 ? o - (nx - 1) : o];
+    const double Kn = K[j == ny - 1 ? o - (long)(ny - 1) * sy : o];
+    const double Ku = K[k == nz - 1 ? o - (long)(nz - 1) * sz : o];
     const double iKc = 1.0 / Kc;
     if (HK) H[o] = 0.5 * iKc;
     double f;
-    if (i > 0) f = 2.0 / (iKc + 1.0 / K[o - 1]);
-    else f = bc.t[0] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + nx - 1]) : bface(bc.t[0], Kc);
+    if (i > 0) f = 2.0 / (iKc + 1.0 / Kw);
+    else f = bc.t[0] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / Kw) : bface(bc.t[0], Kc);
     cstore(LX, so, f * cs);
-    if (j > 0) f = 2.0 / (iKc + 1.0 / K[o - sy]);
-    else f = bc.t[2] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(ny - 1) * sy])
-                                         : bface(bc.t[2], Kc);
+    if (j > 0) f = 2.0 / (iKc + 1.0 / Ks);
+    else f = bc.t[2] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / Ks) : bface(bc.t[2], Kc);
     cstore(LY, so, f * cs);
-    if (k > 0) f = 2.0 / (iKc + 1.0 / K[o - sz]);
-    else f = bc.t[4] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o + (long)(nz - 1) * sz])
-                                         : bface(bc.t[4], Kc);
+    if (k > 0) f = 2.0 / (iKc + 1.0 / Kd);
+    else f = bc.t[4] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / Kd) : bface(bc.t[4], Kc);
     cstore(LZ, so, f * cs);
     if (i == nx - 1) {
-        f = bc.t[1] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (nx - 1)]) : bface(bc.t[1], Kc);
+        f = bc.t[1] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / Ke) : bface(bc.t[1], Kc);
         cstore(UX, j + (long)ny * k, f * cs);
     }
     if (j == ny - 1) {
-        f = bc.t[3] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(ny - 1) * sy])
-                                        : bface(bc.t[3], Kc);
+        f = bc.t[3] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / Kn) : bface(bc.t[3], Kc);
         cstore(UY, i + (long)nx * k, f * cs);
     }
     if (k == nz - 1) {
-        f = bc.t[5] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / K[o - (long)(nz - 1) * sz])
-                                        : bface(bc.t[5], Kc);
+        f = bc.t[5] == MF3D_BC_PERIODIC ? 2.0 / (iKc + 1.0 / Ku) : bface(bc.t[5], Kc);
         cstore(UZ, i + (long)nx * j, f * cs);
     }
 }
