@@ -484,6 +484,16 @@ int grow_pinned(double** p, size_t* cap, size_t need) {
     return MF3D_OK;
 }
 
+// a pending inverse that no solve will use: s waits for the coarse stream's
+// writes to the workspace before it reuses the memory
+int discard_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
+    if (!ws->coarse_pending) return MF3D_OK;
+    ws->coarse_pending = false;
+    ws->join_coarse();
+    if (ws->inv_ev) MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->inv_ev, 0));
+    return MF3D_OK;
+}
+
 // Single solve: start_coarse_inverse enqueues the download of the coarsest K
 // (after the coarsening kernels) and starts the worker thread, which waits for
 // the download, factors, uploads the factor and launches the inverse on the
@@ -494,10 +504,17 @@ int grow_pinned(double** p, size_t* cap, size_t need) {
 int start_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
     const int n = (int)ws->lv.back().n;
     if (n > 4096) return fail(MF3D_ERR_UNSUPPORTED, "dense coarsest solve limited to 4096 cells");
-    ws->join_coarse();  // a set-up not followed by a solve
+    // a set-up not followed by a solve: s waits for its inverse, so that the
+    // download below (kc_ev) completes after that inverse's upload from
+    // host_band, which the new worker overwrites once kc_ev has completed
+    MF3D_TRY(discard_coarse_inverse(ws, s));
     const int w = coarse_bandwidth(ws);
     MF3D_TRY(reserve_coarse(ws, n, w));
-    // the previous upload from host_band was waited for by the previous solve
+    // an earlier consumed inverse's upload from host_band was waited for by the
+    // stream work that consumed it, which precedes the download below; growing
+    // host_band frees it, so then no upload from it may be in flight at all
+    if (ws->coarse_stream && ws->host_band_cap < (size_t)n * (w + 1))
+        MF3D_HIP_TRY(hipStreamSynchronize(ws->coarse_stream));
     MF3D_TRY(grow_pinned(&ws->host_kc, &ws->host_kc_cap, (size_t)n));
     MF3D_TRY(grow_pinned(&ws->host_band, &ws->host_band_cap, (size_t)n * (w + 1)));
     if (!ws->kc_ev) MF3D_HIP_TRY(hipEventCreateWithFlags(&ws->kc_ev, hipEventDisableTiming));
@@ -542,16 +559,6 @@ int finish_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
     ws->join_coarse();
     if (ws->coarse_status != MF3D_OK) return fail(ws->coarse_status, ws->coarse_msg);
     MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->inv_ev, 0));
-    return MF3D_OK;
-}
-
-// a pending inverse that no solve will use: s waits for the coarse stream's
-// writes to the workspace before it reuses the memory
-int discard_coarse_inverse(mf3d_workspace* ws, hipStream_t s) {
-    if (!ws->coarse_pending) return MF3D_OK;
-    ws->coarse_pending = false;
-    ws->join_coarse();
-    if (ws->inv_ev) MF3D_HIP_TRY(hipStreamWaitEvent(s, ws->inv_ev, 0));
     return MF3D_OK;
 }
 

@@ -85,6 +85,34 @@ def test_coarsest_direct_solve_solves_the_operator(ctx, shape, mix):
     ws.close()
 
 
+def test_setup_whose_inverse_is_never_used_then_a_new_setup(ctx):
+    """A set-up whose coarsest inverse no application consumes, then another
+    set-up on the same workspace: the same operator size, a wider band on the
+    same grid (a z wrap: the pinned band staging grows while the unused
+    inverse's upload may still be in flight), a larger grid and a smaller one.
+    The inverse then applied solves the new operator (residual <= 1e-13)."""
+    ws = ph.FlowWorkspace()
+    steps = [((8, 8, 8), "darcy_x"), ((8, 8, 8), "darcy_x"), ((8, 8, 8), "periodic_yz"),
+             ((16, 16, 16), "darcy_x"), ((6, 5, 7), "periodic_yz")]
+    for t, ((shape, mix), nxt) in enumerate(zip(steps, steps[1:])):
+        print(f"unused set-up {shape} {mix}, then {nxt}", flush=True)
+        g, _ = grid_pair(*shape, dx=0.5)
+        bc, _ = bc_pair(mix)
+        nm.MultigridPreconditioner.build(ctx, ws, g, bc, direct_cfg(),
+                                         dev(random_K(g.num_cells(), 90 + t, 1.5), ctx))
+        g, og = grid_pair(*nxt[0], dx=0.5)
+        bc, obc = bc_pair(nxt[1])
+        n = g.num_cells()
+        K = random_K(n, 95 + t, 1.5)
+        M = nm.MultigridPreconditioner.build(ctx, ws, g, bc, direct_cfg(), dev(K, ctx))
+        r = np.random.default_rng(96 + t).standard_normal(n)
+        z = ctx.empty(n)
+        M.apply(ctx, dev(r, ctx), z)
+        res = O.compute_residual_3d(og, obc, pin_for(nxt[1]), host(z), r, K)
+        assert np.linalg.norm(res) <= 1e-13 * np.linalg.norm(r), nxt
+    ws.close()
+
+
 def test_operator_that_is_not_definite_is_reported(ctx):
     g = Grid3D(8, 8, 8)
     bc = BCSpec.darcy_x()
