@@ -170,6 +170,10 @@ struct mf3d_workspace {
     // finest-level operator and preconditioner applications enqueued since
     // the last single solve began (replays of a captured iteration included)
     long n_op = 0, n_pc = 0;
+    // partials of ||b - A x||^2 that a passing true-residual check left in
+    // `partials` for the solve's final x (0: none), so that the reported true
+    // residual needs no second operator pass
+    unsigned true_rr_np = 0;
     // aligned stand-in for a head vector that is not 16-byte aligned
     double* hcopy = nullptr;
     size_t hcopy_n = 0;
@@ -1114,7 +1118,10 @@ int run_pcg(mf3d_workspace* ws, hipStream_t s, const mf3d_grid& g,
         MF3D_TRY(residual_op(ws, s, x, ws->rhs, ws->r, ws->partials, &npo));
         MF3D_TRY(kern::finalize(s, ws->partials, npo, FIN_RESUME, ws->st, nullptr));
         MF3D_TRY(read_state(ws, s));
-        if (ws->host_st->stop == 1) break;  // the true residual meets the tolerance
+        if (ws->host_st->stop == 1) {  // the true residual meets the tolerance
+            ws->true_rr_np = npo;
+            break;
+        }
         ++restarts;
     }
     const IterState& st = *ws->host_st;
@@ -1718,6 +1725,7 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     MF3D_HIP_TRY(hipEventRecord(ws->ev[1], s));
     ws->op_pairs = op_pairs_for(ws, h);
     int st = MF3D_OK;
+    ws->true_rr_np = 0;
     if (cfg->solver_type == MF3D_SOLVER_PCG_MG)
         st = run_pcg(ws, s, *grid, *cfg, K, h, result);
     else if (cfg->solver_type == MF3D_SOLVER_CG)
@@ -1727,8 +1735,10 @@ int mf3d_solve_head(mf3d_workspace* ws, mf3d_stream stream, const mf3d_grid* gri
     if (st != MF3D_OK) return st;
     MF3D_HIP_TRY(hipEventRecord(ws->ev[2], s));
     if (cfg->true_residual) {
-        unsigned npt = 0;
-        MF3D_TRY(residual_op(ws, s, h, ws->rhs, nullptr, ws->partials, &npt));
+        // the same partials, from the same operator pass over this h, when the
+        // solve ended on a passing true-residual check
+        unsigned npt = ws->true_rr_np;
+        if (!npt) MF3D_TRY(residual_op(ws, s, h, ws->rhs, nullptr, ws->partials, &npt));
         MF3D_TRY(kern::finalize(s, ws->partials, npt, FIN_SQRT, nullptr, ws->dscalar));
         MF3D_HIP_TRY(hipMemcpyAsync(&result->true_residual, ws->dscalar, sizeof(double),
                                     hipMemcpyDeviceToHost, s));
