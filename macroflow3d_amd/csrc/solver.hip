@@ -1495,7 +1495,9 @@ int capture_iteration(const BPlan& P, hipStream_t s, GraphHolder& G) {
 }
 
 // PCG of run_pcg for all realisations (FP32 preconditioner: convergence is
-// decided on the true residual, with up to 3 restarts)
+// decided on the true residual, with up to 3 restarts). It returns after a
+// true-residual pass over every realisation's final x, which leaves that
+// residual's partial sums in each workspace.
 int run_pcg_batch(mf3d_workspace* const* ws, int count, hipStream_t s,
                   const mf3d_head_solve_config& cfg, const BPlan& P, bool use_graph,
                   std::vector<int>& restarts) {
@@ -1836,8 +1838,10 @@ int mf3d_solve_head_batched(int count, mf3d_workspace* const* ws, mf3d_stream st
     }
     MF3D_HIP_TRY(hipEventRecord(ev[2], s));
     if (cfg->true_residual) {
+        // run_pcg_batch ends on a true-residual pass over every realisation's
+        // final x, whose partial sums are still in place: only the second
+        // stage runs again
         const BatchVec* V = static_cast<const BatchVec*>(P.vec.desc);
-        MF3D_TRY(kern::batch_op(s, 1, V, P.vec.ostart, P.vec.m, P.vec.onblocks, P.op_lds));
         MF3D_TRY(kern::batch_finalize(s, V, P.vec.m, FIN_SQRT, 0, true));
         for (int r = 0; r < count; ++r)
             MF3D_HIP_TRY(hipMemcpyAsync(&results[r].true_residual, ws[r]->dscalar, sizeof(double),
