@@ -87,7 +87,8 @@ typedef struct {
     int stop_norm;         /* MF3D_STOP_* */
     int precond_precision; /* MF3D_PRECOND_* */
     int coef_format;       /* MF3D_COEF_* (FP32 preconditioner only) */
-    int true_residual;     /* 1: recompute ||b - A h|| in FP64 after the solve */
+    int true_residual;     /* 1: report ||b - A h|| of the returned h in FP64 (the sums of a
+                              final true-residual check when the solve ended on one) */
     int flexible;          /* 1: flexible-CG beta (r_new, z_new - z_old) / (r_old, z_old) */
     int reserved[4];
 } mf3d_head_solve_config;
